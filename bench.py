@@ -1,0 +1,231 @@
+"""Benchmark of the FPI + EFE hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Workloads (seeded synthetic data, generated on the device):
+  plain_fpi_large (default)  1024 agents, S=[16,16,8,8], 6 modalities x 32
+                             outcomes, A Dirichlet(0.5), 16 sweeps: one step =
+                             one fused infer_states launch for all agents.
+  plain_efe                  64 agents, S=[32,16,8], U=[8,4,1], horizon 3
+                             (32768 policies), 4 x 64 outcomes: one step =
+                             one infer_policies (neg-EFE + q_pi) for all agents.
+
+Prints ONE JSON line with value = agent-steps/s over the K timed steps,
+the roofline of the dominant kernel (HIP events on the launch stream) and a
+CPU baseline timed on a bounded sample on this host.
+
+    python bench.py --calibrate-copy   # plain device copy of 1 GiB, for the
+                                       # FETCH_SIZE / WRITE_SIZE scale factor
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def _args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="plain_fpi_large", choices=["plain_fpi_large", "plain_efe"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--calibrate-copy", action="store_true")
+    return ap.parse_args()
+
+
+def _traffic(key: str):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh).get(key)
+    except (OSError, ValueError):
+        return None
+
+
+def calibrate_copy():
+    import torch
+    n = 1 << 28  # 1 GiB of float32
+    x = torch.ones(n, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 10
+    print(json.dumps({"calibration": "device_copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "ms": dt * 1e3, "GBps": 8 * n / dt / 1e9}))
+
+
+def bench_plain_fpi_large(a, torch):
+    import numpy as np
+
+    from pymdp_amd import workloads
+    from pymdp_amd.ffi import infer_states_ffi
+
+    dev = torch.device("cuda:0")
+    w = workloads.plain_fpi_large(seed=0, device=dev)
+    B, S, O, deps = w["batch"], w["S"], w["O"], w["A_dependencies"]
+    obs = torch.stack([o.to(torch.int32) for o in w["obs"]], -1).contiguous()
+    out = torch.empty((B, sum(S)), device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        return infer_states_ffi(w["A"], obs, w["D"], deps, w["num_iter"], distr_obs=False, out=out)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(a.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_events = e0.elapsed_time(e1) / a.steps
+    ms_step = wall * 1e3 / a.steps
+    # algorithmic bytes of one launch: observed A rows + obs indices + prior in + q out
+    K = int(np.prod(S))
+    per_agent = len(O) * K * 4 + len(O) * 4 + 2 * sum(S) * 4
+    alg_bytes = B * per_agent
+    achieved = alg_bytes / (ms_events * 1e-3) / 1e9
+
+    # parity on sampled agents against the oracle (FP32 reference path and FP64)
+    from oracle import numpy_ref as R
+    q = step()
+    torch.cuda.synchronize()
+    e32 = e64 = 0.0
+    for b in (0, B // 2, B - 1):
+        A_b = [x[b].cpu().numpy() for x in w["A"]]
+        ob = [np.eye(o, dtype=np.float32)[int(x[b])] for o, x in zip(O, w["obs"])]
+        D_b = [d[b].cpu().numpy() for d in w["D"]]
+        r32 = R.run_factorized_fpi(A_b, ob, D_b, deps, w["num_iter"], np.float32)
+        r64 = R.run_factorized_fpi(A_b, ob, D_b, deps, w["num_iter"], np.float64)
+        for f in range(len(S)):
+            g = q[f][b].cpu().numpy().astype(np.float64)
+            e32 = max(e32, float(np.abs(g - r32[f]).max()))
+            e64 = max(e64, float(np.abs(g - r64[f]).max()))
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline_fpi(w, a.cpu_seconds)
+    res = {
+        "metric": "agent-steps/s (plain FPI, 1024 agents, S=[16,16,8,8], 6x32 outcomes, 16 sweeps)",
+        "value": B / (ms_step * 1e-3),
+        "unit": "agent-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 data, f64 sweep accumulation",
+        "data": "synthetic (seeded Dirichlet A, sampled observations; generated on device)",
+        "config": {"workload": "plain_fpi_large", "agents": B, "num_states": S, "num_obs": O,
+                   "num_iter": w["num_iter"], "obs": "index (one-hot)"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("fpi_kernel plain_fpi_large"),
+                     "kernel": "fpi_kernel", "kernel_ms": ms_events, "alg_bytes_per_launch": alg_bytes},
+        "parity": {"agents_checked": 3, "max_abs_err_vs_fp32_ref": e32, "max_abs_err_vs_fp64": e64},
+        "cpu_baseline": cpu,
+    }
+    return res
+
+
+def cpu_baseline_fpi(w, seconds):
+    """The oracle's C/OpenMP restatement of the reference CPU FFI FPI, on a
+    bounded sample (the first 64 agents, repeated for ~`seconds`)."""
+    import numpy as np
+
+    from oracle import cpu_ref
+    n = 64
+    A = [x[:n].cpu().numpy() for x in w["A"]]
+    D = [d[:n].cpu().numpy() for d in w["D"]]
+    obs = np.stack([o[:n].cpu().numpy() for o in w["obs"]], 1).astype(np.int32)
+    threads = len(os.sched_getaffinity(0))
+    cpu_ref.infer_states(A, obs, D, w["A_dependencies"], w["num_iter"], threads)  # warm-up
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        cpu_ref.infer_states(A, obs, D, w["A_dependencies"], w["num_iter"], threads)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": reps * n / el, "unit": "agent-steps/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} agents of the workload, {reps} repetitions, {el:.1f} s, "
+                      f"oracle/cpu_baseline.c (OpenMP over agents, -O3 -march=x86-64-v3)"}
+
+
+def bench_plain_efe(a, torch):
+    import numpy as np
+
+    from oracle import numpy_ref as R
+    from pymdp_amd import workloads
+    from pymdp_amd.ffi._efe import update_posterior_policies_ffi
+
+    dev = torch.device("cuda:0")
+    w = workloads.plain_efe(seed=0, device=dev)
+    pm = R.construct_policies(w["S"], w["U"], w["T"])
+    B = w["batch"]
+
+    def step():
+        return update_posterior_policies_ffi(pm, w["qs"], w["A"], w["B"], w["C"], None, w["A_dependencies"],
+                                             w["B_dependencies"], gamma=w["gamma"])
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / a.steps
+    return {
+        "metric": "agent-steps/s (plain EFE, 64 agents, 32768 policies)",
+        "value": B / (ms_step * 1e-3), "unit": "agent-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain_efe", "agents": B, "num_states": w["S"], "num_controls": w["U"],
+                   "horizon": w["T"], "num_obs": w["O"], "num_policies": int(pm.shape[0])},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
+def main():
+    a = _args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
+              "multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if a.calibrate_copy:
+        calibrate_copy()
+        return
+    if a.workload == "plain_fpi_large":
+        res = bench_plain_fpi_large(a, torch)
+    else:
+        res = bench_plain_efe(a, torch)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,188 @@
+/*
+ * pymdp_hip.h - C ABI of the MI355X (gfx950) FPI + EFE hot-path library
+ * (libpymdp_hip.so).
+ *
+ * Every entry point takes plain device pointers, int64 attribute arrays that
+ * live in HOST memory, and a hipStream_t (passed as void*; NULL = the null
+ * stream).  All work is enqueued on that stream; no entry point synchronises
+ * the stream or allocates on the warm path (device scratch is grown once per
+ * shape and reused).
+ *
+ * Return value: 0 on success, PYMDP_HIP_EINVAL for an invalid argument
+ * (the reference's kInvalidArgument), PYMDP_HIP_EDEVICE for a HIP runtime
+ * error.  pymdp_hip_last_error() returns the message of the most recent
+ * failure on the calling thread, prefixed "fpi_ffi: " or "efe_ffi: " like
+ * the reference's ffi::Error messages (pymdp/ffi/common/error_helpers.h:24-27).
+ *
+ * Layout conventions follow the reference's ragged flat-buffer ABI
+ * (pymdp/ffi/registration/xla_register.cc:30-54 and :118-127):
+ *   - per-agent buffers are agent-major: [batch, per_agent_count]
+ *   - *_offsets arrays are prefix sums of length N+1
+ *   - dependency lists are (flat, offsets) pairs of int64
+ */
+#ifndef PYMDP_HIP_H_
+#define PYMDP_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYMDP_HIP_OK 0
+#define PYMDP_HIP_EINVAL 1
+#define PYMDP_HIP_EDEVICE 2
+
+/* Largest dependency rank of one modality / factor (reference
+ * pymdp/ffi/common/modality_dispatch.h:25, kMaxFfiDependencyRank). */
+#define PYMDP_HIP_MAX_DEP_RANK 8
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* pymdp_hip_last_error(void);
+
+/* Library version string and gfx target it was built for. */
+const char* pymdp_hip_version(void);
+
+/*
+ * Generalised fused FPI from log-likelihoods.
+ * Replaces the XLA FFI target `pymdp_fpi` / `pymdp_fpi_cuda_native`
+ * (pymdp/ffi/registration/xla_register.cc:118-158, handler FpiCpu /
+ * FpiCudaDevice, pymdp/ffi/fpi/fpi.h:18-35).
+ *
+ *   ll_flat  [batch, ll_offsets[M]]   per-modality log-likelihood, each ll_m
+ *                                     row-major over S[A_dep[m][0]], ...
+ *   lp_flat  [batch, lp_offsets[F]]   log prior per factor
+ *   q_out    [batch, lp_offsets[F]]   posterior marginals (softmax of log_q)
+ *   ll_count, lp_count, q_count       element counts of the three buffers;
+ *                                     batch = lp_count / lp_offsets[F]
+ *                                     (pymdp/ffi/fpi/fpi_entry.h:55-66)
+ *   S[F], ll_offsets[M+1], lp_offsets[F+1], A_dep_flat[...],
+ *   A_dep_offsets[M+1]                the reference's five int64 attributes
+ *   num_iter                          Jacobi sweeps (always all of them: the
+ *                                     reference JAX path has no early stop)
+ */
+int pymdp_hip_fpi(const float* ll_flat, const float* lp_flat, float* q_out,
+                  int64_t ll_count, int64_t lp_count, int64_t q_count,
+                  const int64_t* S, int64_t F,
+                  const int64_t* ll_offsets, int64_t M,
+                  const int64_t* lp_offsets,
+                  const int64_t* A_dep_flat, const int64_t* A_dep_offsets,
+                  int32_t num_iter, void* stream);
+
+/*
+ * Fused state inference: log-likelihood build + FPI in one launch.
+ * Replaces the body of inference.update_posterior_states's one-step `fpi`
+ * branch (pymdp/inference.py:171-191 -> pymdp/algos.py:88-155), i.e.
+ * compute_log_likelihood_per_modality (pymdp/maths.py:266-287) followed by
+ * the `pymdp_fpi` kernel.
+ *
+ *   A_ptrs[M]          device pointer of A[m], each [batch?, O_m, S_deps...]
+ *   A_batch_stride[M]  elements between consecutive agents' A[m]
+ *                      (O_m * K_m for per-agent A, 0 for an A shared by all)
+ *   obs                obs_is_index == 0: float [batch, obs_offsets[M]]
+ *                      distributional observations (distr_obs=True);
+ *                      obs_is_index == 1: int32 [batch, M] indices, treated
+ *                      as jax.nn.one_hot(index, O_m) (pymdp/agent.py:659-673)
+ *   prior_ptrs[F]      device pointer of each factor's prior probabilities
+ *   prior_batch_stride[F]  elements between consecutive agents' priors
+ *                      (log_stable is applied in-kernel, pymdp/algos.py:110)
+ *   q_out    [batch, lp_offsets[F]]  posterior marginals
+ *   O[M], obs_offsets[M+1]           observation sizes and prefix sums
+ *   method_exact       1 -> num_iter forced to 1 (pymdp/inference.py:182)
+ * With F == 1 the closed form softmax(sum_m ll_m + ln prior) is returned
+ * regardless of num_iter (pymdp/algos.py:101-104).  num_iter == 0 with F > 1
+ * returns uniform marginals (a zero-length scan).
+ */
+int pymdp_hip_infer_states(const float* const* A_ptrs, const int64_t* A_batch_stride,
+                           const void* obs, int32_t obs_is_index,
+                           const float* const* prior_ptrs, const int64_t* prior_batch_stride,
+                           float* q_out, int64_t batch,
+                           const int64_t* S, int64_t F,
+                           const int64_t* O, int64_t M,
+                           const int64_t* obs_offsets, const int64_t* lp_offsets,
+                           const int64_t* A_dep_flat, const int64_t* A_dep_offsets,
+                           int32_t num_iter, int32_t method_exact, void* stream);
+
+/* Flag bits of the neg-EFE entries; identical to the reference
+ * (pymdp/ffi/neg_efe/neg_efe_layout.h:23-42, pymdp/ffi/_efe.py:26-37). */
+#define PYMDP_HIP_FLAG_USE_UTILITY (1 << 0)
+#define PYMDP_HIP_FLAG_USE_STATES_INFO_GAIN (1 << 1)
+#define PYMDP_HIP_FLAG_USE_INDUCTIVE (1 << 2)
+#define PYMDP_HIP_FLAG_USE_PARAM_INFO_GAIN (1 << 3) /* rejected: not implemented */
+#define PYMDP_HIP_FLAG_MODEL_PARAMS_STATIC (1 << 4) /* accepted, no effect */
+
+/*
+ * Negative expected free energy of every policy, optionally with the policy
+ * posterior.  Replaces update_posterior_policies[_inductive]
+ * (pymdp/control.py:198-349, :795-963) = vmap of compute_neg_efe_policy
+ * [_inductive] (pymdp/control.py:604-793) + softmax(gamma G + ln E).
+ *
+ *   policy_host [P, T, F] int32 HOST array (policies are static agent
+ *                      state; the action-history plan built from it is
+ *                      cached by full content hash)
+ *   qs_flat [batch, sum S]            current beliefs qs_init
+ *   A_ptrs[M] / A_batch_stride[M]     A[m] [batch?, O_m, S_deps...]
+ *   B_ptrs[F] / B_batch_stride[F]     B[f] [batch?, S_f, S_Bdeps..., U_f]
+ *   C_ptrs[M] / C_batch_stride[M]     C[m] [batch?, O_m] (C_has_time[m]=0)
+ *                                     or [batch?, T, O_m] (C_has_time[m]=1)
+ *   E [batch, P] or NULL (uniform 1/P), E_batch_stride
+ *   gamma [batch] device or NULL (then gamma_scalar)
+ *   I_ptrs[F], I_batch_stride[F], I_depths[F]   inductive matrices
+ *                                     [depth_f, S_f]; read only with
+ *                                     PYMDP_HIP_FLAG_USE_INDUCTIVE
+ *   inductive_epsilon device pointer holding epsilon_count (1 or batch)
+ *                     values, or NULL (then epsilon_scalar)
+ *   G_out [batch, P]                  neg_efe
+ *   q_pi_out [batch, P] or NULL       softmax(gamma G + log_stable(E))
+ *   S[F], O[M], U[F], A/B dependency (flat, offsets)    model shape
+ *   flags                             PYMDP_HIP_FLAG_* bits
+ */
+int pymdp_hip_neg_efe(const int32_t* policy_host, int64_t P, int64_t T,
+                      const float* qs_flat,
+                      const float* const* A_ptrs, const int64_t* A_batch_stride,
+                      const float* const* B_ptrs, const int64_t* B_batch_stride,
+                      const float* const* C_ptrs, const int64_t* C_batch_stride,
+                      const int64_t* C_has_time,
+                      const float* E, int64_t E_batch_stride,
+                      const float* gamma, float gamma_scalar,
+                      const float* const* I_ptrs, const int64_t* I_batch_stride,
+                      const int64_t* I_depths,
+                      const float* inductive_epsilon, int64_t epsilon_count, float epsilon_scalar,
+                      float* G_out, float* q_pi_out,
+                      int64_t batch,
+                      const int64_t* S, int64_t F, const int64_t* O, int64_t M,
+                      const int64_t* U,
+                      const int64_t* A_dep_flat, const int64_t* A_dep_offsets,
+                      const int64_t* B_dep_flat, const int64_t* B_dep_offsets,
+                      int32_t flags, void* stream);
+
+/*
+ * The reference's packed neg-EFE boundary, `pymdp_neg_efe_all_policies`
+ * (pymdp/ffi/registration/xla_register.cc:30-64, pymdp/ffi/neg_efe/neg_efe.h:54-59,
+ * packing at pymdp/ffi/_efe.py:583-711): ragged flat buffers with the 13
+ * int64 attribute spans and the flags word.  Buffers are agent-major
+ * [batch, per-agent size]; C is the (T, O_m)-broadcast form; the output is
+ * neg_efe [batch, P] (the caller applies gamma, ln E and the softmax, as in
+ * the reference).  policy_host is the [P, T, F] int32 policy matrix in host
+ * memory (identical for every agent).
+ */
+int pymdp_hip_neg_efe_all_policies(const int32_t* policy_host, const float* qs_flat,
+                                   const float* A_flat, const float* B_flat,
+                                   const float* C_flat, const float* I_flat,
+                                   const float* inductive_epsilon, int64_t epsilon_count,
+                                   float* neg_efe_out, int64_t batch,
+                                   int64_t P, int64_t T,
+                                   const int64_t* S, int64_t F, const int64_t* O, int64_t M,
+                                   const int64_t* U,
+                                   const int64_t* qs_offsets, const int64_t* A_offsets,
+                                   const int64_t* B_offsets, const int64_t* C_offsets,
+                                   const int64_t* I_offsets, const int64_t* I_depths,
+                                   const int64_t* A_dep_flat, const int64_t* A_dep_offsets,
+                                   const int64_t* B_dep_flat, const int64_t* B_dep_offsets,
+                                   int32_t flags, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYMDP_HIP_H_ */

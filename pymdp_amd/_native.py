@@ -1,0 +1,146 @@
+"""ctypes binding of libpymdp_hip.so (the C ABI declared in include/pymdp_hip.h).
+
+The library is required: there is no fallback implementation.  Loading fails
+with a RuntimeError that says how to build it.  Mirrors the role of the
+reference's pymdp/ffi/_core.py:41-107 (library path, ctypes.CDLL, symbol
+lookup), minus the XLA registration which has no counterpart without JAX.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+from typing import Sequence
+
+import numpy as np
+
+_LIB: ctypes.CDLL | None = None
+_LOAD_ERROR: Exception | None = None
+
+PYMDP_HIP_OK = 0
+PYMDP_HIP_EINVAL = 1
+PYMDP_HIP_EDEVICE = 2
+
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+# Exported symbol -> (restype, argtypes).  Must match include/pymdp_hip.h.
+SIGNATURES: dict[str, tuple] = {
+    "pymdp_hip_last_error": (ctypes.c_char_p, []),
+    "pymdp_hip_version": (ctypes.c_char_p, []),
+    "pymdp_hip_fpi": (ctypes.c_int, [
+        _vp, _vp, _vp,                       # ll, lp, q
+        ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+        _i64p, ctypes.c_int64,               # S, F
+        _i64p, ctypes.c_int64,               # ll_offsets, M
+        _i64p,                               # lp_offsets
+        _i64p, _i64p,                        # A_dep_flat, A_dep_offsets
+        ctypes.c_int32, _vp]),
+    "pymdp_hip_infer_states": (ctypes.c_int, [
+        _vp, _i64p,                          # A_ptrs, A_batch_stride
+        _vp, ctypes.c_int32,                 # obs, obs_is_index
+        _vp, _i64p,                          # prior_ptrs, prior_batch_stride
+        _vp, ctypes.c_int64,                 # q_out, batch
+        _i64p, ctypes.c_int64,               # S, F
+        _i64p, ctypes.c_int64,               # O, M
+        _i64p, _i64p,                        # obs_offsets, lp_offsets
+        _i64p, _i64p,                        # A_dep_flat, A_dep_offsets
+        ctypes.c_int32, ctypes.c_int32, _vp]),
+    "pymdp_hip_neg_efe": (ctypes.c_int, [
+        _vp, ctypes.c_int64, ctypes.c_int64,     # policy_host, P, T
+        _vp,                                     # qs_flat
+        _vp, _i64p, _vp, _i64p,                  # A ptrs/stride, B ptrs/stride
+        _vp, _i64p, _i64p,                       # C ptrs/stride, C_has_time
+        _vp, ctypes.c_int64,                     # E, E stride
+        _vp, ctypes.c_float,                     # gamma, gamma_scalar
+        _vp, _i64p, _i64p,                       # I ptrs/stride/depths
+        _vp, ctypes.c_int64, ctypes.c_float,     # eps ptr, count, scalar
+        _vp, _vp,                                # G_out, q_pi_out
+        ctypes.c_int64,                          # batch
+        _i64p, ctypes.c_int64, _i64p, ctypes.c_int64, _i64p,  # S, F, O, M, U
+        _i64p, _i64p, _i64p, _i64p,              # A deps, B deps
+        ctypes.c_int32, _vp]),
+    "pymdp_hip_neg_efe_all_policies": (ctypes.c_int, [
+        _vp, _vp, _vp, _vp, _vp, _vp,            # policy, qs, A, B, C, I
+        _vp, ctypes.c_int64,                     # eps, eps count
+        _vp, ctypes.c_int64,                     # out, batch
+        ctypes.c_int64, ctypes.c_int64,          # P, T
+        _i64p, ctypes.c_int64, _i64p, ctypes.c_int64, _i64p,  # S, F, O, M, U
+        _i64p, _i64p, _i64p, _i64p, _i64p, _i64p,            # qs/A/B/C/I offsets, I_depths
+        _i64p, _i64p, _i64p, _i64p,              # A deps, B deps
+        ctypes.c_int32, _vp]),
+}
+
+
+def library_path() -> Path:
+    override = os.environ.get("PYMDP_HIP_LIB")
+    if override:
+        return Path(override)
+    return Path(__file__).resolve().parent / "_lib" / "libpymdp_hip.so"
+
+
+def lib() -> ctypes.CDLL:
+    """Load (once) and return the HIP library; raise if it is missing."""
+    global _LIB, _LOAD_ERROR
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not path.exists():
+        raise RuntimeError(
+            f"pymdp_amd: HIP library not found at {path}. Build it with `python -m pymdp_amd.build`."
+        )
+    try:
+        handle = ctypes.CDLL(str(path))
+    except OSError as exc:  # pragma: no cover - surfaced to the caller
+        _LOAD_ERROR = exc
+        raise RuntimeError(f"pymdp_amd: cannot load {path}: {exc}") from exc
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = handle
+    return _LIB
+
+
+def is_available() -> bool:
+    try:
+        lib()
+        return True
+    except RuntimeError:
+        return False
+
+
+def check(rc: int) -> None:
+    """Map a C-ABI status code to a Python exception (ValueError for
+    kInvalidArgument, RuntimeError for device errors)."""
+    if rc == PYMDP_HIP_OK:
+        return
+    msg = lib().pymdp_hip_last_error().decode(errors="replace")
+    if rc == PYMDP_HIP_EINVAL:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+class I64:
+    """Host int64 array kept alive for the duration of a call."""
+
+    __slots__ = ("arr",)
+
+    def __init__(self, values: Sequence[int] | np.ndarray):
+        self.arr = np.ascontiguousarray(np.asarray(values, dtype=np.int64).reshape(-1))
+
+    @property
+    def ptr(self):
+        return self.arr.ctypes.data_as(_i64p)
+
+
+def ptr_array(ptrs: Sequence[int]):
+    arr = (ctypes.c_void_p * max(1, len(ptrs)))(*[int(p) for p in ptrs])
+    return arr
+
+
+def stream_handle(device=None) -> int:
+    """The current torch HIP stream of `device` as a raw hipStream_t value."""
+    import torch
+
+    return int(torch.cuda.current_stream(device).cuda_stream)

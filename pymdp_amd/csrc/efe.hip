@@ -1,0 +1,851 @@
+// Negative expected free energy over all policies (G = -EFE) for gfx950.
+//
+// Algorithm of record: pymdp/control.py:604-793 (compute_neg_efe_policy[_inductive])
+// vmapped over policies, then q_pi = softmax(gamma G + ln E)
+// (pymdp/control.py:349, :963).  Per policy and planning step t:
+//   qs'_f = B_f[..., u_f(t)] . (x parents qs)                 (:351-389)
+//   qo_m  = A_m . (x_{deps m} qs')                            (:420-447)
+//   G    += H(qo_m) - H_A_m . (x qs') + C_m[t] . qo_m          (:449-509)
+//         (+ inductive value, :1021-1061)
+//
+// MI355X design (see history.hpp for the plan):
+//   1. efe_ha_kernel        HAx_m[k] = sum_o xlogx(A_m[o, k])   (one pass over A)
+//   2. efe_rollout_kernel   predicted marginals once per action history
+//   3. efe_kron_kernel      Kronecker vector of every realised inner tuple
+//   4. efe_gemm1_kernel     W[n] = [A_m ; HAx_m] x_inner (x q_inner[n])
+//   5. efe_score_kernel     qo = W[n] q_outer[h0] for every outer history, fused
+//                           entropy + utility + H_A epilogue -> one score each
+//   6. efe_scatter_kernel   G[p] = sum_{t,m} score[t, m, n(p) H0 + h0(p)]
+//                           (+ inductive), then q_pi = softmax(gamma G + ln E)
+// All per-agent intermediates live in one workspace slab per agent.
+#include "common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <list>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "../../include/pymdp_hip.h"
+#include "history.hpp"
+
+namespace pmh {
+
+namespace efe {
+
+constexpr int kMaxT = 8;
+constexpr int kMaxF = 16;
+constexpr int kMaxM = 32;
+constexpr int kMaxG = 8;
+constexpr int kNT = 256;
+
+constexpr int kFlagUseUtility = 1 << 0;
+constexpr int kFlagUseStatesInfoGain = 1 << 1;
+constexpr int kFlagUseInductive = 1 << 2;
+constexpr int kFlagUseParamInfoGain = 1 << 3;
+
+// Plan descriptor, resident in device memory (one per policy-set key).
+struct Desc {
+  int P, T, F, M, G;
+  int S[kMaxF], U[kMaxF], O[kMaxM];
+  int Brank[kMaxF], Bdeps[kMaxF][8], Kb[kMaxF];
+  int mod_group[kMaxM];
+  int gR[kMaxG], gdeps[kMaxG][8], gS0[kMaxG], gKrest[kMaxG];
+  int fn[kMaxT][kMaxF];
+  int f_par_off[kMaxT][kMaxF];  // int table: parents [n * rank]
+  int f_act_off[kMaxT][kMaxF];  // int table: actions [n]
+  int f_p2h_off[kMaxT][kMaxF];  // int table: p2h [P]
+  int g_nin[kMaxT][kMaxG], g_H0[kMaxT][kMaxG];
+  int g_tup_off[kMaxT][kMaxG];  // int table: tuples [n_in * (R-1)]
+  int g_p2s_off[kMaxT][kMaxG];  // int table: p2score [P]
+  long long qs_off[kMaxT][kMaxF];
+  long long qin_off[kMaxT][kMaxG];
+  long long w_off[kMaxT][kMaxM];
+  long long sc_off[kMaxT][kMaxM];
+  long long fsc_off[kMaxT][kMaxF];  // factor (inductive) scores [n]
+  long long ha_off[kMaxM];
+  long long ws_stride;
+};
+
+// Work items of the two GEMM stages (device arrays, per plan).
+struct Item1 {  // gemm1: one (t, m, n)
+  int t, m, n, pad;
+};
+struct Item2 {  // score: one (t, m, n, block of 256 outer histories)
+  int t, m, n, hb;
+};
+
+// Per-call pointers and scalars (by value).
+struct Args {
+  const float* qs;  // [batch, sum S]
+  long long qs_stride;
+  const float* A[kMaxM];
+  long long A_stride[kMaxM];
+  const float* B[kMaxF];
+  long long B_stride[kMaxF];
+  const float* C[kMaxM];
+  long long C_stride[kMaxM];
+  int C_time[kMaxM];
+  const float* E;  // [batch, P] or null (uniform)
+  long long E_stride;
+  const float* gamma;  // [batch] or null
+  float gamma_scalar;
+  const float* I[kMaxF];  // inductive matrices [batch?, depth_f, S_f]
+  long long I_stride[kMaxF];
+  int I_depth[kMaxF];
+  const float* eps;  // device epsilon: one value, or [batch] when eps_batched
+  int eps_batched;
+  float eps_scalar;  // used when eps is null
+  float* G;     // [batch, P]
+  float* q_pi;  // [batch, P] or null
+  float* ws;    // workspace [batch, ws_stride]
+  int qs_off[kMaxF + 1];
+  int flags;
+};
+
+// ---------------------------------------------------------------------------
+// 1. HAx_m[k] = sum_o xlogx(A_m[o, k])
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kNT) void efe_ha_kernel(const Desc* __restrict__ dp, const Args a) {
+  const Desc& d = *dp;
+  const int m = blockIdx.y;
+  const int b = blockIdx.z;
+  const int O = d.O[m];
+  const int g = d.mod_group[m];
+  const long long K = (long long)d.gS0[g] * d.gKrest[g];
+  const long long k = (long long)blockIdx.x * kNT + threadIdx.x;
+  if (k >= K) return;
+  const float* Am = a.A[m] + (long long)b * a.A_stride[m];
+  float acc = 0.0f;
+  for (int o = 0; o < O; ++o) acc += stable_xlogx(Am[(long long)o * K + k]);
+  a.ws[(long long)b * d.ws_stride + d.ha_off[m] + k] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// 2. Rollout: qs[t][f][h] = B_f[:, parents..., u_h] . (x parent marginals)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kNT) void efe_rollout_kernel(const Desc* __restrict__ dp, const int* __restrict__ tab,
+                                                          const Args a) {
+  const Desc& d = *dp;
+  const int b = blockIdx.x;
+  float* ws = a.ws + (long long)b * d.ws_stride;
+  const float* qs0 = a.qs + (long long)b * a.qs_stride;
+  for (int t = 0; t < d.T; ++t) {
+    for (int f = 0; f < d.F; ++f) {
+      const int n = d.fn[t][f];
+      const int S = d.S[f];
+      const int U = d.U[f];
+      const int rank = d.Brank[f];
+      const int Kb = d.Kb[f];
+      const float* Bf = a.B[f] + (long long)b * a.B_stride[f];
+      const int* par = tab + d.f_par_off[t][f];
+      const int* act = tab + d.f_act_off[t][f];
+      float* out = ws + d.qs_off[t][f];
+      const long long total = (long long)n * S;
+      for (long long i = threadIdx.x; i < total; i += kNT) {
+        const int h = (int)(i / S);
+        const int s = (int)(i - (long long)h * S);
+        const int u = act[h];
+        const float* row = Bf + (long long)s * Kb * U + u;
+        float acc = 0.0f;
+        if (rank == 1) {
+          const int pf = d.Bdeps[f][0];
+          const float* q = (t == 0) ? qs0 + a.qs_off[pf] : ws + d.qs_off[t - 1][pf] + (long long)par[h] * d.S[pf];
+          for (int k = 0; k < Kb; ++k) acc = fmaf(row[(long long)k * U], q[k], acc);
+        } else {
+          const float* qp[8];
+          for (int r = 0; r < rank; ++r) {
+            const int pf = d.Bdeps[f][r];
+            qp[r] = (t == 0) ? qs0 + a.qs_off[pf] : ws + d.qs_off[t - 1][pf] + (long long)par[h * rank + r] * d.S[pf];
+          }
+          for (int k = 0; k < Kb; ++k) {
+            int rem = k;
+            float w = 1.0f;
+            for (int r = rank - 1; r >= 0; --r) {
+              const int sz = d.S[d.Bdeps[f][r]];
+              const int c = rem % sz;
+              rem /= sz;
+              w *= qp[r][c];
+            }
+            acc = fmaf(row[(long long)k * U], w, acc);
+          }
+        }
+        out[(long long)h * S + s] = acc;
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. Kronecker vector of each realised inner tuple:
+//    Qin[t][g][n][k] = prod_{i>=1} qs[t][deps_i][h_i(n)][s_i(k)]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kNT) void efe_kron_kernel(const Desc* __restrict__ dp, const int* __restrict__ tab,
+                                                       const Args a) {
+  const Desc& d = *dp;
+  const int tg = blockIdx.y;
+  const int t = tg / d.G;
+  const int g = tg - t * d.G;
+  const int b = blockIdx.z;
+  const int R = d.gR[g];
+  if (R < 2) return;
+  const int n_in = d.g_nin[t][g];
+  const int Kr = d.gKrest[g];
+  const long long total = (long long)n_in * Kr;
+  float* ws = a.ws + (long long)b * d.ws_stride;
+  const int* tup = tab + d.g_tup_off[t][g];
+  for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < total; i += (long long)gridDim.x * kNT) {
+    const int n = (int)(i / Kr);
+    int rem = (int)(i - (long long)n * Kr);
+    float w = 1.0f;
+    for (int r = R - 1; r >= 1; --r) {
+      const int f = d.gdeps[g][r];
+      const int sz = d.S[f];
+      const int c = rem % sz;
+      rem /= sz;
+      const int h = tup[n * (R - 1) + (r - 1)];
+      w *= ws[d.qs_off[t][f] + (long long)h * sz + c];
+    }
+    ws[d.qin_off[t][g] + i] = w;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 4. W[t][m][n][s0][o'] = sum_k Arow(o', s0)[k] Qin[t][g][n][k]
+//    Arow(o', s0) = A_m[o', s0, :] for o' < O, HAx_m[s0, :] for o' == O.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kNT) void efe_gemm1_kernel(const Desc* __restrict__ dp,
+                                                        const Item1* __restrict__ items, const Args a) {
+  const Desc& d = *dp;
+  const Item1 it = items[blockIdx.x];
+  const int b = blockIdx.y;
+  const int t = it.t, m = it.m, n = it.n;
+  const int g = d.mod_group[m];
+  const int O = d.O[m];
+  const int S0 = d.gS0[g];
+  const int Kr = d.gKrest[g];
+  const int R = d.gR[g];
+  const bool sig = (a.flags & kFlagUseStatesInfoGain) != 0;
+  const int rows = (O + 1) * S0;
+  float* ws = a.ws + (long long)b * d.ws_stride;
+  const float* Am = a.A[m] + (long long)b * a.A_stride[m];
+  const float* ha = ws + d.ha_off[m];
+  const float* q = ws + d.qin_off[t][g] + (long long)n * Kr;
+  float* W = ws + d.w_off[t][m] + (long long)n * S0 * (O + 1);
+  for (int r = threadIdx.x; r < rows; r += kNT) {
+    const int s0 = r / (O + 1);
+    const int o = r - s0 * (O + 1);
+    float acc = 0.0f;
+    if (o < O || sig) {
+      const float* row = (o < O) ? Am + ((long long)o * S0 + s0) * Kr : ha + (long long)s0 * Kr;
+      if (R < 2) {
+        acc = row[0];
+      } else {
+        for (int k = 0; k < Kr; ++k) acc = fmaf(row[k], q[k], acc);
+      }
+    }
+    W[r] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 5. Scores: for every outer history h0 of an inner tuple n,
+//    qo[o'] = sum_s0 W[n][s0][o'] q0[h0][s0];  score = H(qo) + qo[O] + C_t . qo
+// ---------------------------------------------------------------------------
+template <int CH>
+__global__ __launch_bounds__(kNT) void efe_score_kernel(const Desc* __restrict__ dp,
+                                                        const Item2* __restrict__ items, const Args a) {
+  const Desc& d = *dp;
+  const Item2 it = items[blockIdx.x];
+  const int b = blockIdx.y;
+  const int t = it.t, m = it.m, n = it.n;
+  const int g = d.mod_group[m];
+  const int O = d.O[m];
+  const int S0 = d.gS0[g];
+  const int f0 = d.gdeps[g][0];
+  const int H0 = d.g_H0[t][g];
+  const int h0 = it.hb * kNT + threadIdx.x;
+  const bool sig = (a.flags & kFlagUseStatesInfoGain) != 0;
+  const bool util = (a.flags & kFlagUseUtility) != 0;
+  float* ws = a.ws + (long long)b * d.ws_stride;
+  const float* W = ws + d.w_off[t][m] + (long long)n * S0 * (O + 1);
+  const float* Ct = a.C[m] + (long long)b * a.C_stride[m] + (a.C_time[m] ? (long long)t * O : 0);
+  const bool active = h0 < H0;
+  const float* q0 = ws + d.qs_off[t][f0] + (long long)(active ? h0 : 0) * S0;
+  float ent = 0.0f, ut = 0.0f, hq = 0.0f;
+  for (int o0 = 0; o0 <= O; o0 += CH) {
+    float acc[CH];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) acc[i] = 0.0f;
+    for (int s = 0; s < S0; ++s) {
+      const float qv = q0[s];
+      const float* wr = W + (long long)s * (O + 1) + o0;
+#pragma unroll
+      for (int i = 0; i < CH; ++i)
+        if (o0 + i <= O) acc[i] = fmaf(wr[i], qv, acc[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      const int o = o0 + i;
+      if (o < O) {
+        ent += stable_xlogx(acc[i]);
+        ut = fmaf(acc[i], Ct[o], ut);
+      } else if (o == O) {
+        hq = acc[i];
+      }
+    }
+  }
+  if (!active) return;
+  float score = 0.0f;
+  if (sig) score += -ent + hq;
+  if (util) score += ut;
+  ws[d.sc_off[t][m] + (long long)n * H0 + h0] = score;
+}
+
+// ---------------------------------------------------------------------------
+// Inductive value per factor history (pymdp/control.py:1021-1061):
+//   path_f * sum_s (1 - I_f[m_f, s]) ln eps * qs[t][f][h][s]
+// with idx = argmax qs_init_f, m_f = max(argmax_i I_f[i, idx] - 1, 0).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kNT) void efe_inductive_kernel(const Desc* __restrict__ dp, const Args a) {
+  const Desc& d = *dp;
+  const int b = blockIdx.x;
+  const int f = blockIdx.y;
+  float* ws = a.ws + (long long)b * d.ws_stride;
+  const float* qs0 = a.qs + (long long)b * a.qs_stride + a.qs_off[f];
+  const int S = d.S[f];
+  const int depth = a.I_depth[f];
+  const float* I = a.I[f] + (long long)b * a.I_stride[f];
+  // argmax of qs_init (first max), argmax over I[:, idx] (first max)
+  int idx = 0;
+  float best = qs0[0];
+  for (int s = 1; s < S; ++s)
+    if (qs0[s] > best) {
+      best = qs0[s];
+      idx = s;
+    }
+  int am = 0;
+  float bi = I[idx];
+  float colsum = I[idx];
+  for (int i = 1; i < depth; ++i) {
+    const float v = I[(long long)i * S + idx];
+    colsum += v;
+    if (v > bi) {
+      bi = v;
+      am = i;
+    }
+  }
+  const int mrow = am - 1 > 0 ? am - 1 : 0;
+  const float path = fminf(fmaxf(colsum, 0.0f), 1.0f);
+  const float eps = a.eps ? a.eps[a.eps_batched ? b : 0] : a.eps_scalar;
+  const float log_eps = log_stable(eps);
+  for (int t = 0; t < d.T; ++t) {
+    const int n = d.fn[t][f];
+    for (int h = threadIdx.x; h < n; h += kNT) {
+      const float* q = ws + d.qs_off[t][f] + (long long)h * S;
+      float acc = 0.0f;
+      for (int s = 0; s < S; ++s) acc = fmaf((1.0f - I[(long long)mrow * S + s]) * log_eps, q[s], acc);
+      ws[d.fsc_off[t][f] + h] = path * acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 6. G[p] = sum_t sum_m score + factor scores; q_pi = softmax(gamma G + ln E)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void efe_scatter_kernel(const Desc* __restrict__ dp, const int* __restrict__ tab,
+                                                           const Args a) {
+  const Desc& d = *dp;
+  const int b = blockIdx.x;
+  const int P = d.P;
+  const float* ws = a.ws + (long long)b * d.ws_stride;
+  float* G = a.G + (long long)b * P;
+  const bool ind = (a.flags & kFlagUseInductive) != 0;
+  const float gamma = a.gamma ? a.gamma[b] : a.gamma_scalar;
+  const float lnE_uniform = log_stable(1.0f / (float)P);
+  __shared__ float red[16];
+  float mx = -INFINITY;
+  for (int p = threadIdx.x; p < P; p += blockDim.x) {
+    float sum = 0.0f;
+    for (int t = 0; t < d.T; ++t) {
+      float st = 0.0f;
+      for (int m = 0; m < d.M; ++m) {
+        const int g = d.mod_group[m];
+        st += ws[d.sc_off[t][m] + tab[d.g_p2s_off[t][g] + p]];
+      }
+      if (ind) {
+        for (int f = 0; f < d.F; ++f) st += ws[d.fsc_off[t][f] + tab[d.f_p2h_off[t][f] + p]];
+      }
+      sum += st;
+    }
+    G[p] = sum;
+    const float lnE = a.E ? log_stable(a.E[(long long)b * a.E_stride + p]) : lnE_uniform;
+    mx = fmaxf(mx, gamma * sum + lnE);
+  }
+  if (a.q_pi == nullptr) return;
+  // block max
+  mx = wave_max(mx);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0) red[wv] = mx;
+  __syncthreads();
+  mx = -INFINITY;
+  for (int i = 0; i < nw; ++i) mx = fmaxf(mx, red[i]);
+  __syncthreads();
+  float s = 0.0f;
+  float* q = a.q_pi + (long long)b * P;
+  for (int p = threadIdx.x; p < P; p += blockDim.x) {
+    const float lnE = a.E ? log_stable(a.E[(long long)b * a.E_stride + p]) : lnE_uniform;
+    const float e = expf(gamma * G[p] + lnE - mx);
+    q[p] = e;
+    s += e;
+  }
+  s = wave_sum(s);
+  if (lane == 0) red[wv] = s;
+  __syncthreads();
+  s = 0.0f;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  for (int p = threadIdx.x; p < P; p += blockDim.x) q[p] = q[p] / s;
+}
+
+// ---------------------------------------------------------------------------
+// Host: plan cache (device copy of the history plan, per device and key)
+// ---------------------------------------------------------------------------
+struct DevPlan {
+  uint64_t key = 0;
+  int device = -1;
+  EfeHostPlan host;
+  Desc desc{};
+  Desc* d_desc = nullptr;
+  int* d_tab = nullptr;
+  Item1* d_items1 = nullptr;
+  Item2* d_items2 = nullptr;
+  int n_items1 = 0, n_items2 = 0;
+  int kron_blocks = 1;
+  ~DevPlan() {
+    if (d_desc) (void)hipFree(d_desc);
+    if (d_tab) (void)hipFree(d_tab);
+    if (d_items1) (void)hipFree(d_items1);
+    if (d_items2) (void)hipFree(d_items2);
+  }
+};
+
+std::mutex g_mu;
+std::list<std::unique_ptr<DevPlan>> g_plans;  // most recent first, small LRU
+constexpr size_t kMaxPlans = 8;
+
+struct Ws {
+  float* p = nullptr;
+  size_t bytes = 0;
+};
+std::map<int, Ws> g_ws;
+
+static int efe_err(const std::string& msg) {
+  set_error("efe_ffi: " + msg);
+  return PYMDP_HIP_EINVAL;
+}
+
+#define EFE_HIP(expr)                                                                 \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess) {                                                           \
+      set_error(std::string("efe_ffi: ") + #expr + ": " + hipGetErrorString(_e));     \
+      return PYMDP_HIP_EDEVICE;                                                       \
+    }                                                                                 \
+  } while (0)
+
+static int make_dev_plan(EfeHostPlan&& hp, int device, DevPlan** out) {
+  auto dp = std::make_unique<DevPlan>();
+  dp->key = hp.key;
+  dp->device = device;
+  dp->host = std::move(hp);
+  EfeHostPlan& h = dp->host;
+  Desc& d = dp->desc;
+  std::memset(&d, 0, sizeof(d));
+  if (h.T > kMaxT) return efe_err("policy horizon above " + std::to_string(kMaxT) + " is not supported");
+  if (h.F > kMaxF) return efe_err("more than " + std::to_string(kMaxF) + " factors is not supported");
+  if (h.M > kMaxM) return efe_err("more than " + std::to_string(kMaxM) + " modalities is not supported");
+  if ((int64_t)h.groups.size() > kMaxG) return efe_err("more than 8 distinct A_dependencies tuples");
+  d.P = (int)h.P;
+  d.T = (int)h.T;
+  d.F = (int)h.F;
+  d.M = (int)h.M;
+  d.G = (int)h.groups.size();
+  for (int f = 0; f < d.F; ++f) {
+    d.S[f] = (int)h.S[f];
+    d.U[f] = (int)h.U[f];
+    d.Brank[f] = (int)h.B_deps[f].size();
+    long long kb = 1;
+    for (int r = 0; r < d.Brank[f]; ++r) {
+      d.Bdeps[f][r] = (int)h.B_deps[f][r];
+      kb *= h.S[h.B_deps[f][r]];
+    }
+    d.Kb[f] = (int)kb;
+  }
+  for (int m = 0; m < d.M; ++m) {
+    d.O[m] = (int)h.O[m];
+    d.mod_group[m] = h.mod_group[m];
+  }
+  for (int g = 0; g < d.G; ++g) {
+    const auto& deps = h.groups[g].deps;
+    d.gR[g] = (int)deps.size();
+    long long kr = 1;
+    for (size_t i = 0; i < deps.size(); ++i) {
+      d.gdeps[g][i] = (int)deps[i];
+      if (i > 0) kr *= h.S[deps[i]];
+    }
+    d.gS0[g] = (int)h.S[deps[0]];
+    d.gKrest[g] = (int)kr;
+  }
+  // int tables and workspace layout
+  std::vector<int> tab;
+  long long ws = 0;
+  auto walloc = [&](long long n) {
+    const long long off = ws;
+    ws += (n + 15) / 16 * 16;
+    return off;
+  };
+  for (int m = 0; m < d.M; ++m) {
+    const int g = d.mod_group[m];
+    d.ha_off[m] = walloc((long long)d.gS0[g] * d.gKrest[g]);
+  }
+  std::vector<Item1> it1;
+  std::vector<Item2> it2;
+  for (int t = 0; t < d.T; ++t) {
+    for (int f = 0; f < d.F; ++f) {
+      const FactorLevelPlan& lv = h.fac[t][f];
+      d.fn[t][f] = lv.n;
+      d.f_par_off[t][f] = (int)tab.size();
+      tab.insert(tab.end(), lv.parents.begin(), lv.parents.end());
+      d.f_act_off[t][f] = (int)tab.size();
+      tab.insert(tab.end(), lv.action.begin(), lv.action.end());
+      d.f_p2h_off[t][f] = (int)tab.size();
+      tab.insert(tab.end(), lv.p2h.begin(), lv.p2h.end());
+      d.qs_off[t][f] = walloc((long long)lv.n * d.S[f]);
+      d.fsc_off[t][f] = walloc(lv.n);
+    }
+    for (int g = 0; g < d.G; ++g) {
+      const GroupLevelPlan& gl = h.grp[t][g];
+      d.g_nin[t][g] = gl.n_in;
+      d.g_H0[t][g] = gl.H0;
+      d.g_tup_off[t][g] = (int)tab.size();
+      tab.insert(tab.end(), gl.tuples.begin(), gl.tuples.end());
+      d.g_p2s_off[t][g] = (int)tab.size();
+      tab.insert(tab.end(), gl.p2score.begin(), gl.p2score.end());
+      d.qin_off[t][g] = walloc((long long)gl.n_in * d.gKrest[g]);
+    }
+    for (int m = 0; m < d.M; ++m) {
+      const int g = d.mod_group[m];
+      const GroupLevelPlan& gl = h.grp[t][g];
+      d.w_off[t][m] = walloc((long long)gl.n_in * d.gS0[g] * (d.O[m] + 1));
+      d.sc_off[t][m] = walloc((long long)gl.n_in * gl.H0);
+      for (int n = 0; n < gl.n_in; ++n) {
+        it1.push_back(Item1{t, m, n, 0});
+        for (int hb = 0; hb * kNT < gl.H0; ++hb) it2.push_back(Item2{t, m, n, hb});
+      }
+    }
+  }
+  d.ws_stride = (ws + 63) / 64 * 64;
+  if (tab.empty()) tab.push_back(0);
+  dp->n_items1 = (int)it1.size();
+  dp->n_items2 = (int)it2.size();
+  long long max_kron = 1;
+  for (int t = 0; t < d.T; ++t)
+    for (int g = 0; g < d.G; ++g) max_kron = std::max(max_kron, (long long)d.g_nin[t][g] * d.gKrest[g]);
+  dp->kron_blocks = (int)std::min<long long>((max_kron + kNT - 1) / kNT, 4096);
+  EFE_HIP(hipMalloc(&dp->d_desc, sizeof(Desc)));
+  EFE_HIP(hipMalloc(&dp->d_tab, sizeof(int) * tab.size()));
+  EFE_HIP(hipMalloc(&dp->d_items1, sizeof(Item1) * std::max<size_t>(1, it1.size())));
+  EFE_HIP(hipMalloc(&dp->d_items2, sizeof(Item2) * std::max<size_t>(1, it2.size())));
+  EFE_HIP(hipMemcpy(dp->d_desc, &d, sizeof(Desc), hipMemcpyHostToDevice));
+  EFE_HIP(hipMemcpy(dp->d_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+  if (!it1.empty()) EFE_HIP(hipMemcpy(dp->d_items1, it1.data(), sizeof(Item1) * it1.size(), hipMemcpyHostToDevice));
+  if (!it2.empty()) EFE_HIP(hipMemcpy(dp->d_items2, it2.data(), sizeof(Item2) * it2.size(), hipMemcpyHostToDevice));
+  *out = dp.release();
+  return 0;
+}
+
+// Finds or builds the device plan for this policy set (content-keyed).
+static int get_plan(const int32_t* policy, int64_t P, int64_t T, int64_t F, const int64_t* S, const int64_t* U,
+                    const int64_t* O, int64_t M, const int64_t* Adf, const int64_t* Ado, const int64_t* Bdf,
+                    const int64_t* Bdo, DevPlan** out) {
+  int device = 0;
+  EFE_HIP(hipGetDevice(&device));
+  // content key first (cheap relative to the tree build)
+  uint64_t h = fnv1a64(policy, sizeof(int32_t) * (size_t)(P * T * F));
+  h = fnv1a64(S, sizeof(int64_t) * F, h);
+  h = fnv1a64(U, sizeof(int64_t) * F, h);
+  h = fnv1a64(O, sizeof(int64_t) * M, h);
+  h = fnv1a64(Ado, sizeof(int64_t) * (M + 1), h);
+  h = fnv1a64(Adf, sizeof(int64_t) * Ado[M], h);
+  h = fnv1a64(Bdo, sizeof(int64_t) * (F + 1), h);
+  h = fnv1a64(Bdf, sizeof(int64_t) * Bdo[F], h);
+  const int64_t dims[4] = {P, T, F, M};
+  const uint64_t key = fnv1a64(dims, sizeof(dims), h);
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto it = g_plans.begin(); it != g_plans.end(); ++it) {
+    if ((*it)->key == key && (*it)->device == device) {
+      g_plans.splice(g_plans.begin(), g_plans, it);
+      *out = g_plans.front().get();
+      return 0;
+    }
+  }
+  EfeHostPlan hp;
+  std::string err = build_efe_host_plan(policy, P, T, F, S, U, O, M, Adf, Ado, Bdf, Bdo, &hp);
+  if (!err.empty()) return efe_err(err);
+  if (hp.key != key) return efe_err("internal: plan key mismatch");
+  DevPlan* dp = nullptr;
+  int rc = make_dev_plan(std::move(hp), device, &dp);
+  if (rc) return rc;
+  g_plans.emplace_front(dp);
+  while (g_plans.size() > kMaxPlans) g_plans.pop_back();
+  *out = dp;
+  return 0;
+}
+
+static int get_ws(size_t bytes, float** out) {
+  int device = 0;
+  EFE_HIP(hipGetDevice(&device));
+  std::lock_guard<std::mutex> lk(g_mu);
+  Ws& w = g_ws[device];
+  if (w.bytes < bytes) {
+    if (w.p) EFE_HIP(hipFree(w.p));
+    w.p = nullptr;
+    w.bytes = 0;
+    EFE_HIP(hipMalloc(&w.p, bytes));
+    w.bytes = bytes;
+  }
+  *out = w.p;
+  return 0;
+}
+
+struct CallIn {
+  const int32_t* policy;
+  int64_t P, T;
+  const float* qs;
+  const float* const* A;
+  const int64_t* A_stride;
+  const float* const* B;
+  const int64_t* B_stride;
+  const float* const* C;
+  const int64_t* C_stride;
+  const int64_t* C_time;
+  const float* E;
+  int64_t E_stride;
+  const float* gamma;
+  float gamma_scalar;
+  const float* const* I;
+  const int64_t* I_stride;
+  const int64_t* I_depth;
+  const float* eps;
+  int64_t eps_count;
+  float eps_scalar;
+  float* G;
+  float* q_pi;
+  int64_t batch;
+  const int64_t* S;
+  int64_t F;
+  const int64_t* O;
+  int64_t M;
+  const int64_t* U;
+  const int64_t *Adf, *Ado, *Bdf, *Bdo;
+  int32_t flags;
+  hipStream_t stream;
+};
+
+static int run_neg_efe(const CallIn& c) {
+  if (c.flags & kFlagUseParamInfoGain)
+    return efe_err("use_param_info_gain is not supported by the HIP neg-EFE kernel");
+  if (c.batch < 0) return efe_err("batch must be non-negative");
+  if (c.F <= 0 || c.M <= 0) return efe_err("neg-EFE FFI requires at least one factor and one modality");
+  if (c.F > kMaxF || c.M > kMaxM) return efe_err("too many factors or modalities");
+  if (c.P <= 0 || c.T <= 0) return efe_err("policy_matrix shape is inconsistent with the model");
+  DevPlan* plan = nullptr;
+  int rc = get_plan(c.policy, c.P, c.T, c.F, c.S, c.U, c.O, c.M, c.Adf, c.Ado, c.Bdf, c.Bdo, &plan);
+  if (rc) return rc;
+  if (c.batch == 0) return 0;
+  const Desc& d = plan->desc;
+  Args a{};
+  a.qs = c.qs;
+  a.qs_off[0] = 0;
+  for (int f = 0; f < d.F; ++f) a.qs_off[f + 1] = a.qs_off[f] + d.S[f];
+  a.qs_stride = a.qs_off[d.F];
+  for (int m = 0; m < d.M; ++m) {
+    if (!c.A[m] || !c.C[m]) return efe_err("null A or C pointer");
+    a.A[m] = c.A[m];
+    a.A_stride[m] = c.A_stride[m];
+    a.C[m] = c.C[m];
+    a.C_stride[m] = c.C_stride[m];
+    a.C_time[m] = (int)c.C_time[m];
+  }
+  for (int f = 0; f < d.F; ++f) {
+    if (!c.B[f]) return efe_err("null B pointer");
+    a.B[f] = c.B[f];
+    a.B_stride[f] = c.B_stride[f];
+  }
+  const bool ind = (c.flags & kFlagUseInductive) != 0;
+  if (ind) {
+    if (!c.eps && !(c.eps_scalar > 0.0f)) return efe_err("inductive_epsilon must be positive when use_inductive=True");
+    for (int f = 0; f < d.F; ++f) {
+      if (!c.I || !c.I[f] || c.I_depth[f] <= 0) return efe_err("I[f] must have shape (depth_f, S_f)");
+      a.I[f] = c.I[f];
+      a.I_stride[f] = c.I_stride[f];
+      a.I_depth[f] = (int)c.I_depth[f];
+    }
+  }
+  a.E = c.E;
+  a.E_stride = c.E_stride;
+  a.gamma = c.gamma;
+  a.gamma_scalar = c.gamma_scalar;
+  a.eps = c.eps;
+  a.eps_batched = (c.eps && c.eps_count > 1) ? 1 : 0;
+  if (c.eps && c.eps_count != 1 && c.eps_count != c.batch)
+    return efe_err("inductive_epsilon must hold 1 or batch values");
+  a.eps_scalar = c.eps_scalar;
+  a.G = c.G;
+  a.q_pi = c.q_pi;
+  a.flags = c.flags;
+  float* ws = nullptr;
+  rc = get_ws(sizeof(float) * (size_t)d.ws_stride * (size_t)c.batch, &ws);
+  if (rc) return rc;
+  a.ws = ws;
+  const unsigned B = (unsigned)c.batch;
+  hipStream_t st = c.stream;
+  if (c.flags & kFlagUseStatesInfoGain) {
+    long long maxK = 1;
+    for (int g = 0; g < d.G; ++g) maxK = std::max(maxK, (long long)d.gS0[g] * d.gKrest[g]);
+    hipLaunchKernelGGL(efe_ha_kernel, dim3((unsigned)((maxK + kNT - 1) / kNT), d.M, B), dim3(kNT), 0, st,
+                       plan->d_desc, a);
+  }
+  hipLaunchKernelGGL(efe_rollout_kernel, dim3(B), dim3(kNT), 0, st, plan->d_desc, plan->d_tab, a);
+  hipLaunchKernelGGL(efe_kron_kernel, dim3(plan->kron_blocks, d.T * d.G, B), dim3(kNT), 0, st, plan->d_desc,
+                     plan->d_tab, a);
+  hipLaunchKernelGGL(efe_gemm1_kernel, dim3(plan->n_items1, B), dim3(kNT), 0, st, plan->d_desc, plan->d_items1, a);
+  hipLaunchKernelGGL(efe_score_kernel<32>, dim3(plan->n_items2, B), dim3(kNT), 0, st, plan->d_desc,
+                     plan->d_items2, a);
+  if (ind) hipLaunchKernelGGL(efe_inductive_kernel, dim3(B, d.F), dim3(kNT), 0, st, plan->d_desc, a);
+  hipLaunchKernelGGL(efe_scatter_kernel, dim3(B), dim3(1024), 0, st, plan->d_desc, plan->d_tab, a);
+  EFE_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace efe
+}  // namespace pmh
+
+using namespace pmh;
+
+extern "C" int pymdp_hip_neg_efe(const int32_t* policy_host, int64_t P, int64_t T, const float* qs_flat,
+                                 const float* const* A_ptrs, const int64_t* A_batch_stride,
+                                 const float* const* B_ptrs, const int64_t* B_batch_stride,
+                                 const float* const* C_ptrs, const int64_t* C_batch_stride,
+                                 const int64_t* C_has_time, const float* E, int64_t E_batch_stride,
+                                 const float* gamma, float gamma_scalar, const float* const* I_ptrs,
+                                 const int64_t* I_batch_stride, const int64_t* I_depths,
+                                 const float* inductive_epsilon, int64_t epsilon_count, float epsilon_scalar,
+                                 float* G_out, float* q_pi_out,
+                                 int64_t batch, const int64_t* S, int64_t F, const int64_t* O, int64_t M,
+                                 const int64_t* U, const int64_t* A_dep_flat, const int64_t* A_dep_offsets,
+                                 const int64_t* B_dep_flat, const int64_t* B_dep_offsets, int32_t flags,
+                                 void* stream) {
+  set_error("");
+  efe::CallIn c{};
+  c.policy = policy_host;
+  c.P = P;
+  c.T = T;
+  c.qs = qs_flat;
+  c.A = A_ptrs;
+  c.A_stride = A_batch_stride;
+  c.B = B_ptrs;
+  c.B_stride = B_batch_stride;
+  c.C = C_ptrs;
+  c.C_stride = C_batch_stride;
+  c.C_time = C_has_time;
+  c.E = E;
+  c.E_stride = E_batch_stride;
+  c.gamma = gamma;
+  c.gamma_scalar = gamma_scalar;
+  c.I = I_ptrs;
+  c.I_stride = I_batch_stride;
+  c.I_depth = I_depths;
+  c.eps = inductive_epsilon;
+  c.eps_count = epsilon_count;
+  c.eps_scalar = epsilon_scalar;
+  c.G = G_out;
+  c.q_pi = q_pi_out;
+  c.batch = batch;
+  c.S = S;
+  c.F = F;
+  c.O = O;
+  c.M = M;
+  c.U = U;
+  c.Adf = A_dep_flat;
+  c.Ado = A_dep_offsets;
+  c.Bdf = B_dep_flat;
+  c.Bdo = B_dep_offsets;
+  c.flags = flags;
+  c.stream = (hipStream_t)stream;
+  return efe::run_neg_efe(c);
+}
+
+extern "C" int pymdp_hip_neg_efe_all_policies(const int32_t* policy_host, const float* qs_flat, const float* A_flat,
+                                              const float* B_flat, const float* C_flat, const float* I_flat,
+                                              const float* inductive_epsilon, int64_t epsilon_count,
+                                              float* neg_efe_out, int64_t batch,
+                                              int64_t P, int64_t T, const int64_t* S, int64_t F, const int64_t* O,
+                                              int64_t M, const int64_t* U, const int64_t* qs_offsets,
+                                              const int64_t* A_offsets, const int64_t* B_offsets,
+                                              const int64_t* C_offsets, const int64_t* I_offsets,
+                                              const int64_t* I_depths, const int64_t* A_dep_flat,
+                                              const int64_t* A_dep_offsets, const int64_t* B_dep_flat,
+                                              const int64_t* B_dep_offsets, int32_t flags, void* stream) {
+  set_error("");
+  if (F <= 0 || M <= 0 || F > efe::kMaxF || M > efe::kMaxM)
+    return efe::efe_err("neg-EFE FFI requires 1..16 factors and 1..32 modalities");
+  // Layout checks of the reference (pymdp/ffi/neg_efe/neg_efe_layout.h:211-441).
+  for (int64_t f = 0; f < F; ++f) {
+    if (qs_offsets[f + 1] - qs_offsets[f] != S[f]) return efe::efe_err("qs_offsets not consistent with S");
+    int64_t kb = 1;
+    for (int64_t i = B_dep_offsets[f]; i < B_dep_offsets[f + 1]; ++i) {
+      if (B_dep_flat[i] < 0 || B_dep_flat[i] >= F) return efe::efe_err("B_dependencies factor index out of range");
+      kb *= S[B_dep_flat[i]];
+    }
+    if (B_offsets[f + 1] - B_offsets[f] != S[f] * kb * U[f]) return efe::efe_err("B_offsets not consistent");
+    if (I_offsets[f + 1] - I_offsets[f] != I_depths[f] * S[f]) return efe::efe_err("I_offsets not consistent");
+  }
+  for (int64_t m = 0; m < M; ++m) {
+    int64_t k = 1;
+    for (int64_t i = A_dep_offsets[m]; i < A_dep_offsets[m + 1]; ++i) {
+      if (A_dep_flat[i] < 0 || A_dep_flat[i] >= F) return efe::efe_err("A_dependencies factor index out of range");
+      k *= S[A_dep_flat[i]];
+    }
+    if (A_offsets[m + 1] - A_offsets[m] != O[m] * k) return efe::efe_err("A_offsets not consistent");
+    if (C_offsets[m + 1] - C_offsets[m] != T * O[m]) return efe::efe_err("C_offsets not consistent with (T, O_m)");
+  }
+  const float* Ap[efe::kMaxM];
+  const float* Cp[efe::kMaxM];
+  const float* Bp[efe::kMaxF];
+  const float* Ip[efe::kMaxF];
+  int64_t As[efe::kMaxM], Cs[efe::kMaxM], Ct[efe::kMaxM], Bs[efe::kMaxF], Is[efe::kMaxF];
+  for (int64_t m = 0; m < M; ++m) {
+    Ap[m] = A_flat + A_offsets[m];
+    As[m] = A_offsets[M];
+    Cp[m] = C_flat + C_offsets[m];
+    Cs[m] = C_offsets[M];
+    Ct[m] = 1;
+  }
+  for (int64_t f = 0; f < F; ++f) {
+    Bp[f] = B_flat + B_offsets[f];
+    Bs[f] = B_offsets[F];
+    Ip[f] = I_flat ? I_flat + I_offsets[f] : nullptr;
+    Is[f] = I_offsets[F];
+  }
+  return pymdp_hip_neg_efe(policy_host, P, T, qs_flat, Ap, As, Bp, Bs, Cp, Cs, Ct, nullptr, 0, nullptr, 1.0f, Ip, Is,
+                           I_depths, inductive_epsilon, epsilon_count, 0.0f, neg_efe_out, nullptr, batch, S, F, O, M, U, A_dep_flat,
+                           A_dep_offsets, B_dep_flat, B_dep_offsets, flags, stream);
+}

@@ -1,0 +1,627 @@
+// Fused mean-field fixed-point iteration (FPI) for gfx950.
+//
+// Algorithm of record: pymdp/algos.py:88-155 (run_factorized_fpi) with the
+// log-likelihood of pymdp/maths.py:214-287.  One workgroup (4 wave64s) per
+// agent runs the whole thing: log-likelihood build from A and the
+// observation, num_iter Jacobi sweeps, final softmax.
+//
+// Design (MI355X-first, not a translation of the reference CUDA kernel):
+//  * Modalities with the same dependency tuple are summed into one group
+//    log-likelihood LL_g (linearity of the leave-one-out contraction), so a
+//    sweep costs one contraction per distinct tuple, not per modality.
+//  * The leave-one-factor-out contraction of a rank-R group is done as R
+//    "stages".  Stage k views the partially contracted tensor X_k as a
+//    [rows x n] matrix whose columns are the dependency at position
+//    c = R-1-k; it produces both the marginal of that factor
+//    M_c[s] = sum_r w_c[r] X_k[r, s]  (w_c = Kronecker product of the q of
+//    the leading factors) and the next tensor X_{k+1}[r] = X_k[r,:] . q_c.
+//    Cost per sweep ~ 2K instead of R*K.
+//  * Lane mapping: a wave64 is split into 64/npad rows of npad lanes
+//    (npad = next pow2 >= n).  Row sums are butterflies over the low lane
+//    bits; the marginal is a per-lane register accumulator reduced once per
+//    stage over the high lane bits and across waves through LDS.
+//  * Stage 0 input (the big LL) stays in VGPRs for the whole kernel when it
+//    fits (JMAX slots of one element per lane); otherwise it streams from a
+//    per-agent global workspace (L2 / MALL resident).  Everything smaller
+//    (q, log_q, Kronecker weights, X_{k>=1}) lives in LDS.
+//  * Observation rows with zero weight are skipped: for one-hot observations
+//    the build reads one row of A per modality (O_m times less HBM traffic),
+//    and the skipped terms are exact zeros (0 * finite log).
+#include "common.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/pymdp_hip.h"
+
+namespace pmh {
+
+constexpr int kMaxF = 16;
+constexpr int kMaxM = 32;
+constexpr int kMaxG = 8;
+constexpr int kMaxR = PYMDP_HIP_MAX_DEP_RANK;
+constexpr int kMaxDim = 64;            // per-stage column width handled by one wave row
+constexpr int kLdsBudget = 64 * 1024;  // bytes of dynamic LDS per workgroup
+
+struct FpiDesc {
+  int F, M, G, iters;
+  int totS, sumO, mode;  // mode 0: ll_flat + lp_flat (log prior); 1: A + obs + prior probs
+  int obs_index;         // mode 1: obs holds int32 indices [batch, M] instead of distributions
+  int lds_S, lds_O, lds_w, lds_x;
+  int S[kMaxF];
+  int S_off[kMaxF + 1];
+  int mod_group[kMaxM];
+  int O[kMaxM];
+  int obs_off[kMaxM];
+  long long K[kMaxM];
+  long long ll_off[kMaxM];
+  long long A_stride[kMaxM];
+  long long ll_stride;
+  long long ws_stride;
+  long long prior_stride[kMaxF];
+  int R[kMaxG];
+  int fac[kMaxG][kMaxR];
+  int dim[kMaxG][kMaxR];
+  int wt_off[kMaxG][kMaxR];
+  int jbase[kMaxG];
+  int jcount[kMaxG];
+  long long ws_off[kMaxG];
+};
+
+struct FpiPtrs {
+  const float* ll;     // mode 0
+  const float* lp;     // mode 0: packed log prior [batch, totS]
+  const void* obs;     // mode 1: float [batch, sumO] or int32 [batch, M]
+  float* q;
+  double* ws;          // global workspace (JMAX == 0 only)
+  const float* A[kMaxM];
+  const float* prior[kMaxF];  // mode 1: per-factor prior probabilities
+};
+
+// Rows of a group's stage whose column position is c: prod_{j<c} dim[j].
+__device__ __forceinline__ int stage_rows(const FpiDesc& d, int g, int c) {
+  int r = 1;
+  for (int j = 0; j < c; ++j) r *= d.dim[g][j];
+  return r;
+}
+
+template <int NW>
+__device__ void softmax_factors(const FpiDesc& d, const double* src, double* dst, int lane, int wave) {
+  for (int f = wave; f < d.F; f += NW) {
+    const int off = d.S_off[f];
+    const int n = d.S[f];
+    double m = -INFINITY;
+    for (int i = lane; i < n; i += kWave) m = fmax(m, src[off + i]);
+    m = wave_max(m);
+    double s = 0.0;
+    for (int i = lane; i < n; i += kWave) {
+      const double e = exp(src[off + i] - m);
+      dst[off + i] = e;
+      s += e;
+    }
+    s = wave_sum(s);
+    for (int i = lane; i < n; i += kWave) dst[off + i] = dst[off + i] / s;
+  }
+}
+
+// Reduce the per-lane marginal accumulator of one stage and add it into
+// log_q[f].  All lanes of the workgroup must call this.
+template <int NW>
+__device__ __forceinline__ void stage_finish(double acc, int lg, int n, int f, const FpiDesc& d, double* red,
+                                             double* log_q, int tid, int lane, int wave) {
+  acc = group_sum_high(acc, lg);
+  const int col = lane & ((1 << lg) - 1);
+  if ((lane >> lg) == 0 && col < n) red[wave * kWave + col] = acc;
+  __syncthreads();
+  if (tid < n) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += red[w * kWave + tid];
+    log_q[d.S_off[f] + tid] += s;
+  }
+}
+
+// The log-likelihood is built in FP32 (the data type, like log_stable of the
+// reference), held in FP64, and every sweep accumulates in FP64: the
+// mean-field sweep can oscillate (period-2 cycles of the Jacobi update), and
+// there FP32 rounding of the iteration state is amplified sweep by sweep.
+template <int JMAX, int NT>
+__global__ __launch_bounds__(NT) void fpi_kernel(const FpiDesc d, const FpiPtrs p) {
+  constexpr int NW = NT / kWave;
+  extern __shared__ double smd[];
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wave = tid >> 6;
+
+  double* log_q = smd;
+  double* qv = log_q + d.lds_S;
+  double* lp = qv + d.lds_S;
+  double* wbuf = lp + d.lds_S;
+  double* xa = wbuf + d.lds_w;
+  double* xb = xa + d.lds_x;
+  double* red0 = xb + d.lds_x;
+  double* red1 = red0 + NW * kWave;
+  float* obs = reinterpret_cast<float*>(red1 + NW * kWave);
+
+  // ---- priors and observations --------------------------------------------
+  if (d.mode) {
+    for (int f = 0; f < d.F; ++f)
+      for (int i = tid; i < d.S[f]; i += NT)
+        lp[d.S_off[f] + i] = (double)log_stable(p.prior[f][(long long)b * d.prior_stride[f] + i]);
+    if (d.obs_index) {
+      // one_hot(index): an out-of-range index gives an all-zero row (jax.nn.one_hot).
+      const int* oi = static_cast<const int*>(p.obs) + (long long)b * d.M;
+      for (int m = 0; m < d.M; ++m)
+        for (int o = tid; o < d.O[m]; o += NT) obs[d.obs_off[m] + o] = (oi[m] == o) ? 1.0f : 0.0f;
+    } else {
+      const float* of = static_cast<const float*>(p.obs) + (long long)b * d.sumO;
+      for (int i = tid; i < d.sumO; i += NT) obs[i] = of[i];
+    }
+  } else {
+    for (int i = tid; i < d.totS; i += NT) lp[i] = (double)p.lp[(long long)b * d.totS + i];
+  }
+  for (int i = tid; i < d.totS; i += NT) log_q[i] = 0.0;
+  __syncthreads();
+
+  // ---- log-likelihood build -------------------------------------------------
+  double X0[JMAX > 0 ? JMAX : 1];
+#pragma unroll
+  for (int j = 0; j < (JMAX > 0 ? JMAX : 1); ++j) X0[j] = 0.0;
+
+  if constexpr (JMAX > 0) {
+    for (int m = 0; m < d.M; ++m) {
+      const int g = d.mod_group[m];
+      const int R = d.R[g];
+      const int n = d.dim[g][R - 1];
+      const int lg = ceil_log2(n);
+      const int rpw_lg = 6 - lg;
+      const int rows = stage_rows(d, g, R - 1);
+      const int jb = d.jbase[g];
+      const int je = jb + d.jcount[g];
+      const int col = lane & ((1 << lg) - 1);
+      const int lr = lane >> lg;
+      if (d.mode) {
+        const float* Am = p.A[m] + (long long)b * d.A_stride[m];
+        const long long K = d.K[m];
+        for (int o = 0; o < d.O[m]; ++o) {
+          const float w = obs[d.obs_off[m] + o];
+          if (w == 0.0f) continue;  // uniform across the workgroup
+          const float* row = Am + (long long)o * K;
+          // ll_m = sum_o obs[o] * log_stable(A[o]) in FP32 per modality term
+          constexpr int CH = JMAX < 16 ? JMAX : 16;
+#pragma unroll
+          for (int j0 = 0; j0 < JMAX; j0 += CH) {
+            if (j0 + CH <= jb || j0 >= je) continue;  // uniform
+            float t[CH];
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+              const int j = j0 + i;
+              t[i] = 1.0f;
+              if (j >= jb && j < je) {
+                const int r = (((j - jb) * NW + wave) << rpw_lg) + lr;
+                if (r < rows && col < n) t[i] = __builtin_nontemporal_load(row + (long long)r * n + col);
+              }
+            }
+#pragma unroll
+            for (int i = 0; i < CH; ++i) {
+              const int j = j0 + i;
+              if (j >= jb && j < je) {
+                const int r = (((j - jb) * NW + wave) << rpw_lg) + lr;
+                if (r < rows && col < n) X0[j] += (double)(w * log_stable(t[i]));
+              }
+            }
+          }
+        }
+      } else {
+        const float* llm = p.ll + (long long)b * d.ll_stride + d.ll_off[m];
+#pragma unroll
+        for (int j = 0; j < JMAX; ++j) {
+          if (j >= jb && j < je) {
+            const int r = (((j - jb) * NW + wave) << rpw_lg) + lr;
+            if (r < rows && col < n) X0[j] += (double)llm[(long long)r * n + col];
+          }
+        }
+      }
+    }
+  } else {
+    // Streaming fallback: LL_g (FP64) in natural layout in the per-agent workspace.
+    double* ws = p.ws + (long long)b * d.ws_stride;
+    for (int g = 0; g < d.G; ++g) {
+      long long Kg = 1;
+      for (int j = 0; j < d.R[g]; ++j) Kg *= d.dim[g][j];
+      double* out = ws + d.ws_off[g];
+      for (long long e = tid; e < Kg; e += NT) {
+        double acc = 0.0;
+        for (int m = 0; m < d.M; ++m) {
+          if (d.mod_group[m] != g) continue;
+          if (d.mode) {
+            const float* Am = p.A[m] + (long long)b * d.A_stride[m];
+            for (int o = 0; o < d.O[m]; ++o) {
+              const float w = obs[d.obs_off[m] + o];
+              if (w == 0.0f) continue;
+              acc += (double)(w * log_stable(Am[(long long)o * d.K[m] + e]));
+            }
+          } else {
+            acc += (double)p.ll[(long long)b * d.ll_stride + d.ll_off[m] + e];
+          }
+        }
+        out[e] = acc;
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+
+  // ---- Jacobi sweeps ---------------------------------------------------------
+  for (int it = 0; it < d.iters; ++it) {
+    softmax_factors<NW>(d, log_q, qv, lane, wave);
+    __syncthreads();
+    for (int i = tid; i < d.totS; i += NT) log_q[i] = lp[i];
+
+    for (int g = 0; g < d.G; ++g) {
+      const int R = d.R[g];
+      // Kronecker weight tables w_c for c = 1..R-1 (w_c has prod_{j<c} dim[j] entries).
+      for (int c = 1; c < R; ++c) {
+        double* wt = wbuf + d.wt_off[g][c];
+        const int rows = stage_rows(d, g, c);
+        const int dprev = d.dim[g][c - 1];
+        const double* qprev = qv + d.S_off[d.fac[g][c - 1]];
+        const double* wprev = (c > 1) ? wbuf + d.wt_off[g][c - 1] : nullptr;
+        for (int i = tid; i < rows; i += NT) {
+          const int hi = i / dprev;
+          const int lo = i - hi * dprev;
+          wt[i] = (wprev ? wprev[hi] : 1.0) * qprev[lo];
+        }
+        __syncthreads();
+      }
+
+      double* src = nullptr;
+      for (int k = 0; k < R; ++k) {
+        const int c = R - 1 - k;
+        const int n = d.dim[g][c];
+        const int f = d.fac[g][c];
+        const int lg = ceil_log2(n);
+        const int rpw_lg = 6 - lg;
+        const int rows = stage_rows(d, g, c);
+        const double* w = (c > 0) ? wbuf + d.wt_off[g][c] : nullptr;
+        double* dst = (c > 0) ? ((k & 1) ? xb : xa) : nullptr;
+        double* red = (k & 1) ? red1 : red0;
+        const int col = lane & ((1 << lg) - 1);
+        const int lr = lane >> lg;
+        const double qcol = (col < n) ? qv[d.S_off[f] + col] : 0.0;
+        double acc = 0.0;
+
+        auto body = [&](double x, int r) {
+          const bool rv = r < rows;
+          const double wr = rv ? (w ? w[r] : 1.0) : 0.0;
+          acc = fma(wr, x, acc);
+          double v = group_sum_low(x * qcol, lg);
+          if (dst && rv && col == 0) dst[r] = v;
+        };
+
+        if (k == 0) {
+          if constexpr (JMAX > 0) {
+            const int jb = d.jbase[g];
+            const int je = jb + d.jcount[g];
+#pragma unroll
+            for (int j = 0; j < JMAX; ++j) {
+              if (j >= jb && j < je) {
+                const int r = (((j - jb) * NW + wave) << rpw_lg) + lr;
+                body(X0[j], r);
+              }
+            }
+          } else {
+            const double* ws = p.ws + (long long)b * d.ws_stride + d.ws_off[g];
+            const int steps = (rows + (NW << rpw_lg) - 1) / (NW << rpw_lg);
+            for (int s = 0; s < steps; ++s) {
+              const int r = ((s * NW + wave) << rpw_lg) + lr;
+              const double x = (r < rows && col < n) ? ws[(long long)r * n + col] : 0.0;
+              body(x, r);
+            }
+          }
+        } else {
+          const int steps = (rows + (NW << rpw_lg) - 1) / (NW << rpw_lg);
+          for (int s = 0; s < steps; ++s) {
+            const int r = ((s * NW + wave) << rpw_lg) + lr;
+            const double x = (r < rows && col < n) ? src[r * n + col] : 0.0;
+            body(x, r);
+          }
+        }
+        stage_finish<NW>(acc, lg, n, f, d, red, log_q, tid, lane, wave);
+        src = dst;
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+
+  // ---- output: q = softmax(log_q) --------------------------------------------
+  softmax_factors<NW>(d, log_q, qv, lane, wave);
+  __syncthreads();
+  for (int i = tid; i < d.totS; i += NT) p.q[(long long)b * d.totS + i] = (float)qv[i];
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+namespace {
+
+thread_local std::string g_err;
+
+struct Workspace {
+  double* ptr = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_ws_mu;
+std::map<int, Workspace> g_ws;  // per device, grow-only
+
+int einval(const char* prefix, const std::string& msg) {
+  set_error(std::string(prefix) + msg);
+  return PYMDP_HIP_EINVAL;
+}
+
+}  // namespace
+
+void set_error(const std::string& msg) { g_err = msg; }
+const char* get_error() { return g_err.c_str(); }
+
+static int get_workspace(size_t bytes, double** out) {
+  int dev = 0;
+  PMH_HIP_CHECK(hipGetDevice(&dev), "fpi_ffi: ");
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  Workspace& w = g_ws[dev];
+  if (w.bytes < bytes) {
+    if (w.ptr) PMH_HIP_CHECK(hipFree(w.ptr), "fpi_ffi: ");
+    w.ptr = nullptr;
+    w.bytes = 0;
+    PMH_HIP_CHECK(hipMalloc(&w.ptr, bytes), "fpi_ffi: ");
+    w.bytes = bytes;
+  }
+  *out = w.ptr;
+  return 0;
+}
+
+// Validate the reference's FPI attributes (pymdp/ffi/fpi/fpi_entry.h:24-82)
+// and build the launch descriptor.  ll_sizes[m] is the per-agent element count
+// of modality m's log-likelihood.
+static int build_fpi_desc(FpiDesc* d, const int64_t* S, int64_t F, int64_t M, const int64_t* lp_offsets,
+                          const int64_t* A_dep_flat, const int64_t* A_dep_offsets, int32_t num_iter,
+                          int* jslots_out) {
+  const char* pfx = "fpi_ffi: ";
+  std::memset(d, 0, sizeof(*d));
+  if (F <= 0 || M <= 0) return einval(pfx, "invalid F=" + std::to_string(F) + " or M=" + std::to_string(M));
+  if (F > kMaxF) return einval(pfx, "F=" + std::to_string(F) + " exceeds the supported " + std::to_string(kMaxF));
+  if (M > kMaxM) return einval(pfx, "M=" + std::to_string(M) + " exceeds the supported " + std::to_string(kMaxM));
+  if (num_iter < 0) return einval(pfx, "num_iter = " + std::to_string(num_iter) + ", must be non-negative");
+  d->F = (int)F;
+  d->M = (int)M;
+  if (lp_offsets[0] != 0) return einval(pfx, "lp_offsets[0] must be 0");
+  for (int64_t f = 0; f < F; ++f) {
+    if (S[f] <= 0) return einval(pfx, "S[" + std::to_string(f) + "] must be positive");
+    if (lp_offsets[f + 1] - lp_offsets[f] != S[f])
+      return einval(pfx, "lp_offsets not consistent with S at factor " + std::to_string(f));
+    d->S[f] = (int)S[f];
+    d->S_off[f] = (int)lp_offsets[f];
+  }
+  d->S_off[F] = (int)lp_offsets[F];
+  d->totS = (int)lp_offsets[F];
+
+  // Group modalities by identical dependency tuple.
+  std::vector<std::vector<int64_t>> tuples;
+  if (A_dep_offsets[0] != 0) return einval(pfx, "A_dep_offsets[0] must be 0");
+  for (int64_t m = 0; m < M; ++m) {
+    const int64_t a = A_dep_offsets[m], e = A_dep_offsets[m + 1];
+    if (e < a) return einval(pfx, "A_dep_offsets must be monotonic");
+    const int64_t rank = e - a;
+    if (rank < 1 || rank > kMaxR) return einval(pfx, "A_dependencies[m] rank must be in [1, 8]");
+    std::vector<int64_t> t(A_dep_flat + a, A_dep_flat + e);
+    for (int64_t i = 0; i < rank; ++i) {
+      if (t[i] < 0 || t[i] >= F) return einval(pfx, "A_dependencies factor index out of range");
+      for (int64_t j = 0; j < i; ++j)
+        if (t[j] == t[i])
+          return einval(pfx, "modality " + std::to_string(m) + " has duplicate factor in A_dependencies");
+      if (S[t[i]] > kMaxDim)
+        return einval(pfx, "state dimension " + std::to_string(S[t[i]]) + " of a dependency exceeds " +
+                               std::to_string(kMaxDim));
+    }
+    int g = -1;
+    for (size_t k = 0; k < tuples.size(); ++k)
+      if (tuples[k] == t) g = (int)k;
+    if (g < 0) {
+      if ((int)tuples.size() == kMaxG) return einval(pfx, "more than 8 distinct A_dependencies tuples");
+      tuples.push_back(t);
+      g = (int)tuples.size() - 1;
+    }
+    d->mod_group[m] = g;
+    long long K = 1;
+    for (int64_t i = 0; i < rank; ++i) K *= S[t[i]];
+    d->K[m] = K;
+  }
+  d->G = (int)tuples.size();
+  int jslots = 0;
+  long long ws = 0;
+  int lds_w = 1, lds_x = 1;
+  for (int g = 0; g < d->G; ++g) {
+    const auto& t = tuples[g];
+    const int R = (int)t.size();
+    d->R[g] = R;
+    for (int i = 0; i < R; ++i) {
+      d->fac[g][i] = (int)t[i];
+      d->dim[g][i] = (int)S[t[i]];
+    }
+    int off = 0, rows = 1;
+    for (int c = 1; c < R; ++c) {
+      rows *= d->dim[g][c - 1];
+      d->wt_off[g][c] = off;
+      off += rows;
+    }
+    lds_w = std::max(lds_w, off);
+    lds_x = std::max(lds_x, rows);  // rows of stage 0 = prod_{j<R-1} dim[j]
+    const int n = d->dim[g][R - 1];
+    d->jbase[g] = 0;
+    d->jcount[g] = 0;
+    d->ws_off[g] = ws;
+    ws += (long long)rows * n;
+  }
+  d->ws_stride = (ws + 63) / 64 * 64;
+  d->lds_S = (d->totS + 3) / 4 * 4;
+  d->lds_w = (lds_w + 3) / 4 * 4;
+  d->lds_x = (lds_x + 3) / 4 * 4;
+  d->iters = num_iter;
+  *jslots_out = jslots;
+  return 0;
+}
+
+static size_t lds_bytes(const FpiDesc& d, int NW) {
+  return sizeof(double) * (size_t)(3 * d.lds_S + d.lds_w + 2 * d.lds_x + 2 * NW * kWave) +
+         sizeof(float) * (size_t)d.lds_O;
+}
+
+// Register slots per lane of stage 0 when the workgroup has NW waves.
+static int assign_slots(FpiDesc* d, int NW) {
+  int jslots = 0;
+  for (int g = 0; g < d->G; ++g) {
+    const int R = d->R[g];
+    int rows = 1;
+    for (int c = 0; c < R - 1; ++c) rows *= d->dim[g][c];
+    const int rpw = kWave >> ceil_log2(d->dim[g][R - 1]);
+    const int steps = (rows + NW * rpw - 1) / (NW * rpw);
+    d->jbase[g] = jslots;
+    d->jcount[g] = steps;
+    jslots += steps;
+  }
+  return jslots;
+}
+
+static int launch_fpi(const FpiDesc& d0, FpiPtrs p, int /*unused*/, int64_t batch, hipStream_t stream) {
+  const char* pfx = "fpi_ffi: ";
+  FpiDesc d = d0;
+  if (batch == 0) return 0;
+  // 4 waves when the log-likelihood fits 16 FP64 slots per lane, else 8 waves
+  // (up to 32 slots), else the streaming variant.
+  int NW = 4;
+  int jslots = assign_slots(&d, 4);
+  if (jslots > 16) {
+    NW = 8;
+    jslots = assign_slots(&d, 8);
+  }
+  const size_t lds = lds_bytes(d, NW);
+  if (lds > (size_t)kLdsBudget)
+    return einval(pfx, "model needs " + std::to_string(lds) + " bytes of LDS per agent (limit " +
+                           std::to_string(kLdsBudget) + ")");
+  const dim3 grid((unsigned)batch);
+  if (NW == 4 && jslots <= 4) {
+    hipLaunchKernelGGL((fpi_kernel<4, 256>), grid, dim3(256), lds, stream, d, p);
+  } else if (NW == 4) {
+    hipLaunchKernelGGL((fpi_kernel<16, 256>), grid, dim3(256), lds, stream, d, p);
+  } else if (jslots <= 32) {
+    hipLaunchKernelGGL((fpi_kernel<32, 512>), grid, dim3(512), lds, stream, d, p);
+  } else {
+    double* ws = nullptr;
+    int rc = get_workspace(sizeof(double) * (size_t)d.ws_stride * (size_t)batch, &ws);
+    if (rc) return rc;
+    p.ws = ws;
+    hipLaunchKernelGGL((fpi_kernel<0, 512>), grid, dim3(512), lds, stream, d, p);
+  }
+  PMH_HIP_CHECK(hipGetLastError(), pfx);
+  return 0;
+}
+
+}  // namespace pmh
+
+using namespace pmh;
+
+extern "C" const char* pymdp_hip_last_error(void) { return pmh::get_error(); }
+
+extern "C" const char* pymdp_hip_version(void) { return "pymdp_hip 0.1.0 gfx950"; }
+
+extern "C" int pymdp_hip_fpi(const float* ll_flat, const float* lp_flat, float* q_out, int64_t ll_count,
+                             int64_t lp_count, int64_t q_count, const int64_t* S, int64_t F,
+                             const int64_t* ll_offsets, int64_t M, const int64_t* lp_offsets,
+                             const int64_t* A_dep_flat, const int64_t* A_dep_offsets, int32_t num_iter,
+                             void* stream) {
+  const char* pfx = "fpi_ffi: ";
+  pmh::set_error("");
+  if (num_iter <= 0)
+    return einval(pfx, "num_iter = " + std::to_string(num_iter) + ", must be positive");
+  FpiDesc d;
+  int jslots = 0;
+  int rc = build_fpi_desc(&d, S, F, M, lp_offsets, A_dep_flat, A_dep_offsets, num_iter, &jslots);
+  if (rc) return rc;
+  if (ll_offsets[0] != 0) return einval(pfx, "ll_offsets[0] must be 0");
+  for (int64_t m = 0; m < M; ++m) {
+    if (ll_offsets[m + 1] < ll_offsets[m]) return einval(pfx, "ll_offsets must be monotonic");
+    if (ll_offsets[m + 1] - ll_offsets[m] != d.K[m])
+      return einval(pfx, "ll_offsets not consistent with S[A_dependencies] at modality " + std::to_string(m));
+    d.ll_off[m] = ll_offsets[m];
+  }
+  d.ll_stride = ll_offsets[M];
+  const int64_t totS = d.totS;
+  if (lp_count <= 0 || lp_count % totS != 0)
+    return einval(pfx, "lp_flat size = " + std::to_string(lp_count) + " not divisible by total_S = " +
+                           std::to_string(totS));
+  const int64_t batch = lp_count / totS;
+  if (ll_count != batch * d.ll_stride)
+    return einval(pfx, "ll_flat size = " + std::to_string(ll_count) + ", expected " +
+                           std::to_string(batch * d.ll_stride));
+  if (q_count != batch * totS)
+    return einval(pfx, "q_out size = " + std::to_string(q_count) + ", expected " + std::to_string(batch * totS));
+  if (F == 1) d.iters = 1;
+  d.mode = 0;
+  FpiPtrs p{};
+  p.ll = ll_flat;
+  p.lp = lp_flat;
+  p.q = q_out;
+  return launch_fpi(d, p, jslots, batch, (hipStream_t)stream);
+}
+
+extern "C" int pymdp_hip_infer_states(const float* const* A_ptrs, const int64_t* A_batch_stride,
+                                      const void* obs, int32_t obs_is_index,
+                                      const float* const* prior_ptrs, const int64_t* prior_batch_stride,
+                                      float* q_out, int64_t batch,
+                                      const int64_t* S, int64_t F, const int64_t* O, int64_t M,
+                                      const int64_t* obs_offsets, const int64_t* lp_offsets,
+                                      const int64_t* A_dep_flat, const int64_t* A_dep_offsets, int32_t num_iter,
+                                      int32_t method_exact, void* stream) {
+  const char* pfx = "fpi_ffi: ";
+  pmh::set_error("");
+  if (batch < 0) return einval(pfx, "batch must be non-negative");
+  FpiDesc d;
+  int jslots = 0;
+  int rc = build_fpi_desc(&d, S, F, M, lp_offsets, A_dep_flat, A_dep_offsets, num_iter, &jslots);
+  if (rc) return rc;
+  if (obs_offsets[0] != 0) return einval(pfx, "obs_offsets[0] must be 0");
+  for (int64_t m = 0; m < M; ++m) {
+    if (O[m] <= 0) return einval(pfx, "O[" + std::to_string(m) + "] must be positive");
+    if (obs_offsets[m + 1] - obs_offsets[m] != O[m]) return einval(pfx, "obs_offsets not consistent with O");
+    if (A_ptrs[m] == nullptr) return einval(pfx, "A[" + std::to_string(m) + "] is null");
+    if (A_batch_stride[m] < 0) return einval(pfx, "negative A batch stride");
+    d.O[m] = (int)O[m];
+    d.obs_off[m] = (int)obs_offsets[m];
+    d.A_stride[m] = A_batch_stride[m];
+  }
+  d.sumO = (int)obs_offsets[M];
+  d.lds_O = (d.sumO + 3) / 4 * 4;
+  d.mode = 1;
+  d.obs_index = obs_is_index ? 1 : 0;
+  if (method_exact) d.iters = 1;
+  if (F == 1) d.iters = 1;  // closed form, pymdp/algos.py:101-104
+  FpiPtrs p{};
+  p.obs = obs;
+  p.q = q_out;
+  for (int64_t m = 0; m < M; ++m) p.A[m] = A_ptrs[m];
+  for (int64_t f = 0; f < F; ++f) {
+    if (prior_ptrs[f] == nullptr) return einval(pfx, "prior[" + std::to_string(f) + "] is null");
+    if (prior_batch_stride[f] < 0) return einval(pfx, "negative prior batch stride");
+    p.prior[f] = prior_ptrs[f];
+    d.prior_stride[f] = prior_batch_stride[f];
+  }
+  return launch_fpi(d, p, jslots, batch, (hipStream_t)stream);
+}

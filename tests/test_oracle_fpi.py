@@ -1,0 +1,174 @@
+"""Pins of the FPI oracle (CPU only).
+
+The oracle (oracle/numpy_ref.py) restates pymdp/algos.py:53-155 and
+pymdp/maths.py:214-287.  It is pinned here by
+  * an independent brute-force restatement (tests/brute.py, explicit loops),
+  * the reference's own closed forms and invariants:
+      - one factor: softmax(ln A[o] + ln prior)          test/test_fpi.py:38
+      - a factor no modality touches returns its prior    test/test_fpi.py:198
+      - factorized FPI with full deps == vanilla FPI      test/test_message_passing_jax.py:90-118
+      - sparse deps == vanilla on the tiled (full) A      test/test_message_passing_jax.py:120-151
+  * mutation checks: a planted mistake in a copy of the oracle must be caught.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from oracle import numpy_ref as R
+from tests import brute
+
+
+def _rand_model(rng, S, O, deps, alpha=1.0):
+    A = []
+    for m, o in enumerate(O):
+        shp = (o,) + tuple(S[d] for d in deps[m])
+        a = rng.gamma(alpha, size=shp)
+        A.append(a / a.sum(0, keepdims=True))
+    prior = [rng.dirichlet(np.ones(s)) for s in S]
+    obs = [np.eye(o)[rng.integers(o)] for o in O]
+    return A, obs, prior
+
+
+CASES = [
+    ([3], [3], [[0]]),
+    ([3], [3, 2], [[0], [0]]),
+    ([4, 3], [5, 6], [[0, 1], [0, 1]]),
+    ([3, 2, 4], [3, 4, 2], [[0, 2], [1], [2, 1, 0]]),
+    ([2, 3, 2], [2, 2], [[0, 1, 2], [1]]),
+    ([3, 2, 2, 2], [3, 2], [[0, 1, 2, 3], [3, 0]]),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_oracle_matches_bruteforce(case):
+    S, O, deps = CASES[case]
+    rng = np.random.default_rng(100 + case)
+    A, obs, prior = _rand_model(rng, S, O, deps, alpha=0.7)
+    for num_iter in (1, 3, 16):
+        q_o = R.run_factorized_fpi(A, obs, prior, deps, num_iter=num_iter, dt=np.float64)
+        q_b = brute.fpi(A, obs, prior, deps, num_iter)
+        for a, b in zip(q_o, q_b):
+            np.testing.assert_allclose(a, b, rtol=0, atol=1e-12)
+
+
+def test_single_factor_closed_form():
+    rng = np.random.default_rng(1)
+    A = [rng.dirichlet(np.ones(3), size=3).T]
+    prior = [rng.dirichlet(np.ones(3))]
+    o = 2
+    obs = [np.eye(3)[o]]
+    q = R.run_factorized_fpi(A, obs, prior, [[0]], num_iter=16, dt=np.float64)[0]
+    expect = R.softmax(np.log(A[0][o]) + np.log(prior[0]))
+    np.testing.assert_allclose(q, expect, atol=1e-14)
+
+
+def test_unused_factor_returns_prior():
+    rng = np.random.default_rng(2)
+    S, O, deps = [3, 4], [3, 2], [[0], [0]]
+    A, obs, prior = _rand_model(rng, S, O, deps)
+    q = R.run_factorized_fpi(A, obs, prior, deps, num_iter=16, dt=np.float64)
+    np.testing.assert_allclose(q[1], prior[1], atol=1e-14)
+
+
+def test_factorized_full_deps_equals_vanilla():
+    rng = np.random.default_rng(3)
+    S, O = [4, 3, 2], [3, 5]
+    deps = [[0, 1, 2]] * 2
+    A, obs, prior = _rand_model(rng, S, O, deps)
+    qv = R.run_vanilla_fpi(A, obs, prior, num_iter=16, dt=np.float64)
+    qf = R.run_factorized_fpi(A, obs, prior, deps, num_iter=16, dt=np.float64)
+    for a, b in zip(qv, qf):
+        np.testing.assert_allclose(a, b, atol=1e-12)
+
+
+def test_sparse_deps_equals_vanilla_on_tiled_A():
+    rng = np.random.default_rng(4)
+    S, O, deps = [3, 2, 4], [3, 4], [[0], [1, 2]]
+    A, obs, prior = _rand_model(rng, S, O, deps)
+    A_full = []
+    for m, a in enumerate(A):
+        other = [f for f in range(len(S)) if f not in deps[m]]
+        shp = [O[m]] + [1 if f in other else S[f] for f in range(len(S))]
+        # deps are sorted here, so the reshape keeps axis order
+        tile = [1] + [S[f] if f in other else 1 for f in range(len(S))]
+        A_full.append(np.tile(a.reshape(shp), tile))
+    qf = R.run_factorized_fpi(A, obs, prior, deps, num_iter=16, dt=np.float64)
+    qv = R.run_vanilla_fpi(A_full, obs, prior, num_iter=16, dt=np.float64)
+    for a, b in zip(qf, qv):
+        np.testing.assert_allclose(a, b, atol=1e-12)
+
+
+def test_log_stable_floor_is_float64_eps_in_fp32():
+    # pymdp/maths.py:15: MINVAL = float64 eps; ln(MINVAL) = -36.04, not ln(FLT_EPSILON) = -15.94
+    v = R.log_stable(np.array([0.0, 1e-30], dtype=np.float32), np.float32)
+    assert abs(float(v[0]) - np.log(np.float32(2.220446049250313e-16))) < 1e-5
+    assert float(v[0]) < -36.0
+
+
+def test_fp32_oracle_close_to_fp64():
+    rng = np.random.default_rng(5)
+    S, O, deps = [4, 3], [5, 6], [[0, 1], [0, 1]]
+    A, obs, prior = _rand_model(rng, S, O, deps)
+    A32 = [a.astype(np.float32) for a in A]
+    p32 = [p.astype(np.float32) for p in prior]
+    q32 = R.run_factorized_fpi(A32, obs, p32, deps, num_iter=16, dt=np.float32)
+    q64 = R.run_factorized_fpi(A32, obs, p32, deps, num_iter=16, dt=np.float64)
+    for a, b in zip(q32, q64):
+        np.testing.assert_allclose(a, b, atol=1e-5)
+
+
+def test_fpi_from_ll_matches_full_path():
+    rng = np.random.default_rng(6)
+    S, O, deps = [3, 2, 4], [3, 4], [[0, 2], [1, 2]]
+    A, obs, prior = _rand_model(rng, S, O, deps)
+    ll = R.compute_log_likelihood_per_modality(obs, A, np.float64)
+    lp = [R.log_stable(p, np.float64) for p in prior]
+    a = R.fpi_from_ll(ll, lp, deps, 16, np.float64)
+    b = R.run_factorized_fpi(A, obs, prior, deps, num_iter=16, dt=np.float64)
+    for x, y in zip(a, b):
+        np.testing.assert_allclose(x, y, atol=1e-14)
+
+
+# ---- mutation checks: the pins above must catch plausible mistakes ----------
+
+def _gauss_seidel_fpi(A, obs, prior, deps, num_iter):
+    """Planted mistake: sequential (Gauss-Seidel) instead of Jacobi updates."""
+    lls = R.compute_log_likelihood_per_modality(obs, A, np.float64)
+    lp = [R.log_stable(p, np.float64) for p in prior]
+    log_q = [np.zeros_like(x) for x in lp]
+    for _ in range(num_iter):
+        for f in range(len(prior)):
+            q = [R.softmax(l) for l in log_q]
+            acc = np.zeros_like(lp[f])
+            for m, dl in enumerate(deps):
+                if f in dl:
+                    i = dl.index(f)
+                    acc = acc + R.marginal_log_likelihood([q[d] for d in dl], lls[m], i)
+            log_q[f] = acc + lp[f]
+    return [R.softmax(l) for l in log_q]
+
+
+def test_bruteforce_catches_gauss_seidel_mistake():
+    rng = np.random.default_rng(7)
+    S, O, deps = [4, 3], [5, 6], [[0, 1], [0, 1]]
+    A, obs, prior = _rand_model(rng, S, O, deps, alpha=0.3)
+    bad = _gauss_seidel_fpi(A, obs, prior, deps, 2)
+    good = brute.fpi(A, obs, prior, deps, 2)
+    assert max(np.abs(np.asarray(a) - np.asarray(b)).max() for a, b in zip(bad, good)) > 1e-6
+
+
+def test_bruteforce_catches_transposed_dependency():
+    rng = np.random.default_rng(8)
+    S, O, deps = [3, 4], [5], [[0, 1]]
+    A, obs, prior = _rand_model(rng, S, O, deps)
+    good = brute.fpi(A, obs, prior, deps, 4)
+    # planted: the modality's axes read in the wrong order
+    A_t = [np.ascontiguousarray(np.swapaxes(A[0], 1, 2))]
+    bad = R.run_factorized_fpi(A_t, obs, prior, [[1, 0]], num_iter=4, dt=np.float64)
+    # a transposed tensor with swapped deps is the SAME model (must agree) ...
+    for a, b in zip(bad, good):
+        np.testing.assert_allclose(a, b, atol=1e-12)
+    # ... but reading it with the original deps must not
+    with pytest.raises(Exception):
+        R.run_factorized_fpi(A_t, obs, prior, [[0, 1]], num_iter=4, dt=np.float64)
