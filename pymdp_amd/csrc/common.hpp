@@ -42,26 +42,105 @@ __device__ __forceinline__ float stable_xlogx(float x) {
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
-__device__ __forceinline__ float wave_sum(float v) {
+// ---------------------------------------------------------------------------
+// ln(x) of a float, evaluated to FP64 accuracy (~1e-16 relative).
+// x = 2^e * m with m in [1, 2); m = c_i * (1 + r), c_i = 1 + i/128 (i = top 7
+// mantissa bits), so ln x = e ln2 + ln c_i + ln(1 + r) with 0 <= r < 2^-7 and
+// ln(1 + r) from its Taylor series (degree 7: truncation < 2^-59).  The two
+// 128-entry tables (ln c_i, 1/c_i) are computed on the host in double and
+// copied to constant memory once per device (ensure_log_table).
+// ---------------------------------------------------------------------------
+static __constant__ double g_logtab[256];  // [0,128): ln(1+i/128); [128,256): 1/(1+i/128)
+
+// Per translation unit (internal linkage: each TU has its own g_logtab and
+// its own "initialised" flags).
+namespace {
+inline int ensure_log_table() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 2;
+  static bool done[64] = {false};
+  if (dev < 0 || dev >= 64) return 2;
+  if (done[dev]) return 0;
+  double h[256];
+  for (int i = 0; i < 128; ++i) {
+    const double c = 1.0 + i / 128.0;
+    h[i] = __builtin_log(c);
+    h[128 + i] = 1.0 / c;
+  }
+  if (hipMemcpyToSymbol(HIP_SYMBOL(g_logtab), h, sizeof(h)) != hipSuccess) return 2;
+  done[dev] = true;
+  return 0;
+}
+}  // namespace
+
+// x must be a positive normal float (callers clip at kMinVal first).
+__device__ __forceinline__ double log_f64(float x) {
+  const unsigned bits = __float_as_uint(x);
+  const int e = (int)(bits >> 23) - 127;
+  const unsigned mb = bits & 0x7FFFFFu;
+  const int i = (int)(mb >> 16);
+  const double m = 1.0 + (double)mb * 0x1p-23;
+  const double c = 1.0 + (double)i * 0x1p-7;
+  const double r = (m - c) * g_logtab[128 + i];
+  const double p =
+      r * (1.0 + r * (-0.5 + r * (1.0 / 3.0 + r * (-0.25 + r * (0.2 + r * (-1.0 / 6.0 + r * (1.0 / 7.0)))))));
+  return (double)e * 0.69314718055994530942 + g_logtab[i] + p;
+}
+
+// ln(x) of a positive normal double to ~1e-16 relative, with the two tables
+// given by pointer (e.g. staged in LDS): tab[0,128) = ln(1 + i/128),
+// tab[128,256) = 1/(1 + i/128).  Degree-6 series, truncation < 2^-49 / 7.
+__device__ __forceinline__ double log_f64d(double x, const double* tab) {
+  const long long bits = __double_as_longlong(x);
+  const int e = (int)((bits >> 52) & 0x7FF) - 1023;
+  const long long mb = bits & 0xFFFFFFFFFFFFFll;
+  const int i = (int)(mb >> 45);
+  const double m = __longlong_as_double(mb | 0x3FF0000000000000ll);
+  const double c = 1.0 + (double)i * 0x1p-7;
+  const double r = (m - c) * tab[128 + i];
+  const double p = r * (1.0 + r * (-0.5 + r * (1.0 / 3.0 + r * (-0.25 + r * (0.2 + r * (-1.0 / 6.0))))));
+  return (double)e * 0.69314718055994530942 + tab[i] + p;
+}
+
+// log_stable in FP64 accuracy: ln(max(x, MINVAL)).
+__device__ __forceinline__ double log_stable_f64(float x) { return log_f64(fmaxf(x, kMinVal)); }
+
+// x ln x with the reference's clip (0 for x == 0), in FP64 accuracy.
+__device__ __forceinline__ double xlogx_f64(float x) { return x == 0.0f ? 0.0 : (double)x * log_stable_f64(x); }
+
+// q ln q of an FP64 value; the log argument is rounded to float first
+// (relative 6e-8, i.e. an absolute 6e-8 on ln q), the product stays FP64.
+__device__ __forceinline__ double xlogx_d(double q) {
+  return q == 0.0 ? 0.0 : q * log_stable_f64((float)q);
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
   return v;
 }
 
-__device__ __forceinline__ float wave_max(float v) {
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
+  for (int o = 32; o > 0; o >>= 1) {
+    const T u = __shfl_xor(v, o, kWave);
+    v = u > v ? u : v;
+  }
   return v;
 }
 
 // Sum over the low `log2w` lane bits (groups of 2^log2w adjacent lanes).
-__device__ __forceinline__ float group_sum_low(float v, int log2w) {
+template <typename T>
+__device__ __forceinline__ T group_sum_low(T v, int log2w) {
   for (int b = 0; b < log2w; ++b) v += __shfl_xor(v, 1 << b, kWave);
   return v;
 }
 
 // Sum over lane bits [log2w, 6): lanes that share (lane & (2^log2w - 1)).
-__device__ __forceinline__ float group_sum_high(float v, int log2w) {
+template <typename T>
+__device__ __forceinline__ T group_sum_high(T v, int log2w) {
   for (int b = log2w; b < 6; ++b) v += __shfl_xor(v, 1 << b, kWave);
   return v;
 }

@@ -44,6 +44,10 @@ constexpr int kMaxM = 32;
 constexpr int kMaxG = 8;
 constexpr int kNT = 256;
 
+constexpr int G1_BM = 64;  // GEMM1 tile: rows x inner tuples x k
+constexpr int G1_BN = 64;
+constexpr int G1_BK = 16;
+
 constexpr int kFlagUseUtility = 1 << 0;
 constexpr int kFlagUseStatesInfoGain = 1 << 1;
 constexpr int kFlagUseInductive = 1 << 2;
@@ -69,6 +73,9 @@ struct Desc {
   long long sc_off[kMaxT][kMaxM];
   long long fsc_off[kMaxT][kMaxF];  // factor (inductive) scores [n]
   long long ha_off[kMaxM];
+  int g_ntot[kMaxG];  // inner tuples of all levels
+  long long qs0_off;  // qs_init in FP64
+  long long g_off;    // FP64 logits / exp of the policy posterior [P]
   long long ws_stride;
 };
 
@@ -103,14 +110,19 @@ struct Args {
   float eps_scalar;  // used when eps is null
   float* G;     // [batch, P]
   float* q_pi;  // [batch, P] or null
-  float* ws;    // workspace [batch, ws_stride]
+  double* ws;   // FP64 workspace [batch, ws_stride]
   int qs_off[kMaxF + 1];
   int flags;
 };
 
 // ---------------------------------------------------------------------------
-// 1. HAx_m[k] = sum_o xlogx(A_m[o, k])
+// All intermediates are FP64 (inputs are FP32): the policy scores sum 2 T M
+// entropy / ambiguity terms of opposite sign, and gamma = 16 amplifies their
+// rounding in q_pi.  FP64 vector FMA runs at the FP32 (non-packed) rate on
+// gfx950, so the accuracy costs register space, not issue slots.
 // ---------------------------------------------------------------------------
+
+// 1. HAx_m[k] = sum_o xlogx(A_m[o, k])
 __global__ __launch_bounds__(kNT) void efe_ha_kernel(const Desc* __restrict__ dp, const Args a) {
   const Desc& d = *dp;
   const int m = blockIdx.y;
@@ -121,20 +133,23 @@ __global__ __launch_bounds__(kNT) void efe_ha_kernel(const Desc* __restrict__ dp
   const long long k = (long long)blockIdx.x * kNT + threadIdx.x;
   if (k >= K) return;
   const float* Am = a.A[m] + (long long)b * a.A_stride[m];
-  float acc = 0.0f;
-  for (int o = 0; o < O; ++o) acc += stable_xlogx(Am[(long long)o * K + k]);
+  double acc = 0.0;
+  for (int o = 0; o < O; ++o) acc += xlogx_f64(Am[(long long)o * K + k]);
   a.ws[(long long)b * d.ws_stride + d.ha_off[m] + k] = acc;
 }
 
-// ---------------------------------------------------------------------------
 // 2. Rollout: qs[t][f][h] = B_f[:, parents..., u_h] . (x parent marginals)
-// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kNT) void efe_rollout_kernel(const Desc* __restrict__ dp, const int* __restrict__ tab,
                                                           const Args a) {
   const Desc& d = *dp;
   const int b = blockIdx.x;
-  float* ws = a.ws + (long long)b * d.ws_stride;
-  const float* qs0 = a.qs + (long long)b * a.qs_stride;
+  double* ws = a.ws + (long long)b * d.ws_stride;
+  // level -1 (qs_init) into the workspace in FP64
+  const float* qin = a.qs + (long long)b * a.qs_stride;
+  for (int i = threadIdx.x; i < a.qs_off[d.F]; i += kNT) ws[d.qs0_off + i] = (double)qin[i];
+  __threadfence_block();
+  __syncthreads();
+  const double* qs0 = ws + d.qs0_off;
   for (int t = 0; t < d.T; ++t) {
     for (int f = 0; f < d.F; ++f) {
       const int n = d.fn[t][f];
@@ -145,34 +160,34 @@ __global__ __launch_bounds__(kNT) void efe_rollout_kernel(const Desc* __restrict
       const float* Bf = a.B[f] + (long long)b * a.B_stride[f];
       const int* par = tab + d.f_par_off[t][f];
       const int* act = tab + d.f_act_off[t][f];
-      float* out = ws + d.qs_off[t][f];
+      double* out = ws + d.qs_off[t][f];
       const long long total = (long long)n * S;
       for (long long i = threadIdx.x; i < total; i += kNT) {
         const int h = (int)(i / S);
         const int s = (int)(i - (long long)h * S);
         const int u = act[h];
         const float* row = Bf + (long long)s * Kb * U + u;
-        float acc = 0.0f;
+        double acc = 0.0;
         if (rank == 1) {
           const int pf = d.Bdeps[f][0];
-          const float* q = (t == 0) ? qs0 + a.qs_off[pf] : ws + d.qs_off[t - 1][pf] + (long long)par[h] * d.S[pf];
-          for (int k = 0; k < Kb; ++k) acc = fmaf(row[(long long)k * U], q[k], acc);
+          const double* q = (t == 0) ? qs0 + a.qs_off[pf] : ws + d.qs_off[t - 1][pf] + (long long)par[h] * d.S[pf];
+          for (int k = 0; k < Kb; ++k) acc = fma((double)row[(long long)k * U], q[k], acc);
         } else {
-          const float* qp[8];
+          const double* qp[8];
           for (int r = 0; r < rank; ++r) {
             const int pf = d.Bdeps[f][r];
             qp[r] = (t == 0) ? qs0 + a.qs_off[pf] : ws + d.qs_off[t - 1][pf] + (long long)par[h * rank + r] * d.S[pf];
           }
           for (int k = 0; k < Kb; ++k) {
             int rem = k;
-            float w = 1.0f;
+            double w = 1.0;
             for (int r = rank - 1; r >= 0; --r) {
               const int sz = d.S[d.Bdeps[f][r]];
               const int c = rem % sz;
               rem /= sz;
               w *= qp[r][c];
             }
-            acc = fmaf(row[(long long)k * U], w, acc);
+            acc = fma((double)row[(long long)k * U], w, acc);
           }
         }
         out[(long long)h * S + s] = acc;
@@ -183,10 +198,8 @@ __global__ __launch_bounds__(kNT) void efe_rollout_kernel(const Desc* __restrict
   }
 }
 
-// ---------------------------------------------------------------------------
 // 3. Kronecker vector of each realised inner tuple:
 //    Qin[t][g][n][k] = prod_{i>=1} qs[t][deps_i][h_i(n)][s_i(k)]
-// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kNT) void efe_kron_kernel(const Desc* __restrict__ dp, const int* __restrict__ tab,
                                                        const Args a) {
   const Desc& d = *dp;
@@ -199,12 +212,12 @@ __global__ __launch_bounds__(kNT) void efe_kron_kernel(const Desc* __restrict__ 
   const int n_in = d.g_nin[t][g];
   const int Kr = d.gKrest[g];
   const long long total = (long long)n_in * Kr;
-  float* ws = a.ws + (long long)b * d.ws_stride;
+  double* ws = a.ws + (long long)b * d.ws_stride;
   const int* tup = tab + d.g_tup_off[t][g];
   for (long long i = (long long)blockIdx.x * kNT + threadIdx.x; i < total; i += (long long)gridDim.x * kNT) {
     const int n = (int)(i / Kr);
     int rem = (int)(i - (long long)n * Kr);
-    float w = 1.0f;
+    double w = 1.0;
     for (int r = R - 1; r >= 1; --r) {
       const int f = d.gdeps[g][r];
       const int sz = d.S[f];
@@ -217,108 +230,189 @@ __global__ __launch_bounds__(kNT) void efe_kron_kernel(const Desc* __restrict__ 
   }
 }
 
-// ---------------------------------------------------------------------------
-// 4. W[t][m][n][s0][o'] = sum_k Arow(o', s0)[k] Qin[t][g][n][k]
-//    Arow(o', s0) = A_m[o', s0, :] for o' < O, HAx_m[s0, :] for o' == O.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kNT) void efe_gemm1_kernel(const Desc* __restrict__ dp,
-                                                        const Item1* __restrict__ items, const Args a) {
+// 4. W[m][nn][s0][o'] = sum_k Arow(o', s0)[k] Qin[g][nn][k] for the inner
+//    tuples nn of every level at once (one LDS-tiled FP64 GEMM per agent and
+//    modality, A read once per 64-tuple column tile).  Arow(o', s0) = A_m[o', s0, :]
+//    for o' < O and HAx_m[s0, :] for o' == O; rank-1 modalities copy A.
+__global__ __launch_bounds__(256) void efe_gemm1_kernel(const Desc* __restrict__ dp, const Args a) {
   const Desc& d = *dp;
-  const Item1 it = items[blockIdx.x];
-  const int b = blockIdx.y;
-  const int t = it.t, m = it.m, n = it.n;
+  const int m = blockIdx.y;
+  const int b = blockIdx.z;
   const int g = d.mod_group[m];
   const int O = d.O[m];
   const int S0 = d.gS0[g];
   const int Kr = d.gKrest[g];
   const int R = d.gR[g];
   const bool sig = (a.flags & kFlagUseStatesInfoGain) != 0;
-  const int rows = (O + 1) * S0;
-  float* ws = a.ws + (long long)b * d.ws_stride;
+  const int rows = (O + 1) * S0;  // A rows j = o' * S0 + s0
+  const int ntot = d.g_ntot[g];
+  const int tiles_m = (rows + G1_BM - 1) / G1_BM;
+  const int tiles_n = (ntot + G1_BN - 1) / G1_BN;
+  if ((int)blockIdx.x >= tiles_m * tiles_n) return;
+  const int row0 = ((int)blockIdx.x % tiles_m) * G1_BM;
+  const int col0 = ((int)blockIdx.x / tiles_m) * G1_BN;
+  double* ws = a.ws + (long long)b * d.ws_stride;
   const float* Am = a.A[m] + (long long)b * a.A_stride[m];
-  const float* ha = ws + d.ha_off[m];
-  const float* q = ws + d.qin_off[t][g] + (long long)n * Kr;
-  float* W = ws + d.w_off[t][m] + (long long)n * S0 * (O + 1);
-  for (int r = threadIdx.x; r < rows; r += kNT) {
-    const int s0 = r / (O + 1);
-    const int o = r - s0 * (O + 1);
-    float acc = 0.0f;
-    if (o < O || sig) {
-      const float* row = (o < O) ? Am + ((long long)o * S0 + s0) * Kr : ha + (long long)s0 * Kr;
-      if (R < 2) {
-        acc = row[0];
-      } else {
-        for (int k = 0; k < Kr; ++k) acc = fmaf(row[k], q[k], acc);
-      }
+  const double* ha = ws + d.ha_off[m];
+  const double* Q = ws + d.qin_off[0][g];  // all levels, [ntot, Kr]
+  double* W = ws + d.w_off[0][m];           // all levels, [ntot, S0, O + 1]
+  const int tid = threadIdx.x;
+  if (R < 2) {
+    // W[nn][s0][o'] = Arow(o', s0) for every level
+    for (int e = tid; e < G1_BM * G1_BN; e += 256) {
+      const int r = row0 + e % G1_BM;
+      const int nn = col0 + e / G1_BM;
+      if (r >= rows || nn >= ntot) continue;
+      const int o = r / S0, s0 = r - o * S0;
+      const double v = (o < O) ? (double)Am[r] : (sig ? ha[s0] : 0.0);
+      W[((long long)nn * S0 + s0) * (O + 1) + o] = v;
     }
-    W[r] = acc;
+    return;
+  }
+  __shared__ double As[G1_BK][G1_BM + 1];
+  __shared__ double Qs[G1_BK][G1_BN + 1];
+  const int tr = tid >> 4;  // 16 x 16 threads, 4 x 4 outputs each
+  const int tc = tid & 15;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+  for (int k0 = 0; k0 < Kr; k0 += G1_BK) {
+#pragma unroll
+    for (int l = 0; l < (G1_BM * G1_BK) / 256; ++l) {
+      const int e = tid + 256 * l;
+      const int kk = e % G1_BK, rr = e / G1_BK;
+      const int r = row0 + rr, k = k0 + kk;
+      double v = 0.0;
+      if (r < rows && k < Kr) {
+        if (r < O * S0) v = (double)Am[(long long)r * Kr + k];
+        else if (sig) v = ha[(long long)(r - O * S0) * Kr + k];
+      }
+      As[kk][rr] = v;
+    }
+#pragma unroll
+    for (int l = 0; l < (G1_BN * G1_BK) / 256; ++l) {
+      const int e = tid + 256 * l;
+      const int kk = e % G1_BK, cc = e / G1_BK;
+      const int nn = col0 + cc, k = k0 + kk;
+      Qs[kk][cc] = (nn < ntot && k < Kr) ? Q[(long long)nn * Kr + k] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < G1_BK; ++kk) {
+      double av[4], qv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = As[kk][tr * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) qv[j] = Qs[kk][tc * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fma(av[i], qv[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = row0 + tr * 4 + i;
+    if (r >= rows) continue;
+    const int o = r / S0, s0 = r - o * S0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int nn = col0 + tc * 4 + j;
+      if (nn < ntot) W[((long long)nn * S0 + s0) * (O + 1) + o] = acc[i][j];
+    }
   }
 }
 
-// ---------------------------------------------------------------------------
 // 5. Scores: for every outer history h0 of an inner tuple n,
 //    qo[o'] = sum_s0 W[n][s0][o'] q0[h0][s0];  score = H(qo) + qo[O] + C_t . qo
-// ---------------------------------------------------------------------------
-template <int CH>
+//    One workgroup per (t, m, n, block of 512 outer histories); W[n] is staged
+//    in LDS (read as broadcasts), each thread owns two outer histories and
+//    walks the outcomes in chunks of 16 FP64 accumulators per history.
+constexpr int kScoreCols = 2 * kNT;
+template <bool LDSW>
 __global__ __launch_bounds__(kNT) void efe_score_kernel(const Desc* __restrict__ dp,
                                                         const Item2* __restrict__ items, const Args a) {
+  constexpr int CH = 16;
+  extern __shared__ double wl[];
   const Desc& d = *dp;
   const Item2 it = items[blockIdx.x];
   const int b = blockIdx.y;
   const int t = it.t, m = it.m, n = it.n;
   const int g = d.mod_group[m];
   const int O = d.O[m];
+  const int O1 = O + 1;
   const int S0 = d.gS0[g];
   const int f0 = d.gdeps[g][0];
   const int H0 = d.g_H0[t][g];
-  const int h0 = it.hb * kNT + threadIdx.x;
   const bool sig = (a.flags & kFlagUseStatesInfoGain) != 0;
   const bool util = (a.flags & kFlagUseUtility) != 0;
-  float* ws = a.ws + (long long)b * d.ws_stride;
-  const float* W = ws + d.w_off[t][m] + (long long)n * S0 * (O + 1);
+  double* ws = a.ws + (long long)b * d.ws_stride;
+  const double* Wg = ws + d.w_off[t][m] + (long long)n * S0 * O1;
+  const double* W = Wg;
+  if constexpr (LDSW) {
+    for (int i = threadIdx.x; i < S0 * O1; i += kNT) wl[i] = Wg[i];
+    __syncthreads();
+    W = wl;
+  }
   const float* Ct = a.C[m] + (long long)b * a.C_stride[m] + (a.C_time[m] ? (long long)t * O : 0);
-  const bool active = h0 < H0;
-  const float* q0 = ws + d.qs_off[t][f0] + (long long)(active ? h0 : 0) * S0;
-  float ent = 0.0f, ut = 0.0f, hq = 0.0f;
-  for (int o0 = 0; o0 <= O; o0 += CH) {
-    float acc[CH];
+  const int ha = it.hb * kScoreCols + threadIdx.x;
+  const int hb = ha + kNT;
+  if (ha >= H0) return;
+  const bool has_b = hb < H0;
+  const double* qa = ws + d.qs_off[t][f0] + (long long)ha * S0;
+  const double* qb = ws + d.qs_off[t][f0] + (long long)(has_b ? hb : ha) * S0;
+  double ent_a = 0.0, ent_b = 0.0, ut_a = 0.0, ut_b = 0.0;
+  for (int o0 = 0; o0 < O; o0 += CH) {
+    double acc_a[CH], acc_b[CH];
 #pragma unroll
-    for (int i = 0; i < CH; ++i) acc[i] = 0.0f;
+    for (int i = 0; i < CH; ++i) acc_a[i] = acc_b[i] = 0.0;
     for (int s = 0; s < S0; ++s) {
-      const float qv = q0[s];
-      const float* wr = W + (long long)s * (O + 1) + o0;
+      const double xa = qa[s];
+      const double xb = qb[s];
+      const double* wr = W + s * O1 + o0;
 #pragma unroll
-      for (int i = 0; i < CH; ++i)
-        if (o0 + i <= O) acc[i] = fmaf(wr[i], qv, acc[i]);
+      for (int i = 0; i < CH; ++i) {
+        const double w = (o0 + i < O) ? wr[i] : 0.0;
+        acc_a[i] = fma(w, xa, acc_a[i]);
+        acc_b[i] = fma(w, xb, acc_b[i]);
+      }
     }
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       const int o = o0 + i;
       if (o < O) {
-        ent += stable_xlogx(acc[i]);
-        ut = fmaf(acc[i], Ct[o], ut);
-      } else if (o == O) {
-        hq = acc[i];
+        const double c = util ? (double)Ct[o] : 0.0;
+        ent_a += xlogx_d(acc_a[i]);
+        ent_b += xlogx_d(acc_b[i]);
+        ut_a = fma(acc_a[i], c, ut_a);
+        ut_b = fma(acc_b[i], c, ut_b);
       }
     }
   }
-  if (!active) return;
-  float score = 0.0f;
-  if (sig) score += -ent + hq;
-  if (util) score += ut;
-  ws[d.sc_off[t][m] + (long long)n * H0 + h0] = score;
+  double hq_a = 0.0, hq_b = 0.0;
+  if (sig) {
+    for (int s = 0; s < S0; ++s) {
+      const double w = W[s * O1 + O];
+      hq_a = fma(w, qa[s], hq_a);
+      hq_b = fma(w, qb[s], hq_b);
+    }
+  }
+  double* out = ws + d.sc_off[t][m] + (long long)n * H0;
+  out[ha] = (sig ? -ent_a + hq_a : 0.0) + (util ? ut_a : 0.0);
+  if (has_b) out[hb] = (sig ? -ent_b + hq_b : 0.0) + (util ? ut_b : 0.0);
 }
 
-// ---------------------------------------------------------------------------
 // Inductive value per factor history (pymdp/control.py:1021-1061):
 //   path_f * sum_s (1 - I_f[m_f, s]) ln eps * qs[t][f][h][s]
 // with idx = argmax qs_init_f, m_f = max(argmax_i I_f[i, idx] - 1, 0).
-// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kNT) void efe_inductive_kernel(const Desc* __restrict__ dp, const Args a) {
   const Desc& d = *dp;
   const int b = blockIdx.x;
   const int f = blockIdx.y;
-  float* ws = a.ws + (long long)b * d.ws_stride;
+  double* ws = a.ws + (long long)b * d.ws_stride;
   const float* qs0 = a.qs + (long long)b * a.qs_stride + a.qs_off[f];
   const int S = d.S[f];
   const int depth = a.I_depth[f];
@@ -333,7 +427,7 @@ __global__ __launch_bounds__(kNT) void efe_inductive_kernel(const Desc* __restri
     }
   int am = 0;
   float bi = I[idx];
-  float colsum = I[idx];
+  double colsum = I[idx];
   for (int i = 1; i < depth; ++i) {
     const float v = I[(long long)i * S + idx];
     colsum += v;
@@ -343,39 +437,38 @@ __global__ __launch_bounds__(kNT) void efe_inductive_kernel(const Desc* __restri
     }
   }
   const int mrow = am - 1 > 0 ? am - 1 : 0;
-  const float path = fminf(fmaxf(colsum, 0.0f), 1.0f);
+  const double path = fmin(fmax(colsum, 0.0), 1.0);
   const float eps = a.eps ? a.eps[a.eps_batched ? b : 0] : a.eps_scalar;
-  const float log_eps = log_stable(eps);
+  const double log_eps = log_stable_f64(eps);
   for (int t = 0; t < d.T; ++t) {
     const int n = d.fn[t][f];
     for (int h = threadIdx.x; h < n; h += kNT) {
-      const float* q = ws + d.qs_off[t][f] + (long long)h * S;
-      float acc = 0.0f;
-      for (int s = 0; s < S; ++s) acc = fmaf((1.0f - I[(long long)mrow * S + s]) * log_eps, q[s], acc);
+      const double* q = ws + d.qs_off[t][f] + (long long)h * S;
+      double acc = 0.0;
+      for (int s = 0; s < S; ++s) acc = fma((1.0 - (double)I[(long long)mrow * S + s]) * log_eps, q[s], acc);
       ws[d.fsc_off[t][f] + h] = path * acc;
     }
   }
 }
 
-// ---------------------------------------------------------------------------
 // 6. G[p] = sum_t sum_m score + factor scores; q_pi = softmax(gamma G + ln E)
-// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void efe_scatter_kernel(const Desc* __restrict__ dp, const int* __restrict__ tab,
                                                            const Args a) {
   const Desc& d = *dp;
   const int b = blockIdx.x;
   const int P = d.P;
-  const float* ws = a.ws + (long long)b * d.ws_stride;
+  const double* ws = a.ws + (long long)b * d.ws_stride;
   float* G = a.G + (long long)b * P;
+  double* Gd = a.ws + (long long)b * d.ws_stride + d.g_off;
   const bool ind = (a.flags & kFlagUseInductive) != 0;
-  const float gamma = a.gamma ? a.gamma[b] : a.gamma_scalar;
-  const float lnE_uniform = log_stable(1.0f / (float)P);
-  __shared__ float red[16];
-  float mx = -INFINITY;
+  const double gamma = a.gamma ? (double)a.gamma[b] : (double)a.gamma_scalar;
+  const double lnE_uniform = log_stable_f64(1.0f / (float)P);
+  __shared__ double red[16];
+  double mx = -INFINITY;
   for (int p = threadIdx.x; p < P; p += blockDim.x) {
-    float sum = 0.0f;
+    double sum = 0.0;
     for (int t = 0; t < d.T; ++t) {
-      float st = 0.0f;
+      double st = 0.0;
       for (int m = 0; m < d.M; ++m) {
         const int g = d.mod_group[m];
         st += ws[d.sc_off[t][m] + tab[d.g_p2s_off[t][g] + p]];
@@ -385,33 +478,33 @@ __global__ __launch_bounds__(1024) void efe_scatter_kernel(const Desc* __restric
       }
       sum += st;
     }
-    G[p] = sum;
-    const float lnE = a.E ? log_stable(a.E[(long long)b * a.E_stride + p]) : lnE_uniform;
-    mx = fmaxf(mx, gamma * sum + lnE);
+    G[p] = (float)sum;
+    const double lnE = a.E ? log_stable_f64(a.E[(long long)b * a.E_stride + p]) : lnE_uniform;
+    const double logit = gamma * sum + lnE;
+    Gd[p] = logit;
+    mx = fmax(mx, logit);
   }
   if (a.q_pi == nullptr) return;
-  // block max
   mx = wave_max(mx);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if (lane == 0) red[wv] = mx;
   __syncthreads();
   mx = -INFINITY;
-  for (int i = 0; i < nw; ++i) mx = fmaxf(mx, red[i]);
+  for (int i = 0; i < nw; ++i) mx = fmax(mx, red[i]);
   __syncthreads();
-  float s = 0.0f;
-  float* q = a.q_pi + (long long)b * P;
+  double s = 0.0;
   for (int p = threadIdx.x; p < P; p += blockDim.x) {
-    const float lnE = a.E ? log_stable(a.E[(long long)b * a.E_stride + p]) : lnE_uniform;
-    const float e = expf(gamma * G[p] + lnE - mx);
-    q[p] = e;
+    const double e = exp(Gd[p] - mx);
+    Gd[p] = e;
     s += e;
   }
   s = wave_sum(s);
   if (lane == 0) red[wv] = s;
   __syncthreads();
-  s = 0.0f;
+  s = 0.0;
   for (int i = 0; i < nw; ++i) s += red[i];
-  for (int p = threadIdx.x; p < P; p += blockDim.x) q[p] = q[p] / s;
+  float* q = a.q_pi + (long long)b * P;
+  for (int p = threadIdx.x; p < P; p += blockDim.x) q[p] = (float)(Gd[p] / s);
 }
 
 // ---------------------------------------------------------------------------
@@ -428,6 +521,7 @@ struct DevPlan {
   Item2* d_items2 = nullptr;
   int n_items1 = 0, n_items2 = 0;
   int kron_blocks = 1;
+  int g1_blocks = 1;
   ~DevPlan() {
     if (d_desc) (void)hipFree(d_desc);
     if (d_tab) (void)hipFree(d_tab);
@@ -441,7 +535,7 @@ std::list<std::unique_ptr<DevPlan>> g_plans;  // most recent first, small LRU
 constexpr size_t kMaxPlans = 8;
 
 struct Ws {
-  float* p = nullptr;
+  double* p = nullptr;
   size_t bytes = 0;
 };
 std::map<int, Ws> g_ws;
@@ -515,6 +609,12 @@ static int make_dev_plan(EfeHostPlan&& hp, int device, DevPlan** out) {
     const int g = d.mod_group[m];
     d.ha_off[m] = walloc((long long)d.gS0[g] * d.gKrest[g]);
   }
+  {
+    long long totS = 0;
+    for (int f = 0; f < d.F; ++f) totS += d.S[f];
+    d.qs0_off = walloc(totS);
+    d.g_off = walloc(d.P);
+  }
   std::vector<Item1> it1;
   std::vector<Item2> it2;
   for (int t = 0; t < d.T; ++t) {
@@ -538,19 +638,42 @@ static int make_dev_plan(EfeHostPlan&& hp, int device, DevPlan** out) {
       tab.insert(tab.end(), gl.tuples.begin(), gl.tuples.end());
       d.g_p2s_off[t][g] = (int)tab.size();
       tab.insert(tab.end(), gl.p2score.begin(), gl.p2score.end());
-      d.qin_off[t][g] = walloc((long long)gl.n_in * d.gKrest[g]);
     }
     for (int m = 0; m < d.M; ++m) {
       const int g = d.mod_group[m];
       const GroupLevelPlan& gl = h.grp[t][g];
-      d.w_off[t][m] = walloc((long long)gl.n_in * d.gS0[g] * (d.O[m] + 1));
       d.sc_off[t][m] = walloc((long long)gl.n_in * gl.H0);
-      for (int n = 0; n < gl.n_in; ++n) {
-        it1.push_back(Item1{t, m, n, 0});
-        for (int hb = 0; hb * kNT < gl.H0; ++hb) it2.push_back(Item2{t, m, n, hb});
-      }
+      for (int n = 0; n < gl.n_in; ++n)
+        for (int hb = 0; hb * kScoreCols < gl.H0; ++hb) it2.push_back(Item2{t, m, n, hb});
     }
   }
+  // Qin of every level of a group, and W of every level of a modality, are
+  // contiguous so that one GEMM covers all levels.
+  for (int g = 0; g < d.G; ++g) {
+    long long ntot = 0;
+    for (int t = 0; t < d.T; ++t) ntot += d.g_nin[t][g];
+    d.g_ntot[g] = (int)ntot;
+    const long long base = walloc(ntot * d.gKrest[g]);
+    long long cum = 0;
+    for (int t = 0; t < d.T; ++t) {
+      d.qin_off[t][g] = base + cum * d.gKrest[g];
+      cum += d.g_nin[t][g];
+    }
+  }
+  int g1_blocks = 1;
+  for (int m = 0; m < d.M; ++m) {
+    const int g = d.mod_group[m];
+    const long long per = (long long)d.gS0[g] * (d.O[m] + 1);
+    const long long base = walloc((long long)d.g_ntot[g] * per);
+    long long cum = 0;
+    for (int t = 0; t < d.T; ++t) {
+      d.w_off[t][m] = base + cum * per;
+      cum += d.g_nin[t][g];
+    }
+    const int tiles = (int)(((per + G1_BM - 1) / G1_BM) * ((d.g_ntot[g] + G1_BN - 1) / G1_BN));
+    g1_blocks = std::max(g1_blocks, tiles);
+  }
+  dp->g1_blocks = g1_blocks;
   d.ws_stride = (ws + 63) / 64 * 64;
   if (tab.empty()) tab.push_back(0);
   dp->n_items1 = (int)it1.size();
@@ -609,7 +732,7 @@ static int get_plan(const int32_t* policy, int64_t P, int64_t T, int64_t F, cons
   return 0;
 }
 
-static int get_ws(size_t bytes, float** out) {
+static int get_ws(size_t bytes, double** out) {
   int device = 0;
   EFE_HIP(hipGetDevice(&device));
   std::lock_guard<std::mutex> lk(g_mu);
@@ -670,6 +793,7 @@ static int run_neg_efe(const CallIn& c) {
   int rc = get_plan(c.policy, c.P, c.T, c.F, c.S, c.U, c.O, c.M, c.Adf, c.Ado, c.Bdf, c.Bdo, &plan);
   if (rc) return rc;
   if (c.batch == 0) return 0;
+  if (ensure_log_table()) return efe_err("cannot initialise the log table");
   const Desc& d = plan->desc;
   Args a{};
   a.qs = c.qs;
@@ -711,8 +835,8 @@ static int run_neg_efe(const CallIn& c) {
   a.G = c.G;
   a.q_pi = c.q_pi;
   a.flags = c.flags;
-  float* ws = nullptr;
-  rc = get_ws(sizeof(float) * (size_t)d.ws_stride * (size_t)c.batch, &ws);
+  double* ws = nullptr;
+  rc = get_ws(sizeof(double) * (size_t)d.ws_stride * (size_t)c.batch, &ws);
   if (rc) return rc;
   a.ws = ws;
   const unsigned B = (unsigned)c.batch;
@@ -726,9 +850,18 @@ static int run_neg_efe(const CallIn& c) {
   hipLaunchKernelGGL(efe_rollout_kernel, dim3(B), dim3(kNT), 0, st, plan->d_desc, plan->d_tab, a);
   hipLaunchKernelGGL(efe_kron_kernel, dim3(plan->kron_blocks, d.T * d.G, B), dim3(kNT), 0, st, plan->d_desc,
                      plan->d_tab, a);
-  hipLaunchKernelGGL(efe_gemm1_kernel, dim3(plan->n_items1, B), dim3(kNT), 0, st, plan->d_desc, plan->d_items1, a);
-  hipLaunchKernelGGL(efe_score_kernel<32>, dim3(plan->n_items2, B), dim3(kNT), 0, st, plan->d_desc,
-                     plan->d_items2, a);
+  hipLaunchKernelGGL(efe_gemm1_kernel, dim3(plan->g1_blocks, d.M, B), dim3(256), 0, st, plan->d_desc, a);
+  {
+    size_t wmax = 0;
+    for (int m = 0; m < d.M; ++m) wmax = std::max(wmax, (size_t)d.gS0[d.mod_group[m]] * (d.O[m] + 1));
+    if (wmax * sizeof(double) <= 48 * 1024) {
+      hipLaunchKernelGGL(efe_score_kernel<true>, dim3(plan->n_items2, B), dim3(kNT), wmax * sizeof(double), st,
+                         plan->d_desc, plan->d_items2, a);
+    } else {
+      hipLaunchKernelGGL(efe_score_kernel<false>, dim3(plan->n_items2, B), dim3(kNT), 0, st, plan->d_desc,
+                         plan->d_items2, a);
+    }
+  }
   if (ind) hipLaunchKernelGGL(efe_inductive_kernel, dim3(B, d.F), dim3(kNT), 0, st, plan->d_desc, a);
   hipLaunchKernelGGL(efe_scatter_kernel, dim3(B), dim3(1024), 0, st, plan->d_desc, plan->d_tab, a);
   EFE_HIP(hipGetLastError());

@@ -33,6 +33,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -53,7 +54,7 @@ struct FpiDesc {
   int F, M, G, iters;
   int totS, sumO, mode;  // mode 0: ll_flat + lp_flat (log prior); 1: A + obs + prior probs
   int obs_index;         // mode 1: obs holds int32 indices [batch, M] instead of distributions
-  int lds_S, lds_O, lds_w, lds_x;
+  int lds_S, lds_O, lds_w, lds_x, lds_x2;
   int S[kMaxF];
   int S_off[kMaxF + 1];
   int mod_group[kMaxM];
@@ -127,10 +128,11 @@ __device__ __forceinline__ void stage_finish(double acc, int lg, int n, int f, c
   }
 }
 
-// The log-likelihood is built in FP32 (the data type, like log_stable of the
-// reference), held in FP64, and every sweep accumulates in FP64: the
-// mean-field sweep can oscillate (period-2 cycles of the Jacobi update), and
-// there FP32 rounding of the iteration state is amplified sweep by sweep.
+// The log-likelihood is built from the FP32 data with an FP64-accurate log
+// and held in FP64, and every sweep accumulates in FP64: the mean-field
+// sweep can oscillate (period-2 cycles of the Jacobi update), and there any
+// FP32 rounding of the likelihood or of the iteration state is amplified
+// sweep by sweep.
 template <int JMAX, int NT>
 __global__ __launch_bounds__(NT) void fpi_kernel(const FpiDesc d, const FpiPtrs p) {
   constexpr int NW = NT / kWave;
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(NT) void fpi_kernel(const FpiDesc d, const FpiPtrs 
   if (d.mode) {
     for (int f = 0; f < d.F; ++f)
       for (int i = tid; i < d.S[f]; i += NT)
-        lp[d.S_off[f] + i] = (double)log_stable(p.prior[f][(long long)b * d.prior_stride[f] + i]);
+        lp[d.S_off[f] + i] = log_stable_f64(p.prior[f][(long long)b * d.prior_stride[f] + i]);
     if (d.obs_index) {
       // one_hot(index): an out-of-range index gives an all-zero row (jax.nn.one_hot).
       const int* oi = static_cast<const int*>(p.obs) + (long long)b * d.M;
@@ -194,7 +196,7 @@ __global__ __launch_bounds__(NT) void fpi_kernel(const FpiDesc d, const FpiPtrs 
           const float w = obs[d.obs_off[m] + o];
           if (w == 0.0f) continue;  // uniform across the workgroup
           const float* row = Am + (long long)o * K;
-          // ll_m = sum_o obs[o] * log_stable(A[o]) in FP32 per modality term
+          // ll_m = sum_o obs[o] * log_stable(A[o]), FP64-accurate log
           constexpr int CH = JMAX < 16 ? JMAX : 16;
 #pragma unroll
           for (int j0 = 0; j0 < JMAX; j0 += CH) {
@@ -214,7 +216,7 @@ __global__ __launch_bounds__(NT) void fpi_kernel(const FpiDesc d, const FpiPtrs 
               const int j = j0 + i;
               if (j >= jb && j < je) {
                 const int r = (((j - jb) * NW + wave) << rpw_lg) + lr;
-                if (r < rows && col < n) X0[j] += (double)(w * log_stable(t[i]));
+                if (r < rows && col < n) X0[j] = fma((double)w, log_stable_f64(t[i]), X0[j]);
               }
             }
           }
@@ -246,7 +248,7 @@ __global__ __launch_bounds__(NT) void fpi_kernel(const FpiDesc d, const FpiPtrs 
             for (int o = 0; o < d.O[m]; ++o) {
               const float w = obs[d.obs_off[m] + o];
               if (w == 0.0f) continue;
-              acc += (double)(w * log_stable(Am[(long long)o * d.K[m] + e]));
+              acc = fma((double)w, log_stable_f64(Am[(long long)o * d.K[m] + e]), acc);
             }
           } else {
             acc += (double)p.ll[(long long)b * d.ll_stride + d.ll_off[m] + e];
@@ -344,6 +346,259 @@ __global__ __launch_bounds__(NT) void fpi_kernel(const FpiDesc d, const FpiPtrs 
 
   // ---- output: q = softmax(log_q) --------------------------------------------
   softmax_factors<NW>(d, log_q, qv, lane, wave);
+  __syncthreads();
+  for (int i = tid; i < d.totS; i += NT) p.q[(long long)b * d.totS + i] = (float)qv[i];
+}
+
+// ===========================================================================
+// Single-dependency-tuple fast path: every modality depends on the same
+// factors (the reference's default, fully connected A_dependencies, and all
+// of the plain workloads).  Row-per-lane layout: lane t owns rows
+// r = t + 256 i of the stage-0 matrix [rows0 x n] and keeps them in VGPRs
+// (FP64 sums of FP64-accurate logs of the FP32 data); stage 0 then needs
+// no cross-lane traffic per element (row sums and weighted column sums are
+// both lane-local FP64 FMAs) and one LDS reduction per sweep.  The smaller
+// stages run from LDS with thread-per-row sums and thread-per-(column,
+// chunk) weighted sums, each with a single reduction.
+// ===========================================================================
+// X [rows x n] in LDS -> dst[r] = X[r,:] . qc (if dst), log_q_f += sum_r w[r] X[r,:]
+template <int kFastNT>
+__device__ void lds_stage(const double* X, int rows, int n, const double* qc, const double* w, double* dst,
+                          double* red, double* log_q_f, int tid, int lane, int wave) {
+  if (dst) {
+    for (int r = tid; r < rows; r += kFastNT) {
+      double t = 0.0;
+      for (int s = 0; s < n; ++s) t = fma(X[r * n + s], qc[s], t);
+      dst[r] = t;
+    }
+  }
+  const int lg = ceil_log2(n);
+  const int s = tid & ((1 << lg) - 1);
+  const int nch = kFastNT >> lg;
+  double acc = 0.0;
+  if (s < n) {
+    for (int r = tid >> lg; r < rows; r += nch) acc = fma(w ? w[r] : 1.0, X[r * n + s], acc);
+  }
+  acc = group_sum_high(acc, lg);
+  if ((lane >> lg) == 0 && s < n) red[wave * kWave + s] = acc;
+  __syncthreads();
+  if (tid < n) {
+    double m = 0.0;
+#pragma unroll
+    for (int v = 0; v < kFastNT / kWave; ++v) m += red[v * kWave + tid];
+    log_q_f[tid] += m;
+  }
+}
+
+template <int N0, int RPL, int NT>
+__global__ __launch_bounds__(NT) void fpi_fast_kernel(const FpiDesc d, const FpiPtrs p) {
+  constexpr int kFastNW = NT / kWave;
+  constexpr int LGN = N0 == 4 ? 2 : N0 == 8 ? 3 : 4;  // log2(N0)
+  extern __shared__ double smd[];
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wave = tid >> 6;
+
+  double* ltab = smd;  // ln / reciprocal tables for log_f64d
+  double* log_q = ltab + 256;
+  double* qv = log_q + d.lds_S;
+  double* lp = qv + d.lds_S;
+  double* wbuf = lp + d.lds_S;
+  double* xa = wbuf + d.lds_w;
+  double* xb = xa + d.lds_x;
+  double* part = xb + d.lds_x2;
+  double* red0 = part + N0 * NT;
+  double* red1 = red0 + kFastNW * kWave;
+  float* obs = reinterpret_cast<float*>(red1 + kFastNW * kWave);
+
+  if (d.mode) {
+    for (int f = 0; f < d.F; ++f)
+      for (int i = tid; i < d.S[f]; i += NT)
+        lp[d.S_off[f] + i] = log_stable_f64(p.prior[f][(long long)b * d.prior_stride[f] + i]);
+    if (d.obs_index) {
+      const int* oi = static_cast<const int*>(p.obs) + (long long)b * d.M;
+      for (int m = 0; m < d.M; ++m)
+        for (int o = tid; o < d.O[m]; o += NT) obs[d.obs_off[m] + o] = (oi[m] == o) ? 1.0f : 0.0f;
+    } else {
+      const float* of = static_cast<const float*>(p.obs) + (long long)b * d.sumO;
+      for (int i = tid; i < d.sumO; i += NT) obs[i] = of[i];
+    }
+  } else {
+    for (int i = tid; i < d.totS; i += NT) lp[i] = (double)p.lp[(long long)b * d.totS + i];
+  }
+  for (int i = tid; i < d.totS; i += NT) log_q[i] = 0.0;
+  for (int i = tid; i < 256; i += NT) ltab[i] = g_logtab[i];
+  __syncthreads();
+
+  const int R = d.R[0];
+  const int n = d.dim[0][R - 1];
+  int rows0 = 1;
+  for (int j = 0; j < R - 1; ++j) rows0 *= d.dim[0][j];
+  const int dprev = (R >= 2) ? d.dim[0][R - 2] : 1;
+
+  // ---- log-likelihood rows (FP64-accurate log per term, FP64 accumulation) --
+  double X0[RPL][N0];
+#pragma unroll
+  for (int i = 0; i < RPL; ++i)
+#pragma unroll
+    for (int s = 0; s < N0; ++s) X0[i][s] = 0.0;
+  if (d.mode && d.obs_index) {
+    // One-hot observations: LL[s] = sum_m ln max(A_m[o_m, s], MINVAL)
+    //                              = ln prod_m max(A_m[o_m, s], MINVAL),
+    // the product in FP64 (>= MINVAL^M, far inside the FP64 range), so each
+    // state costs one FP64 log instead of one per modality.
+    const int* oi = static_cast<const int*>(p.obs) + (long long)b * d.M;
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+      const int r = tid + NT * i;
+      double pr[N0];
+#pragma unroll
+      for (int s = 0; s < N0; ++s) pr[s] = 1.0;
+      for (int m = 0; m < d.M; ++m) {
+        const int o = oi[m];
+        if (o < 0 || o >= d.O[m]) continue;  // all-zero one_hot row contributes 0
+        const float* row = p.A[m] + (long long)b * d.A_stride[m] + (long long)o * d.K[m] + (long long)r * n;
+        float t[N0];
+        if (N0 % 4 == 0 && n == N0) {
+#pragma unroll
+          for (int s4 = 0; s4 < N0; s4 += 4) {
+            typedef float v4f __attribute__((ext_vector_type(4)));
+            v4f v = {1.f, 1.f, 1.f, 1.f};
+            if (r < rows0) v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(row + s4));
+            t[s4] = v.x;
+            t[s4 + 1] = v.y;
+            t[s4 + 2] = v.z;
+            t[s4 + 3] = v.w;
+          }
+        } else {
+#pragma unroll
+          for (int s = 0; s < N0; ++s) t[s] = (r < rows0 && s < n) ? __builtin_nontemporal_load(row + s) : 1.0f;
+        }
+#pragma unroll
+        for (int s = 0; s < N0; ++s) pr[s] *= (double)fmaxf(t[s], kMinVal);
+      }
+#pragma unroll
+      for (int s = 0; s < N0; ++s) X0[i][s] = (r < rows0 && s < n) ? log_f64d(pr[s], ltab) : 0.0;
+    }
+  } else {
+    for (int m = 0; m < d.M; ++m) {
+      const long long K = d.K[m];
+      const int nO = d.mode ? d.O[m] : 1;
+      for (int o = 0; o < nO; ++o) {
+        const float w = d.mode ? obs[d.obs_off[m] + o] : 1.0f;
+        if (w == 0.0f) continue;  // uniform
+        const float* row = d.mode ? p.A[m] + (long long)b * d.A_stride[m] + (long long)o * K
+                                  : p.ll + (long long)b * d.ll_stride + d.ll_off[m];
+#pragma unroll
+        for (int i = 0; i < RPL; ++i) {
+          const int r = tid + NT * i;
+#pragma unroll
+          for (int s = 0; s < N0; ++s) {
+            if (r < rows0 && s < n) {
+              const float a = __builtin_nontemporal_load(row + (long long)r * n + s);
+              X0[i][s] += d.mode ? (double)w * log_f64d((double)fmaxf(a, kMinVal), ltab) : (double)a;
+            }
+          }
+        }
+      }
+    }
+  }
+
+  // ---- Jacobi sweeps ---------------------------------------------------------
+  const int f_last = d.fac[0][R - 1];
+  const int f_prev = (R >= 2) ? d.fac[0][R - 2] : 0;
+  int hi[RPL], lo[RPL];  // stage-0 row r = hi * dim[R-2] + lo
+#pragma unroll
+  for (int i = 0; i < RPL; ++i) {
+    const int r = tid + NT * i;
+    hi[i] = r / dprev;
+    lo[i] = r - hi[i] * dprev;
+  }
+  for (int it = 0; it < d.iters; ++it) {
+    softmax_factors<kFastNW>(d, log_q, qv, lane, wave);
+    __syncthreads();
+    for (int i = tid; i < d.totS; i += NT) log_q[i] = lp[i];
+    // W_c = q_{d0} x ... x q_{d_{c-1}} for c = 2 .. R-2 (W_1 is q_{d0} itself)
+    for (int c = 2; c <= R - 2; ++c) {
+      double* wt = wbuf + d.wt_off[0][c];
+      const int rows = stage_rows(d, 0, c);
+      const int dp = d.dim[0][c - 1];
+      const double* qp = qv + d.S_off[d.fac[0][c - 1]];
+      const double* wp = (c > 2) ? wbuf + d.wt_off[0][c - 1] : qv + d.S_off[d.fac[0][0]];
+      for (int i = tid; i < rows; i += NT) {
+        const int hi = i / dp;
+        wt[i] = wp[hi] * qp[i - hi * dp];
+      }
+      __syncthreads();
+    }
+    const double* Wp = (R >= 4) ? wbuf + d.wt_off[0][R - 2] : qv + d.S_off[d.fac[0][0]];  // W_{R-2}
+    const double* qprev = qv + d.S_off[f_prev];
+    // ---- stage 0 from registers: row sums tr[i] = X0[i,:] . q_last and
+    //      weighted column sums part[s] = sum_i w_i X0[i, s]
+    double wv[RPL], tr[RPL];
+#pragma unroll
+    for (int i = 0; i < RPL; ++i) {
+      const int r = tid + NT * i;
+      double w = 0.0;
+      if (r < rows0) w = (R == 1) ? 1.0 : (R == 2) ? qprev[r] : Wp[hi[i]] * qprev[lo[i]];
+      wv[i] = w;
+      tr[i] = 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < N0; ++s) {
+      const double qs_ = (s < n) ? qv[d.S_off[f_last] + s] : 0.0;
+      double a = 0.0;
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) {
+        const double x = X0[i][s];
+        tr[i] = fma(x, qs_, tr[i]);
+        a = fma(wv[i], x, a);
+      }
+      part[s * NT + tid] = a;
+    }
+    if (R >= 2) {
+#pragma unroll
+      for (int i = 0; i < RPL; ++i) {
+        const int r = tid + NT * i;
+        if (r < rows0) xa[r] = tr[i];
+      }
+    }
+    __syncthreads();
+    {
+      // column sums of part [NT x N0]: thread (column, chunk), one butterfly,
+      // then across waves
+      const int s = tid & (N0 - 1);
+      const int nch = NT >> LGN;
+      double v = 0.0;
+      for (int r = tid >> LGN; r < NT; r += nch) v += part[s * NT + r];
+      v = group_sum_high(v, LGN);
+      if ((lane >> LGN) == 0) red0[wave * N0 + s] = v;
+      __syncthreads();
+      if (tid < n) {
+        double m = 0.0;
+#pragma unroll
+        for (int w2 = 0; w2 < kFastNW; ++w2) m += red0[w2 * N0 + tid];
+        log_q[d.S_off[f_last] + tid] += m;
+      }
+    }
+    // ---- stages 1 .. R-1 from LDS
+    double* src = xa;
+    for (int k = 1; k < R; ++k) {
+      const int c = R - 1 - k;
+      const int nk = d.dim[0][c];
+      const int f = d.fac[0][c];
+      const int rows = stage_rows(d, 0, c);
+      const double* w = (c == 0) ? nullptr : (c == 1) ? qv + d.S_off[d.fac[0][0]] : wbuf + d.wt_off[0][c];
+      double* dst = (c > 0) ? ((k & 1) ? xb : xa) : nullptr;
+      lds_stage<NT>(src, rows, nk, qv + d.S_off[f], w, dst, (k & 1) ? red1 : red0, log_q + d.S_off[f], tid, lane,
+                    wave);
+      src = dst;
+    }
+    __syncthreads();
+  }
+
+  softmax_factors<kFastNW>(d, log_q, qv, lane, wave);
   __syncthreads();
   for (int i = tid; i < d.totS; i += NT) p.q[(long long)b * d.totS + i] = (float)qv[i];
 }
@@ -500,10 +755,188 @@ static int assign_slots(FpiDesc* d, int NW) {
   return jslots;
 }
 
+// Fast-path eligibility and launch configurations.  Candidates are
+// (N0 = padded innermost width, RPL = rows per lane, NT = workgroup size) with
+// RPL * N0 <= 64 FP64 values per lane; the default picks the fewest threads
+// that keep RPL * N0 <= 32, the autotuner times every candidate once per
+// shape (see autotune_fast).
+struct FastChoice {
+  int N0, RPL, NT;
+};
+
+#define PMH_FAST_INSTANCES(X)                                                            \
+  X(4, 1, 256) X(4, 2, 256) X(4, 4, 256) X(4, 8, 256) X(4, 16, 256) X(4, 4, 512) X(4, 8, 512) \
+  X(8, 1, 256) X(8, 2, 256) X(8, 4, 256) X(8, 8, 256) X(8, 2, 512) X(8, 4, 512)              \
+  X(16, 1, 256) X(16, 2, 256) X(16, 4, 256) X(16, 1, 512) X(16, 2, 512)
+
+static bool has_fast_instance(int n0, int rpl, int nt) {
+#define PMH_HAS(a, b, c) if (n0 == a && rpl == b && nt == c) return true;
+  PMH_FAST_INSTANCES(PMH_HAS)
+#undef PMH_HAS
+  return false;
+}
+
+static bool fast_shape(FpiDesc* d, int* n0_out, long long* rows0_out) {
+  if (d->G != 1) return false;
+  const int R = d->R[0];
+  for (int j = 0; j < R; ++j)
+    if (d->dim[0][j] > kMaxDim) return false;
+  const int n = d->dim[0][R - 1];
+  if (n > 16) return false;
+  int n0 = 4;
+  while (n0 < n) n0 *= 2;
+  long long rows0 = 1;
+  for (int j = 0; j < R - 1; ++j) rows0 *= d->dim[0][j];
+  long long w = 2;
+  if (R >= 4) {
+    long long rr = 1;
+    for (int j = 0; j < R - 2; ++j) rr *= d->dim[0][j];
+    w = d->wt_off[0][R - 2] + rr;
+  }
+  d->lds_w = (int)((w + 1) / 2 * 2);
+  d->lds_x = (int)((rows0 + 1) / 2 * 2);
+  const long long x2 = (R >= 2) ? rows0 / d->dim[0][R - 2] : 1;
+  d->lds_x2 = (int)((x2 + 1) / 2 * 2);
+  *n0_out = n0;
+  *rows0_out = rows0;
+  return true;
+}
+
+static size_t fast_lds(const FpiDesc& d, int n0, int nt) {
+  return sizeof(double) * (size_t)(256 + 3 * d.lds_S + d.lds_w + d.lds_x + d.lds_x2 + n0 * nt + 2 * nt) +
+         sizeof(float) * (size_t)d.lds_O;
+}
+
+static std::vector<FastChoice> fast_candidates(const FpiDesc& d, int n0, long long rows0) {
+  std::vector<FastChoice> c;
+  for (int nt : {256, 512}) {
+    int rpl = 1;
+    while ((long long)rpl * nt < rows0) rpl *= 2;
+    if (rpl * n0 > 64 || !has_fast_instance(n0, rpl, nt)) continue;
+    if (fast_lds(d, n0, nt) > (size_t)kLdsBudget) continue;
+    c.push_back(FastChoice{n0, rpl, nt});
+  }
+  return c;
+}
+
+static int launch_fast(const FpiDesc& d, const FpiPtrs& p, const FastChoice& fc, int64_t batch, hipStream_t stream) {
+  const dim3 grid((unsigned)batch);
+  const size_t lds = fast_lds(d, fc.N0, fc.NT);
+#define PMH_FAST(n0, rpl, nt)                                                                    \
+  if (fc.N0 == n0 && fc.RPL == rpl && fc.NT == nt) {                                             \
+    hipLaunchKernelGGL((fpi_fast_kernel<n0, rpl, nt>), grid, dim3(nt), lds, stream, d, p);       \
+    return 0;                                                                                    \
+  }
+  PMH_FAST_INSTANCES(PMH_FAST)
+#undef PMH_FAST
+  return -1;
+}
+
+// Per-shape choice cache (key: FNV-1a of the descriptor's shape fields and
+// the batch size).  With PYMDP_HIP_AUTOTUNE=1 the first call of a shape on a
+// stream that is not capturing times each candidate (3 launches after one
+// warm-up, hipEvents on the caller's stream) and keeps the fastest; otherwise
+// the default rule picks.
+std::mutex g_tune_mu;
+std::map<uint64_t, FastChoice> g_tune;
+
+static uint64_t shape_key(const FpiDesc& d, int64_t batch) {
+  uint64_t h = 14695981039346656037ull;
+  auto mix = [&](long long v) {
+    for (int i = 0; i < 8; ++i) {
+      h ^= (uint64_t)((v >> (8 * i)) & 0xff);
+      h *= 1099511628211ull;
+    }
+  };
+  mix(d.G);
+  mix(d.R[0]);
+  for (int j = 0; j < d.R[0]; ++j) mix(d.dim[0][j]);
+  mix(d.M);
+  mix(d.F);
+  mix(d.mode);
+  mix(d.obs_index);
+  mix(d.iters);
+  mix(batch);
+  return h;
+}
+
+static bool autotune_enabled() {
+  const char* e = getenv("PYMDP_HIP_AUTOTUNE");
+  return e && e[0] == '1';
+}
+
+static int autotune_fast(const FpiDesc& d, const FpiPtrs& p, const std::vector<FastChoice>& cands, int64_t batch,
+                         hipStream_t stream, FastChoice* best) {
+  hipEvent_t e0, e1;
+  PMH_HIP_CHECK(hipEventCreate(&e0), "fpi_ffi: ");
+  PMH_HIP_CHECK(hipEventCreate(&e1), "fpi_ffi: ");
+  float best_ms = 1e30f;
+  for (const FastChoice& fc : cands) {
+    if (launch_fast(d, p, fc, batch, stream)) continue;
+    PMH_HIP_CHECK(hipEventRecord(e0, stream), "fpi_ffi: ");
+    for (int r = 0; r < 3; ++r) launch_fast(d, p, fc, batch, stream);
+    PMH_HIP_CHECK(hipEventRecord(e1, stream), "fpi_ffi: ");
+    PMH_HIP_CHECK(hipEventSynchronize(e1), "fpi_ffi: ");
+    float ms = 0.f;
+    PMH_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1), "fpi_ffi: ");
+    if (ms < best_ms) {
+      best_ms = ms;
+      *best = fc;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
 static int launch_fpi(const FpiDesc& d0, FpiPtrs p, int /*unused*/, int64_t batch, hipStream_t stream) {
   const char* pfx = "fpi_ffi: ";
   FpiDesc d = d0;
   if (batch == 0) return 0;
+  if (ensure_log_table()) {
+    set_error("fpi_ffi: cannot initialise the log table");
+    return PYMDP_HIP_EDEVICE;
+  }
+  {
+    FpiDesc df = d0;
+    int n0 = 0;
+    long long rows0 = 0;
+    if (fast_shape(&df, &n0, &rows0)) {
+      const std::vector<FastChoice> cands = fast_candidates(df, n0, rows0);
+      if (!cands.empty()) {
+        // default: fewest threads with <= 32 FP64 row values per lane
+        FastChoice fc = cands.back();
+        for (const FastChoice& c : cands)
+          if (c.RPL * c.N0 <= 32) {
+            fc = c;
+            break;
+          }
+        const uint64_t key = shape_key(df, batch);
+        bool cached = false;
+        {
+          std::lock_guard<std::mutex> lk(g_tune_mu);
+          auto it = g_tune.find(key);
+          if (it != g_tune.end()) {
+            fc = it->second;
+            cached = true;
+          }
+        }
+        if (!cached && autotune_enabled() && cands.size() > 1) {
+          hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+          if (hipStreamIsCapturing(stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+            int rc = autotune_fast(df, p, cands, batch, stream, &fc);
+            if (rc) return rc;
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune[key] = fc;
+          }
+        }
+        if (launch_fast(df, p, fc, batch, stream) == 0) {
+          PMH_HIP_CHECK(hipGetLastError(), pfx);
+          return 0;
+        }
+      }
+    }
+  }
   // 4 waves when the log-likelihood fits 16 FP64 slots per lane, else 8 waves
   // (up to 32 slots), else the streaming variant.
   int NW = 4;

@@ -1,9 +1,12 @@
 """GPU parity of the neg-EFE pipeline against the oracle.
 
-Bar: max |G_gpu - G_oracle_fp32| <= 1e-5 and max |q_pi_gpu - q_pi_oracle| <=
-1e-5 (FP32), argmax policies identical; errors against the FP64 oracle are
-printed.  The full 32768-policy workload is checked on sampled agents with
-the history-dedup oracle (oracle/dedup_ref.py).
+Bar (per agent): |G - G_fp64| <= 1e-5 and |q_pi - q_pi_fp64| <= 1e-5, and
+against the FP32 restatement of the reference path |x - x_fp32| <= 1e-5 + r
+where r = |x_fp32 - x_fp64| is that path's own FP32 error (G sums 2 T M
+entropy / ambiguity terms of opposite sign; two FP32 summation orders
+legitimately differ by ~1e-5 at 32768 policies).  Argmax policies must be
+identical.  The full 32768-policy workload is checked on sampled agents
+with the history-dedup oracle (oracle/dedup_ref.py).
 """
 import numpy as np
 import pytest
@@ -21,6 +24,17 @@ TOL = 1e-5
 
 def _np(t):
     return t.detach().float().cpu().numpy()
+
+
+def _bar(x, x32, x64, tol=TOL):
+    """Asserts the bar of this module; returns (e32, e64)."""
+    x = np.asarray(x, dtype=np.float64)
+    e32 = float(np.abs(x - x32).max())
+    e64 = float(np.abs(x - x64).max())
+    r = float(np.abs(np.asarray(x32, dtype=np.float64) - x64).max())
+    assert e64 <= tol, e64
+    assert e32 <= tol + r, (e32, r)
+    return (e32, e64)
 
 
 def _agent(xs, b, core):
@@ -60,21 +74,18 @@ def test_neg_efe_random_parity(dev, case, shared):
     gamma = 16.0
     q_pi, G = _efe.update_posterior_policies_ffi(pm, qs, A, B, C, None, Ad, Bd, gamma=gamma)
     torch.cuda.synchronize()
-    eG32 = eG64 = eq32 = 0.0
+    worst = np.zeros(4)
     for b in range(batch):
         Ab = _agent(A, b, [len(d) + 1 for d in Ad])
         Bb = _agent(B, b, [len(d) + 2 for d in Bd])
         Cb = [_np(c[b]) for c in C]
         qb = [_np(q[b]) for q in qs]
-        q32, G32 = R.update_posterior_policies(pm, qb, Ab, Bb, Cb, np.full(pm.shape[0], 1.0 / pm.shape[0]),
-                                               Ad, Bd, gamma=gamma, dt=np.float32)
-        G64 = R.neg_efe_all_policies(pm, qb, Ab, Bb, Cb, Ad, Bd, dt=np.float64)
-        eG32 = max(eG32, float(np.abs(_np(G[b]) - G32).max()))
-        eG64 = max(eG64, float(np.abs(_np(G[b]) - G64).max()))
-        eq32 = max(eq32, float(np.abs(_np(q_pi[b]) - q32).max()))
-        assert int(torch.argmax(q_pi[b])) == int(np.argmax(q32))
-    print(f"max|G-G32|={eG32:.2e} max|G-G64|={eG64:.2e} max|q-q32|={eq32:.2e}")
-    assert eG32 <= TOL and eq32 <= TOL
+        E = np.full(pm.shape[0], 1.0 / pm.shape[0])
+        q32, G32 = R.update_posterior_policies(pm, qb, Ab, Bb, Cb, E, Ad, Bd, gamma=gamma, dt=np.float32)
+        q64, G64 = R.update_posterior_policies(pm, qb, Ab, Bb, Cb, E, Ad, Bd, gamma=gamma, dt=np.float64)
+        worst = np.maximum(worst, np.array(_bar(_np(G[b]), G32, G64) + _bar(_np(q_pi[b]), q32, q64)))
+        assert int(torch.argmax(q_pi[b])) == int(np.argmax(q64))
+    print(f"max|G-G32|={worst[0]:.2e} max|G-G64|={worst[1]:.2e} max|q-q32|={worst[2]:.2e} max|q-q64|={worst[3]:.2e}")
 
 
 def test_term_flags_and_reference_packed_abi(dev):
@@ -84,10 +95,11 @@ def test_term_flags_and_reference_packed_abi(dev):
         _, G = _efe.update_posterior_policies_ffi(pm, qs, A, B, C, None, Ad, Bd, use_utility=use_u,
                                                   use_states_info_gain=use_ig, return_q_pi=False)
         for b in range(batch):
-            Gr = R.neg_efe_all_policies(pm, [_np(q[b]) for q in qs], [_np(a[b]) for a in A],
-                                        [_np(x[b]) for x in B], [_np(c[b]) for c in C], Ad, Bd,
-                                        use_utility=use_u, use_states_info_gain=use_ig, dt=np.float32)
-            assert np.abs(_np(G[b]) - Gr).max() <= TOL
+            args = (pm, [_np(q[b]) for q in qs], [_np(a[b]) for a in A], [_np(x[b]) for x in B],
+                    [_np(c[b]) for c in C], Ad, Bd)
+            kw = dict(use_utility=use_u, use_states_info_gain=use_ig)
+            _bar(_np(G[b]), R.neg_efe_all_policies(*args, dt=np.float32, **kw),
+                 R.neg_efe_all_policies(*args, dt=np.float64, **kw))
     # the reference's packed flat-buffer ABI gives the same G as the pointer entry
     from pymdp_amd.ffi import flatten_dep_lists, flatten_ragged_f32
     qs_flat, qs_off = flatten_ragged_f32(qs, batch)
@@ -118,10 +130,11 @@ def test_inductive_parity(dev):
                                                  inductive_epsilon=eps, use_inductive=True)
     torch.cuda.synchronize()
     for b in range(batch):
-        Gr = R.neg_efe_all_policies(pm, [_np(q[b]) for q in qs], [_np(a[b]) for a in A], [_np(x[b]) for x in B],
-                                    [_np(c[b]) for c in C], Ad, Bd, I=[_np(i[b]) for i in I],
-                                    inductive_epsilon=1e-3, use_inductive=True, dt=np.float32)
-        assert np.abs(_np(G[b]) - Gr).max() <= TOL
+        args = (pm, [_np(q[b]) for q in qs], [_np(a[b]) for a in A], [_np(x[b]) for x in B],
+                [_np(c[b]) for c in C], Ad, Bd)
+        kw = dict(I=[_np(i[b]) for i in I], inductive_epsilon=1e-3, use_inductive=True)
+        _bar(_np(G[b]), R.neg_efe_all_policies(*args, dt=np.float32, **kw),
+             R.neg_efe_all_policies(*args, dt=np.float64, **kw))
 
 
 def test_plain_efe_full_size(dev):
@@ -134,19 +147,18 @@ def test_plain_efe_full_size(dev):
     torch.cuda.synchronize()
     assert torch.isfinite(G).all() and torch.isfinite(q_pi).all()
     assert torch.allclose(q_pi.sum(-1), torch.ones(w["batch"], device=dev), atol=1e-4)
-    eG32 = eG64 = eq64 = 0.0
+    worst = np.zeros(4)
     for b in (0, 63):
         args = (pm, [_np(q[b]) for q in w["qs"]], [_np(a[b]) for a in w["A"]], [_np(x[b]) for x in w["B"]],
                 [_np(c[b]) for c in w["C"]], w["A_dependencies"], w["B_dependencies"])
         G64 = D.neg_efe_dedup(*args, dt=np.float64)
         G32 = D.neg_efe_dedup(*args, dt=np.float32)
         q64 = D.q_pi_from_G(G64, w["gamma"])
-        eG32 = max(eG32, float(np.abs(_np(G[b]) - G32).max()))
-        eG64 = max(eG64, float(np.abs(_np(G[b]) - G64).max()))
-        eq64 = max(eq64, float(np.abs(_np(q_pi[b]) - q64).max()))
+        q32 = D.q_pi_from_G(G32, w["gamma"], dt=np.float32)
+        worst = np.maximum(worst, np.array(_bar(_np(G[b]), G32, G64) + _bar(_np(q_pi[b]), q32, q64)))
         assert int(torch.argmax(q_pi[b])) == int(np.argmax(q64))
-    print(f"plain_efe max|G-G32|={eG32:.2e} max|G-G64|={eG64:.2e} max|q-q64|={eq64:.2e}")
-    assert eG32 <= TOL and eq64 <= TOL
+    print(f"plain_efe max|G-G32|={worst[0]:.2e} max|G-G64|={worst[1]:.2e} max|q-q32|={worst[2]:.2e} "
+          f"max|q-q64|={worst[3]:.2e}")
 
 
 def test_agent_step_matches_oracle(dev):
@@ -165,14 +177,17 @@ def test_agent_step_matches_oracle(dev):
         Bb = [_np(x[b]) for x in w["B"]]
         o_b = [np.eye(a.shape[0], dtype=np.float32)[int(o[b])] for a, o in zip(Ab, obs)]
         D_b = [_np(d[b]) for d in agent.D]
-        qr = R.run_factorized_fpi(Ab, o_b, D_b, w["A_dependencies"], num_iter=16, dt=np.float32)
+        qr = R.run_factorized_fpi(Ab, o_b, D_b, w["A_dependencies"], num_iter=16, dt=np.float64)
         for f in range(2):
             assert np.abs(_np(qs[f][b, 0]) - qr[f]).max() <= TOL
-        q32, G32 = R.update_posterior_policies(pm, qr, Ab, Bb, [_np(c[b]) for c in w["C"]],
+        # policy inference from the GPU posterior (the step's own input)
+        qg = [_np(q[b, 0]) for q in qs]
+        q64, G64 = R.update_posterior_policies(pm, qg, Ab, Bb, [_np(c[b]) for c in w["C"]],
                                                np.full(pm.shape[0], 1.0 / pm.shape[0]), w["A_dependencies"],
-                                               w["B_dependencies"], gamma=16.0, dt=np.float32)
-        assert np.abs(_np(G[b]) - G32).max() <= 1e-4
-        assert np.array_equal(_np(act[b]).astype(int), R.sample_policy(pm, q32))
+                                               w["B_dependencies"], gamma=16.0, dt=np.float64)
+        assert np.abs(_np(G[b]) - G64).max() <= TOL
+        assert np.abs(_np(q_pi[b]) - q64).max() <= TOL
+        assert np.array_equal(_np(act[b]).astype(int), R.sample_policy(pm, q64))
     nxt = agent.update_empirical_prior(act, qs)
     for f in range(2):
         for b in range(4):
