@@ -16,6 +16,13 @@ CPU baseline timed on a bounded sample on this host.
 
     python bench.py --calibrate-copy   # plain device copy of 1 GiB, for the
                                        # FETCH_SIZE / WRITE_SIZE scale factor
+    python bench.py --write-traffic KEY --kernel SUBSTR \
+        --pmc-dirs FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR
+                                       # HBM bytes per launch of the kernel
+                                       # whose name contains SUBSTR, from four
+                                       # rocprofv3 --pmc csv runs, scaled by
+                                       # the copy calibration; stored under KEY
+                                       # in profiles/traffic.json
 """
 from __future__ import annotations
 
@@ -42,7 +49,65 @@ def _args():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--calibrate-copy", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--write-traffic", metavar="KEY")
+    ap.add_argument("--kernel", default="fpi_fast_kernel")
+    ap.add_argument("--pmc-dirs", nargs=4)
     return ap.parse_args()
+
+
+COPY_BYTES = 4 << 28  # bytes read (and written) by calibrate_copy
+
+
+def _pmc_rows(d: str, counter: str):
+    """(kernel name, value) per dispatch for `counter` from a rocprofv3 csv dir."""
+    import csv
+    import glob
+    rows = []
+    for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") == counter:
+                    rows.append((r.get("Kernel_Name", ""), float(r["Counter_Value"])))
+    return rows
+
+
+def write_traffic(a):
+    """HBM bytes per launch = FETCH_SIZE + WRITE_SIZE (KiB per dispatch, last
+    dispatch of the named kernel), each scaled by bytes / counter-reading of a
+    plain 1 GiB device copy measured the same way."""
+    fdir, wdir, cfdir, cwdir = a.pmc_dirs
+
+    def last(d, counter, sub):
+        vals = [v for k, v in _pmc_rows(d, counter) if sub in k]
+        if not vals:
+            raise SystemExit(f"no {counter} rows for kernels matching {sub!r} under {d}")
+        return vals[-1]
+
+    def biggest(d, counter):
+        vals = [v for _, v in _pmc_rows(d, counter)]
+        if not vals:
+            raise SystemExit(f"no {counter} rows under {d}")
+        return max(vals)
+
+    fetch_kib, write_kib = last(fdir, "FETCH_SIZE", a.kernel), last(wdir, "WRITE_SIZE", a.kernel)
+    cf_kib, cw_kib = biggest(cfdir, "FETCH_SIZE"), biggest(cwdir, "WRITE_SIZE")
+    sf, sw = COPY_BYTES / (cf_kib * 1024.0), COPY_BYTES / (cw_kib * 1024.0)
+    traffic = fetch_kib * 1024.0 * sf + write_kib * 1024.0 * sw
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            db = json.load(fh)
+    except (OSError, ValueError):
+        db = {}
+    db[a.write_traffic] = traffic
+    db[a.write_traffic + " (detail)"] = {
+        "kernel": a.kernel, "FETCH_SIZE_KiB": fetch_kib, "WRITE_SIZE_KiB": write_kib,
+        "copy_FETCH_SIZE_KiB": cf_kib, "copy_WRITE_SIZE_KiB": cw_kib, "fetch_scale": sf, "write_scale": sw,
+        "copy_bytes_each_way": COPY_BYTES}
+    with open(path, "w") as fh:
+        json.dump(db, fh, indent=1, sort_keys=True)
+    print(json.dumps({a.write_traffic: traffic}))
 
 
 def _traffic(key: str):
@@ -102,6 +167,8 @@ def bench_plain_fpi_large(a, torch):
     wall = time.perf_counter() - t0
     ms_events = e0.elapsed_time(e1) / a.steps
     ms_step = wall * 1e3 / a.steps
+    from pymdp_amd import _native
+    kernel = _native.lib().pymdp_hip_last_launch().decode()
     # algorithmic bytes of one launch: observed A rows + obs indices + prior in + q out
     K = int(np.prod(S))
     per_agent = len(O) * K * 4 + len(O) * 4 + 2 * sum(S) * 4
@@ -143,8 +210,8 @@ def bench_plain_fpi_large(a, torch):
         "config": {"workload": "plain_fpi_large", "agents": B, "num_states": S, "num_obs": O,
                    "num_iter": w["num_iter"], "obs": "index (one-hot)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("fpi_kernel plain_fpi_large"),
-                     "kernel": "fpi_kernel", "kernel_ms": ms_events, "alg_bytes_per_launch": alg_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("fpi plain_fpi_large"),
+                     "kernel": kernel, "kernel_ms": ms_events, "alg_bytes_per_launch": alg_bytes},
         "parity": {"agents_checked": 3, "max_abs_err_vs_fp32_ref": e32, "max_abs_err_vs_fp64": e64},
         "cpu_baseline": cpu,
     }
@@ -212,10 +279,15 @@ def bench_plain_efe(a, torch):
 
 def main():
     a = _args()
+    if a.write_traffic:
+        write_traffic(a)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {a.gpus}); "
               "multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
+    if not a.no_autotune:
+        os.environ.setdefault("PYMDP_HIP_AUTOTUNE", "1")
     import torch
     if a.calibrate_copy:
         calibrate_copy()

@@ -43,6 +43,11 @@ const char* pymdp_hip_last_error(void);
 /* Library version string and gfx target it was built for. */
 const char* pymdp_hip_version(void);
 
+/* Name of the FPI kernel instance (template arguments = innermost width,
+ * rows per lane, workgroup size) used by the most recent FPI call on the
+ * calling thread.  Diagnostic only; no counterpart in the reference. */
+const char* pymdp_hip_last_launch(void);
+
 /*
  * Generalised fused FPI from log-likelihoods.
  * Replaces the XLA FFI target `pymdp_fpi` / `pymdp_fpi_cuda_native`

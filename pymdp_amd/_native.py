@@ -28,6 +28,7 @@ _vp = ctypes.c_void_p
 SIGNATURES: dict[str, tuple] = {
     "pymdp_hip_last_error": (ctypes.c_char_p, []),
     "pymdp_hip_version": (ctypes.c_char_p, []),
+    "pymdp_hip_last_launch": (ctypes.c_char_p, []),
     "pymdp_hip_fpi": (ctypes.c_int, [
         _vp, _vp, _vp,                       # ll, lp, q
         ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,

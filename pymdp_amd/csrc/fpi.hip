@@ -819,8 +819,12 @@ static std::vector<FastChoice> fast_candidates(const FpiDesc& d, int n0, long lo
   return c;
 }
 
+// Name of the kernel instance of the most recent FPI launch on this thread.
+thread_local char g_last_launch[96] = "";
+
 static int launch_fast(const FpiDesc& d, const FpiPtrs& p, const FastChoice& fc, int64_t batch, hipStream_t stream) {
   const dim3 grid((unsigned)batch);
+  snprintf(g_last_launch, sizeof(g_last_launch), "fpi_fast_kernel<%d,%d,%d>", fc.N0, fc.RPL, fc.NT);
   const size_t lds = fast_lds(d, fc.N0, fc.NT);
 #define PMH_FAST(n0, rpl, nt)                                                                    \
   if (fc.N0 == n0 && fc.RPL == rpl && fc.NT == nt) {                                             \
@@ -950,6 +954,8 @@ static int launch_fpi(const FpiDesc& d0, FpiPtrs p, int /*unused*/, int64_t batc
     return einval(pfx, "model needs " + std::to_string(lds) + " bytes of LDS per agent (limit " +
                            std::to_string(kLdsBudget) + ")");
   const dim3 grid((unsigned)batch);
+  snprintf(g_last_launch, sizeof(g_last_launch), "fpi_kernel<%d,%d>",
+           NW == 4 ? (jslots <= 4 ? 4 : 16) : (jslots <= 32 ? 32 : 0), NW == 4 ? 256 : 512);
   if (NW == 4 && jslots <= 4) {
     hipLaunchKernelGGL((fpi_kernel<4, 256>), grid, dim3(256), lds, stream, d, p);
   } else if (NW == 4) {
@@ -974,6 +980,8 @@ using namespace pmh;
 extern "C" const char* pymdp_hip_last_error(void) { return pmh::get_error(); }
 
 extern "C" const char* pymdp_hip_version(void) { return "pymdp_hip 0.1.0 gfx950"; }
+
+extern "C" const char* pymdp_hip_last_launch(void) { return pmh::g_last_launch; }
 
 extern "C" int pymdp_hip_fpi(const float* ll_flat, const float* lp_flat, float* q_out, int64_t ll_count,
                              int64_t lp_count, int64_t q_count, const int64_t* S, int64_t F,
