@@ -186,6 +186,64 @@ int pymdp_hip_neg_efe_all_policies(const int32_t* policy_host, const float* qs_f
                                    const int64_t* B_dep_flat, const int64_t* B_dep_offsets,
                                    int32_t flags, void* stream);
 
+/*
+ * Batched ragged agents: one infer_states -> infer_policies -> sample_action
+ * step for many independent agents of different shapes.  No counterpart in
+ * the reference (its FFI handles one agent shape per call); it composes the
+ * reference semantics of pymdp/algos.py:88-155 (FPI),
+ * pymdp/control.py:604-793 (neg-EFE per policy), :963 (q_pi) and :115-150
+ * (deterministic action selection) per agent.
+ *
+ * Per agent a (in order), hdr[4a..4a+3] = (F, M, T, P).  Concatenated over
+ * agents: S, U [F each], O [M each], A_dep_rank [M each] with A_dep_flat,
+ * B_dep_rank [F each] with B_dep_flat, C_time [M each] (1 = C_m is [T, O_m]),
+ * E_present [1 each] (0 = uniform E), gamma [1 each], policies [P*T*F each,
+ * int32].  Limits: F, M <= 8, T <= 4, S, O <= 64, dependency ranks <= 3.
+ * The plan is built on the host; pymdp_hip_ragged_plan_upload allocates and
+ * fills its device tables on the current device (the only allocation).
+ */
+int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr,
+                                 const int64_t* S, const int64_t* U, const int64_t* O,
+                                 const int64_t* A_dep_rank, const int64_t* A_dep_flat,
+                                 const int64_t* B_dep_rank, const int64_t* B_dep_flat,
+                                 const int64_t* C_time, const int32_t* E_present,
+                                 const float* gamma, const int32_t* policies,
+                                 int32_t num_iter, void** plan_out);
+int pymdp_hip_ragged_plan_upload(void* plan);
+
+/* Packed layout chosen by the plan, as int64 values:
+ *   [params_floats, obs_ints, out_floats, act_ints]
+ *   A_off [sum M]            offset of A_m in the params arena, stored
+ *                            [O_m][storage axes] row-major
+ *   A_perm [sum M * 3]       storage axis j = position A_perm[j] of the
+ *                            modality's given dependency list (-1 = unused)
+ *   B_off [sum F]            B_f stored [S_f][S_deps...][U_f] (as given)
+ *   C_off [sum M]            C_m [O_m] or [T, O_m]
+ *   D_off [sum F]            prior D_f [S_f]
+ *   E_off [n]                E [P] (or -1)
+ *   obs_off [n]              M int32 observation indices in the obs arena
+ *   qs_off, G_off, qpi_off [n each]   outputs in the float out arena
+ *   act_off [n]              F int32 actions in the action arena
+ * With out == NULL returns the number of values. */
+int pymdp_hip_ragged_layout(void* plan, int64_t* out, int64_t n_out);
+/* out8 = (agents, EFE work items, FPI grid, EFE grid, history doubles,
+ *         score doubles, FPI workspace per workgroup, EFE workspace per
+ *         workgroup). */
+int pymdp_hip_ragged_info(void* plan, int64_t* out8);
+/* Host copies of the plan's device tables (for host-side verification):
+ * sizes8 = (sizeof agent record, sizeof item, items, int arena, max grid). */
+int pymdp_hip_ragged_tables(void* plan, int64_t* sizes8, void* agents_out,
+                            int32_t* order_out, int32_t* items_out, int32_t* iarena_out);
+/* One step on `stream`: three launches, no allocation, no synchronisation.
+ * Steps that overlap on one device race on the plan's scratch. */
+int pymdp_hip_ragged_step(void* plan, const float* params, const int32_t* obs,
+                          float* out, int32_t* actions, void* stream);
+/* Times the launch grids (2, 4, 8 workgroups per CU for each persistent
+ * kernel) and keeps the fastest; not while `stream` is being captured. */
+int pymdp_hip_ragged_autotune(void* plan, const float* params, const int32_t* obs,
+                              float* out, int32_t* actions, void* stream, float* best_ms);
+int pymdp_hip_ragged_plan_destroy(void* plan);
+
 #ifdef __cplusplus
 }
 #endif

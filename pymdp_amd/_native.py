@@ -70,6 +70,20 @@ SIGNATURES: dict[str, tuple] = {
         _i64p, _i64p, _i64p, _i64p, _i64p, _i64p,            # qs/A/B/C/I offsets, I_depths
         _i64p, _i64p, _i64p, _i64p,              # A deps, B deps
         ctypes.c_int32, _vp]),
+    # batched ragged agents (no counterpart in the reference)
+    "pymdp_hip_ragged_plan_create": (ctypes.c_int, [
+        ctypes.c_int64, _i64p,                   # n_agents, hdr [n, 4]
+        _i64p, _i64p, _i64p,                     # S, U, O
+        _i64p, _i64p, _i64p, _i64p,              # A dep rank/flat, B dep rank/flat
+        _i64p, _vp, _vp, _vp,                    # C_time, E_present, gamma, policies
+        ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    "pymdp_hip_ragged_plan_upload": (ctypes.c_int, [_vp]),
+    "pymdp_hip_ragged_layout": (ctypes.c_int, [_vp, _i64p, ctypes.c_int64]),
+    "pymdp_hip_ragged_info": (ctypes.c_int, [_vp, _i64p]),
+    "pymdp_hip_ragged_tables": (ctypes.c_int, [_vp, _i64p, _vp, _vp, _vp, _vp]),
+    "pymdp_hip_ragged_step": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "pymdp_hip_ragged_autotune": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_float)]),
+    "pymdp_hip_ragged_plan_destroy": (ctypes.c_int, [_vp]),
 }
 
 

@@ -9,6 +9,15 @@
   plain_efe         64 agents, S=[32,16,8], U=[8,4,1], horizon 3 (32768
                     policies), 4 modalities of 64 outcomes, A Dirichlet(0.5),
                     B Dirichlet(0.2), C ~ N(0,1), gamma 16, E uniform
+  ragged            4096 agents of different shapes: per agent F ~ U{1..4}
+                    factors of S ~ U{2..64} states and U ~ U{1..8} controls,
+                    M ~ U{1..5} modalities of O ~ U{2..64} outcomes, each
+                    depending on r ~ U{1..min(F,3)} distinct factors, horizon
+                    T ~ U{1,2}; an agent is redrawn while its policy count
+                    (prod U)^T exceeds 4096 or its A exceeds 4 MiB.  A and B
+                    Dirichlet(0.5) columns, factor-local B, C ~ N(0,1),
+                    D Dirichlet(1), observation indices uniform, 16 sweeps,
+                    gamma 16, E uniform
   grad_fpi_efe      512 agents, S=[16,8], U=[16,4], horizon 1, 3 modalities
                     of 16 outcomes, softmax-parameterised A, B, C, D
 
@@ -19,9 +28,10 @@ from __future__ import annotations
 
 import math
 
+import numpy as np
 import torch
 
-from . import utils
+from . import control, utils
 
 
 def plain_fpi_small(seed: int = 0, device="cpu", batch: int = 64) -> dict:
@@ -60,3 +70,44 @@ def plain_efe(seed: int = 0, device="cpu", batch: int = 64, S=(32, 16, 8), U=(8,
     return dict(name="plain_efe", S=S, U=U, O=O, T=T, A=A, B=B, C=C, qs=qs, A_dependencies=A_deps,
                 B_dependencies=B_deps, gamma=16.0, batch=batch,
                 num_policies=int(math.prod(U) ** T))
+
+
+def ragged(seed: int = 0, device="cpu", n_agents: int = 4096, max_policies: int = 4096,
+           max_A_bytes: int = 4 << 20, state_range=(2, 64), obs_range=(2, 64), max_factors: int = 4,
+           max_controls: int = 8, max_modalities: int = 5, horizons=(1, 2), num_iter: int = 16,
+           gamma: float = 16.0, tensors: bool = True) -> dict:
+    """Independent agents of different shapes (see the module docstring).
+    Shapes come from a NumPy PCG64 stream, tensors from a torch generator on
+    `device`, both seeded by `seed`; tensors=False keeps only the shapes
+    (same shapes, policies and observations)."""
+    rng = np.random.default_rng(seed)
+    g = utils._gen(seed, device)
+    agents = []
+    for _ in range(n_agents):
+        while True:
+            F = int(rng.integers(1, max_factors + 1))
+            S = [int(x) for x in rng.integers(state_range[0], state_range[1] + 1, F)]
+            U = [int(x) for x in rng.integers(1, max_controls + 1, F)]
+            M = int(rng.integers(1, max_modalities + 1))
+            O = [int(x) for x in rng.integers(obs_range[0], obs_range[1] + 1, M)]
+            T = int(rng.choice(list(horizons)))
+            A_deps = []
+            for _m in range(M):
+                r = int(rng.integers(1, min(F, 3) + 1))
+                A_deps.append(sorted(int(x) for x in rng.choice(F, r, replace=False)))
+            P = math.prod(U) ** T
+            A_bytes = sum(4 * O[m] * math.prod(S[d] for d in A_deps[m]) for m in range(M))
+            if P <= max_policies and A_bytes <= max_A_bytes:
+                break
+        B_deps = [[f] for f in range(F)]
+        A = B = C = D = None
+        if tensors:
+            A = [utils.dirichlet([O[m]] + [S[d] for d in A_deps[m]], 0.5, 0, g, device) for m in range(M)]
+            B = [utils.dirichlet([S[f], S[f], U[f]], 0.5, 0, g, device) for f in range(F)]
+            C = [torch.randn(O[m], generator=g, device=device) for m in range(M)]
+            D = [utils.dirichlet([S[f]], 1.0, 0, g, device) for f in range(F)]
+        obs = [int(rng.integers(0, O[m])) for m in range(M)]
+        policies = control.construct_policies(S, U, T)
+        agents.append(dict(S=S, U=U, O=O, T=T, A=A, B=B, C=C, D=D, E=None, obs=obs, A_dependencies=A_deps,
+                           B_dependencies=B_deps, policies=policies, gamma=float(gamma)))
+    return dict(name="ragged", agents=agents, num_iter=num_iter, n_agents=n_agents)

@@ -1,0 +1,153 @@
+"""GPU parity of the batched ragged step (pymdp_hip_ragged_*) against the
+oracle, per agent: qs, G and q_pi to the bar |x - x_fp64| <= 1e-5 and
+|x - x_fp32| <= 1e-5 + |x_fp32 - x_fp64|; actions equal to the oracle's
+first-index argmax policy's first actions.  The full 4096-agent batch is
+also checked through size-independent properties (normalisation,
+finiteness, bitwise determinism, graph replay == eager step)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import dedup_ref as DR
+from oracle import numpy_ref as R
+from pymdp_amd import ragged, workloads
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _np(t):
+    return t.detach().float().cpu().numpy()
+
+
+def _bar(x, x32, x64, tol=TOL):
+    x = np.asarray(x, dtype=np.float64)
+    e32 = float(np.abs(x - x32).max())
+    e64 = float(np.abs(x - x64).max())
+    r = float(np.abs(np.asarray(x32, dtype=np.float64) - x64).max())
+    assert e64 <= tol, e64
+    assert e32 <= tol + r, (e32, r)
+    return e32, e64
+
+
+def _check_agent(rb, i, use_dedup=False):
+    a = rb.agents[i]
+    qs, G, q_pi, act = rb.agent_outputs(i)
+    A = [_np(x) for x in a["A"]]
+    B = [_np(x) for x in a["B"]]
+    C = [_np(x) for x in a["C"]]
+    D = [_np(x) for x in a["D"]]
+    ob = [np.eye(o, dtype=np.float32)[x] for o, x in zip(a["O"], a["obs"])]
+    errs = []
+    q32 = R.run_factorized_fpi(A, ob, D, a["A_dependencies"], num_iter=16, dt=np.float32)
+    q64 = R.run_factorized_fpi(A, ob, D, a["A_dependencies"], num_iter=16, dt=np.float64)
+    for f in range(len(a["S"])):
+        errs.append(_bar(_np(qs[f]), q32[f], q64[f]))
+    pm = a["policies"]
+    # the policy stage starts from the kernel's posterior (as infer_policies does)
+    qg = [_np(q).astype(np.float64) for q in qs]
+    if use_dedup:
+        G32 = DR.neg_efe_dedup(pm, qg, A, B, C, a["A_dependencies"], a["B_dependencies"], dt=np.float32)
+        G64 = DR.neg_efe_dedup(pm, qg, A, B, C, a["A_dependencies"], a["B_dependencies"], dt=np.float64)
+        p32 = DR.q_pi_from_G(G32, a["gamma"], dt=np.float32)
+        p64 = DR.q_pi_from_G(G64, a["gamma"], dt=np.float64)
+    else:
+        P = pm.shape[0]
+        E = np.full(P, 1.0 / P)
+        p32, G32 = R.update_posterior_policies(pm, qg, A, B, C, E, a["A_dependencies"], a["B_dependencies"],
+                                               gamma=a["gamma"], dt=np.float32)
+        p64, G64 = R.update_posterior_policies(pm, qg, A, B, C, E, a["A_dependencies"], a["B_dependencies"],
+                                               gamma=a["gamma"], dt=np.float64)
+    errs.append(_bar(_np(G), G32, G64))
+    errs.append(_bar(_np(q_pi), p32, p64))
+    np.testing.assert_array_equal(act.cpu().numpy(), R.sample_policy(pm, _np(q_pi)))
+    np.testing.assert_array_equal(act.cpu().numpy(), R.sample_policy(pm, p64))
+    return max(e[0] for e in errs), max(e[1] for e in errs)
+
+
+def test_ragged_small_parity(dev):
+    w = workloads.ragged(seed=1, device=dev, n_agents=64, state_range=(2, 9), obs_range=(2, 9), max_policies=256,
+                         max_A_bytes=1 << 16)
+    print("ragged small: 64 agents", flush=True)
+    rb = ragged.RaggedBatch(w["agents"], num_iter=16, device=dev)
+    rb.step()
+    torch.cuda.synchronize()
+    worst = [0.0, 0.0]
+    for i in range(64):
+        e32, e64 = _check_agent(rb, i)
+        worst = [max(worst[0], e32), max(worst[1], e64)]
+    print(f"ragged small max|x-x32|={worst[0]:.3e} max|x-x64|={worst[1]:.3e}")
+
+
+def test_ragged_wide_shapes_parity(dev):
+    """Full-range shapes (rank-3 dependencies, 64-state factors, workspace
+    paths) with the policy count capped so the oracle stays quick."""
+    w = workloads.ragged(seed=5, device=dev, n_agents=24, max_policies=64)
+    print("ragged wide: 24 agents", flush=True)
+    rb = ragged.RaggedBatch(w["agents"], num_iter=16, device=dev)
+    rb.step()
+    torch.cuda.synchronize()
+    for i in range(24):
+        _check_agent(rb, i, use_dedup=True)
+
+
+def test_ragged_full_batch(dev):
+    w = workloads.ragged(seed=0, device=dev, n_agents=4096)
+    print("ragged full: 4096 agents", flush=True)
+    rb = ragged.RaggedBatch(w["agents"], num_iter=16, device=dev)
+    rb.step()
+    torch.cuda.synchronize()
+    out1 = rb.out.clone()
+    act1 = rb.act.clone()
+    assert torch.isfinite(out1).all()
+    for i in range(0, 4096, 7):
+        qs, G, q_pi, act = rb.agent_outputs(i)
+        for q in qs:
+            assert abs(float(q.sum()) - 1.0) < 1e-5
+        assert abs(float(q_pi.sum()) - 1.0) < 1e-4
+    # a second step is bitwise identical
+    rb.step()
+    torch.cuda.synchronize()
+    assert torch.equal(out1, rb.out) and torch.equal(act1, rb.act)
+    # oracle on a sample: the most expensive FPI agents and a spread of others
+    cost = [sum(a_.numel() for a_ in a["A"]) * np.asarray(a["policies"]).shape[0] for a in w["agents"]]
+    order = np.argsort(cost)
+    sample = list(order[:: len(order) // 10][:10]) + [int(np.argmax([sum(a_.numel() for a_ in a["A"])
+                                                                       for a in w["agents"]]))]
+    for i in sample:
+        _check_agent(rb, int(i), use_dedup=True)
+
+
+def test_ragged_graph_replay_and_autotune(dev):
+    w = workloads.ragged(seed=3, device=dev, n_agents=512)
+    print("ragged graph: 512 agents", flush=True)
+    rb = ragged.RaggedBatch(w["agents"], num_iter=16, device=dev)
+    rb.step()
+    torch.cuda.synchronize()
+    ref_out, ref_act = rb.out.clone(), rb.act.clone()
+    rb.autotune()
+    rb.step()
+    torch.cuda.synchronize()
+    assert torch.equal(ref_out, rb.out) and torch.equal(ref_act, rb.act)
+    g = rb.capture()
+    torch.cuda.synchronize()
+    for _ in range(3):
+        rb.out.zero_()
+        rb.act.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(ref_out, rb.out) and torch.equal(ref_act, rb.act)
+        rb.step()  # eager work between replays
+        torch.cuda.synchronize()
+    # new observations through the same graph
+    rng = np.random.default_rng(11)
+    new_obs = [[int(rng.integers(0, o)) for o in a["O"]] for a in w["agents"]]
+    for a, o in zip(w["agents"], new_obs):
+        a["obs"] = o
+    rb.set_obs(new_obs)
+    g.replay()
+    torch.cuda.synchronize()
+    for i in (0, 100, 511):
+        _check_agent(rb, i, use_dedup=True)

@@ -1,0 +1,143 @@
+"""Host-side checks of the batched ragged plan (no GPU): the plan's tables
+replayed by the kernel emulation in tests/ragged_emu.py reproduce the
+oracle per agent, every access of the full 4096-agent plan stays in bounds,
+and invalid agents are rejected with a "ragged:" ValueError."""
+import numpy as np
+import pytest
+
+from oracle import dedup_ref as DR
+from oracle import numpy_ref as R
+from pymdp_amd import ragged, workloads
+from tests import ragged_emu as E
+
+
+def _emulate(agents):
+    plan = ragged.RaggedPlan(agents, 16)
+    params = ragged.pack_params(plan, agents, "cpu").numpy()
+    obs = ragged.pack_obs(plan, [a["obs"] for a in agents], "cpu").numpy()
+    return plan, E.Emu(plan, params, obs).run()
+
+
+@pytest.mark.parametrize("kw", [
+    dict(seed=1, n_agents=24, state_range=(2, 9), obs_range=(2, 9), max_policies=256, max_A_bytes=1 << 16),
+    dict(seed=5, n_agents=10, max_policies=64),
+])
+def test_emulated_step_matches_oracle(kw):
+    w = workloads.ragged(**kw)
+    ags = w["agents"]
+    plan, em = _emulate(ags)
+    lay = plan.layout
+    for i, a in enumerate(ags):
+        A = [x.numpy() for x in a["A"]]
+        B = [x.numpy() for x in a["B"]]
+        C = [x.numpy() for x in a["C"]]
+        D = [x.numpy() for x in a["D"]]
+        ob = [np.eye(o, dtype=np.float32)[x] for o, x in zip(a["O"], a["obs"])]
+        q64 = R.run_factorized_fpi(A, ob, D, a["A_dependencies"], num_iter=16, dt=np.float64)
+        S = a["S"]
+        qs = np.split(em.out[lay["qs_off"][i]:lay["qs_off"][i] + sum(S)], np.cumsum(S)[:-1])
+        for x, y in zip(qs, q64):
+            assert np.abs(x - y).max() <= 1e-6
+        pm = a["policies"]
+        P = pm.shape[0]
+        G64 = DR.neg_efe_dedup(pm, [x.astype(np.float64) for x in qs], A, B, C, a["A_dependencies"],
+                               a["B_dependencies"], dt=np.float64)
+        p64 = DR.q_pi_from_G(G64, a["gamma"])
+        G = em.out[lay["G_off"][i]:lay["G_off"][i] + P]
+        q = em.out[lay["qpi_off"][i]:lay["qpi_off"][i] + P]
+        assert np.abs(G - G64).max() <= 2e-6
+        assert np.abs(q - p64).max() <= 2e-6
+        act = em.act[lay["act_off"][i]:lay["act_off"][i] + len(S)]
+        np.testing.assert_array_equal(act, R.sample_policy(pm, p64))
+
+
+def test_emulation_detects_planted_index_error():
+    """The emulation is a real check: swapping the two score strides of a
+    rank-2 modality (policies read the wrong history combination) breaks
+    agreement with the oracle."""
+    w = workloads.ragged(seed=5, n_agents=10, max_policies=64)
+    ags = w["agents"]
+    plan = ragged.RaggedPlan(ags, 16)
+    params = ragged.pack_params(plan, ags, "cpu").numpy()
+    obs = ragged.pack_obs(plan, [a["obs"] for a in ags], "cpu").numpy()
+    em = E.Emu(plan, params, obs)
+    planted = 0
+    for a in em.ag:
+        for t in range(int(a["T"])):
+            for m in range(int(a["M"])):
+                g = int(a["mod_group"][m])
+                st = a["score_stride"][t][m]
+                if a["gR"][g] == 2 and st[0] != st[1]:
+                    st[0], st[1] = int(st[1]), int(st[0])
+                    planted += 1
+    assert planted > 0
+    worst = 0.0
+    try:
+        em.run()
+    except AssertionError:
+        return  # an access left its buffer: detected
+    lay = plan.layout
+    for i, a in enumerate(ags):
+        qs = np.split(em.out[lay["qs_off"][i]:lay["qs_off"][i] + sum(a["S"])], np.cumsum(a["S"])[:-1])
+        G64 = DR.neg_efe_dedup(a["policies"], [x.astype(np.float64) for x in qs], [x.numpy() for x in a["A"]],
+                               [x.numpy() for x in a["B"]], [x.numpy() for x in a["C"]], a["A_dependencies"],
+                               a["B_dependencies"], dt=np.float64)
+        P = a["policies"].shape[0]
+        worst = max(worst, float(np.abs(em.out[lay["G_off"][i]:lay["G_off"][i] + P] - G64).max()))
+    assert worst > 1e-3
+
+
+def test_full_plan_bounds():
+    w = workloads.ragged(seed=0, n_agents=4096, tensors=False)
+    plan = ragged.RaggedPlan(w["agents"], 16)
+    info = E.check_bounds_only(plan)
+    assert info["agents"] == 4096 and info["items"] > 4096
+    ag, order, items, iarena, _ = E.tables(plan)
+    # work queues are ordered by decreasing cost and cover everything once
+    assert sorted(order.tolist()) == list(range(4096))
+    per = np.zeros(4096, np.int64)
+    for it in items:
+        per[it[0]] += 1
+    assert (per >= 1).all()
+
+
+def _one_agent(**over):
+    a = dict(S=[3, 2], U=[2, 1], O=[4], T=1, A=None, B=None, C=None, D=None, E=None, obs=[0],
+             A_dependencies=[[0, 1]], B_dependencies=[[0], [1]],
+             policies=np.array([[[0, 0]], [[1, 0]]], np.int32), gamma=1.0)
+    a.update(over)
+    return a
+
+
+@pytest.mark.parametrize("over,msg", [
+    (dict(S=[3, 65], U=[2, 1]), "state counts"),
+    (dict(O=[65]), "outcome counts"),
+    (dict(A_dependencies=[[0, 0]]), "duplicate factor"),
+    (dict(A_dependencies=[[0, 2]]), "out of range"),
+    (dict(policies=np.array([[[2, 0]]], np.int32)), "out of bounds"),
+])
+def test_invalid_agents_rejected(over, msg):
+    with pytest.raises(ValueError, match="ragged: agent 0: .*" + msg):
+        ragged.RaggedPlan([_one_agent(**over)], 16)
+
+
+def test_plan_layout_is_disjoint():
+    w = workloads.ragged(seed=2, n_agents=50, tensors=False)
+    plan = ragged.RaggedPlan(w["agents"], 16)
+    lay = plan.layout
+    spans = []
+    mi = fi = 0
+    for a in w["agents"]:
+        for m, o in enumerate(a["O"]):
+            K = int(np.prod([a["S"][d] for d in a["A_dependencies"][m]]))
+            spans.append((int(lay["A_off"][mi]), o * K))
+            spans.append((int(lay["C_off"][mi]), o))
+            mi += 1
+        for f, s in enumerate(a["S"]):
+            spans.append((int(lay["B_off"][fi]), s * s * a["U"][f]))
+            spans.append((int(lay["D_off"][fi]), s))
+            fi += 1
+    spans.sort()
+    for (o0, n0), (o1, _) in zip(spans, spans[1:]):
+        assert o0 + n0 <= o1
+    assert spans[-1][0] + spans[-1][1] <= int(plan.sizes[0])
