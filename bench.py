@@ -9,6 +9,12 @@ Workloads (seeded synthetic data, generated on the device):
   plain_efe                  64 agents, S=[32,16,8], U=[8,4,1], horizon 3
                              (32768 policies), 4 x 64 outcomes: one step =
                              one infer_policies (neg-EFE + q_pi) for all agents.
+  ragged                     4096 agents of different shapes (1-4 factors of
+                             2-64 states, 1-8 controls, 1-5 modalities of 2-64
+                             outcomes, horizon 1-2): one step = infer_states ->
+                             infer_policies -> action for every agent, one HIP
+                             graph replay of the batched kernels; compared with
+                             the plain per-agent entry points on a sample.
 
 Prints ONE JSON line with value = agent-steps/s over the K timed steps,
 the roofline of the dominant kernel (HIP events on the launch stream) and a
@@ -45,7 +51,7 @@ def _args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="plain_fpi_large", choices=["plain_fpi_large", "plain_efe"])
+    ap.add_argument("--workload", default="plain_fpi_large", choices=["plain_fpi_large", "plain_efe", "ragged"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -277,6 +283,148 @@ def bench_plain_efe(a, torch):
     }
 
 
+def _plain_agent_step(ag, tensors, torch):
+    """One agent through the plain per-agent entry points (infer_states_ffi,
+    update_posterior_policies_ffi, argmax action) at batch size 1."""
+    from pymdp_amd.ffi import infer_states_ffi
+    from pymdp_amd.ffi._efe import update_posterior_policies_ffi
+    A, B, C, D, obs, pol0 = tensors
+    qs = infer_states_ffi(A, obs, D, ag["A_dependencies"], 16, distr_obs=False)
+    q_pi, G = update_posterior_policies_ffi(ag["policies"], qs, A, B, C, None, ag["A_dependencies"],
+                                            ag["B_dependencies"], gamma=ag["gamma"])
+    return pol0[torch.argmax(q_pi[0])]
+
+
+def bench_ragged(a, torch):
+    import numpy as np
+
+    from pymdp_amd import ragged, workloads
+
+    dev = torch.device("cuda:0")
+    w = workloads.ragged(seed=0, device=dev, n_agents=4096)
+    ags = w["agents"]
+    rb = ragged.RaggedBatch(ags, num_iter=w["num_iter"], device=dev)
+    rb.step()
+    torch.cuda.synchronize()
+    if not a.no_autotune:
+        rb.autotune()
+    g = rb.capture()
+    for _ in range(a.warmup):
+        g.replay()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(a.steps):
+        g.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_events = e0.elapsed_time(e1) / a.steps
+    ms_step = wall * 1e3 / a.steps
+    n = len(ags)
+    alg = ragged.algorithmic_bytes(ags)
+    achieved = alg / (ms_events * 1e-3) / 1e9
+
+    # plain per-agent entry points on a bounded sample (every 16th agent)
+    sample = list(range(0, n, 16))
+    tens = []
+    for i in sample:
+        ag = ags[i]
+        tens.append(([x[None] for x in ag["A"]], [x[None] for x in ag["B"]], [x[None] for x in ag["C"]],
+                     [x[None] for x in ag["D"]], [torch.tensor([o], device=dev) for o in ag["obs"]],
+                     torch.as_tensor(ag["policies"][:, 0, :], device=dev)))
+    for i, tt in zip(sample[:8], tens[:8]):
+        _plain_agent_step(ags[i], tt, torch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_act = [_plain_agent_step(ags[i], tt, torch) for i, tt in zip(sample, tens)]
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    plain_rate = len(sample) / plain_s
+    same = all(torch.equal(pa.to(torch.int32), rb.agent_outputs(i)[3]) for pa, i in zip(plain_act, sample))
+
+    # parity of a few agents against the oracle (FPI restatement + history-dedup EFE, FP64)
+    from oracle import dedup_ref as DR
+    from oracle import numpy_ref as R
+    e64 = 0.0
+    for i in (0, n // 3, 2 * n // 3, n - 1):
+        ag = ags[i]
+        qs, G, q_pi, act = rb.agent_outputs(i)
+        A = [x.cpu().numpy() for x in ag["A"]]
+        ob = [np.eye(o, dtype=np.float32)[x] for o, x in zip(ag["O"], ag["obs"])]
+        q64 = R.run_factorized_fpi(A, ob, [d.cpu().numpy() for d in ag["D"]], ag["A_dependencies"], 16, np.float64)
+        e64 = max(e64, max(float(np.abs(q.cpu().numpy() - r).max()) for q, r in zip(qs, q64)))
+        qg = [q.cpu().numpy().astype(np.float64) for q in qs]
+        G64 = DR.neg_efe_dedup(ag["policies"], qg, A, [x.cpu().numpy() for x in ag["B"]],
+                               [x.cpu().numpy() for x in ag["C"]], ag["A_dependencies"], ag["B_dependencies"])
+        p64 = DR.q_pi_from_G(G64, ag["gamma"])
+        e64 = max(e64, float(np.abs(G.cpu().numpy() - G64).max()), float(np.abs(q_pi.cpu().numpy() - p64).max()))
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline_ragged(ags, a.cpu_seconds)
+    info = rb.plan.info()
+    return {
+        "metric": "agent-steps/s (batched ragged FPI+EFE step, 4096 agents, HIP graph)",
+        "value": n / (ms_step * 1e-3),
+        "unit": "agent-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 data, f64 accumulation",
+        "data": "synthetic (seeded ragged shapes, Dirichlet A/B, N(0,1) C; generated on device)",
+        "config": {"workload": "ragged", "agents": n, "policies_total": ragged.num_policies(ags),
+                   "num_iter": w["num_iter"], "gamma": ags[0]["gamma"], "efe_work_items": info["items"],
+                   "grids": [info["grid_fpi"], info["grid_efe"]]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("ragged step"),
+                     "kernel": "rg_fpi_kernel + rg_efe_kernel + rg_assemble_kernel (one graph)",
+                     "kernel_ms": ms_events, "alg_bytes_per_launch": alg},
+        "plain_per_agent": {"value": plain_rate, "unit": "agent-steps/s",
+                            "sample": f"every 16th agent ({len(sample)}), batch-1 infer_states_ffi + "
+                                      f"update_posterior_policies_ffi + argmax, {plain_s:.2f} s",
+                            "same_actions_as_batched": bool(same)},
+        "speedup_vs_plain": (n / (ms_step * 1e-3)) / plain_rate,
+        "parity": {"agents_checked": 4, "max_abs_err_vs_fp64": e64},
+        "cpu_baseline": cpu,
+    }
+
+
+def cpu_baseline_ragged(ags, seconds):
+    """The oracle on the host (one core): FPI restatement + the reference CPU
+    FFI's history-dedup neg-EFE, agents in order until ~`seconds` elapse."""
+    import numpy as np
+
+    from oracle import dedup_ref as DR
+    from oracle import numpy_ref as R
+    host = []
+    for ag in ags[:2048]:
+        host.append(([x.cpu().numpy() for x in ag["A"]], [x.cpu().numpy() for x in ag["B"]],
+                     [x.cpu().numpy() for x in ag["C"]], [x.cpu().numpy() for x in ag["D"]]))
+    done, t0 = 0, time.perf_counter()
+    while done < len(host) and time.perf_counter() - t0 < seconds:
+        ag = ags[done]
+        A, B, C, D = host[done]
+        ob = [np.eye(o, dtype=np.float32)[x] for o, x in zip(ag["O"], ag["obs"])]
+        qs = R.run_factorized_fpi(A, ob, D, ag["A_dependencies"], 16, np.float32)
+        G = DR.neg_efe_dedup(ag["policies"], qs, A, B, C, ag["A_dependencies"], ag["B_dependencies"],
+                             dt=np.float32)
+        q = DR.q_pi_from_G(G, ag["gamma"], dt=np.float32)
+        R.sample_policy(ag["policies"], q)
+        done += 1
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "agent-steps/s", "cores": 1, "kind": "port",
+            "sample": f"first {done} agents of the workload, {el:.1f} s, NumPy FP32 restatement (FPI + "
+                      f"history-dedup neg-EFE + q_pi + action), single thread"}
+
+
 def main():
     a = _args()
     if a.write_traffic:
@@ -294,6 +442,8 @@ def main():
         return
     if a.workload == "plain_fpi_large":
         res = bench_plain_fpi_large(a, torch)
+    elif a.workload == "ragged":
+        res = bench_ragged(a, torch)
     else:
         res = bench_plain_efe(a, torch)
     print(json.dumps(res))

@@ -242,6 +242,11 @@ int pymdp_hip_ragged_step(void* plan, const float* params, const int32_t* obs,
  * kernel) and keeps the fastest; not while `stream` is being captured. */
 int pymdp_hip_ragged_autotune(void* plan, const float* params, const int32_t* obs,
                               float* out, int32_t* actions, void* stream, float* best_ms);
+/* Optional timing trace: a device buffer of 2 * (agents + items) int64
+ * values that later steps fill with wall-clock ticks (100 MHz): [2a, 2a+1]
+ * = start / end of agent a's FPI + rollout, [2(agents + i), ...] = start /
+ * end of EFE work item i.  NULL switches the trace off. */
+int pymdp_hip_ragged_set_trace(void* plan, int64_t* trace);
 int pymdp_hip_ragged_plan_destroy(void* plan);
 
 #ifdef __cplusplus

@@ -73,6 +73,8 @@ struct FpiDesc {
   int jbase[kMaxG];
   int jcount[kMaxG];
   long long ws_off[kMaxG];
+  int scr_global;        // JMAX == 0: weight tables and stage buffers in the workspace
+  long long scr_ws_off;  //   at this offset of the agent's workspace row
 };
 
 struct FpiPtrs {
@@ -145,12 +147,13 @@ __global__ __launch_bounds__(NT) void fpi_kernel(const FpiDesc d, const FpiPtrs 
   double* log_q = smd;
   double* qv = log_q + d.lds_S;
   double* lp = qv + d.lds_S;
-  double* wbuf = lp + d.lds_S;
-  double* xa = wbuf + d.lds_w;
-  double* xb = xa + d.lds_x;
-  double* red0 = xb + d.lds_x;
+  double* red0 = lp + d.lds_S;
   double* red1 = red0 + NW * kWave;
   float* obs = reinterpret_cast<float*>(red1 + NW * kWave);
+  double* wbuf = (JMAX == 0 && d.scr_global) ? p.ws + (long long)b * d.ws_stride + d.scr_ws_off
+                                             : reinterpret_cast<double*>(obs + ((d.lds_O + 1) & ~1));
+  double* xa = wbuf + d.lds_w;
+  double* xb = xa + d.lds_x;
 
   // ---- priors and observations --------------------------------------------
   if (d.mode) {
@@ -735,8 +738,8 @@ static int build_fpi_desc(FpiDesc* d, const int64_t* S, int64_t F, int64_t M, co
 }
 
 static size_t lds_bytes(const FpiDesc& d, int NW) {
-  return sizeof(double) * (size_t)(3 * d.lds_S + d.lds_w + 2 * d.lds_x + 2 * NW * kWave) +
-         sizeof(float) * (size_t)d.lds_O;
+  return sizeof(double) * (size_t)(3 * d.lds_S + 2 * NW * kWave + (d.scr_global ? 0 : d.lds_w + 2 * d.lds_x)) +
+         sizeof(float) * (size_t)((d.lds_O + 1) & ~1);
 }
 
 // Register slots per lane of stage 0 when the workgroup has NW waves.
@@ -944,10 +947,19 @@ static int launch_fpi(const FpiDesc& d0, FpiPtrs p, int /*unused*/, int64_t batc
   // 4 waves when the log-likelihood fits 16 FP64 slots per lane, else 8 waves
   // (up to 32 slots), else the streaming variant.
   int NW = 4;
+  d.scr_global = 0;
   int jslots = assign_slots(&d, 4);
   if (jslots > 16) {
     NW = 8;
     jslots = assign_slots(&d, 8);
+  }
+  if (lds_bytes(d, NW) > (size_t)kLdsBudget) {
+    // weight tables and stage buffers move to the workspace (streaming variant)
+    NW = 8;
+    jslots = 33;
+    d.scr_global = 1;
+    d.scr_ws_off = d.ws_stride;
+    d.ws_stride = (d.ws_stride + d.lds_w + 2LL * d.lds_x + 63) / 64 * 64;
   }
   const size_t lds = lds_bytes(d, NW);
   if (lds > (size_t)kLdsBudget)

@@ -55,7 +55,6 @@ constexpr int kNW = kNT / kWave;
 constexpr int kHC = 16;                 // max first-axis histories per EFE item
 constexpr int kFpiScr = 2048;           // FP64 LL + stage scratch kept in LDS
 constexpr int kTileFloats = 4096;       // A tile of an EFE item (floats)
-constexpr int kHaDoubles = 2048;        // sum_o A ln A of the tile's columns
 
 struct Agent {
   int F, M, T, P, G, iters, totS, has_E;
@@ -94,6 +93,7 @@ struct Bufs {
   double* ws2;
   long long ws1_slot, ws2_slot;
   int* cnt;
+  long long* trace;  // optional: [agent or item][start, end] wall clock ticks
   int n_agents, n_items;
 };
 
@@ -113,13 +113,41 @@ __device__ void rg_softmax(const Agent& a, const double* src, double* dst, int l
   }
 }
 
-// Kronecker weight of row r over the axes before c: prod_{j<c} q_{fac_j}[digit_j(r)].
-__device__ __forceinline__ double rg_weight(const Agent& a, int g, int c, int r, const double* qv) {
-  if (c == 0) return 1.0;
-  if (c == 1) return qv[a.S_off[a.gfac[g][0]] + r];
-  const int d1 = a.gdim[g][1];
-  const int hi = r / d1;
-  return qv[a.S_off[a.gfac[g][0]] + hi] * qv[a.S_off[a.gfac[g][1]] + (r - hi * d1)];
+// One stage of the staged leave-one-out contraction over rows [0, rows) of
+// X [rows x n] (lane-per-column mapping): acc += w[r] X[r, col] (weights =
+// Kronecker product of the leading factors' q), dst[r] = X[r, :] . q_c.
+// U rows per lane are loaded before any is used.
+template <int U>
+__device__ __forceinline__ void rg_stage_rows(const double* src, int rows, int n, int c, int lg, int per, int rbase,
+                                              int col, const double* q0p, const double* q1p, int d1, double inv1,
+                                              double qcol, double* dst, double& acc) {
+  for (int r0 = 0; r0 < rows; r0 += U * per) {
+    double x[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + u * per + rbase;
+      x[u] = (r < rows && col < n) ? src[(long long)r * n + col] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int r = r0 + u * per + rbase;
+      const bool rv = r < rows;
+      double w = 0.0;
+      if (rv) {
+        if (c == 0) {
+          w = 1.0;
+        } else if (c == 1) {
+          w = q0p[r];
+        } else {
+          const int hi = (int)(r * inv1 + 1e-9);  // r / d1 (exact: r < 2^22)
+          w = q0p[hi] * q1p[r - hi * d1];
+        }
+      }
+      acc = fma(w, x[u], acc);
+      const double v = group_sum_low(x[u] * qcol, lg);
+      if (dst && rv && col == 0) dst[r] = v;
+    }
+  }
 }
 
 __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
@@ -140,6 +168,7 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
     if (idx >= b.n_agents) break;
     const Agent& a = b.ag[b.fpi_order[idx]];
     double* scr = a.scr_lds ? s_scr : wsl;
+    if (b.trace && tid == 0) b.trace[2 * b.fpi_order[idx]] = wall_clock64();
 
     for (int f = 0; f < a.F; ++f)
       for (int i = tid; i < a.S[f]; i += kNT) {
@@ -193,14 +222,16 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
           const double qcol = col < n ? s_qv[a.S_off[f] + col] : 0.0;
           double acc = 0.0;
           const int per = kNW << rpw_lg;
-          for (int r0 = 0; r0 < rows; r0 += per) {
-            const int r = r0 + (wave << rpw_lg) + lr;
-            const bool rv = r < rows;
-            const double x = (rv && col < n) ? src[(long long)r * n + col] : 0.0;
-            acc = fma(rv ? rg_weight(a, g, c, r, s_qv) : 0.0, x, acc);
-            const double v = group_sum_low(x * qcol, lg);
-            if (dst && rv && col == 0) dst[r] = v;
-          }
+          const int rbase = (wave << rpw_lg) + lr;
+          const double* q0p = c >= 1 ? s_qv + a.S_off[a.gfac[g][0]] : nullptr;
+          const double* q1p = c == 2 ? s_qv + a.S_off[a.gfac[g][1]] : nullptr;
+          const int d1 = c == 2 ? a.gdim[g][1] : 1;
+          const double inv1 = 1.0 / d1;
+          // streamed stage 0 of a large agent: 16 row loads in flight per lane
+          if (!a.scr_lds && k == 0 && rows >= 16 * per)
+            rg_stage_rows<16>(src, rows, n, c, lg, per, rbase, col, q0p, q1p, d1, inv1, qcol, dst, acc);
+          else
+            rg_stage_rows<4>(src, rows, n, c, lg, per, rbase, col, q0p, q1p, d1, inv1, qcol, dst, acc);
           acc = group_sum_high(acc, lg);
           if ((lane >> lg) == 0 && col < n) red[wave * kWave + col] = acc;
           __threadfence_block();
@@ -225,7 +256,6 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
       b.out[a.qs_off + i] = (float)s_qv[i];
       hist[i] = s_qv[i];
     }
-    __threadfence();
     __syncthreads();
 
     // rollout: q_f[t][h] = B_f[:, k, u_h] . (x_j q_{dep_j}[t-1][parent_j(h)])[k]
@@ -264,20 +294,85 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
           hist[a.hist_lvl[t][f] + h * Sf + s] = acc;
         }
       }
-      __threadfence();
       __syncthreads();
     }
+    if (b.trace && tid == 0) b.trace[2 * b.fpi_order[idx] + 1] = wall_clock64();
   }
 }
 
 // ---------------------------------------------------------------------------
 // EFE items
 // ---------------------------------------------------------------------------
+// Every contraction stage is a batched small GEMM Y[b*H + h][row] =
+// sum_i X[b][row][i] q_h[i] over a row-major X whose last axis is the one
+// being contracted; X tiles are staged through LDS with coalesced loads and
+// each thread computes four histories of one row from LDS.
+constexpr int kTileBytes = (kTileFloats + 2 * kDim) * 4;
+constexpr int kHaPart = 512;
+
+// X (FP64, global) [rows x n]; row R belongs to slab b = R / slab, and its
+// outputs for histories [h0, h0 + hn) go to Y[(b * H + h) * slab + R % slab].
+__device__ void rg_stage_tile(const double* X, long long rows, long long slab, int n, const double* Q, int h0,
+                              int hn, int H, double* Y, double* tile, double* qh, int tid) {
+  constexpr int kTD = kTileBytes / 8;
+  constexpr int kPF = (kTD + kNT - 1) / kNT;
+  const int np1 = n + 1;
+  int rt = kTD / np1;
+  if (rt > rows) rt = (int)rows;
+  for (int i = tid; i < hn * n; i += kNT) qh[i] = Q[(long long)h0 * n + i];
+  const int hb_n = (hn + 3) / 4;
+  // the next tile's values are loaded into registers while this one is used
+  double pf[kPF];
+  auto fetch = [&](long long r0, int nr) {
+#pragma unroll
+    for (int j = 0; j < kPF; ++j) {
+      const int e = tid + j * kNT;
+      pf[j] = e < nr * n ? X[r0 * n + e] : 0.0;
+    }
+  };
+  fetch(0, (int)min((long long)rt, rows));
+  for (long long r0 = 0; r0 < rows; r0 += rt) {
+    const int nr = (int)min((long long)rt, rows - r0);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPF; ++j) {
+      const int e = tid + j * kNT;
+      if (e < nr * n) {
+        const int r = e / n, i = e - r * n;
+        tile[r * np1 + i] = pf[j];
+      }
+    }
+    __syncthreads();
+    if (r0 + rt < rows) fetch(r0 + rt, (int)min((long long)rt, rows - r0 - rt));
+    for (int e = tid; e < hb_n * nr; e += kNT) {
+      const int hb = e / nr, r = e - hb * nr;
+      const int hh0 = hb * 4;
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int i = 0; i < n; ++i) {
+        const double x = tile[r * np1 + i];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          if (hh0 + u < hn) acc[u] = fma(x, qh[(hh0 + u) * n + i], acc[u]);
+      }
+      const long long R = r0 + r;
+      const long long bb = R / slab;
+      double* y = Y + (bb * H + h0 + hh0) * slab + (R - bb * slab);
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (hh0 + u < hn) y[(long long)u * slab] = acc[u];
+    }
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(kNT) void rg_efe_kernel(Bufs b) {
-  __shared__ float s_tile[kTileFloats + 2 * kDim];
-  __shared__ double s_ha[kHaDoubles];
+  __shared__ __align__(16) unsigned char s_tile_raw[kTileBytes];
+  __shared__ double s_hap[kHaPart];
+  __shared__ double s_ha[kHaPart];
   __shared__ double s_qh[kHC * kDim];
   __shared__ int s_item;
+  float* s_tile = reinterpret_cast<float*>(s_tile_raw);
+  double* s_tiled = reinterpret_cast<double*>(s_tile_raw);
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
   double* Z = b.ws2 + (long long)blockIdx.x * b.ws2_slot;
 
@@ -289,6 +384,8 @@ __global__ __launch_bounds__(kNT) void rg_efe_kernel(Bufs b) {
     const Item itm = b.items[idx];
     const Agent& a = b.ag[itm.agent];
     const int m = itm.m, t = itm.t;
+    long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_agents + idx) : nullptr;
+    if (tr && tid == 0) tr[0] = wall_clock64();
     const int g = a.mod_group[m];
     const int R = a.gR[g];
     const int O = a.O[m], O1 = O + 1;
@@ -306,26 +403,62 @@ __global__ __launch_bounds__(kNT) void rg_efe_kernel(Bufs b) {
     // stage 1: Z1[h][o'][p] = sum_i X[o'][p][i] q_h[i], X = [A ; sum_o A ln A]
     const int np1 = n + 1;
     int pt = kTileFloats / (O * np1);
-    if (pt > kHaDoubles / np1) pt = kHaDoubles / np1;
+    if (pt > kHaPart / np1) pt = kHaPart / np1;
     if (pt < 1) pt = 1;
     if (pt > pre) pt = pre;
     const int hb_n = (hc + 3) / 4;
+    constexpr int kPF1 = (kTileFloats + 2 * kDim + kNT - 1) / kNT;
+    float pf[kPF1];
+    auto fetch1 = [&](int p0, int np) {
+      const int ncol = np * n;
+#pragma unroll
+      for (int j = 0; j < kPF1; ++j) {
+        const int e = tid + j * kNT;
+        if (e < O * ncol) {
+          const int o = e / ncol;
+          const int rem = e - o * ncol;
+          pf[j] = __builtin_nontemporal_load(Am + (long long)o * K + (long long)p0 * n + rem);
+        }
+      }
+    };
+    fetch1(0, min(pt, pre));
     for (int p0 = 0; p0 < pre; p0 += pt) {
       const int np = min(pt, pre - p0);
+      const int ncol = np * n;
       __syncthreads();
-      const int tile_n = O * np * n;
-      for (int e = tid; e < tile_n; e += kNT) {
-        const int o = e / (np * n);
-        const int rem = e - o * (np * n);
-        const int p = rem / n, i = rem - p * n;
-        s_tile[(o * np + p) * np1 + i] = __builtin_nontemporal_load(Am + (long long)o * K + (long long)p0 * n + rem);
+#pragma unroll
+      for (int j = 0; j < kPF1; ++j) {
+        const int e = tid + j * kNT;
+        if (e < O * ncol) {
+          const int o = e / ncol;
+          const int rem = e - o * ncol;
+          const int p = rem / n, i = rem - p * n;
+          s_tile[(o * np + p) * np1 + i] = pf[j];
+        }
       }
       __syncthreads();
-      for (int e = tid; e < np * n; e += kNT) {
-        const int p = e / n, i = e - p * n;
-        double s = 0.0;
-        for (int o = 0; o < O; ++o) s += (double)xlogx_f(s_tile[(o * np + p) * np1 + i]);
-        s_ha[p * np1 + i] = s;
+      if (p0 + pt < pre) fetch1(p0 + pt, min(pt, pre - p0 - pt));
+      // sum_o x ln x per column, the outcome axis split over oc chunks
+      {
+        int oc = kNT / ncol;
+        if (oc < 1) oc = 1;
+        if (oc > O) oc = O;
+        const int ochunk = (O + oc - 1) / oc;
+        for (int e = tid; e < oc * ncol; e += kNT) {
+          const int ch = e / ncol, col = e - ch * ncol;
+          const int p = col / n, i = col - p * n;
+          double sacc = 0.0;
+          const int o1 = min(O, (ch + 1) * ochunk);
+          for (int o = ch * ochunk; o < o1; ++o) sacc += (double)xlogx_f(s_tile[(o * np + p) * np1 + i]);
+          s_hap[e] = sacc;
+        }
+        __syncthreads();
+        for (int col = tid; col < ncol; col += kNT) {
+          double sacc = 0.0;
+          for (int ch = 0; ch < oc; ++ch) sacc += s_hap[ch * ncol + col];
+          const int p = col / n, i = col - p * n;
+          s_ha[p * np1 + i] = sacc;
+        }
       }
       __syncthreads();
       const int outs = hb_n * O1 * np;
@@ -351,29 +484,17 @@ __global__ __launch_bounds__(kNT) void rg_efe_kernel(Bufs b) {
 
     // stages 2..R: contract the next axis with every history of its factor
     double* zin = Z;
-    long long Bin = hc, rows_in = pre;
+    long long Bin = hc, rows_in = pre;  // zin: [Bin][O1 * rows_in], last axis contracted next
     for (int j = R - 2; j >= 0; --j) {
       const int nj = a.gdim[g][j];
       const int fj = a.gfac[g][j];
       const int Hj = a.H[t][fj];
       const double* qj = hist + a.hist_lvl[t][fj];
       const long long rows_out = rows_in / nj;
+      const long long slab = O1 * rows_out;  // output values per (b, h)
       double* zout = zin + Bin * O1 * rows_in;
-      const long long total = Bin * Hj * O1 * rows_out;
-      for (long long e = tid; e < total; e += kNT) {
-        long long rest = e;
-        const long long r = rest % rows_out;
-        rest /= rows_out;
-        const int o = (int)(rest % O1);
-        rest /= O1;
-        const int h = (int)(rest % Hj);
-        const long long bb = rest / Hj;
-        const double* x = zin + (bb * O1 + o) * rows_in + r * nj;
-        const double* q = qj + (long long)h * nj;
-        double acc = 0.0;
-        for (int i = 0; i < nj; ++i) acc = fma(x[i], q[i], acc);
-        zout[((bb * Hj + h) * O1 + o) * rows_out + r] = acc;
-      }
+      for (int h0 = 0; h0 < Hj; h0 += kHC)
+        rg_stage_tile(zin, Bin * slab, slab, nj, qj, h0, min(kHC, Hj - h0), Hj, zout, s_tiled, s_qh, tid);
       __threadfence_block();
       __syncthreads();
       zin = zout;
@@ -397,6 +518,7 @@ __global__ __launch_bounds__(kNT) void rg_efe_kernel(Bufs b) {
       v = wave_sum(v);
       if (lane == 0) sc[c] = v + z[O];
     }
+    if (tr && tid == 0) tr[1] = wall_clock64();
     __syncthreads();
   }
 }
@@ -500,6 +622,7 @@ struct Plan {
   long long hist_total = 0, score_total = 0, ws1_need = 0, ws2_need = 0;
   int grid1 = 0, grid2 = 0, grid_max = 0;
   bool tuned = false;
+  long long* trace = nullptr;  // caller-owned device buffer, see pymdp_hip_ragged_set_trace
   // device
   Agent* d_ag = nullptr;
   int* d_order = nullptr;
@@ -996,6 +1119,7 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
   b.cnt = pl->d_cnt;
   b.n_agents = (int)pl->n_agents;
   b.n_items = (int)pl->items.size();
+  b.trace = pl->trace;
   hipLaunchKernelGGL(rg_fpi_kernel, dim3(grid1), dim3(kNT), 0, s, b);
   hipLaunchKernelGGL(rg_efe_kernel, dim3(grid2), dim3(kNT), 0, s, b);
   hipLaunchKernelGGL(rg_assemble_kernel, dim3((unsigned)pl->n_agents), dim3(kNT), 0, s, b);
@@ -1065,6 +1189,13 @@ extern "C" int pymdp_hip_ragged_autotune(void* plan, const float* params, const 
   pl->grid2 = b2;
   pl->tuned = true;
   if (best_ms_out) *best_ms_out = best / 3.0f;
+  return PYMDP_HIP_OK;
+}
+
+extern "C" int pymdp_hip_ragged_set_trace(void* plan, int64_t* trace) {
+  Plan* pl = static_cast<Plan*>(plan);
+  if (!pl) return einval("plan is NULL");
+  pl->trace = reinterpret_cast<long long*>(trace);
   return PYMDP_HIP_OK;
 }
 

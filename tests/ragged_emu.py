@@ -18,7 +18,7 @@ from pymdp_amd import _native
 
 MINVAL = np.float32(2.220446049250313e-16)
 kF, kM, kR, kT, kDim, kNT = 8, 8, 3, 4, 64, 256
-kHC, kFpiScr, kTileFloats, kHaDoubles = 16, 2048, 4096, 2048
+kHC, kFpiScr, kTileFloats, kHaPart = 16, 2048, 4096, 512
 TILE_LDS = kTileFloats + 2 * kDim
 
 AGENT_DT = np.dtype([
@@ -213,13 +213,16 @@ class Emu:
         Z = np.full(ws_n, np.nan)
         np1 = n + 1
         pt = kTileFloats // (O * np1)
-        pt = min(pt, kHaDoubles // np1)
+        pt = min(pt, kHaPart // np1)
         pt = max(pt, 1)
         pt = min(pt, pre)
         for p0 in range(0, pre, pt):
             np_ = min(pt, pre - p0)
+            ncol = np_ * n
+            oc = min(max(kNT // ncol, 1), O)
             _chk(np.array([0, (O * np_ - 1) * np1 + n - 1]), TILE_LDS, "s_tile")
-            _chk(np.array([0, (np_ - 1) * np1 + n - 1]), kHaDoubles, "s_ha")
+            _chk(np.array([0, (np_ - 1) * np1 + n - 1]), kHaPart, "s_ha")
+            _chk(np.array([0, oc * ncol - 1]), kHaPart, "s_hap")
             gidx = int(a["A_off"][m]) + np.arange(O)[:, None] * K + p0 * n + np.arange(np_ * n)[None, :]
             tile = P[_chk(gidx, P.size, "A tile")].reshape(O, np_, n).astype(np.float64)
             t32 = tile.astype(np.float32)
@@ -239,6 +242,9 @@ class Emu:
             qj = qj.reshape(Hj, nj)
             rows_out = rows_in // nj
             assert rows_out * nj == rows_in
+            rt = min((TILE_LDS * 4 // 8) // (nj + 1), O1 * rows_out)
+            _chk(np.array([0, (rt - 1) * (nj + 1) + nj - 1]), TILE_LDS * 4 // 8, "stage tile")
+            _chk(np.array([0, min(kHC, Hj) * nj - 1]), kHC * kDim, "stage qh")
             zout_off = zin_off + Bin * O1 * rows_in
             zin = Z[_chk(zin_off + np.arange(Bin * O1 * rows_in), ws_n, "zin")].reshape(Bin, O1, rows_out, nj)
             assert np.isfinite(zin).all()
@@ -340,8 +346,10 @@ def check_bounds_only(plan, total_hist=None):
             need += Bin * O1 * rows
         assert need <= info["ws_efe"], (need, info["ws_efe"])
         np1 = n + 1
-        pt = max(1, min(kTileFloats // ((O1 - 1) * np1), kHaDoubles // np1))
+        pt = max(1, min(kTileFloats // ((O1 - 1) * np1), kHaPart // np1))
         pt = min(pt, K // n)
         assert ((O1 - 1) * pt - 1) * np1 + n - 1 < TILE_LDS
-        assert (pt - 1) * np1 + n - 1 < kHaDoubles
+        assert (pt - 1) * np1 + n - 1 < kHaPart
+        ncol = pt * n
+        assert min(max(kNT // ncol, 1), O1 - 1) * ncol <= kHaPart
     return info
