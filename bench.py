@@ -9,6 +9,11 @@ Workloads (seeded synthetic data, generated on the device):
   plain_efe                  64 agents, S=[32,16,8], U=[8,4,1], horizon 3
                              (32768 policies), 4 x 64 outcomes: one step =
                              one infer_policies (neg-EFE + q_pi) for all agents.
+  grad                       512 agents, S=[16,8], U=[16,4], horizon 1, 3 x 16
+                             outcomes, 32-step trajectories: one step = the
+                             gradient of sum_t ln q_pi(a_t) w.r.t. the logits of
+                             A, B, C, D through the HIP adjoints, compared with
+                             torch autograd through the pure-torch forward.
   ragged                     4096 agents of different shapes (1-4 factors of
                              2-64 states, 1-8 controls, 1-5 modalities of 2-64
                              outcomes, horizon 1-2): one step = infer_states ->
@@ -51,7 +56,7 @@ def _args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="plain_fpi_large", choices=["plain_fpi_large", "plain_efe", "ragged"])
+    ap.add_argument("--workload", default="plain_fpi_large", choices=["plain_fpi_large", "plain_efe", "ragged", "grad"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -397,6 +402,65 @@ def bench_ragged(a, torch):
     }
 
 
+def bench_grad(a, torch):
+    from pymdp_amd import autograd_baseline as AB
+    from pymdp_amd import differentiable as DF
+    from pymdp_amd import workloads
+
+    dev = torch.device("cuda:0")
+    w = workloads.grad_fpi_efe(seed=0, device=dev)
+    keys = ("A_logits", "B_logits", "C_logits", "D_logits")
+    ph = [[x.clone().requires_grad_() for x in w[k]] for k in keys]
+    pr = [[x.clone().requires_grad_() for x in w[k]] for k in keys]
+
+    def run(params, fn, **kw):
+        for grp in params:
+            for x in grp:
+                x.grad = None
+        loss = fn(*params, w["obs"], w["actions"], w["policies"], w["A_dependencies"], num_iter=w["num_iter"],
+                  gamma=w["gamma"], **kw)
+        loss.backward()
+        return loss
+
+    def hip():
+        return run(ph, DF.trajectory_loss, B_dependencies=w["B_dependencies"])
+
+    def ref():
+        return run(pr, AB.trajectory_loss)
+
+    def timed(fn, k):
+        for _ in range(max(1, a.warmup)):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / k
+
+    ms_hip = timed(hip, a.steps)
+    ms_ref = timed(ref, max(2, a.steps // 5))
+    num = max(float((x.grad - y.grad).abs().max()) for gx, gy in zip(ph, pr) for x, y in zip(gx, gy))
+    den = max(float(y.grad.abs().max()) for gy in pr for y in gy)
+    B = w["batch"]
+    return {
+        "metric": "agent-trajectory gradients/s (512 agents x 32 steps, dL/d logits of A, B, C, D)",
+        "value": B / (ms_hip * 1e-3), "unit": "agent-gradients/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_hip, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 data, f64 adjoint accumulation",
+        "data": "synthetic (N(0,1) logits, uniform observation / policy trajectories)",
+        "config": {"workload": "grad", "agents": B, "steps_per_trajectory": w["T"], "num_states": w["S"],
+                   "num_controls": w["U"], "num_obs": w["O"], "num_iter": w["num_iter"],
+                   "adjoint": "implicit where converged, unrolled otherwise"},
+        "grads_per_s": 1e3 / ms_hip,
+        "autograd_baseline": {"ms_per_gradient": ms_ref, "grads_per_s": 1e3 / ms_ref,
+                              "what": "torch autograd through the pure-torch FP32 forward on the same GPU"},
+        "speedup_vs_autograd": ms_ref / ms_hip,
+        "parity": {"max_abs_grad_diff_vs_autograd": num, "max_abs_grad": den},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def cpu_baseline_ragged(ags, seconds):
     """The oracle on the host (one core): FPI restatement + the reference CPU
     FFI's history-dedup neg-EFE, agents in order until ~`seconds` elapse."""
@@ -444,6 +508,8 @@ def main():
         res = bench_plain_fpi_large(a, torch)
     elif a.workload == "ragged":
         res = bench_ragged(a, torch)
+    elif a.workload == "grad":
+        res = bench_grad(a, torch)
     else:
         res = bench_plain_efe(a, torch)
     print(json.dumps(res))

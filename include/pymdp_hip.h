@@ -187,6 +187,57 @@ int pymdp_hip_neg_efe_all_policies(const int32_t* policy_host, const float* qs_f
                                    int32_t flags, void* stream);
 
 /*
+ * Adjoint (vector-Jacobian product) of pymdp_hip_infer_states with index
+ * observations.  Replaces the backward of the reference's with_jax_grad
+ * wrapper around the FPI FFI call (pymdp/ffi/_core.py:282-310, which runs
+ * jax.vjp of pymdp/algos.py:88-155); no HIP counterpart in the reference.
+ *
+ *   q_bar [batch, sum S]            dL/dq of the posterior
+ *   A_bar_ptrs[m] [batch, O_m*K_m]  dL/dA_m, written at the observed rows
+ *                                   only (caller zero-fills the rest)
+ *   prior_bar [batch, sum S]        dL/dprior
+ *   q_out, mode_out (optional)      recomputed posterior; 1 where the
+ *                                   implicit adjoint was used, 0 unrolled
+ *   mode                            0 = unrolled reverse sweep; 1 = implicit
+ *                                   differentiation at the fixed point when
+ *                                   max|q^(N) - q^(N-1)| <= implicit_tol,
+ *                                   else (or if the adjoint iteration does
+ *                                   not converge) the unrolled sweep
+ * Limits: F, M <= 8, sum S <= 256, ranks <= 3, sum over dependency groups
+ * of the joint state counts <= 2048, per-agent A (batch stride > 0).
+ */
+int pymdp_hip_fpi_adjoint(const float* const* A_ptrs, const int64_t* A_batch_stride,
+                          const int32_t* obs_idx,
+                          const float* const* prior_ptrs, const int64_t* prior_batch_stride,
+                          const float* q_bar, float* const* A_bar_ptrs, float* prior_bar,
+                          float* q_out, int32_t* mode_out,
+                          int64_t batch, const int64_t* S, int64_t F, const int64_t* O, int64_t M,
+                          const int64_t* A_dep_flat, const int64_t* A_dep_offsets,
+                          int32_t num_iter, int32_t mode, float implicit_tol, void* stream);
+
+/*
+ * Adjoint of the neg-EFE of every policy (horizon 1, factor-local B,
+ * utility and state information gain) w.r.t. qs, A, B and C given
+ * G_bar = dL/dG [batch, P].  Replaces the backward of with_jax_grad around
+ * the neg-EFE FFI call (pymdp/ffi/_core.py:282-310 over
+ * pymdp/control.py:604-690).  policy_dev: int32 [P, 1, F] in DEVICE memory.
+ * A_bar / B_bar / C_bar / qs_bar are written in full.
+ */
+int pymdp_hip_neg_efe_adjoint(const int32_t* policy_dev, int64_t P, int64_t T,
+                              const float* qs_flat,
+                              const float* const* A_ptrs, const int64_t* A_batch_stride,
+                              const float* const* B_ptrs, const int64_t* B_batch_stride,
+                              const float* const* C_ptrs, const int64_t* C_batch_stride,
+                              const float* G_bar, float* qs_bar,
+                              float* const* A_bar_ptrs, float* const* B_bar_ptrs,
+                              float* const* C_bar_ptrs,
+                              int64_t batch, const int64_t* S, int64_t F,
+                              const int64_t* O, int64_t M, const int64_t* U,
+                              const int64_t* A_dep_flat, const int64_t* A_dep_offsets,
+                              const int64_t* B_dep_flat, const int64_t* B_dep_offsets,
+                              int32_t flags, void* stream);
+
+/*
  * Batched ragged agents: one infer_states -> infer_policies -> sample_action
  * step for many independent agents of different shapes.  No counterpart in
  * the reference (its FFI handles one agent shape per call); it composes the

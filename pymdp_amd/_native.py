@@ -70,6 +70,19 @@ SIGNATURES: dict[str, tuple] = {
         _i64p, _i64p, _i64p, _i64p, _i64p, _i64p,            # qs/A/B/C/I offsets, I_depths
         _i64p, _i64p, _i64p, _i64p,              # A deps, B deps
         ctypes.c_int32, _vp]),
+    "pymdp_hip_fpi_adjoint": (ctypes.c_int, [
+        _vp, _i64p, _vp, _vp, _i64p,             # A ptrs/stride, obs, prior ptrs/stride
+        _vp, _vp, _vp, _vp, _vp,                 # q_bar, A_bar ptrs, prior_bar, q_out, mode_out
+        ctypes.c_int64, _i64p, ctypes.c_int64, _i64p, ctypes.c_int64,  # batch, S, F, O, M
+        _i64p, _i64p,                            # A deps
+        ctypes.c_int32, ctypes.c_int32, ctypes.c_float, _vp]),
+    "pymdp_hip_neg_efe_adjoint": (ctypes.c_int, [
+        _vp, ctypes.c_int64, ctypes.c_int64, _vp,  # policy (device), P, T, qs
+        _vp, _i64p, _vp, _i64p, _vp, _i64p,      # A, B, C ptrs/strides
+        _vp, _vp, _vp, _vp, _vp,                 # G_bar, qs_bar, A_bar, B_bar, C_bar
+        ctypes.c_int64, _i64p, ctypes.c_int64, _i64p, ctypes.c_int64, _i64p,  # batch, S, F, O, M, U
+        _i64p, _i64p, _i64p, _i64p,              # A deps, B deps
+        ctypes.c_int32, _vp]),
     # batched ragged agents (no counterpart in the reference)
     "pymdp_hip_ragged_plan_create": (ctypes.c_int, [
         ctypes.c_int64, _i64p,                   # n_agents, hdr [n, 4]

@@ -111,3 +111,27 @@ def ragged(seed: int = 0, device="cpu", n_agents: int = 4096, max_policies: int 
         agents.append(dict(S=S, U=U, O=O, T=T, A=A, B=B, C=C, D=D, E=None, obs=obs, A_dependencies=A_deps,
                            B_dependencies=B_deps, policies=policies, gamma=float(gamma)))
     return dict(name="ragged", agents=agents, num_iter=num_iter, n_agents=n_agents)
+
+
+def grad_fpi_efe(seed: int = 0, device="cpu", batch: int = 512, T: int = 32, S=(16, 8), U=(16, 4),
+                 O=(16, 16, 16)) -> dict:
+    """Model fitting through FPI + EFE: per agent, logits of A [O_m, S...]
+    (every modality depends on every factor), B [S_f, S_f, U_f]
+    (factor-local), C [O_m] and D [S_f], all N(0,1); policies of horizon 1
+    over U (P = prod U); T-step trajectories of observation indices and
+    taken policy indices, uniform."""
+    g = utils._gen(seed, device)
+    S, U, O = list(S), list(U), list(O)
+    F = len(S)
+    A_deps = [list(range(F)) for _ in O]
+    B_deps = [[f] for f in range(F)]
+    A_logits = [torch.randn((batch, o, *S), generator=g, device=device) for o in O]
+    B_logits = [torch.randn((batch, s, s, u), generator=g, device=device) for s, u in zip(S, U)]
+    C_logits = [torch.randn((batch, o), generator=g, device=device) for o in O]
+    D_logits = [torch.randn((batch, s), generator=g, device=device) for s in S]
+    policies = control.construct_policies(S, U, 1)
+    obs = torch.stack([torch.randint(0, o, (T, batch), generator=g, device=device) for o in O], -1).to(torch.int32)
+    actions = torch.randint(0, policies.shape[0], (T, batch), generator=g, device=device)
+    return dict(name="grad_fpi_efe", S=S, U=U, O=O, T=T, batch=batch, A_logits=A_logits, B_logits=B_logits,
+                C_logits=C_logits, D_logits=D_logits, policies=policies, obs=obs, actions=actions,
+                A_dependencies=A_deps, B_dependencies=B_deps, gamma=1.0, num_iter=16)
