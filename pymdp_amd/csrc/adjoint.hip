@@ -40,7 +40,6 @@ constexpr int kS = 256;      // total states of all factors
 constexpr int kLL = 2048;    // sum of the dependency groups' joint state counts
 constexpr int kEl = kLL / kNT;
 constexpr int kQ1 = 2048;    // sum_f U_f * S_f (predicted states per action)
-constexpr int kK = 2048;     // joint states of one modality (EFE)
 constexpr int kCO = 4096;    // history combinations x outcomes of one modality
 
 struct Desc {
@@ -110,9 +109,14 @@ __device__ __forceinline__ int gidx(const Desc& d, int g, const int* s) {
   return e;
 }
 
-// M_f(q)[s_f] = sum_{g contains f} sum_{s: s_f} LL_g[s] prod_{j != axis f} q_{f_j}[s_j]   (bin i)
-// or, with lb != nullptr, (dF/dq)^T: sum_g sum_{a != b} sum_{s: s_b} LL_g[s] lb_{f_a}[s_a] prod_{j != a, b} q
-__device__ double bin_contract(const Desc& d, const double* LL, const double* q, const double* lb, int i) {
+constexpr int kGrpF = 8;  // lanes per bin of the leave-one-out contractions
+
+// Partial (lanes part, part + kGrpF, ...) of, for bin i = (f, s_f):
+//   lb == nullptr: M_f(q)[s_f] = sum_{g contains f} sum_{s: s_f} LL_g[s] prod_{j != axis f} q_{f_j}[s_j]
+//   else (dF/dq)^T: sum_g sum_{a != b} sum_{s: s_b} LL_g[s] lb_{f_a}[s_a] prod_{j != a, b} q_{f_j}[s_j]
+// gd holds the digits of every LL element (3 bytes each).
+__device__ double bin_contract(const Desc& d, const double* LL, const unsigned char* gd, const double* q,
+                               const double* lb, int i, int part) {
   const int f = factor_of(d, i);
   const int sf = i - d.S_off[f];
   double acc = 0.0;
@@ -122,28 +126,29 @@ __device__ double bin_contract(const Desc& d, const double* LL, const double* q,
     for (int j = 0; j < R; ++j)
       if (d.gfac[g][j] == f) b = j;
     if (b < 0) continue;
-    const int others = d.gK[g] / d.gdim[g][b];
-    for (int r = 0; r < others; ++r) {
-      int s[kR];
-      int rr = r;
-      for (int j = R - 1; j >= 0; --j) {
-        if (j == b) continue;
-        s[j] = rr % d.gdim[g][j];
-        rr /= d.gdim[g][j];
-      }
-      s[b] = sf;
-      const double x = LL[d.gll[g] + gidx(d, g, s)];
+    const int nb = d.gdim[g][b];
+    int stride = 1;
+    for (int j = b + 1; j < R; ++j) stride *= d.gdim[g][j];
+    const float inv = 1.0f / (float)stride;
+    const int others = d.gK[g] / nb;
+    int qoff[kR];
+    for (int j = 0; j < R; ++j) qoff[j] = d.S_off[d.gfac[g][j]];
+    for (int r = part; r < others; r += kGrpF) {
+      const int hi = (int)(((float)r + 0.5f) * inv);  // r / stride (exact: r < 2^12)
+      const int e = d.gll[g] + (hi * nb + sf) * stride + (r - hi * stride);
+      const unsigned char* s = gd + 3 * e;
+      const double x = LL[e];
       if (!lb) {
         double w = 1.0;
         for (int j = 0; j < R; ++j)
-          if (j != b) w *= q[d.S_off[d.gfac[g][j]] + s[j]];
+          if (j != b) w *= q[qoff[j] + s[j]];
         acc = fma(x, w, acc);
       } else {
         for (int a = 0; a < R; ++a) {
           if (a == b) continue;
-          double w = lb[d.S_off[d.gfac[g][a]] + s[a]];
+          double w = lb[qoff[a] + s[a]];
           for (int j = 0; j < R; ++j)
-            if (j != a && j != b) w *= q[d.S_off[d.gfac[g][j]] + s[j]];
+            if (j != a && j != b) w *= q[qoff[j] + s[j]];
           acc = fma(x, w, acc);
         }
       }
@@ -152,8 +157,21 @@ __device__ double bin_contract(const Desc& d, const double* LL, const double* q,
   return acc;
 }
 
+// out[i] = base[i] (or 0) + bin_contract(i) for every bin, kGrpF lanes per bin.
+__device__ void all_bins(const Desc& d, const double* LL, const unsigned char* gd, const double* q, const double* lb,
+                         const double* base, double* out, int tid) {
+  for (int b0 = 0; b0 < d.totS * kGrpF; b0 += kNT) {
+    const int slot = b0 + tid;
+    const int bin = slot / kGrpF, part = slot % kGrpF;
+    double v = bin < d.totS ? bin_contract(d, LL, gd, q, lb, bin, part) : 0.0;
+    v = group_sum_low(v, 3);
+    if (bin < d.totS && part == 0) out[bin] = (base ? base[bin] : 0.0) + v;
+  }
+}
+
 // d/dLL_g[s] of sum_f lb_f . M_f(q): sum_a lb_{f_a}[s_a] prod_{j != a} q_{f_j}[s_j]
-__device__ __forceinline__ double ll_grad(const Desc& d, int g, const int* s, const double* q, const double* lb) {
+__device__ __forceinline__ double ll_grad(const Desc& d, int g, const unsigned char* s, const double* q,
+                                          const double* lb) {
   const int R = d.gR[g];
   double acc = 0.0;
   for (int a = 0; a < R; ++a) {
@@ -168,6 +186,7 @@ __device__ __forceinline__ double ll_grad(const Desc& d, int g, const int* s, co
 __global__ __launch_bounds__(kNT) void fpi_adjoint_kernel(const Desc d, const Ptrs p) {
   __shared__ double s_LL[kLL];
   __shared__ double s_lp[kS], s_q[kS], s_lq[kS], s_lb[kS], s_v[kS], s_t[kS], s_lpb[kS];
+  __shared__ unsigned char s_gd[3 * kLL];
   __shared__ int s_obs[kM];
   __shared__ int s_flag;
   const int b = blockIdx.x, tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
@@ -192,6 +211,11 @@ __global__ __launch_bounds__(kNT) void fpi_adjoint_kernel(const Desc d, const Pt
         acc += lstab((double)p.A[m][(long long)b * d.A_stride[m] + (long long)o * d.gK[g] + e]);
       }
       s_LL[d.gll[g] + e] = acc;
+      int ee = e;
+      for (int j = d.gR[g] - 1; j >= 0; --j) {
+        s_gd[3 * (d.gll[g] + e) + j] = (unsigned char)(ee % d.gdim[g][j]);
+        ee /= d.gdim[g][j];
+      }
     }
   __syncthreads();
 
@@ -201,7 +225,7 @@ __global__ __launch_bounds__(kNT) void fpi_adjoint_kernel(const Desc d, const Pt
     __syncthreads();
     for (int i = tid; i < d.totS; i += kNT) qit[(long long)it * d.totS + i] = s_q[i];
     if (it == d.iters) break;
-    for (int i = tid; i < d.totS; i += kNT) s_t[i] = s_lp[i] + bin_contract(d, s_LL, s_q, nullptr, i);
+    all_bins(d, s_LL, s_gd, s_q, nullptr, s_lp, s_t, tid);
     __syncthreads();
     for (int i = tid; i < d.totS; i += kNT) s_lq[i] = s_t[i];
     __syncthreads();
@@ -239,12 +263,12 @@ __global__ __launch_bounds__(kNT) void fpi_adjoint_kernel(const Desc d, const Pt
       __syncthreads();
       if (tid == 0) s_flag = 0;
       __syncthreads();
+      all_bins(d, s_LL, s_gd, s_q, s_lb, s_t, s_lq, tid);
+      __syncthreads();
       double dv = 0.0, vm = 0.0;
       for (int i = tid; i < d.totS; i += kNT) {
-        const double nv = s_t[i] + bin_contract(d, s_LL, s_q, s_lb, i);
-        dv = fmax(dv, fabs(nv - s_v[i]));
-        vm = fmax(vm, fabs(nv));
-        s_lq[i] = nv;
+        dv = fmax(dv, fabs(s_lq[i] - s_v[i]));
+        vm = fmax(vm, fabs(s_lq[i]));
       }
       dv = wave_max(dv);
       vm = wave_max(vm);
@@ -267,13 +291,7 @@ __global__ __launch_bounds__(kNT) void fpi_adjoint_kernel(const Desc d, const Pt
         for (int k = 0; k < kEl; ++k) {
           const int e = tid + k * kNT - d.gll[g];
           if (e < 0 || e >= d.gK[g]) continue;
-          int s[kR];
-          int rr = e;
-          for (int j = d.gR[g] - 1; j >= 0; --j) {
-            s[j] = rr % d.gdim[g][j];
-            rr /= d.gdim[g][j];
-          }
-          llb[k] += ll_grad(d, g, s, s_q, s_lb);
+          llb[k] += ll_grad(d, g, s_gd + 3 * (tid + k * kNT), s_q, s_lb);
         }
     } else {
       use_implicit = false;
@@ -297,16 +315,10 @@ __global__ __launch_bounds__(kNT) void fpi_adjoint_kernel(const Desc d, const Pt
         for (int k = 0; k < kEl; ++k) {
           const int e = tid + k * kNT - d.gll[g];
           if (e < 0 || e >= d.gK[g]) continue;
-          int s[kR];
-          int rr = e;
-          for (int j = d.gR[g] - 1; j >= 0; --j) {
-            s[j] = rr % d.gdim[g][j];
-            rr /= d.gdim[g][j];
-          }
-          llb[k] += ll_grad(d, g, s, s_q, s_lb);
+          llb[k] += ll_grad(d, g, s_gd + 3 * (tid + k * kNT), s_q, s_lb);
         }
       if (it == 1) break;  // q^(0) = softmax(0) is constant
-      for (int i = tid; i < d.totS; i += kNT) s_v[i] = bin_contract(d, s_LL, s_q, s_lb, i);
+      all_bins(d, s_LL, s_gd, s_q, s_lb, nullptr, s_v, tid);
       __syncthreads();
       sbp(d, s_q, s_v, s_lb, lane, wave);
       __syncthreads();
@@ -340,38 +352,32 @@ __global__ __launch_bounds__(kNT) void fpi_adjoint_kernel(const Desc d, const Pt
 // ---------------------------------------------------------------------------
 // neg-EFE adjoint (horizon 1, factor-local B)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double w_of(const Desc& d, int m, int c, const int* s, const double* q1) {
-  double w = 1.0;
-  int cc = c;
-  for (int j = d.mR[m] - 1; j >= 0; --j) {
-    const int f = d.mdep[m][j];
-    const int u = cc % d.U[f];
-    cc /= d.U[f];
-    w *= q1[d.q1_off[f] + u * d.S[f] + s[j]];
-  }
-  return w;
-}
+// LDS (doubles unless noted): q1, q1b [Q]; qs [totS]; HA [Kmax]; SB [Cmax];
+// QO, GQ [COmax]; A of the current modality as floats [OKmax]; digit tables
+// (bytes) ud [Cmax][3] = u_j(c), sd [Kmax][3] = s_j(k).
+constexpr int kGrp = 8;  // lanes per output bin of the dq1 reduction
 
-__device__ __forceinline__ void msplit(const Desc& d, int m, int e, int* s) {
-  for (int j = d.mR[m] - 1; j >= 0; --j) {
-    const int n = d.S[d.mdep[m][j]];
-    s[j] = e % n;
-    e /= n;
-  }
-}
-
-__global__ __launch_bounds__(kNT) void efe_adjoint_kernel(const Desc d, const Ptrs p) {
-  __shared__ double s_qs[kS], s_q1[kQ1], s_q1b[kQ1];
+template <int kMaxEl>
+__global__ __launch_bounds__(kNT) void efe_adjoint_kernel(const Desc d, const Ptrs p, int Kmax, int Cmax, int COmax,
+                                                          int OKmax) {
+  extern __shared__ double sm[];
+  const int Q = d.q1_off[d.F];
+  double* q1 = sm;
+  double* q1b = q1 + Q;
+  double* qs = q1b + Q;
+  double* HA = qs + d.totS;
+  double* SB = HA + Kmax;
+  double* QO = SB + Cmax;
+  double* GQ = QO + COmax;
+  float* As = reinterpret_cast<float*>(GQ + COmax);
+  unsigned char* ud = reinterpret_cast<unsigned char*>(As + OKmax);
+  unsigned char* sd = ud + 3 * Cmax;
   const int b = blockIdx.x, tid = threadIdx.x;
-  double* HA = p.ws + (long long)b * d.ws_stride;
-  double* QO = HA + kK;
-  double* GQ = QO + kCO;
-  double* SB = GQ + kCO;
   const bool util = d.flags & PYMDP_HIP_FLAG_USE_UTILITY;
   const bool info = d.flags & PYMDP_HIP_FLAG_USE_STATES_INFO_GAIN;
 
-  for (int i = tid; i < d.totS; i += kNT) s_qs[i] = (double)p.qs[(long long)b * d.totS + i];
-  for (int i = tid; i < d.q1_off[d.F]; i += kNT) s_q1b[i] = 0.0;
+  for (int i = tid; i < d.totS; i += kNT) qs[i] = (double)p.qs[(long long)b * d.totS + i];
+  for (int i = tid; i < Q; i += kNT) q1b[i] = 0.0;
   __syncthreads();
   // predicted states per action: q1_f[u] = B_f[:, :, u] qs_f
   for (int f = 0; f < d.F; ++f) {
@@ -380,48 +386,74 @@ __global__ __launch_bounds__(kNT) void efe_adjoint_kernel(const Desc d, const Pt
     for (int e = tid; e < U * n; e += kNT) {
       const int u = e / n, s2 = e - u * n;
       double acc = 0.0;
-      for (int s = 0; s < n; ++s) acc = fma((double)Bf[((long long)s2 * n + s) * U + u], s_qs[d.S_off[f] + s], acc);
-      s_q1[d.q1_off[f] + e] = acc;
+      for (int s = 0; s < n; ++s) acc = fma((double)Bf[((long long)s2 * n + s) * U + u], qs[d.S_off[f] + s], acc);
+      q1[d.q1_off[f] + e] = acc;
     }
   }
-  __syncthreads();
 
   for (int m = 0; m < d.M; ++m) {
     const int O = d.O[m], K = d.mK[m], C = d.mC[m], R = d.mR[m];
     const float* Am = p.A[m] + (long long)b * d.A_stride[m];
     const float* Cm = p.C[m] + (long long)b * d.C_stride[m];
-    // sum_o A ln A, and dL/dscore per history combination
-    for (int s = tid; s < K; s += kNT) {
-      double acc = 0.0;
-      for (int o = 0; o < O; ++o) {
-        const double a = (double)Am[(long long)o * K + s];
-        if (a != 0.0) acc += a * lstab(a);
-      }
-      HA[s] = acc;
+    int qo_[kR], nj[kR];
+    for (int j = 0; j < R; ++j) {
+      qo_[j] = d.q1_off[d.mdep[m][j]];
+      nj[j] = d.S[d.mdep[m][j]];
     }
+    __syncthreads();
+    for (int e = tid; e < O * K; e += kNT) As[e] = Am[e];
     for (int c = tid; c < C; c += kNT) {
+      int cc = c;
+      for (int j = R - 1; j >= 0; --j) {
+        const int U = d.U[d.mdep[m][j]];
+        ud[3 * c + j] = (unsigned char)(cc % U);
+        cc /= U;
+      }
       double acc = 0.0;
       for (int q = 0; q < d.P; ++q) {
-        int cc = 0;
-        for (int j = 0; j < R; ++j) cc += p.policy[q * d.F + d.mdep[m][j]] * d.cstride[m][j];
-        if (cc == c) acc += (double)p.Gbar[(long long)b * d.P + q];
+        int x = 0;
+        for (int j = 0; j < R; ++j) x += p.policy[q * d.F + d.mdep[m][j]] * d.cstride[m][j];
+        if (x == c) acc += (double)p.Gbar[(long long)b * d.P + q];
       }
       SB[c] = acc;
     }
+    for (int k = tid; k < K; k += kNT) {
+      int kk = k;
+      for (int j = R - 1; j >= 0; --j) {
+        sd[3 * k + j] = (unsigned char)(kk % nj[j]);
+        kk /= nj[j];
+      }
+    }
     __syncthreads();
-    // qo per combination and dL/dqo
+    // W[c, k] = prod_j q1_{d_j}[u_j(c)][s_j(k)]
+    auto W = [&](int c, int k, int skip) {
+      double w = 1.0;
+      for (int j = 0; j < R; ++j)
+        if (j != skip) w *= q1[qo_[j] + (int)ud[3 * c + j] * nj[j] + (int)sd[3 * k + j]];
+      return w;
+    };
+    for (int k = tid; k < K; k += kNT) {
+      double acc = 0.0;
+      for (int o = 0; o < O; ++o) {
+        const double a = (double)As[o * K + k];
+        if (a != 0.0) acc += a * lstab(a);
+      }
+      HA[k] = acc;
+    }
+    // qo[c, o] = sum_k A[o, k] W[c, k]
     for (int e = tid; e < C * O; e += kNT) {
       const int c = e / O, o = e - c * O;
       double acc = 0.0;
-      for (int k = 0; k < K; ++k) {
-        int s[kR];
-        msplit(d, m, k, s);
-        acc = fma((double)Am[(long long)o * K + k], w_of(d, m, c, s, s_q1), acc);
-      }
+      for (int k = 0; k < K; ++k) acc = fma((double)As[o * K + k], W(c, k, -1), acc);
       QO[e] = acc;
+    }
+    __syncthreads();
+    for (int e = tid; e < C * O; e += kNT) {
+      const int c = e / O, o = e - c * O;
+      const double qo = QO[e];
       double g = 0.0;
       if (util) g += (double)Cm[o];
-      if (info) g -= acc >= (double)kMinVal ? lstab(acc) + 1.0 : lstab(acc);
+      if (info) g -= qo >= (double)kMinVal ? lstab(qo) + 1.0 : lstab(qo);
       GQ[e] = SB[c] * g;
     }
     __syncthreads();
@@ -432,51 +464,57 @@ __global__ __launch_bounds__(kNT) void efe_adjoint_kernel(const Desc d, const Pt
           for (int c = 0; c < C; ++c) acc = fma(SB[c], QO[c * O + o], acc);
         p.Cbar[m][(long long)b * d.C_stride[m] + o] = (float)acc;
       }
-    // dA[o, s] = sum_c w_c[s] (GQ[c, o] + SB[c] (ln A + 1))
-    for (int e = tid; e < O * K; e += kNT) {
-      const int o = e / K, k = e - o * K;
-      int s[kR];
-      msplit(d, m, k, s);
-      double g1 = 0.0, g2 = 0.0;
-      for (int c = 0; c < C; ++c) {
-        const double w = w_of(d, m, c, s, s_q1);
-        g1 = fma(GQ[c * O + o], w, g1);
-        g2 = fma(SB[c], w, g2);
-      }
-      const double a = (double)Am[e];
-      double v = g1;
-      if (info) v += g2 * (a >= (double)kMinVal ? lstab(a) + 1.0 : lstab(a));
-      p.Abar[m][(long long)b * d.A_stride[m] + e] = (float)v;
-    }
-    // d q1_{d_j}[u][s_j] += sum_{c: u_j(c) = u} sum_{s: s_j} wbar_c[s] prod_{i != j} q1_{d_i}[u_i(c)][s_i]
-    for (int j = 0; j < R; ++j) {
-      const int f = d.mdep[m][j];
-      const int n = d.S[f], U = d.U[f];
-      for (int e = tid; e < U * n; e += kNT) {
-        const int u = e / n, sj = e - u * n;
-        double acc = 0.0;
-        for (int c = 0; c < C; ++c) {
-          if ((c / d.cstride[m][j]) % U != u) continue;
-          for (int k = 0; k < K; ++k) {
-            int s[kR];
-            msplit(d, m, k, s);
-            if (s[j] != sj) continue;
-            double wb = info ? SB[c] * HA[k] : 0.0;
-            for (int o = 0; o < O; ++o) wb = fma((double)Am[(long long)o * K + k], GQ[c * O + o], wb);
-            double w = 1.0;
-            int cc = c;
-            for (int i = R - 1; i >= 0; --i) {
-              const int fi = d.mdep[m][i];
-              const int ui = cc % d.U[fi];
-              cc /= d.U[fi];
-              if (i != j) w *= s_q1[d.q1_off[fi] + ui * d.S[fi] + s[i]];
-            }
-            acc = fma(wb, w, acc);
-          }
+    // dA[o, k] = sum_c (GQ[c, o] + SB[c] (ln A + 1)) W[c, k]
+#pragma unroll
+    for (int i = 0; i < kMaxEl; ++i) {
+      const int e = tid + i * kNT;
+      if (e < O * K) {
+        const int o = e / K, k = e - o * K;
+        double la = 0.0;
+        if (info) {
+          const double a = (double)As[e];
+          la = a >= (double)kMinVal ? lstab(a) + 1.0 : lstab(a);
         }
-        s_q1b[d.q1_off[f] + e] += acc;
+        double acc = 0.0;
+        for (int c = 0; c < C; ++c) acc = fma(GQ[c * O + o] + SB[c] * la, W(c, k, -1), acc);
+        p.Abar[m][(long long)b * d.A_stride[m] + e] = (float)acc;
       }
-      __syncthreads();
+    }
+    // dq1_{d_j}[u][s] = sum_{c: u_j(c) = u} sum_{k: s_j(k) = s} wb[c, k] W_{-j}[c, k],
+    // wb[c, k] = sum_o A[o, k] GQ[c, o] + SB[c] HA[k]; kGrp lanes per bin
+    int nb = 0;
+    for (int j = 0; j < R; ++j) nb += d.U[d.mdep[m][j]] * nj[j];
+    for (int base = 0; base < nb * kGrp; base += kNT) {
+      const int slot = base + tid;
+      const int bin = slot / kGrp, part = slot % kGrp;
+      double acc = 0.0;
+      int j = -1, u = 0, sj = 0;
+      if (bin < nb) {
+        int t = bin;
+        j = 0;
+        while (t >= d.U[d.mdep[m][j]] * nj[j]) {
+          t -= d.U[d.mdep[m][j]] * nj[j];
+          ++j;
+        }
+        u = t / nj[j];
+        sj = t - u * nj[j];
+        int stride = 1;
+        for (int i = j + 1; i < R; ++i) stride *= nj[i];
+        const int others = K / nj[j];
+        const int nc = C / d.U[d.mdep[m][j]];
+        // combinations with u_j(c) = u: c = (ch * U_j + u) * cstride_j + cl
+        const int cstr = d.cstride[m][j];
+        for (int w = part; w < nc * others; w += kGrp) {
+          const int ci = w / others, r = w - ci * others;
+          const int c = ((ci / cstr) * d.U[d.mdep[m][j]] + u) * cstr + (ci % cstr);
+          const int k = ((r / stride) * nj[j] + sj) * stride + (r % stride);
+          double wb = info ? SB[c] * HA[k] : 0.0;
+          for (int o = 0; o < O; ++o) wb = fma((double)As[o * K + k], GQ[c * O + o], wb);
+          acc = fma(wb, W(c, k, j), acc);
+        }
+      }
+      acc = group_sum_low(acc, 3);
+      if (bin < nb && part == 0) q1b[qo_[j] + u * nj[j] + sj] += acc;
     }
     __syncthreads();
   }
@@ -489,13 +527,13 @@ __global__ __launch_bounds__(kNT) void efe_adjoint_kernel(const Desc d, const Pt
       const int u = e % U;
       const int s = (e / U) % n;
       const int s2 = e / (U * n);
-      Bb[e] = (float)(s_q1b[d.q1_off[f] + u * n + s2] * s_qs[d.S_off[f] + s]);
+      Bb[e] = (float)(q1b[d.q1_off[f] + u * n + s2] * qs[d.S_off[f] + s]);
     }
     for (int s = tid; s < n; s += kNT) {
       double acc = 0.0;
       for (int u = 0; u < U; ++u)
         for (int s2 = 0; s2 < n; ++s2)
-          acc = fma((double)Bf[((long long)s2 * n + s) * U + u], s_q1b[d.q1_off[f] + u * n + s2], acc);
+          acc = fma((double)Bf[((long long)s2 * n + s) * U + u], q1b[d.q1_off[f] + u * n + s2], acc);
       p.qsbar[(long long)b * d.totS + d.S_off[f] + s] = (float)acc;
     }
   }
@@ -685,7 +723,6 @@ extern "C" int pymdp_hip_neg_efe_adjoint(const int32_t* policy_dev, int64_t P, i
       d.cstride[m][j] = (int)C;
       C *= d.U[d.mdep[m][j]];
     }
-    if (d.mK[m] > kK) return einval(pfx, "adjoint supports modalities of at most 2048 joint states");
     if (C * d.O[m] > kCO) return einval(pfx, "adjoint supports (action combinations x outcomes) <= 4096 per modality");
     d.mC[m] = (int)C;
     if ((A_batch_stride[m] == 0 || C_batch_stride[m] == 0) && batch > 1)
@@ -710,11 +747,23 @@ extern "C" int pymdp_hip_neg_efe_adjoint(const int32_t* policy_dev, int64_t P, i
   p.qsbar = qs_bar;
   p.policy = policy_dev;
   if (batch == 0) return 0;
-  d.ws_stride = kK + 3 * kCO;
-  double* ws = nullptr;
-  if ((rc = workspace(sizeof(double) * (size_t)d.ws_stride * (size_t)batch, &ws, pfx))) return rc;
-  p.ws = ws;
-  hipLaunchKernelGGL(efe_adjoint_kernel, dim3((unsigned)batch), dim3(kNT), 0, static_cast<hipStream_t>(stream), d, p);
+  int Kmax = 1, Cmax = 1, COmax = 1, OKmax = 1;
+  for (int m = 0; m < M; ++m) {
+    Kmax = std::max(Kmax, d.mK[m]);
+    Cmax = std::max(Cmax, d.mC[m]);
+    COmax = std::max(COmax, d.mC[m] * d.O[m]);
+    OKmax = std::max(OKmax, d.mK[m] * d.O[m]);
+  }
+  if (OKmax > 16 * kNT) return einval(pfx, "adjoint supports O_m * prod S_deps <= 4096 per modality");
+  const size_t lds = sizeof(double) * (size_t)(2 * d.q1_off[F] + d.totS + Kmax + Cmax + 2 * COmax) +
+                     sizeof(float) * (size_t)OKmax + 3 * (size_t)(Cmax + Kmax);
+  if (lds > 64 * 1024) return einval(pfx, "adjoint model needs " + std::to_string(lds) + " bytes of LDS (limit 65536)");
+  p.ws = nullptr;
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  if (OKmax <= 4 * kNT)
+    hipLaunchKernelGGL((efe_adjoint_kernel<4>), dim3((unsigned)batch), dim3(kNT), lds, st, d, p, Kmax, Cmax, COmax, OKmax);
+  else
+    hipLaunchKernelGGL((efe_adjoint_kernel<16>), dim3((unsigned)batch), dim3(kNT), lds, st, d, p, Kmax, Cmax, COmax, OKmax);
   PMH_HIP_CHECK(hipGetLastError(), pfx);
   return 0;
 }
