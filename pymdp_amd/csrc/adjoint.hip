@@ -354,12 +354,13 @@ __global__ __launch_bounds__(kNT) void fpi_adjoint_kernel(const Desc d, const Pt
 // ---------------------------------------------------------------------------
 // LDS (doubles unless noted): q1, q1b [Q]; qs [totS]; HA [Kmax]; SB [Cmax];
 // QO, GQ [COmax]; A of the current modality as floats [OKmax]; digit tables
-// (bytes) ud [Cmax][3] = u_j(c), sd [Kmax][3] = s_j(k).
+// tables cof [Cmax][3] (uint16: offset in q1 of q1_{d_j}[u_j(c)]) and sd [Kmax][3]
+// (bytes: s_j(k)).
 constexpr int kGrp = 8;  // lanes per output bin of the dq1 reduction
 
 template <int kMaxEl>
 __global__ __launch_bounds__(kNT) void efe_adjoint_kernel(const Desc d, const Ptrs p, int Kmax, int Cmax, int COmax,
-                                                          int OKmax) {
+                                                          int OKmax, int Bmax) {
   extern __shared__ double sm[];
   const int Q = d.q1_off[d.F];
   double* q1 = sm;
@@ -370,23 +371,34 @@ __global__ __launch_bounds__(kNT) void efe_adjoint_kernel(const Desc d, const Pt
   double* QO = SB + Cmax;
   double* GQ = QO + COmax;
   float* As = reinterpret_cast<float*>(GQ + COmax);
-  unsigned char* ud = reinterpret_cast<unsigned char*>(As + OKmax);
-  unsigned char* sd = ud + 3 * Cmax;
+  float* Bs = As + ((OKmax + 1) & ~1);          // B_f of one factor [Bmax] (Bmax = 0: read from HBM)
+  double* Gb = reinterpret_cast<double*>(Bs + ((Bmax + 1) & ~1));  // dL/dG [P]
+  unsigned short* cof = reinterpret_cast<unsigned short*>(Gb + d.P);  // q1 offset of u_j(c) [Cmax][3]
+  unsigned char* sd = reinterpret_cast<unsigned char*>(cof + 3 * Cmax);
+  unsigned char* pol = sd + 3 * Kmax;           // first actions [P][F]
   const int b = blockIdx.x, tid = threadIdx.x;
   const bool util = d.flags & PYMDP_HIP_FLAG_USE_UTILITY;
   const bool info = d.flags & PYMDP_HIP_FLAG_USE_STATES_INFO_GAIN;
 
   for (int i = tid; i < d.totS; i += kNT) qs[i] = (double)p.qs[(long long)b * d.totS + i];
   for (int i = tid; i < Q; i += kNT) q1b[i] = 0.0;
-  __syncthreads();
+  for (int i = tid; i < d.P; i += kNT) Gb[i] = (double)p.Gbar[(long long)b * d.P + i];
+  for (int i = tid; i < d.P * d.F; i += kNT) pol[i] = (unsigned char)p.policy[i];
   // predicted states per action: q1_f[u] = B_f[:, :, u] qs_f
   for (int f = 0; f < d.F; ++f) {
-    const float* Bf = p.B[f] + (long long)b * d.B_stride[f];
+    const float* Bg = p.B[f] + (long long)b * d.B_stride[f];
     const int n = d.S[f], U = d.U[f];
+    const float* Bf = Bg;
+    if (Bmax) {
+      __syncthreads();
+      for (int e = tid; e < n * n * U; e += kNT) Bs[e] = Bg[e];
+      Bf = Bs;
+    }
+    __syncthreads();
     for (int e = tid; e < U * n; e += kNT) {
       const int u = e / n, s2 = e - u * n;
       double acc = 0.0;
-      for (int s = 0; s < n; ++s) acc = fma((double)Bf[((long long)s2 * n + s) * U + u], qs[d.S_off[f] + s], acc);
+      for (int s = 0; s < n; ++s) acc = fma((double)Bf[(s2 * n + s) * U + u], qs[d.S_off[f] + s], acc);
       q1[d.q1_off[f] + e] = acc;
     }
   }
@@ -406,14 +418,14 @@ __global__ __launch_bounds__(kNT) void efe_adjoint_kernel(const Desc d, const Pt
       int cc = c;
       for (int j = R - 1; j >= 0; --j) {
         const int U = d.U[d.mdep[m][j]];
-        ud[3 * c + j] = (unsigned char)(cc % U);
+        cof[3 * c + j] = (unsigned short)(qo_[j] + (cc % U) * nj[j]);
         cc /= U;
       }
       double acc = 0.0;
       for (int q = 0; q < d.P; ++q) {
         int x = 0;
-        for (int j = 0; j < R; ++j) x += p.policy[q * d.F + d.mdep[m][j]] * d.cstride[m][j];
-        if (x == c) acc += (double)p.Gbar[(long long)b * d.P + q];
+        for (int j = 0; j < R; ++j) x += (int)pol[q * d.F + d.mdep[m][j]] * d.cstride[m][j];
+        if (x == c) acc += Gb[q];
       }
       SB[c] = acc;
     }
@@ -429,7 +441,7 @@ __global__ __launch_bounds__(kNT) void efe_adjoint_kernel(const Desc d, const Pt
     auto W = [&](int c, int k, int skip) {
       double w = 1.0;
       for (int j = 0; j < R; ++j)
-        if (j != skip) w *= q1[qo_[j] + (int)ud[3 * c + j] * nj[j] + (int)sd[3 * k + j]];
+        if (j != skip) w *= q1[(int)cof[3 * c + j] + (int)sd[3 * k + j]];
       return w;
     };
     for (int k = tid; k < K; k += kNT) {
@@ -440,12 +452,29 @@ __global__ __launch_bounds__(kNT) void efe_adjoint_kernel(const Desc d, const Pt
       }
       HA[k] = acc;
     }
-    // qo[c, o] = sum_k A[o, k] W[c, k]
+    // qo[c, o] = sum_k A[o, k] W[c, k]  (neighbouring lanes: same o, next c)
     for (int e = tid; e < C * O; e += kNT) {
-      const int c = e / O, o = e - c * O;
-      double acc = 0.0;
-      for (int k = 0; k < K; ++k) acc = fma((double)As[o * K + k], W(c, k, -1), acc);
-      QO[e] = acc;
+      const int o = e / C, c = e - o * C;
+      int cb[kR];
+      for (int j = 0; j < R; ++j) cb[j] = cof[3 * c + j];
+      const float* Ao = As + o * K;
+      double a0 = 0.0, a1 = 0.0;
+      int k = 0;
+      for (; k + 1 < K; k += 2) {
+        double w0 = 1.0, w1 = 1.0;
+        for (int j = 0; j < R; ++j) {
+          w0 *= q1[cb[j] + (int)sd[3 * k + j]];
+          w1 *= q1[cb[j] + (int)sd[3 * k + 3 + j]];
+        }
+        a0 = fma((double)Ao[k], w0, a0);
+        a1 = fma((double)Ao[k + 1], w1, a1);
+      }
+      if (k < K) {
+        double w0 = 1.0;
+        for (int j = 0; j < R; ++j) w0 *= q1[cb[j] + (int)sd[3 * k + j]];
+        a0 = fma((double)Ao[k], w0, a0);
+      }
+      QO[c * O + o] = a0 + a1;
     }
     __syncthreads();
     for (int e = tid; e < C * O; e += kNT) {
@@ -475,8 +504,25 @@ __global__ __launch_bounds__(kNT) void efe_adjoint_kernel(const Desc d, const Pt
           const double a = (double)As[e];
           la = a >= (double)kMinVal ? lstab(a) + 1.0 : lstab(a);
         }
-        double acc = 0.0;
-        for (int c = 0; c < C; ++c) acc = fma(GQ[c * O + o] + SB[c] * la, W(c, k, -1), acc);
+        int sk[kR];
+        for (int j = 0; j < R; ++j) sk[j] = sd[3 * k + j];
+        double acc0 = 0.0, acc1 = 0.0;
+        int c = 0;
+        for (; c + 1 < C; c += 2) {
+          double w0 = 1.0, w1 = 1.0;
+          for (int j = 0; j < R; ++j) {
+            w0 *= q1[(int)cof[3 * c + j] + sk[j]];
+            w1 *= q1[(int)cof[3 * c + 3 + j] + sk[j]];
+          }
+          acc0 = fma(GQ[c * O + o] + SB[c] * la, w0, acc0);
+          acc1 = fma(GQ[(c + 1) * O + o] + SB[c + 1] * la, w1, acc1);
+        }
+        if (c < C) {
+          double w0 = 1.0;
+          for (int j = 0; j < R; ++j) w0 *= q1[(int)cof[3 * c + j] + sk[j]];
+          acc0 = fma(GQ[c * O + o] + SB[c] * la, w0, acc0);
+        }
+        const double acc = acc0 + acc1;
         p.Abar[m][(long long)b * d.A_stride[m] + e] = (float)acc;
       }
     }
@@ -520,21 +566,34 @@ __global__ __launch_bounds__(kNT) void efe_adjoint_kernel(const Desc d, const Pt
   }
   // dB_f[s2, s, u] = q1b_f[u][s2] qs_f[s];  dqs_f[s] = sum_{u, s2} B_f[s2, s, u] q1b_f[u][s2]
   for (int f = 0; f < d.F; ++f) {
-    const float* Bf = p.B[f] + (long long)b * d.B_stride[f];
+    const float* Bg = p.B[f] + (long long)b * d.B_stride[f];
     float* Bb = p.Bbar[f] + (long long)b * d.B_stride[f];
     const int n = d.S[f], U = d.U[f];
+    const float* Bf = Bg;
+    __syncthreads();
+    if (Bmax) {
+      for (int e = tid; e < n * n * U; e += kNT) Bs[e] = Bg[e];
+      Bf = Bs;
+      __syncthreads();
+    }
     for (int e = tid; e < n * n * U; e += kNT) {
       const int u = e % U;
       const int s = (e / U) % n;
       const int s2 = e / (U * n);
       Bb[e] = (float)(q1b[d.q1_off[f] + u * n + s2] * qs[d.S_off[f] + s]);
     }
-    for (int s = tid; s < n; s += kNT) {
+    // dqs_f[s]: kGrp lanes per s over (u, s2)
+    for (int b0 = 0; b0 < n * kGrp; b0 += kNT) {
+      const int slot = b0 + tid;
+      const int s = slot / kGrp, part = slot % kGrp;
       double acc = 0.0;
-      for (int u = 0; u < U; ++u)
-        for (int s2 = 0; s2 < n; ++s2)
-          acc = fma((double)Bf[((long long)s2 * n + s) * U + u], q1b[d.q1_off[f] + u * n + s2], acc);
-      p.qsbar[(long long)b * d.totS + d.S_off[f] + s] = (float)acc;
+      if (s < n)
+        for (int w = part; w < U * n; w += kGrp) {
+          const int u = w / n, s2 = w - u * n;
+          acc = fma((double)Bf[(s2 * n + s) * U + u], q1b[d.q1_off[f] + u * n + s2], acc);
+        }
+      acc = group_sum_low(acc, 3);
+      if (s < n && part == 0) p.qsbar[(long long)b * d.totS + d.S_off[f] + s] = (float)acc;
     }
   }
 }
@@ -755,15 +814,25 @@ extern "C" int pymdp_hip_neg_efe_adjoint(const int32_t* policy_dev, int64_t P, i
     OKmax = std::max(OKmax, d.mK[m] * d.O[m]);
   }
   if (OKmax > 16 * kNT) return einval(pfx, "adjoint supports O_m * prod S_deps <= 4096 per modality");
-  const size_t lds = sizeof(double) * (size_t)(2 * d.q1_off[F] + d.totS + Kmax + Cmax + 2 * COmax) +
-                     sizeof(float) * (size_t)OKmax + 3 * (size_t)(Cmax + Kmax);
+  if (P > 4096) return einval(pfx, "adjoint supports at most 4096 policies");
+  for (int f = 0; f < F; ++f)
+    if (d.U[f] > 255) return einval(pfx, "adjoint supports at most 255 controls per factor");
+  int Bmax = 0;
+  for (int f = 0; f < F; ++f) Bmax = std::max(Bmax, d.S[f] * d.S[f] * d.U[f]);
+  auto lds_of = [&](int bm) {
+    return sizeof(double) * (size_t)(2 * d.q1_off[F] + d.totS + Kmax + Cmax + 2 * COmax + P) +
+           sizeof(float) * (size_t)(((OKmax + 1) & ~1) + ((bm + 1) & ~1)) + 6 * (size_t)Cmax + 3 * (size_t)Kmax +
+           (size_t)P * F;
+  };
+  if (lds_of(Bmax) > 64 * 1024) Bmax = 0;  // B read from HBM
+  const size_t lds = lds_of(Bmax);
   if (lds > 64 * 1024) return einval(pfx, "adjoint model needs " + std::to_string(lds) + " bytes of LDS (limit 65536)");
   p.ws = nullptr;
   const hipStream_t st = static_cast<hipStream_t>(stream);
   if (OKmax <= 4 * kNT)
-    hipLaunchKernelGGL((efe_adjoint_kernel<4>), dim3((unsigned)batch), dim3(kNT), lds, st, d, p, Kmax, Cmax, COmax, OKmax);
+    hipLaunchKernelGGL((efe_adjoint_kernel<4>), dim3((unsigned)batch), dim3(kNT), lds, st, d, p, Kmax, Cmax, COmax, OKmax, Bmax);
   else
-    hipLaunchKernelGGL((efe_adjoint_kernel<16>), dim3((unsigned)batch), dim3(kNT), lds, st, d, p, Kmax, Cmax, COmax, OKmax);
+    hipLaunchKernelGGL((efe_adjoint_kernel<16>), dim3((unsigned)batch), dim3(kNT), lds, st, d, p, Kmax, Cmax, COmax, OKmax, Bmax);
   PMH_HIP_CHECK(hipGetLastError(), pfx);
   return 0;
 }
