@@ -54,21 +54,31 @@ class _FPI(torch.autograd.Function):
         deps, num_iter, mode, tol, M = ctx.meta
         saved = ctx.saved_tensors
         obs_idx, A, prior = saved[0], list(saved[1:1 + M]), list(saved[1 + M:])
-        batch = int(prior[0].shape[0])
-        S = np.array([int(p.shape[-1]) for p in prior], np.int64)
-        O = np.array([int(a.shape[1]) for a in A], np.int64)
-        A_bar = [torch.zeros_like(a) for a in A]
-        prior_bar = torch.empty((batch, int(S.sum())), dtype=torch.float32, device=prior[0].device)
-        df, do = flatten_dep_lists(deps)
-        k = {n: _native.I64(v) for n, v in dict(As=[a[0].numel() for a in A], Ps=[p.stride(0) for p in prior], S=S,
-                                                 O=O, df=df, do=do).items()}
-        _native.check(_native.lib().pymdp_hip_fpi_adjoint(
-            _ptrs(A), k["As"].ptr, obs_idx.data_ptr(), _ptrs(prior), k["Ps"].ptr, qbar.contiguous().data_ptr(),
-            _ptrs(A_bar), prior_bar.data_ptr(), None, None, batch, k["S"].ptr, len(S), k["O"].ptr, M,
-            k["df"].ptr, k["do"].ptr, int(num_iter), 1 if mode == "auto" else 0, float(tol),
-            _native.stream_handle(prior[0].device)))
-        pb = torch.split(prior_bar, [int(s) for s in S], dim=1)
+        A_bar, pb, _ = fpi_adjoint(A, obs_idx, prior, deps, num_iter, qbar, mode=mode, implicit_tol=tol)
         return (None, None, *A_bar, *pb)
+
+
+def fpi_adjoint(A, obs_idx, prior, A_dependencies, num_iter, q_bar, *, mode="auto", implicit_tol=1e-6):
+    """dL/dA (observed rows), dL/dprior and the per-agent choice (1 =
+    implicit, 0 = unrolled) for dL/dq = q_bar [batch, sum S]."""
+    M = len(A)
+    batch = int(prior[0].shape[0])
+    S = np.array([int(p.shape[-1]) for p in prior], np.int64)
+    O = np.array([int(a.shape[1]) for a in A], np.int64)
+    A = [a.contiguous() for a in A]
+    prior = [p.contiguous() for p in prior]
+    A_bar = [torch.zeros_like(a) for a in A]
+    prior_bar = torch.empty((batch, int(S.sum())), dtype=torch.float32, device=prior[0].device)
+    used = torch.empty(batch, dtype=torch.int32, device=prior[0].device)
+    df, do = flatten_dep_lists(A_dependencies)
+    k = {n: _native.I64(v) for n, v in dict(As=[a[0].numel() for a in A], Ps=[p.stride(0) for p in prior], S=S,
+                                             O=O, df=df, do=do).items()}
+    _native.check(_native.lib().pymdp_hip_fpi_adjoint(
+        _ptrs(A), k["As"].ptr, obs_idx.to(torch.int32).contiguous().data_ptr(), _ptrs(prior), k["Ps"].ptr,
+        q_bar.contiguous().data_ptr(), _ptrs(A_bar), prior_bar.data_ptr(), None, used.data_ptr(), batch,
+        k["S"].ptr, len(S), k["O"].ptr, M, k["df"].ptr, k["do"].ptr, int(num_iter), 1 if mode == "auto" else 0,
+        float(implicit_tol), _native.stream_handle(prior[0].device)))
+    return A_bar, list(torch.split(prior_bar, [int(s) for s in S], dim=1)), used
 
 
 class _EFE(torch.autograd.Function):

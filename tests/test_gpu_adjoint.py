@@ -96,3 +96,27 @@ def test_adjoint_full_size_matches_autograd_baseline(dev):
     for grp in g:
         for x in grp:
             assert torch.isfinite(x).all()
+
+
+def test_implicit_adjoint_on_converged_agents(dev):
+    """With many sweeps the forward reaches its fixed point and the implicit
+    adjoint is used; it agrees with the unrolled reverse sweep."""
+    from pymdp_amd import utils
+    g = torch.Generator(device=dev).manual_seed(7)
+    S, O, deps = [5, 4], [6, 3], [[0, 1], [1]]
+    batch = 32
+    A = utils.random_A_array(O, S, deps, batch=batch, alpha=2.0, generator=g, device=dev)
+    D = utils.random_factorized_categorical(S, batch=batch, alpha=1.0, generator=g, device=dev)
+    obs = torch.stack([torch.randint(0, o, (batch,), generator=g, device=dev) for o in O], -1).to(torch.int32)
+    qbar = torch.randn((batch, sum(S)), generator=g, device=dev)
+    Ai, pi, used_i = DF.fpi_adjoint(A, obs, D, deps, 200, qbar, mode="auto", implicit_tol=1e-9)
+    Au, pu, used_u = DF.fpi_adjoint(A, obs, D, deps, 200, qbar, mode="unrolled")
+    torch.cuda.synchronize()
+    n_imp = int(used_i.sum())
+    print(f"implicit adjoint used for {n_imp}/{batch} agents")
+    assert n_imp >= batch // 2 and int(used_u.sum()) == 0
+    sel = used_i.bool()
+    for a, b in zip(Ai, Au):
+        assert torch.allclose(a[sel], b[sel], atol=1e-5, rtol=1e-4)
+    for a, b in zip(pi, pu):
+        assert torch.allclose(a[sel], b[sel], atol=1e-5, rtol=1e-4)
