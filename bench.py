@@ -3,6 +3,10 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
 Workloads (seeded synthetic data, generated on the device):
+  headline (default)         the ragged batched step below (value, roofline,
+                             plain per-agent comparison, CPU baseline), with
+                             the plain_fpi_large, plain_efe and grad results
+                             attached as sub-objects of the same JSON line.
   plain_fpi_large (default)  1024 agents, S=[16,16,8,8], 6 modalities x 32
                              outcomes, A Dirichlet(0.5), 16 sweeps: one step =
                              one fused infer_states launch for all agents.
@@ -30,7 +34,8 @@ CPU baseline timed on a bounded sample on this host.
     python bench.py --write-traffic KEY --kernel SUBSTR \
         --pmc-dirs FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR
                                        # HBM bytes per launch of the kernel
-                                       # whose name contains SUBSTR, from four
+                                       # whose name contains SUBSTR (comma-
+                                       # separated: summed), from four
                                        # rocprofv3 --pmc csv runs, scaled by
                                        # the copy calibration; stored under KEY
                                        # in profiles/traffic.json
@@ -56,7 +61,8 @@ def _args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="plain_fpi_large", choices=["plain_fpi_large", "plain_efe", "ragged", "grad"])
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "plain_fpi_large", "plain_efe", "ragged", "grad"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -101,7 +107,9 @@ def write_traffic(a):
             raise SystemExit(f"no {counter} rows under {d}")
         return max(vals)
 
-    fetch_kib, write_kib = last(fdir, "FETCH_SIZE", a.kernel), last(wdir, "WRITE_SIZE", a.kernel)
+    subs = [k for k in a.kernel.split(",") if k]
+    fetch_kib = sum(last(fdir, "FETCH_SIZE", k) for k in subs)
+    write_kib = sum(last(wdir, "WRITE_SIZE", k) for k in subs)
     cf_kib, cw_kib = biggest(cfdir, "FETCH_SIZE"), biggest(cwdir, "WRITE_SIZE")
     sf, sw = COPY_BYTES / (cf_kib * 1024.0), COPY_BYTES / (cw_kib * 1024.0)
     traffic = fetch_kib * 1024.0 * sf + write_kib * 1024.0 * sw
@@ -489,6 +497,31 @@ def cpu_baseline_ragged(ags, seconds):
                       f"history-dedup neg-EFE + q_pi + action), single thread"}
 
 
+HEADLINE_METRIC = "agent-steps/s & % HBM peak: batched vs plain FPI+EFE at <=1e-5 err; grads/s"
+
+
+def bench_headline(a, torch):
+    import copy
+    res = bench_ragged(a, torch)
+    res["metric"] = HEADLINE_METRIC
+    keep = ("value", "unit", "ms_per_step", "steps", "config", "roofline", "parity", "cpu_baseline",
+            "autograd_baseline", "speedup_vs_autograd", "grads_per_s")
+    b = copy.copy(a)
+    b.steps, b.warmup = 20, 3
+    fpi = bench_plain_fpi_large(b, torch)
+    res["plain_fpi_large"] = {k: fpi[k] for k in keep if k in fpi}
+    b = copy.copy(a)
+    b.steps, b.warmup, b.no_cpu_baseline = 5, 1, True
+    efe = bench_plain_efe(b, torch)
+    res["plain_efe"] = {k: efe[k] for k in keep if k in efe}
+    b = copy.copy(a)
+    b.steps, b.warmup = 5, 1
+    g = bench_grad(b, torch)
+    res["grad"] = {k: g[k] for k in keep if k in g}
+    res["grads_per_s"] = g["grads_per_s"]
+    return res
+
+
 def main():
     a = _args()
     if a.write_traffic:
@@ -504,7 +537,9 @@ def main():
     if a.calibrate_copy:
         calibrate_copy()
         return
-    if a.workload == "plain_fpi_large":
+    if a.workload == "headline":
+        res = bench_headline(a, torch)
+    elif a.workload == "plain_fpi_large":
         res = bench_plain_fpi_large(a, torch)
     elif a.workload == "ragged":
         res = bench_ragged(a, torch)
