@@ -394,18 +394,20 @@ def bench_ragged(a, torch):
         "dtype": "f32 data, f64 accumulation",
         "data": "synthetic (seeded ragged shapes, Dirichlet A/B, N(0,1) C; generated on device)",
         "config": {"workload": "ragged", "agents": n, "policies_total": ragged.num_policies(ags),
-                   "num_iter": w["num_iter"], "gamma": ags[0]["gamma"], "efe_work_items": info["items"],
-                   "grids": [info["grid_fpi"], info["grid_efe"]]},
+                   "num_iter": w["num_iter"], "gamma": ags[0]["gamma"], "fpi_units": info["units"],
+                   "fpi_team_max": info["team_max"], "fpi_grid": info["grid_fpi"],
+                   "efe_tiles": [info["tiles1"], info["tiles2"], info["tiles3"]],
+                   "efe_macs_per_step": info["efe_macs"], "fpi_macs_per_step": info["fpi_macs"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("ragged step"),
-                     "kernel": "rg_fpi_kernel + rg_efe_kernel + rg_assemble_kernel (one graph)",
+                     "kernel": "rg_fpi + rg_e1 + rg_e2 + rg_e3 + rg_assemble kernels (one graph)",
                      "kernel_ms": ms_events, "alg_bytes_per_launch": alg},
         "plain_per_agent": {"value": plain_rate, "unit": "agent-steps/s",
                             "sample": f"every 16th agent ({len(sample)}), batch-1 infer_states_ffi + "
                                       f"update_posterior_policies_ffi + argmax, {plain_s:.2f} s",
                             "same_actions_as_batched": bool(same)},
         "speedup_vs_plain": (n / (ms_step * 1e-3)) / plain_rate,
-        "parity": {"agents_checked": 4, "max_abs_err_vs_fp64": e64},
+        "parity": {"agents_checked": 4, "max_abs_err_vs_fp64": e64, "team_barrier_timeouts": rb.status()},
         "cpu_baseline": cpu,
     }
 

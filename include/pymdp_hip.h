@@ -268,7 +268,8 @@ int pymdp_hip_ragged_plan_upload(void* plan);
  *                            [O_m][storage axes] row-major
  *   A_perm [sum M * 3]       storage axis j = position A_perm[j] of the
  *                            modality's given dependency list (-1 = unused)
- *   B_off [sum F]            B_f stored [S_f][S_deps...][U_f] (as given)
+ *   B_off [sum F]            B_f stored [U_f][S_f][S_deps...] (control axis
+ *                            first; given as [S_f][S_deps...][U_f])
  *   C_off [sum M]            C_m [O_m] or [T, O_m]
  *   D_off [sum F]            prior D_f [S_f]
  *   E_off [n]                E [P] (or -1)
@@ -277,26 +278,35 @@ int pymdp_hip_ragged_plan_upload(void* plan);
  *   act_off [n]              F int32 actions in the action arena
  * With out == NULL returns the number of values. */
 int pymdp_hip_ragged_layout(void* plan, int64_t* out, int64_t n_out);
-/* out8 = (agents, EFE work items, FPI grid, EFE grid, history doubles,
- *         score doubles, FPI workspace per workgroup, EFE workspace per
- *         workgroup). */
-int pymdp_hip_ragged_info(void* plan, int64_t* out8);
+/* out16 = (agents, FPI units, stage-1 tiles, stage-2 tiles, stage-3 tiles,
+ *          FPI grid, history doubles, score doubles, stage-1 doubles,
+ *          stage-2 doubles, team-exchange doubles, largest FPI team,
+ *          EFE multiply-adds per step, FPI multiply-adds per step, FPI grid
+ *          capacity (workgroups resident at once), 0). */
+int pymdp_hip_ragged_info(void* plan, int64_t* out16);
 /* Host copies of the plan's device tables (for host-side verification):
- * sizes8 = (sizeof agent record, sizeof item, items, int arena, max grid). */
-int pymdp_hip_ragged_tables(void* plan, int64_t* sizes8, void* agents_out,
-                            int32_t* order_out, int32_t* items_out, int32_t* iarena_out);
-/* One step on `stream`: three launches, no allocation, no synchronisation.
- * Steps that overlap on one device race on the plan's scratch. */
+ * sizes16 = (sizeof agent record, sizeof unit, units, sizeof tile, stage-1
+ * tiles, stage-2 tiles, stage-3 tiles, int arena, 0...); tiles_out receives
+ * the three tile lists concatenated. */
+int pymdp_hip_ragged_tables(void* plan, int64_t* sizes16, void* agents_out,
+                            void* units_out, void* tiles_out, int32_t* iarena_out);
+/* One step on `stream`: five launches (FPI units, three EFE stages, per-agent
+ * assembly), no allocation, no synchronisation.  Steps that overlap on one
+ * device race on the plan's scratch. */
 int pymdp_hip_ragged_step(void* plan, const float* params, const int32_t* obs,
                           float* out, int32_t* actions, void* stream);
-/* Times the launch grids (2, 4, 8 workgroups per CU for each persistent
- * kernel) and keeps the fastest; not while `stream` is being captured. */
+/* Times the persistent FPI grid (1, 2, ... workgroups per CU up to the
+ * occupancy limit) and keeps the fastest; not while `stream` is being
+ * captured. */
 int pymdp_hip_ragged_autotune(void* plan, const float* params, const int32_t* obs,
                               float* out, int32_t* actions, void* stream, float* best_ms);
-/* Optional timing trace: a device buffer of 2 * (agents + items) int64
- * values that later steps fill with wall-clock ticks (100 MHz): [2a, 2a+1]
- * = start / end of agent a's FPI + rollout, [2(agents + i), ...] = start /
- * end of EFE work item i.  NULL switches the trace off. */
+/* *flag_out = 1 if a team barrier of an earlier step gave up waiting (the
+ * posteriors of that agent are NaN), else 0.  Synchronises the device. */
+int pymdp_hip_ragged_status(void* plan, int32_t* flag_out);
+/* Optional timing trace: a device buffer of 2 * (units + stage-1 tiles +
+ * stage-2 tiles + stage-3 tiles) int64 values that later steps fill with
+ * wall-clock ticks (100 MHz): [2u, 2u+1] = start / end of FPI unit u, then
+ * the tiles of the three EFE stages in order.  NULL switches the trace off. */
 int pymdp_hip_ragged_set_trace(void* plan, int64_t* trace);
 int pymdp_hip_ragged_plan_destroy(void* plan);
 

@@ -114,6 +114,17 @@ __device__ __forceinline__ double xlogx_d(double q) {
   return q == 0.0 ? 0.0 : q * log_stable_f64((float)q);
 }
 
+// Workgroup barrier that orders LDS accesses only (s_waitcnt lgkmcnt(0);
+// s_barrier): global loads issued before it stay in flight, unlike
+// __syncthreads(), whose workgroup-scope fences also wait for them.  Use it
+// only where no thread reads global memory another thread of the
+// workgroup wrote since the last full barrier.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

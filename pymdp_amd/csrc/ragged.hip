@@ -1,22 +1,27 @@
 // Batched ragged agents: one step of infer_states -> infer_policies ->
 // sample_action for thousands of agents whose factor, modality, control and
-// policy counts all differ, as three stream-ordered launches that a caller can
+// policy counts all differ, as five stream-ordered launches that a caller can
 // capture in one hipGraph.
 //
-//   rg_fpi_kernel       persistent; a workgroup takes the next agent from an
-//                       atomic queue ordered by decreasing cost, runs the
-//                       fused log-likelihood + Jacobi FPI of
-//                       pymdp/algos.py:88-155 and then the action-history
-//                       rollout of pymdp/control.py:351-389 (B . q per
-//                       distinct history), writing the posterior (FP32 out,
-//                       FP64 internal) and every history vector.
-//   rg_efe_kernel       persistent; work items are (agent, modality, level,
-//                       chunk of first-axis histories), largest first.  An item
-//                       streams its modality's A once through LDS tiles, forms
-//                       sum_o A ln A on the fly, contracts [A ; that row] with
-//                       every history of each dependency in turn and scores
-//                       every history combination: entropy of qo + ambiguity
-//                       + qo . C (pymdp/control.py:420-509).
+//   rg_fpi_kernel       persistent; a workgroup takes the next FPI unit from
+//                       an atomic queue.  A unit holds (a row range of) an
+//                       agent's log-likelihood tensors in LDS for all sweeps
+//                       of the fused log-likelihood + Jacobi FPI of
+//                       pymdp/algos.py:88-155.  An agent too large for one
+//                       unit is split over a team of units whose workgroups
+//                       exchange partial marginals once per sweep; slice 0
+//                       then runs the action-history rollout of
+//                       pymdp/control.py:351-389 (B . q per distinct history).
+//   rg_e1_kernel        one tile per workgroup: rows (o, i, j) of
+//                       [A ; sum_o A ln A] of one modality contracted over
+//                       its last storage axis with the history vectors of
+//                       every level at once (A is read once per step).
+//   rg_e2_kernel        the middle axis contracted with the middle factor's
+//                       histories of one level.
+//   rg_e3_kernel        the first axis contracted with the first factor's
+//                       histories and every history combination scored:
+//                       entropy of qo + ambiguity + qo . C
+//                       (pymdp/control.py:420-509).
 //   rg_assemble_kernel  one workgroup per agent: G[p] = sum_t sum_m of the
 //                       combination scores its histories select, q_pi =
 //                       softmax(gamma G + ln E) (pymdp/control.py:963), the
@@ -52,36 +57,56 @@ constexpr int kT = 4;     // policy horizon
 constexpr int kDim = 64;  // state / outcome count
 constexpr int kNT = 256;
 constexpr int kNW = kNT / kWave;
-constexpr int kHC = 16;                 // max first-axis histories per EFE item
-constexpr int kFpiScr = 2048;           // FP64 LL + stage scratch kept in LDS
-constexpr int kTileFloats = 4096;       // A tile of an EFE item (floats)
+constexpr int kCap = 6144;        // log-likelihood doubles one FPI unit keeps in LDS
+constexpr int kRowsCap = 1024;    // rows of one FPI segment
+constexpr int kTeamMax = 128;     // FPI units of one agent
+constexpr int kTile = 4608;       // doubles of an EFE operand tile in LDS
+constexpr int kQTab = 64 * 65;    // doubles of an EFE history table in LDS
+constexpr int kCols = 64;         // histories per EFE tile
+constexpr int kE3Combos = 128;    // history combinations scored per E3 tile
+constexpr long long kSpinMax = 1LL << 22;
 
 struct Agent {
-  int F, M, T, P, G, iters, totS, has_E;
+  int F, M, T, P, G, iters, totS, has_E, team;
   float gamma;
-  int scr_lds;  // LL and stage buffers in LDS (1) or the workgroup's workspace slot (0)
-  int ll_total, xa_off, xb_off, scr_total;
   int S[kF], S_off[kF + 1];
   int O[kM], mod_group[kM];
-  int gR[kM], gfac[kM][kR], gdim[kM][kR], gll[kM];
+  int gR[kM], gfac[kM][kR], gdim[kM][kR];
   int BR[kF], Bdep[kF][kRB], BK[kF], U[kF];
   int C_time[kM];
   int H[kT][kF], hist_lvl[kT][kF], roll_lvl[kT][kF];
   int score_base[kT][kM], score_stride[kT][kM][kR];
+  // EFE view of modality m: padded dims (ed0, ed1, ed2), ed2 contracted first;
+  // ef = factor of each padded dim (-1: padding, one state, one history, q = 1)
+  int ed[kM][3], ef[kM][3];
+  int tstart[kM][kT + 1];  // first concatenated history column of each level
   long long K[kM];
   long long A_off[kM], B_off[kF], C_off[kM], D_off[kF], E_off;
   long long obs_off, qs_off, G_off, qpi_off, act_off;
-  long long hist_off, roll_off, p2h_off, pol0_off, score_off, logit_off;
+  long long hist_off, roll_off, p2h_off, pol0_off, score_off, logit_off, part_off;
+  long long z1_off[kM], z2_off[kM][kT];
 };
 
-struct Item {
-  int agent, m, t, c0, c1;
+// One FPI unit: up to kM segments, each a row range [r0, r1) of the stage-0
+// matrix [rows x n] (n = last storage dim) of dependency group g, held in
+// LDS at offset lds with row stride n | 1.
+struct Unit {
+  int agent, slice, nseg, pad;
+  int g[kM], r0[kM], r1[kM], lds[kM];
+};
+
+// One EFE tile: rows [r0, r1) and columns [c0, c1) of one stage of modality
+// m of `agent` at level t (E1 tiles span every level).
+struct Tile {
+  int agent, m, t, r0, r1, c0, c1, pad;
 };
 
 struct Bufs {
   const Agent* ag;
-  const int* fpi_order;
-  const Item* items;
+  const Unit* units;
+  const Tile* t1;
+  const Tile* t2;
+  const Tile* t3;
   const int* iarena;
   const float* params;
   const int* obs;
@@ -89,18 +114,39 @@ struct Bufs {
   int* act;
   double* hist;
   double* score;
-  double* ws1;
-  double* ws2;
-  long long ws1_slot, ws2_slot;
-  int* cnt;
-  long long* trace;  // optional: [agent or item][start, end] wall clock ticks
-  int n_agents, n_items;
+  double* z1;
+  double* z2;
+  double* part;
+  int* team_cnt;
+  int* cnt;          // [0] FPI queue, [3] team-barrier timeout flag
+  long long* trace;  // optional: [unit or tile][start, end] wall clock ticks
+  int n_agents, n_units, n_t1, n_t2, n_t3;
 };
 
 __device__ __forceinline__ float xlogx_f(float x) { return x == 0.0f ? 0.0f : x * logf(fmaxf(x, kMinVal)); }
 
+// dst(e) = src(e) for e in [0, n): every thread issues eight global loads
+// before it stores any of them (one memory round trip per eight elements
+// instead of one per element).
+template <typename T, typename Ld, typename St>
+__device__ __forceinline__ void rg_stage(long long n, int tid, Ld ld, St st) {
+  for (long long e0 = tid; e0 < n; e0 += 8 * kNT) {
+    T v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const long long e = e0 + (long long)q * kNT;
+      if (e < n) v[q] = ld(e);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const long long e = e0 + (long long)q * kNT;
+      if (e < n) st(e, v[q]);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
-// FPI + rollout, one agent per workgroup iteration
+// FPI units
 // ---------------------------------------------------------------------------
 __device__ void rg_softmax(const Agent& a, const double* src, double* dst, int lane, int wave) {
   for (int f = wave; f < a.F; f += kNW) {
@@ -113,414 +159,639 @@ __device__ void rg_softmax(const Agent& a, const double* src, double* dst, int l
   }
 }
 
-// One stage of the staged leave-one-out contraction over rows [0, rows) of
-// X [rows x n] (lane-per-column mapping): acc += w[r] X[r, col] (weights =
-// Kronecker product of the leading factors' q), dst[r] = X[r, :] . q_c.
-// U rows per lane are loaded before any is used.
-template <int U>
-__device__ __forceinline__ void rg_stage_rows(const double* src, int rows, int n, int c, int lg, int per, int rbase,
-                                              int col, const double* q0p, const double* q1p, int d1, double inv1,
-                                              double qcol, double* dst, double& acc) {
-  for (int r0 = 0; r0 < rows; r0 += U * per) {
-    double x[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int r = r0 + u * per + rbase;
-      x[u] = (r < rows && col < n) ? src[(long long)r * n + col] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int r = r0 + u * per + rbase;
-      const bool rv = r < rows;
-      double w = 0.0;
-      if (rv) {
-        if (c == 0) {
-          w = 1.0;
-        } else if (c == 1) {
-          w = q0p[r];
-        } else {
-          const int hi = (int)(r * inv1 + 1e-9);  // r / d1 (exact: r < 2^22)
-          w = q0p[hi] * q1p[r - hi * d1];
-        }
+// log2 of the threads that share one row's dot product (about 8 columns each)
+__device__ __forceinline__ int rows_lg(int n) { return n <= 8 ? 0 : ceil_log2(n) - 3; }
+
+// Adds one segment's contribution to the partial marginals s_part of every
+// factor of group g (one pass over the rows; pymdp/algos.py:26-51 for all
+// factors of the group at once):
+//   R = 1: m_0[k]  += X[0, k]
+//   R = 2: rows i: m_1[k] += sum_i q_0[i] X[i, k];  m_0[i] += X[i, :] . q_1
+//   R = 3: rows r = i*d1 + j: m_2[k] += sum_r q_0[i] q_1[j] X[r, k];
+//          y_r = X[r, :] . q_2;  m_1[j] += sum_i q_0[i] y_ij;  m_0[i] += sum_j q_1[j] y_ij
+// Two barriers; every sum has a fixed order (bitwise reproducible).
+__device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const double* X, double* s_y,
+                               double* s_red, const double* s_qv, double* s_part, int tid) {
+  const int R = a.gR[g];
+  const int n = a.gdim[g][R - 1], ld = n | 1;
+  const int fl = a.gfac[g][R - 1];
+  const double* qc = s_qv + a.S_off[fl];
+  const double* q0 = R >= 2 ? s_qv + a.S_off[a.gfac[g][0]] : nullptr;
+  const double* q1 = R == 3 ? s_qv + a.S_off[a.gfac[g][1]] : nullptr;
+  const int d1 = R == 3 ? a.gdim[g][1] : 1;
+  // row dot products (R >= 2)
+  if (R >= 2) {
+    const int lt = rows_lg(n), tpr = 1 << lt;
+    for (int e = tid; e < nr * tpr; e += kNT) {
+      const int rl = e >> lt, sub = e & (tpr - 1);
+      double v = 0.0;
+      for (int k = sub; k < n; k += tpr) v = fma(X[rl * ld + k], qc[k], v);
+      v = group_sum_low(v, lt);
+      if (sub == 0) {
+        if (R == 2)
+          s_part[a.S_off[a.gfac[g][0]] + r0 + rl] += v;
+        else
+          s_y[rl] = v;
       }
-      acc = fma(w, x[u], acc);
-      const double v = group_sum_low(x[u] * qcol, lg);
-      if (dst && rv && col == 0) dst[r] = v;
     }
   }
-}
-
-__global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
-  __shared__ double s_logq[kF * kDim], s_qv[kF * kDim], s_lp[kF * kDim];
-  __shared__ double s_red[2][kNW * kWave];
-  __shared__ double s_scr[kFpiScr];
-  __shared__ double s_ltab[256];
-  __shared__ int s_obs[kM];
-  __shared__ int s_item;
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
-  for (int i = tid; i < 256; i += kNT) s_ltab[i] = g_logtab[i];
-  double* wsl = b.ws1 + (long long)blockIdx.x * b.ws1_slot;
-
-  for (;;) {
-    if (tid == 0) s_item = atomicAdd(&b.cnt[0], 1);
-    __syncthreads();
-    const int idx = s_item;
-    if (idx >= b.n_agents) break;
-    const Agent& a = b.ag[b.fpi_order[idx]];
-    double* scr = a.scr_lds ? s_scr : wsl;
-    if (b.trace && tid == 0) b.trace[2 * b.fpi_order[idx]] = wall_clock64();
-
-    for (int f = 0; f < a.F; ++f)
-      for (int i = tid; i < a.S[f]; i += kNT) {
-        s_lp[a.S_off[f] + i] = log_stable_f64(b.params[a.D_off[f] + i]);
-        s_logq[a.S_off[f] + i] = 0.0;
+  // weighted column sums
+  const int nrg = kNT / n;
+  const int col = tid % n, rg = tid / n;
+  if (rg < nrg) {
+    double acc = 0.0;
+    for (int rl = rg; rl < nr; rl += nrg) {
+      const int r = r0 + rl;
+      double w = 1.0;
+      if (R == 2) {
+        w = q0[r];
+      } else if (R == 3) {
+        const int i = r / d1;
+        w = q0[i] * q1[r - i * d1];
       }
-    if (tid < a.M) s_obs[tid] = b.obs[a.obs_off + tid];
-    __syncthreads();
-
-    // log-likelihood per dependency group: sum over its modalities of
-    // ln max(A_m[o_m, s], MINVAL), formed as one FP64 log of the product
-    // (an out-of-range index is an all-zero one-hot row: no contribution).
-    for (int g = 0; g < a.G; ++g) {
-      long long Kg = 1;
-      for (int j = 0; j < a.gR[g]; ++j) Kg *= a.gdim[g][j];
-      double* ll = scr + a.gll[g];
-      for (long long e = tid; e < Kg; e += kNT) {
-        double prod = 1.0;
-        for (int m = 0; m < a.M; ++m) {
-          if (a.mod_group[m] != g) continue;
-          const int o = s_obs[m];
-          if (o < 0 || o >= a.O[m]) continue;
-          prod *= (double)fmaxf(b.params[a.A_off[m] + (long long)o * a.K[m] + e], kMinVal);
-        }
-        ll[e] = log_f64d(prod, s_ltab);
-      }
+      acc = fma(w, X[rl * ld + col], acc);
     }
-    __threadfence_block();
-    __syncthreads();
-
-    for (int it = 0; it < a.iters; ++it) {
-      rg_softmax(a, s_logq, s_qv, lane, wave);
-      __syncthreads();
-      for (int i = tid; i < a.totS; i += kNT) s_logq[i] = s_lp[i];
-      __syncthreads();
-      for (int g = 0; g < a.G; ++g) {
-        const int R = a.gR[g];
-        const double* src = scr + a.gll[g];
-        for (int k = 0; k < R; ++k) {
-          const int c = R - 1 - k;
-          const int n = a.gdim[g][c];
-          const int f = a.gfac[g][c];
-          const int lg = ceil_log2(n);
-          const int rpw_lg = 6 - lg;
-          int rows = 1;
-          for (int j = 0; j < c; ++j) rows *= a.gdim[g][j];
-          double* dst = c > 0 ? scr + ((k & 1) ? a.xb_off : a.xa_off) : nullptr;
-          double* red = s_red[k & 1];
-          const int col = lane & ((1 << lg) - 1);
-          const int lr = lane >> lg;
-          const double qcol = col < n ? s_qv[a.S_off[f] + col] : 0.0;
-          double acc = 0.0;
-          const int per = kNW << rpw_lg;
-          const int rbase = (wave << rpw_lg) + lr;
-          const double* q0p = c >= 1 ? s_qv + a.S_off[a.gfac[g][0]] : nullptr;
-          const double* q1p = c == 2 ? s_qv + a.S_off[a.gfac[g][1]] : nullptr;
-          const int d1 = c == 2 ? a.gdim[g][1] : 1;
-          const double inv1 = 1.0 / d1;
-          // streamed stage 0 of a large agent: 16 row loads in flight per lane
-          if (!a.scr_lds && k == 0 && rows >= 16 * per)
-            rg_stage_rows<16>(src, rows, n, c, lg, per, rbase, col, q0p, q1p, d1, inv1, qcol, dst, acc);
-          else
-            rg_stage_rows<4>(src, rows, n, c, lg, per, rbase, col, q0p, q1p, d1, inv1, qcol, dst, acc);
-          acc = group_sum_high(acc, lg);
-          if ((lane >> lg) == 0 && col < n) red[wave * kWave + col] = acc;
-          __threadfence_block();
-          __syncthreads();
-          if (tid < n) {
-            double s = 0.0;
-#pragma unroll
-            for (int w = 0; w < kNW; ++w) s += red[w * kWave + tid];
-            s_logq[a.S_off[f] + tid] += s;
-          }
-          src = dst;
-        }
-        __syncthreads();
-      }
-    }
-    rg_softmax(a, s_logq, s_qv, lane, wave);
-    __syncthreads();
-
-    // posterior out (FP32) and as the level -1 history vector (FP64)
-    double* hist = b.hist + a.hist_off;
-    for (int i = tid; i < a.totS; i += kNT) {
-      b.out[a.qs_off + i] = (float)s_qv[i];
-      hist[i] = s_qv[i];
-    }
-    __syncthreads();
-
-    // rollout: q_f[t][h] = B_f[:, k, u_h] . (x_j q_{dep_j}[t-1][parent_j(h)])[k]
-    const int* roll = b.iarena + a.roll_off;
-    for (int t = 0; t < a.T; ++t) {
-      for (int f = 0; f < a.F; ++f) {
-        const int Sf = a.S[f], rb = a.BR[f], KB = a.BK[f], Uf = a.U[f];
-        const int nh = a.H[t][f];
-        const float* Bf = b.params + a.B_off[f];
-        for (int e = tid; e < nh * Sf; e += kNT) {
-          const int h = e / Sf, s = e - h * Sf;
-          const int* rec = roll + a.roll_lvl[t][f] + h * (1 + rb);
-          const int u = rec[0];
-          const double* qp[kRB];
-          int dd[kRB];
-          for (int j = 0; j < rb; ++j) {
-            const int d = a.Bdep[f][j];
-            dd[j] = a.S[d];
-            qp[j] = hist + (t == 0 ? a.S_off[d] : a.hist_lvl[t - 1][d] + rec[1 + j] * a.S[d]);
-          }
-          double acc = 0.0;
-          for (int k = 0; k < KB; ++k) {
-            double w;
-            if (rb == 1) {
-              w = qp[0][k];
-            } else if (rb == 2) {
-              const int k0 = k / dd[1];
-              w = qp[0][k0] * qp[1][k - k0 * dd[1]];
-            } else {
-              const int k01 = k / dd[2], k2 = k - k01 * dd[2];
-              const int k0 = k01 / dd[1];
-              w = qp[0][k0] * qp[1][k01 - k0 * dd[1]] * qp[2][k2];
-            }
-            acc = fma((double)Bf[((long long)s * KB + k) * Uf + u], w, acc);
-          }
-          hist[a.hist_lvl[t][f] + h * Sf + s] = acc;
-        }
-      }
-      __syncthreads();
-    }
-    if (b.trace && tid == 0) b.trace[2 * b.fpi_order[idx] + 1] = wall_clock64();
+    s_red[tid] = acc;
   }
-}
-
-// ---------------------------------------------------------------------------
-// EFE items
-// ---------------------------------------------------------------------------
-// Every contraction stage is a batched small GEMM Y[b*H + h][row] =
-// sum_i X[b][row][i] q_h[i] over a row-major X whose last axis is the one
-// being contracted; X tiles are staged through LDS with coalesced loads and
-// each thread computes four histories of one row from LDS.
-constexpr int kTileBytes = (kTileFloats + 2 * kDim) * 4;
-constexpr int kHaPart = 512;
-
-// X (FP64, global) [rows x n]; row R belongs to slab b = R / slab, and its
-// outputs for histories [h0, h0 + hn) go to Y[(b * H + h) * slab + R % slab].
-__device__ void rg_stage_tile(const double* X, long long rows, long long slab, int n, const double* Q, int h0,
-                              int hn, int H, double* Y, double* tile, double* qh, int tid) {
-  constexpr int kTD = kTileBytes / 8;
-  constexpr int kPF = (kTD + kNT - 1) / kNT;
-  const int np1 = n + 1;
-  int rt = kTD / np1;
-  if (rt > rows) rt = (int)rows;
-  for (int i = tid; i < hn * n; i += kNT) qh[i] = Q[(long long)h0 * n + i];
-  const int hb_n = (hn + 3) / 4;
-  // the next tile's values are loaded into registers while this one is used
-  double pf[kPF];
-  auto fetch = [&](long long r0, int nr) {
-#pragma unroll
-    for (int j = 0; j < kPF; ++j) {
-      const int e = tid + j * kNT;
-      pf[j] = e < nr * n ? X[r0 * n + e] : 0.0;
-    }
-  };
-  fetch(0, (int)min((long long)rt, rows));
-  for (long long r0 = 0; r0 < rows; r0 += rt) {
-    const int nr = (int)min((long long)rt, rows - r0);
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kPF; ++j) {
-      const int e = tid + j * kNT;
-      if (e < nr * n) {
-        const int r = e / n, i = e - r * n;
-        tile[r * np1 + i] = pf[j];
+  __syncthreads();
+  if (tid < n) {
+    double s = 0.0;
+    for (int k = 0; k < nrg; ++k) s += s_red[k * n + tid];
+    s_part[a.S_off[fl] + tid] += s;
+  }
+  if (R == 3) {
+    const int i_lo = r0 / d1, i_hi = (r0 + nr - 1) / d1;
+    if (tid >= 64 && tid < 64 + d1) {
+      const int j = tid - 64;
+      double s = 0.0;
+      for (int i = i_lo; i <= i_hi; ++i) {
+        const int r = i * d1 + j;
+        if (r >= r0 && r < r0 + nr) s = fma(q0[i], s_y[r - r0], s);
       }
-    }
-    __syncthreads();
-    if (r0 + rt < rows) fetch(r0 + rt, (int)min((long long)rt, rows - r0 - rt));
-    for (int e = tid; e < hb_n * nr; e += kNT) {
-      const int hb = e / nr, r = e - hb * nr;
-      const int hh0 = hb * 4;
-      double acc[4] = {0.0, 0.0, 0.0, 0.0};
-      for (int i = 0; i < n; ++i) {
-        const double x = tile[r * np1 + i];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-          if (hh0 + u < hn) acc[u] = fma(x, qh[(hh0 + u) * n + i], acc[u]);
+      s_part[a.S_off[a.gfac[g][1]] + j] += s;
+    } else if (tid >= 128 && tid < 128 + (i_hi - i_lo + 1)) {
+      const int i = i_lo + (tid - 128);
+      double s = 0.0;
+      for (int j = 0; j < d1; ++j) {
+        const int r = i * d1 + j;
+        if (r >= r0 && r < r0 + nr) s = fma(q1[j], s_y[r - r0], s);
       }
-      const long long R = r0 + r;
-      const long long bb = R / slab;
-      double* y = Y + (bb * H + h0 + hh0) * slab + (R - bb * slab);
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (hh0 + u < hn) y[(long long)u * slab] = acc[u];
+      s_part[a.S_off[a.gfac[g][0]] + i] += s;
     }
   }
   __syncthreads();
 }
 
-__global__ __launch_bounds__(kNT) void rg_efe_kernel(Bufs b) {
-  __shared__ __align__(16) unsigned char s_tile_raw[kTileBytes];
-  __shared__ double s_hap[kHaPart];
-  __shared__ double s_ha[kHaPart];
-  __shared__ double s_qh[kHC * kDim];
-  __shared__ int s_item;
-  float* s_tile = reinterpret_cast<float*>(s_tile_raw);
-  double* s_tiled = reinterpret_cast<double*>(s_tile_raw);
+__global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
+  __shared__ double s_x[kCap];
+  __shared__ double s_y[kRowsCap];
+  __shared__ double s_logq[kF * kDim], s_qv[kF * kDim], s_lp[kF * kDim], s_part[kF * kDim];
+  __shared__ double s_red[kNT];
+  __shared__ double s_ltab[256];
+  __shared__ int s_obs[kM];
+  __shared__ int s_item, s_broken;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
-  double* Z = b.ws2 + (long long)blockIdx.x * b.ws2_slot;
+  for (int i = tid; i < 256; i += kNT) s_ltab[i] = g_logtab[i];
 
   for (;;) {
-    if (tid == 0) s_item = atomicAdd(&b.cnt[1], 1);
+    if (tid == 0) {
+      s_item = atomicAdd(&b.cnt[0], 1);
+      s_broken = 0;
+    }
     __syncthreads();
     const int idx = s_item;
-    if (idx >= b.n_items) break;
-    const Item itm = b.items[idx];
-    const Agent& a = b.ag[itm.agent];
-    const int m = itm.m, t = itm.t;
-    long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_agents + idx) : nullptr;
-    if (tr && tid == 0) tr[0] = wall_clock64();
-    const int g = a.mod_group[m];
-    const int R = a.gR[g];
-    const int O = a.O[m], O1 = O + 1;
-    const long long K = a.K[m];
-    const int n = a.gdim[g][R - 1];
-    const int pre = (int)(K / n);
-    const int hc = itm.c1 - itm.c0;
-    const double* hist = b.hist + a.hist_off;
-    const float* Am = b.params + a.A_off[m];
+    if (idx >= b.n_units) break;
+    const Unit& u = b.units[idx];
+    const Agent& a = b.ag[u.agent];
+    const int totS = a.totS;
+    if (b.trace && tid == 0) b.trace[2 * idx] = wall_clock64();
 
-    {
-      const double* q = hist + a.hist_lvl[t][a.gfac[g][R - 1]] + (long long)itm.c0 * n;
-      for (int i = tid; i < hc * n; i += kNT) s_qh[i] = q[i];
+    for (int f = 0; f < a.F; ++f)
+      for (int i = tid; i < a.S[f]; i += kNT) s_lp[a.S_off[f] + i] = log_stable_f64(b.params[a.D_off[f] + i]);
+    for (int i = tid; i < totS; i += kNT) {
+      s_logq[i] = 0.0;
+      s_part[i] = 0.0;
     }
-    // stage 1: Z1[h][o'][p] = sum_i X[o'][p][i] q_h[i], X = [A ; sum_o A ln A]
-    const int np1 = n + 1;
-    int pt = kTileFloats / (O * np1);
-    if (pt > kHaPart / np1) pt = kHaPart / np1;
-    if (pt < 1) pt = 1;
-    if (pt > pre) pt = pre;
-    const int hb_n = (hc + 3) / 4;
-    constexpr int kPF1 = (kTileFloats + 2 * kDim + kNT - 1) / kNT;
-    float pf[kPF1];
-    auto fetch1 = [&](int p0, int np) {
-      const int ncol = np * n;
+    if (tid < a.M) s_obs[tid] = b.obs[a.obs_off + tid];
+    __syncthreads();
+
+    // log-likelihood rows of every segment: sum over the group's modalities
+    // of ln max(A_m[o_m, s], MINVAL), formed as one FP64 log of the product
+    // (an out-of-range index is an all-zero one-hot row: no contribution)
+    for (int sg = 0; sg < u.nseg; ++sg) {
+      const int g = u.g[sg], R = a.gR[g];
+      const int n = a.gdim[g][R - 1], ld = n | 1;
+      const int r0 = u.r0[sg], nr = u.r1[sg] - r0;
+      double* X = s_x + u.lds[sg];
+      // four elements per thread in flight: every load of a batch is issued
+      // before any product is used
+      for (int e0 = tid; e0 < nr * n; e0 += 4 * kNT) {
+        double prod[4] = {1.0, 1.0, 1.0, 1.0};
+        for (int m = 0; m < a.M; ++m) {
+          if (a.mod_group[m] != g) continue;
+          const int o = s_obs[m];
+          if (o < 0 || o >= a.O[m]) continue;
+          const float* Ar = b.params + a.A_off[m] + (long long)o * a.K[m] + (long long)r0 * n;
+          float v[4];
 #pragma unroll
-      for (int j = 0; j < kPF1; ++j) {
-        const int e = tid + j * kNT;
-        if (e < O * ncol) {
-          const int o = e / ncol;
-          const int rem = e - o * ncol;
-          pf[j] = __builtin_nontemporal_load(Am + (long long)o * K + (long long)p0 * n + rem);
+          for (int q = 0; q < 4; ++q) {
+            const int e = e0 + q * kNT;
+            v[q] = e < nr * n ? Ar[e] : 1.0f;
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) prod[q] *= (double)fmaxf(v[q], kMinVal);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int e = e0 + q * kNT;
+          if (e < nr * n) {
+            const int rl = e / n, k = e - rl * n;
+            X[rl * ld + k] = log_f64d(prod[q], s_ltab);
+          }
         }
       }
-    };
-    fetch1(0, min(pt, pre));
-    for (int p0 = 0; p0 < pre; p0 += pt) {
-      const int np = min(pt, pre - p0);
-      const int ncol = np * n;
+    }
+    __syncthreads();
+
+    for (int it = 0; it < a.iters; ++it) {
+      rg_softmax(a, s_logq, s_qv, lane, wave);
       __syncthreads();
-#pragma unroll
-      for (int j = 0; j < kPF1; ++j) {
-        const int e = tid + j * kNT;
-        if (e < O * ncol) {
-          const int o = e / ncol;
-          const int rem = e - o * ncol;
-          const int p = rem / n, i = rem - p * n;
-          s_tile[(o * np + p) * np1 + i] = pf[j];
-        }
-      }
-      __syncthreads();
-      if (p0 + pt < pre) fetch1(p0 + pt, min(pt, pre - p0 - pt));
-      // sum_o x ln x per column, the outcome axis split over oc chunks
-      {
-        int oc = kNT / ncol;
-        if (oc < 1) oc = 1;
-        if (oc > O) oc = O;
-        const int ochunk = (O + oc - 1) / oc;
-        for (int e = tid; e < oc * ncol; e += kNT) {
-          const int ch = e / ncol, col = e - ch * ncol;
-          const int p = col / n, i = col - p * n;
-          double sacc = 0.0;
-          const int o1 = min(O, (ch + 1) * ochunk);
-          for (int o = ch * ochunk; o < o1; ++o) sacc += (double)xlogx_f(s_tile[(o * np + p) * np1 + i]);
-          s_hap[e] = sacc;
+      for (int sg = 0; sg < u.nseg; ++sg)
+        rg_fpi_segment(a, u.g[sg], u.r0[sg], u.r1[sg] - u.r0[sg], s_x + u.lds[sg], s_y, s_red, s_qv, s_part, tid);
+      if (a.team > 1) {
+        // exchange partial marginals with the other units of this agent
+        // (double-buffered by sweep parity: a unit overwrites its slot of
+        // buffer it & 1 only after every unit has read it at sweep it - 1)
+        double* part = b.part + a.part_off + (long long)(it & 1) * a.team * totS;
+        // wave 0 publishes this unit's partials: its stores, an agent-scope
+        // release fence, the arrival count; lane 0 spins with relaxed loads
+        // and the wave acquires once; the barrier below orders every other
+        // thread's loads after that acquire
+        if (wave == 0) {
+          for (int i = lane; i < totS; i += kWave) part[u.slice * totS + i] = s_part[i];
+          if (!s_broken) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (lane == 0) {
+              int* cnt = b.team_cnt + u.agent;
+              __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              const int target = a.team * (it + 1);
+              long long spins = 0;
+              while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > kSpinMax) {
+                  s_broken = 1;
+                  atomicOr(&b.cnt[3], 1);
+                  break;
+                }
+              }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          }
         }
         __syncthreads();
-        for (int col = tid; col < ncol; col += kNT) {
-          double sacc = 0.0;
-          for (int ch = 0; ch < oc; ++ch) sacc += s_hap[ch * ncol + col];
-          const int p = col / n, i = col - p * n;
-          s_ha[p * np1 + i] = sacc;
+        for (int i = tid; i < totS; i += kNT) {
+          const double* pp = part + i;
+          double s = 0.0;
+          for (int s0 = 0; s0 < a.team; s0 += 16) {
+            double v[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) v[q] = s0 + q < a.team ? pp[(long long)(s0 + q) * totS] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+              if (s0 + q < a.team) s += v[q];
+          }
+          s_logq[i] = s_lp[i] + s;
+          s_part[i] = 0.0;
+        }
+      } else {
+        for (int i = tid; i < totS; i += kNT) {
+          s_logq[i] = s_lp[i] + s_part[i];
+          s_part[i] = 0.0;
         }
       }
       __syncthreads();
-      const int outs = hb_n * O1 * np;
-      for (int e = tid; e < outs; e += kNT) {
-        const int hb = e / (O1 * np);
-        const int rem = e - hb * (O1 * np);
-        const int o = rem / np, p = rem - o * np;
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        const int h0 = hb * 4;
-        for (int i = 0; i < n; ++i) {
-          const double x = o < O ? (double)s_tile[(o * np + p) * np1 + i] : s_ha[p * np1 + i];
-#pragma unroll
-          for (int hh = 0; hh < 4; ++hh)
-            if (h0 + hh < hc) acc[hh] = fma(x, s_qh[(h0 + hh) * n + i], acc[hh]);
-        }
-#pragma unroll
-        for (int hh = 0; hh < 4; ++hh)
-          if (h0 + hh < hc) Z[((long long)(h0 + hh) * O1 + o) * pre + p0 + p] = acc[hh];
-      }
     }
-    __threadfence_block();
+    rg_softmax(a, s_logq, s_qv, lane, wave);
     __syncthreads();
 
-    // stages 2..R: contract the next axis with every history of its factor
-    double* zin = Z;
-    long long Bin = hc, rows_in = pre;  // zin: [Bin][O1 * rows_in], last axis contracted next
-    for (int j = R - 2; j >= 0; --j) {
-      const int nj = a.gdim[g][j];
-      const int fj = a.gfac[g][j];
-      const int Hj = a.H[t][fj];
-      const double* qj = hist + a.hist_lvl[t][fj];
-      const long long rows_out = rows_in / nj;
-      const long long slab = O1 * rows_out;  // output values per (b, h)
-      double* zout = zin + Bin * O1 * rows_in;
-      for (int h0 = 0; h0 < Hj; h0 += kHC)
-        rg_stage_tile(zin, Bin * slab, slab, nj, qj, h0, min(kHC, Hj - h0), Hj, zout, s_tiled, s_qh, tid);
-      __threadfence_block();
-      __syncthreads();
-      zin = zout;
-      Bin *= Hj;
-      rows_in = rows_out;
-    }
-
-    // score every combination: H(qo) + sum_s (sum_o A ln A) x q + qo . C
-    const float* Cm = b.params + a.C_off[m] + (a.C_time[m] ? (long long)t * O : 0);
-    double* sc = b.score + a.score_off + a.score_base[t][m] + (long long)itm.c0 * a.score_stride[t][m][R - 1];
-    const float cl = lane < O ? Cm[lane] : 0.0f;
-    for (long long c = wave; c < Bin; c += kNW) {
-      const double* z = zin + c * O1;
-      double v = 0.0;
-      if (lane < O) {
-        const double q = z[lane];
-        const float qf = (float)q;
-        v = q * (double)cl;
-        if (qf != 0.0f) v -= q * (double)logf(fmaxf(qf, kMinVal));
+    if (u.slice == 0) {
+      // posterior out (FP32) and as the level -1 history vector (FP64)
+      double* hist = b.hist + a.hist_off;
+      const bool bad = s_broken != 0;
+      for (int i = tid; i < totS; i += kNT) {
+        b.out[a.qs_off + i] = bad ? NAN : (float)s_qv[i];
+        hist[i] = s_qv[i];
       }
-      v = wave_sum(v);
-      if (lane == 0) sc[c] = v + z[O];
+      __syncthreads();
+      // rollout: q_f[t][h] = B_f[u_h][:, k] . (x_j q_{dep_j}[t-1][parent_j(h)])[k].  B_f is
+      // stored u-major ([U_f][S_f][K_B]); s_x is free after the sweeps and
+      // holds, per (t, f), the level t-1 vectors of a one-dependency B's
+      // parent at its end and as many whole action blocks as fit before them
+      // (one global round trip for both).
+      const int* roll = b.iarena + a.roll_off;
+      float* s_b = reinterpret_cast<float*>(s_x);
+      for (int t = 0; t < a.T; ++t) {
+        for (int f = 0; f < a.F; ++f) {
+          const int Sf = a.S[f], rb = a.BR[f], KB = a.BK[f], Uf = a.U[f];
+          const int nh = a.H[t][f];
+          const float* Bf = b.params + a.B_off[f];
+          const double* par = nullptr;  // one-dependency parents: vector h at par + h * S_d
+          int Sd = 0, nv = 0;
+          const double* par_src = nullptr;
+          if (rb == 1) {
+            const int d = a.Bdep[f][0];
+            Sd = a.S[d];
+            if (t == 0) {
+              par = s_qv + a.S_off[d];
+            } else {
+              par_src = hist + a.hist_lvl[t - 1][d];
+              if ((long long)a.H[t - 1][d] * Sd <= kCap / 2) {
+                nv = a.H[t - 1][d] * Sd;
+                par = s_x + (kCap - nv);
+              } else {
+                par = par_src;
+              }
+            }
+          }
+          const long long blk = (long long)Sf * KB;  // floats per action block in global memory
+          const int ldb = KB | 1;                    // LDS row stride (odd: no bank conflicts)
+          const long long blkl = (long long)Sf * ldb;  // floats per action block in LDS
+          const long long room = 2LL * (kCap - nv);  // floats before the parents
+          const int per = (int)std::max<long long>(0, std::min<long long>(Uf, room / std::max<long long>(blkl, 1)));
+          const int chunk = per > 0 ? per : 1;
+          for (int u0 = 0; u0 < Uf; u0 += chunk) {
+            const int u1 = std::min(Uf, u0 + chunk);
+            __syncthreads();
+            if (per > 0)
+              rg_stage<float>(
+                  (u1 - u0) * blk, tid, [&](long long i) { return Bf[u0 * blk + i]; },
+                  [&](long long i, float v) {
+                    const long long row = i / KB;  // (u - u0) * S_f + s
+                    s_b[row * ldb + (i - row * KB)] = v;
+                  });
+            if (u0 == 0 && nv > 0)
+              rg_stage<double>(
+                  nv, tid, [&](long long i) { return par_src[i]; }, [&](long long i, double v) { s_x[kCap - nv + i] = v; });
+            __syncthreads();
+            for (int e = tid; e < nh * Sf; e += kNT) {
+              const int h = e / Sf, s = e - h * Sf;
+              const int* rec = roll + a.roll_lvl[t][f] + h * (1 + rb);
+              const int uu = rec[0];
+              if (uu < u0 || uu >= u1) continue;
+              const float* br = per > 0 ? s_b + ((long long)(uu - u0) * Sf + s) * ldb : Bf + uu * blk + (long long)s * KB;
+              double acc = 0.0;
+              if (rb == 1) {
+                const double* w = par + (t == 0 ? 0 : (long long)rec[1] * Sd);
+                for (int k = 0; k < KB; ++k) acc = fma((double)br[k], w[k], acc);
+              } else {
+                const double* qp[kRB];
+                int dd[kRB];
+                for (int j = 0; j < rb; ++j) {
+                  const int d = a.Bdep[f][j];
+                  dd[j] = a.S[d];
+                  qp[j] = hist + (t == 0 ? a.S_off[d] : a.hist_lvl[t - 1][d] + rec[1 + j] * a.S[d]);
+                }
+                for (int k = 0; k < KB; ++k) {
+                  double w;
+                  if (rb == 2) {
+                    const int k0 = k / dd[1];
+                    w = qp[0][k0] * qp[1][k - k0 * dd[1]];
+                  } else {
+                    const int k01 = k / dd[2], k2 = k - k01 * dd[2];
+                    const int k0 = k01 / dd[1];
+                    w = qp[0][k0] * qp[1][k01 - k0 * dd[1]] * qp[2][k2];
+                  }
+                  acc = fma((double)br[k], w, acc);
+                }
+              }
+              hist[a.hist_lvl[t][f] + h * Sf + s] = acc;
+            }
+          }
+          __syncthreads();
+        }
+      }
     }
-    if (tr && tid == 0) tr[1] = wall_clock64();
+    if (b.trace && tid == 0) b.trace[2 * idx + 1] = wall_clock64();
     __syncthreads();
   }
+}
+
+// ---------------------------------------------------------------------------
+// EFE stages: batched small GEMMs Y[row, col] = X[row, :] . Q[col, :] with
+// X (rows x nk) and Q (cols x nk) staged in LDS (row stride nk | 1); each
+// thread computes a 4 x 4 block of (row, col) pairs strided by the block
+// counts, so the lanes of a wave read consecutive X rows (no bank
+// conflicts) and mostly the same Q rows (broadcast).
+// ---------------------------------------------------------------------------
+template <int RB, int CB, typename Store>
+__device__ __forceinline__ void rg_gemm_blk(const double* X, int rows, const double* Q, int cols, int nk, int tid,
+                                            Store store) {
+  const int ld = nk | 1;
+  const int nrb = (rows + RB - 1) / RB, ncb = (cols + CB - 1) / CB;
+  for (int e = tid; e < nrb * ncb; e += kNT) {
+    const int rb = e % nrb, cb = e / nrb;
+    double acc[RB][CB];
+#pragma unroll
+    for (int u = 0; u < RB; ++u)
+#pragma unroll
+      for (int v = 0; v < CB; ++v) acc[u][v] = 0.0;
+    int ro[RB], co[CB];
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+      const int r = rb + u * nrb;
+      ro[u] = (r < rows ? r : 0) * ld;
+    }
+#pragma unroll
+    for (int v = 0; v < CB; ++v) {
+      const int c = cb + v * ncb;
+      co[v] = (c < cols ? c : 0) * ld;
+    }
+    for (int k = 0; k < nk; ++k) {
+      double x[RB], q[CB];
+#pragma unroll
+      for (int u = 0; u < RB; ++u) x[u] = X[ro[u] + k];
+#pragma unroll
+      for (int v = 0; v < CB; ++v) q[v] = Q[co[v] + k];
+#pragma unroll
+      for (int u = 0; u < RB; ++u)
+#pragma unroll
+        for (int v = 0; v < CB; ++v) acc[u][v] = fma(x[u], q[v], acc[u][v]);
+    }
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+      const int r = rb + u * nrb;
+      if (r >= rows) continue;
+#pragma unroll
+      for (int v = 0; v < CB; ++v) {
+        const int c = cb + v * ncb;
+        if (c < cols) store(r, c, acc[u][v]);
+      }
+    }
+  }
+}
+
+// Sixteen (row, column) pairs per thread; the block is tall when the
+// column count is small so that more threads have work.
+template <typename Store>
+__device__ __forceinline__ void rg_gemm_lds(const double* X, int rows, const double* Q, int cols, int nk, int tid,
+                                            Store store) {
+  if (cols >= 4)
+    rg_gemm_blk<4, 4>(X, rows, Q, cols, nk, tid, store);
+  else if (cols >= 2)
+    rg_gemm_blk<8, 2>(X, rows, Q, cols, nk, tid, store);
+  else
+    rg_gemm_blk<16, 1>(X, rows, Q, cols, nk, tid, store);
+}
+
+__device__ __forceinline__ int rg_hcount(const Agent& a, int m, int t, int p) {
+  const int f = a.ef[m][p];
+  return f < 0 ? 1 : a.H[t][f];
+}
+
+// History vectors [c0, c1) of padded dim p of modality m at level t into
+// Q [c][k] (row stride d | 1); padding dims have one history, q = 1.
+__device__ __forceinline__ void rg_load_hist(const Agent& a, const double* hist, int m, int t, int p, int c0,
+                                             int c1, double* Q, int tid) {
+  const int d = a.ed[m][p], ld = d | 1, f = a.ef[m][p];
+  if (f < 0) {
+    for (int e = tid; e < c1 - c0; e += kNT) Q[e * ld] = 1.0;
+    return;
+  }
+  const double* src = hist + a.hist_lvl[t][f] + (long long)c0 * a.S[f];
+  rg_stage<double>(
+      (long long)(c1 - c0) * d, tid, [&](long long e) { return src[e]; },
+      [&](long long e, double v) {
+        const int c = (int)e / d, k = (int)e - c * d;
+        Q[c * ld + k] = v;
+      });
+}
+
+// Stage 1: for positions ij in [r0, r1) (ij = i * d1 + j) and concatenated
+// last-dim histories n in [c0, c1) (at most 16):
+//   Z1[((n * (O+1) + o) * d0 + i) * d1 + j] = sum_k A[o, ij, k] q_n[k]   (o < O)
+//   Z1[((n * (O+1) + O) * d0 + i) * d1 + j] = sum_k (sum_o A ln A)[ij, k] q_n[k]
+// The tile's A block is staged in LDS as floats, sum_o A ln A as doubles;
+// each wave multiplies 16-row blocks of [A ; sum_o A ln A] by the 16
+// history columns with FP64 MFMA (v_mfma_f64_16x16x4f64: lane l holds
+// X[l % 16][k + l / 16] and Q[col l % 16][k + l / 16]; accumulator element v
+// of lane l is row 4 v + l / 16, column l % 16).
+typedef double rg_d4 __attribute__((ext_vector_type(4)));
+constexpr int kE1A = 4352;    // floats of the A block
+constexpr int kE1H = 1024;    // doubles of the sum_o A ln A rows
+constexpr int kE1Q = 16 * 65;  // doubles of the history block
+
+__host__ __device__ __forceinline__ int rg_e1_rows(int O, int d2, int d01) {
+  // (i, j) positions of one tile: the A block and the sum_o A ln A rows fit
+  const int lda = ((d2 + 3) & ~3) + 1;
+  int nr = kE1A / (O * lda);
+  nr = nr < kE1H / lda ? nr : kE1H / lda;
+  if (nr < 1) nr = 1;
+  return nr < d01 ? nr : d01;
+}
+
+__global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
+  __shared__ float s_a[kE1A];
+  __shared__ double s_h[kE1H];
+  __shared__ double s_q[kE1Q];
+  __shared__ double s_hap[kNT];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
+  const Tile tl = b.t1[blockIdx.x];
+  const Agent& a = b.ag[tl.agent];
+  long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_units + blockIdx.x) : nullptr;
+  if (tr && tid == 0) tr[0] = wall_clock64();
+  const int m = tl.m, O = a.O[m], O1 = O + 1;
+  const int d01 = a.ed[m][0] * a.ed[m][1], d2 = a.ed[m][2];
+  const int d2p = (d2 + 3) & ~3, lda = d2p + 1;
+  const int nr = tl.r1 - tl.r0, nc = tl.c1 - tl.c0;
+  const long long K = a.K[m];
+  {
+    const float* Am = b.params + a.A_off[m] + (long long)tl.r0 * d2;
+    const int per = nr * d2;
+    rg_stage<float>(
+        (long long)O * per, tid,
+        [&](long long e) {
+          const int o = (int)(e / per);
+          return __builtin_nontemporal_load(Am + (long long)o * K + (e - (long long)o * per));
+        },
+        [&](long long e, float v) {
+          const int o = (int)(e / per), rem = (int)(e - (long long)o * per);
+          const int rl = rem / d2, k = rem - rl * d2;
+          s_a[(o * nr + rl) * lda + k] = v;
+        });
+    const int f = a.ef[m][2];
+    const double* hist = b.hist + a.hist_off;
+    rg_stage<double>(
+        16 * d2p, tid,
+        [&](long long e) {
+          const int c = (int)e / d2p, k = (int)e - c * d2p;
+          double q = 0.0;
+          if (c < nc && k < d2) {
+            const int n = tl.c0 + c;
+            int t = 0;
+            while (t + 1 < a.T && n >= a.tstart[m][t + 1]) ++t;
+            q = hist[a.hist_lvl[t][f] + (long long)(n - a.tstart[m][t]) * a.S[f] + k];
+          }
+          return q;
+        },
+        [&](long long e, double q) {
+          const int c = (int)e / d2p, k = (int)e - c * d2p;
+          s_q[c * lda + k] = q;
+        });
+  }
+  lds_barrier();
+  // sum_o A ln A per (ij, k): outcome axis split over thread groups, partial
+  // sums combined in a fixed order
+  {
+    const int P = nr * d2;
+    const int G = std::max(1, std::min(O, kNT / P));
+    const int och = (O + G - 1) / G;
+    for (int e = tid; e < P * G; e += kNT) {
+      const int g = e / P, pp = e - g * P;
+      const int rl = pp / d2, k = pp - rl * d2;
+      const float* col = s_a + rl * lda + k;
+      const int o1 = std::min(O, (g + 1) * och);
+      double s0 = 0.0, s1 = 0.0;
+      int o = g * och;
+      for (; o + 2 <= o1; o += 2) {
+        const float v0 = col[(o * nr) * lda], v1 = col[((o + 1) * nr) * lda];
+        s0 += (double)xlogx_f(v0);
+        s1 += (double)xlogx_f(v1);
+      }
+      if (o < o1) s0 += (double)xlogx_f(col[(o * nr) * lda]);
+      if (G == 1)
+        s_h[rl * lda + k] = s0 + s1;
+      else
+        s_hap[e] = s0 + s1;
+    }
+    if (G > 1) {
+      lds_barrier();
+      for (int pp = tid; pp < P; pp += kNT) {
+        double sacc = 0.0;
+        for (int g = 0; g < G; ++g) sacc += s_hap[g * P + pp];
+        const int rl = pp / d2, k = pp - rl * d2;
+        s_h[rl * lda + k] = sacc;
+      }
+    }
+  }
+  lds_barrier();
+  // MFMA over 16-row blocks of the O1 * nr rows
+  const int R = O1 * nr;
+  const int nblk = (R + 15) >> 4;
+  const int li = lane & 15, lk = lane >> 4;
+  double* Z = b.z1 + a.z1_off[m] + tl.r0;
+  const double* qrow = s_q + li * lda + lk;
+  for (int blk = wave; blk < nblk; blk += kNW) {
+    const int r = (blk << 4) + li;  // this lane's A-operand row
+    const bool rv = r < R;
+    const bool isha = r >= O * nr;
+    const float* xa = s_a + (rv && !isha ? r : 0) * lda + lk;
+    const double* xh = s_h + (rv && isha ? r - O * nr : 0) * lda + lk;
+    rg_d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < d2p; k0 += 4) {
+      const bool kv = rv && (k0 + lk) < d2;
+      double x = 0.0;
+      if (kv) x = isha ? xh[k0] : (double)xa[k0];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, qrow[k0], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int rr = (blk << 4) + 4 * v + lk;
+      if (rr < R && li < nc) {
+        const int o = rr / nr, rl = rr - o * nr;
+        Z[((long long)(tl.c0 + li) * O1 + o) * d01 + rl] = acc[v];
+      }
+    }
+  }
+  if (tr && tid == 0) tr[1] = wall_clock64();
+}
+
+// Stage 2 at level t: rows R = (h2, o, i) in [r0, r1) of Z1 (length d1),
+// columns = middle-dim histories [c0, c1).  Z2[((h2 * H1 + h1) * (O+1) + o) * d0 + i].
+__global__ __launch_bounds__(kNT) void rg_e2_kernel(Bufs b) {
+  __shared__ double s_t[kTile];
+  __shared__ double s_q[kQTab];
+  const int tid = threadIdx.x;
+  const Tile tl = b.t2[blockIdx.x];
+  const Agent& a = b.ag[tl.agent];
+  long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_units + b.n_t1 + blockIdx.x) : nullptr;
+  if (tr && tid == 0) tr[0] = wall_clock64();
+  const int m = tl.m, t = tl.t, O1 = a.O[m] + 1;
+  const int d0 = a.ed[m][0], d1 = a.ed[m][1], ld1 = d1 | 1;
+  const int nr = tl.r1 - tl.r0, nc = tl.c1 - tl.c0;
+  const int H1 = rg_hcount(a, m, t, 1);
+  const double* src = b.z1 + a.z1_off[m] + ((long long)a.tstart[m][t] * O1 * d0 + tl.r0) * d1;
+  rg_stage<double>(
+      (long long)nr * d1, tid, [&](long long e) { return src[e]; },
+      [&](long long e, double v) {
+        const int rl = (int)e / d1, j = (int)e - rl * d1;
+        s_t[rl * ld1 + j] = v;
+      });
+  rg_load_hist(a, b.hist + a.hist_off, m, t, 1, tl.c0, tl.c1, s_q, tid);
+  __syncthreads();
+  double* Z = b.z2 + a.z2_off[m][t];
+  const int r0 = tl.r0, c0 = tl.c0, od = O1 * d0;
+  rg_gemm_lds(s_t, nr, s_q, nc, d1, tid, [&](int r, int c, double v) {
+    const int R = r0 + r;
+    const int h2 = R / od, oi = R - h2 * od;
+    Z[((long long)h2 * H1 + c0 + c) * od + oi] = v;
+  });
+  if (tr && tid == 0) tr[1] = wall_clock64();
+}
+
+// Stage 3 at level t: pairs (h2, h1) in [r0, r1) (pair = h2 * H1 + h1),
+// first-dim histories h0 in [c0, c1): qo[o] = Z2[pair, o, :] . q0[h0], then
+// score = sum_{o<O} (qo C[o] - qo ln qo) + qo[O]  (one wave per combination).
+__global__ __launch_bounds__(kNT) void rg_e3_kernel(Bufs b) {
+  __shared__ double s_t[kTile];
+  __shared__ double s_q[kQTab];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
+  const Tile tl = b.t3[blockIdx.x];
+  const Agent& a = b.ag[tl.agent];
+  long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_units + b.n_t1 + b.n_t2 + blockIdx.x) : nullptr;
+  if (tr && tid == 0) tr[0] = wall_clock64();
+  const int m = tl.m, t = tl.t, O = a.O[m], O1 = O + 1;
+  const int d0 = a.ed[m][0], ld0 = d0 | 1;
+  const int np = tl.r1 - tl.r0, nh = tl.c1 - tl.c0;
+  const int H0 = rg_hcount(a, m, t, 0);
+  // a rank-1 modality's stage-1 output already has the stage-2 layout
+  const double* zb = a.gR[a.mod_group[m]] == 1 ? b.z1 : b.z2;
+  const double* src = zb + a.z2_off[m][t] + (long long)tl.r0 * O1 * d0;
+  rg_stage<double>(
+      (long long)np * O1 * d0, tid, [&](long long e) { return src[e]; },
+      [&](long long e, double v) {
+        const int row = (int)e / d0, i = (int)e - row * d0;
+        s_t[row * ld0 + i] = v;
+      });
+  rg_load_hist(a, b.hist + a.hist_off, m, t, 0, tl.c0, tl.c1, s_q, tid);
+  const float* Cm = b.params + a.C_off[m] + (a.C_time[m] ? (long long)t * O : 0);
+  const float c_lo = lane < O ? Cm[lane] : 0.0f;
+  const float c_hi = lane + kWave < O ? Cm[lane + kWave] : 0.0f;
+  __syncthreads();
+  double* sc = b.score + a.score_off + a.score_base[t][m];
+  for (int cc = wave; cc < np * nh; cc += kNW) {
+    const int pl = cc / nh, hl = cc - pl * nh;
+    const double* q0 = s_q + hl * ld0;
+    double v = 0.0;
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const int o = lane + half * kWave;
+      if (o > O) continue;
+      const double* z = s_t + (pl * O1 + o) * ld0;
+      double qo = 0.0;
+      for (int i = 0; i < d0; ++i) qo = fma(z[i], q0[i], qo);
+      if (o < O) {
+        const float qf = (float)qo;
+        double w = qo * (double)(half ? c_hi : c_lo);
+        if (qf != 0.0f) w -= qo * (double)logf(fmaxf(qf, kMinVal));
+        v += w;
+      } else {
+        v += qo;
+      }
+    }
+    v = wave_sum(v);
+    if (lane == 0) sc[(long long)(tl.r0 + pl) * H0 + tl.c0 + hl] = v;
+  }
+  if (tr && tid == 0) tr[1] = wall_clock64();
 }
 
 // ---------------------------------------------------------------------------
@@ -600,12 +871,11 @@ __global__ __launch_bounds__(kNT) void rg_assemble_kernel(Bufs b) {
       }
     const int* pol0 = b.iarena + a.pol0_off;
     for (int f = 0; f < F; ++f) b.act[a.act_off + f] = pol0[(long long)bx * F + f];
+    // this agent's team barrier starts from zero at the next step
+    b.team_cnt[blockIdx.x] = 0;
   }
-  // the queues of the two persistent kernels start from zero at the next step
-  if (blockIdx.x == 0 && tid == 0) {
-    b.cnt[0] = 0;
-    b.cnt[1] = 0;
-  }
+  // the FPI queue starts from zero at the next step
+  if (blockIdx.x == 0 && tid == 0) b.cnt[0] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -615,20 +885,23 @@ struct Plan {
   int device = 0;
   int64_t n_agents = 0, sumF = 0, sumM = 0;
   std::vector<Agent> agents;
-  std::vector<int> fpi_order;
-  std::vector<Item> items;
+  std::vector<Unit> units;
+  std::vector<Tile> t1, t2, t3;
   std::vector<int32_t> iarena;
   std::vector<int64_t> layout;  // see pymdp_hip_ragged_layout
-  long long hist_total = 0, score_total = 0, ws1_need = 0, ws2_need = 0;
-  int grid1 = 0, grid2 = 0, grid_max = 0;
+  long long hist_total = 0, score_total = 0, z1_total = 0, z2_total = 0, part_total = 0;
+  double efe_macs = 0.0, fpi_macs = 0.0;
+  int team_max = 1;
+  int grid1 = 0, grid_cap = 0;
   bool tuned = false;
   long long* trace = nullptr;  // caller-owned device buffer, see pymdp_hip_ragged_set_trace
   // device
   Agent* d_ag = nullptr;
-  int* d_order = nullptr;
-  Item* d_items = nullptr;
+  Unit* d_units = nullptr;
+  Tile *d_t1 = nullptr, *d_t2 = nullptr, *d_t3 = nullptr;
   int32_t* d_iarena = nullptr;
-  double *d_hist = nullptr, *d_score = nullptr, *d_ws1 = nullptr, *d_ws2 = nullptr;
+  double *d_hist = nullptr, *d_score = nullptr, *d_z1 = nullptr, *d_z2 = nullptr, *d_part = nullptr;
+  int* d_team = nullptr;
   int* d_cnt = nullptr;
 };
 
@@ -642,15 +915,10 @@ static void free_plan(Plan* p) {
   int prev = 0;
   (void)hipGetDevice(&prev);
   (void)hipSetDevice(p->device);
-  (void)hipFree(p->d_ag);
-  (void)hipFree(p->d_order);
-  (void)hipFree(p->d_items);
-  (void)hipFree(p->d_iarena);
-  (void)hipFree(p->d_hist);
-  (void)hipFree(p->d_score);
-  (void)hipFree(p->d_ws1);
-  (void)hipFree(p->d_ws2);
-  (void)hipFree(p->d_cnt);
+  for (void* x : {(void*)p->d_ag, (void*)p->d_units, (void*)p->d_t1, (void*)p->d_t2, (void*)p->d_t3,
+                  (void*)p->d_iarena, (void*)p->d_hist, (void*)p->d_score, (void*)p->d_z1, (void*)p->d_z2,
+                  (void*)p->d_part, (void*)p->d_team, (void*)p->d_cnt})
+    (void)hipFree(x);
   (void)hipSetDevice(prev);
   delete p;
 }
@@ -688,16 +956,18 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
   pl->n_agents = n_agents;
   long long pA = 0, pO = 0, pS = 0;  // running offsets: A_dep_flat, modality, factor
   long long pB = 0, pPol = 0;
-  long long params = 0, obs = 0, outf = 0, acts = 0, hist = 0, score = 0;
+  long long params = 0, obs = 0, outf = 0, acts = 0, hist = 0, score = 0, z1 = 0, z2 = 0, part = 0;
   std::vector<int64_t> A_off, A_perm, B_off, C_off, D_off, E_off, obs_off, qs_off, G_off, qpi_off, act_off;
-  std::vector<double> fpi_cost(n_agents);
-  struct ItemC {
-    Item it;
+  struct UnitC {
+    std::vector<Unit> units;
     double cost;
   };
-  std::vector<ItemC> items;
-  double total_cost = 0.0;
-  std::vector<std::vector<ItemC>> per_agent_items(n_agents);
+  std::vector<UnitC> teams, singles;
+  struct TileC {
+    Tile t;
+    double cost;
+  };
+  std::vector<TileC> t1, t2, t3;
   for (int64_t ai = 0; ai < n_agents; ++ai) {
     const int64_t F = hdr[4 * ai], M = hdr[4 * ai + 1], T = hdr[4 * ai + 2], P = hdr[4 * ai + 3];
     auto fail = [&](const std::string& m) {
@@ -801,28 +1071,70 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
       }
     }
     a.G = (int)sets.size();
-    int ll = 0;
-    int xa = 0, xb = 0;
     for (int g = 0; g < a.G; ++g) {
       a.gR[g] = (int)set_order[g].size();
-      int K = 1;
       for (int j = 0; j < a.gR[g]; ++j) {
         a.gfac[g][j] = set_order[g][j];
         a.gdim[g][j] = a.S[set_order[g][j]];
-        K *= a.gdim[g][j];
       }
-      a.gll[g] = ll;
-      ll += K;
-      if (a.gR[g] >= 2) xa = std::max(xa, K / a.gdim[g][a.gR[g] - 1]);
-      if (a.gR[g] >= 3) xb = std::max(xb, a.gdim[g][0]);
     }
-    a.ll_total = ll;
-    a.xa_off = ll;
-    a.xb_off = ll + xa;
-    a.scr_total = ll + xa + xb;
-    a.scr_lds = a.scr_total <= kFpiScr ? 1 : 0;
-    if (!a.scr_lds) pl->ws1_need = std::max(pl->ws1_need, (long long)a.scr_total);
     a.iters = (F == 1 && num_iter > 0) ? 1 : num_iter;
+    // FPI units: segments of the groups' stage-0 rows, packed into LDS slices
+    {
+      std::vector<Unit> us;
+      Unit cur;
+      std::memset(&cur, 0, sizeof(cur));
+      int used = 0;
+      double fc = 0.0;
+      auto flush = [&]() {
+        if (cur.nseg > 0) {
+          us.push_back(cur);
+          std::memset(&cur, 0, sizeof(cur));
+          used = 0;
+        }
+      };
+      for (int g = 0; g < a.G; ++g) {
+        const int R = a.gR[g];
+        const int n = a.gdim[g][R - 1], ld = n | 1;
+        int rows = 1;
+        for (int j = 0; j + 1 < R; ++j) rows *= a.gdim[g][j];
+        fc += (double)rows * n * (1.0 + 2.0 * a.iters);
+        int r = 0;
+        while (r < rows) {
+          const int cap = std::min((kCap - used) / ld, kRowsCap);
+          if (cap <= 0 || cur.nseg == kM) {
+            flush();
+            continue;
+          }
+          const int take = std::min(rows - r, cap);
+          cur.g[cur.nseg] = g;
+          cur.r0[cur.nseg] = r;
+          cur.r1[cur.nseg] = r + take;
+          cur.lds[cur.nseg] = used;
+          cur.nseg++;
+          used += take * ld;
+          r += take;
+        }
+      }
+      flush();
+      if (us.empty()) return fail("no FPI work");
+      if ((int)us.size() > kTeamMax) return fail("log-likelihood too large for one agent (more than 128 units)");
+      a.team = (int)us.size();
+      for (int i = 0; i < a.team; ++i) {
+        us[i].agent = (int)ai;
+        us[i].slice = i;
+      }
+      pl->fpi_macs += fc;
+      if (a.team > 1) {
+        a.part_off = part;
+        part += 2LL * a.team * a.totS;
+        teams.push_back(UnitC{us, fc});
+      } else {
+        a.part_off = 0;
+        singles.push_back(UnitC{us, fc});
+      }
+      pl->team_max = std::max(pl->team_max, a.team);
+    }
     // params arena
     for (int m = 0; m < M; ++m) {
       long long K = 1;
@@ -894,10 +1206,9 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
     a.pol0_off = (long long)pl->iarena.size();
     for (int64_t p = 0; p < P; ++p)
       for (int f = 0; f < F; ++f) pl->iarena.push_back(policy_flat[pPol + (p * T + 0) * F + f]);
-    // score layout and EFE items
+    // score layout and EFE tiles
     a.score_off = score;
     long long sl = 0;
-    double acost = 0.0;
     for (int t = 0; t < T; ++t)
       for (int m = 0; m < M; ++m) {
         const int g = a.mod_group[m], R = a.gR[g];
@@ -909,63 +1220,98 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
         if (sl + stride > INT32_MAX) return fail("too many history combinations");
         a.score_base[t][m] = (int)sl;
         sl += stride;
-        // items: chunks of the first contracted axis' histories
-        std::vector<int> dims, Hs;
-        for (int j = 0; j < R; ++j) {
-          dims.push_back(a.gdim[g][j]);
-          Hs.push_back(a.H[t][a.gfac[g][j]]);
-        }
-        const double c = efe_cost(dims, Hs, a.O[m]);
-        const int HR = Hs[R - 1];
-        // workspace (doubles) per first-axis history
-        double per_h = (a.O[m] + 1.0) * (double)(a.K[m] / dims[R - 1]);
-        {
-          double rows = (double)(a.K[m] / dims[R - 1]), mult = 1.0, acc = per_h;
-          for (int j = R - 2; j >= 0; --j) {
-            mult *= Hs[j];
-            rows /= dims[j];
-            acc += mult * (a.O[m] + 1.0) * rows;
-          }
-          per_h = acc;
-        }
-        const int hc_max = std::min(HR, kHC);
-        for (int c0 = 0; c0 < HR; c0 += hc_max) {
-          const int c1 = std::min(HR, c0 + hc_max);
-          ItemC ic{Item{(int)ai, m, t, c0, c1}, c * (c1 - c0) / HR};
-          per_agent_items[ai].push_back(ic);
-          pl->ws2_need = std::max(pl->ws2_need, (long long)std::ceil(per_h * (c1 - c0)));
-        }
-        acost += c;
       }
     a.logit_off = score + sl;
     score += sl + P;
-    total_cost += acost;
-    {
-      double fc = 0.0;
-      for (int g = 0; g < a.G; ++g) {
-        double K = 1;
-        for (int j = 0; j < a.gR[g]; ++j) K *= a.gdim[g][j];
-        fc += K * (2.0 + 3.0 * a.iters * a.gR[g]);
+    for (int m = 0; m < M; ++m) {
+      const int g = a.mod_group[m], R = a.gR[g];
+      // padded rank-3 view
+      for (int p = 0; p < 3; ++p) {
+        const int j = p - (3 - R);
+        a.ed[m][p] = j >= 0 ? a.gdim[g][j] : 1;
+        a.ef[m][p] = j >= 0 ? a.gfac[g][j] : -1;
       }
-      fpi_cost[ai] = fc;
+      const int O1 = a.O[m] + 1;
+      const int d0 = a.ed[m][0], d1 = a.ed[m][1], d2 = a.ed[m][2];
+      auto Hc = [&](int t, int p) { return a.ef[m][p] < 0 ? 1 : a.H[t][a.ef[m][p]]; };
+      a.tstart[m][0] = 0;
+      for (int t = 0; t < T; ++t) a.tstart[m][t + 1] = a.tstart[m][t] + Hc(t, 2);
+      for (int t = T + 1; t <= kT; ++t) a.tstart[m][t] = a.tstart[m][T];
+      const int N1 = a.tstart[m][T];
+      a.z1_off[m] = z1;
+      z1 += (long long)N1 * O1 * d0 * d1;
+      // E1 tiles: blocks of (i, j) positions whose A rows and sum_o A ln A
+      // rows fit the tile's LDS x blocks of at most 16 concatenated history
+      // columns
+      {
+        const int d01 = d0 * d1;
+        const int nr = rg_e1_rows(a.O[m], d2, d01);
+        if (a.O[m] * (((d2 + 3) & ~3) + 1) > kE1A) return fail("EFE tile does not fit");
+        for (int r0 = 0; r0 < d01; r0 += nr)
+          for (int c0 = 0; c0 < N1; c0 += 16) {
+            const int r1 = std::min(d01, r0 + nr), c1 = std::min(N1, c0 + 16);
+            t1.push_back(TileC{Tile{(int)ai, m, 0, r0, r1, c0, c1, 0}, (double)(r1 - r0) * O1 * d2 * 16});
+          }
+        pl->efe_macs += (double)d01 * O1 * d2 * N1;
+      }
+      for (int t = 0; t < T; ++t) {
+        const int H0 = Hc(t, 0), H1 = Hc(t, 1), H2 = Hc(t, 2);
+        if (R == 1) {
+          // Z1 of a rank-1 modality already has the stage-2 layout
+          a.z2_off[m][t] = a.z1_off[m] + (long long)a.tstart[m][t] * O1;
+        } else {
+          a.z2_off[m][t] = z2;
+          z2 += (long long)H2 * H1 * O1 * d0;
+          const int rows = H2 * O1 * d0, ld1 = d1 | 1;
+          const int nr = std::max(1, std::min(std::min(rows, kTile / ld1), 1024));
+          for (int r0 = 0; r0 < rows; r0 += nr)
+            for (int c0 = 0; c0 < H1; c0 += kCols) {
+              const int r1 = std::min(rows, r0 + nr), c1 = std::min(H1, c0 + kCols);
+              t2.push_back(TileC{Tile{(int)ai, m, t, r0, r1, c0, c1, 0}, (double)(r1 - r0) * d1 * (c1 - c0)});
+            }
+          pl->efe_macs += (double)rows * d1 * H1;
+        }
+        // E3 tiles: (h2, h1) pairs x first-dim histories
+        const int pairs = H2 * H1, ld0 = d0 | 1;
+        if (O1 * ld0 > kTile) return fail("EFE tile does not fit");
+        const int hb = std::min(H0, kCols);
+        const int np = std::max(1, std::min({pairs, kTile / (O1 * ld0), std::max(1, kE3Combos / hb)}));
+        for (int r0 = 0; r0 < pairs; r0 += np)
+          for (int c0 = 0; c0 < H0; c0 += kCols) {
+            const int r1 = std::min(pairs, r0 + np), c1 = std::min(H0, c0 + kCols);
+            t3.push_back(TileC{Tile{(int)ai, m, t, r0, r1, c0, c1, 0}, (double)(r1 - r0) * (c1 - c0) * O1 * d0});
+          }
+        pl->efe_macs += (double)pairs * H0 * O1 * d0;
+      }
     }
     pl->agents.push_back(a);
     pS += F;
     pO += M;
     pPol += P * T * F;
   }
-  for (auto& v : per_agent_items)
-    for (auto& ic : v) items.push_back(ic);
-  std::stable_sort(items.begin(), items.end(), [](const ItemC& x, const ItemC& y) { return x.cost > y.cost; });
-  for (auto& ic : items) pl->items.push_back(ic.it);
-  pl->fpi_order.resize(n_agents);
-  std::iota(pl->fpi_order.begin(), pl->fpi_order.end(), 0);
-  std::stable_sort(pl->fpi_order.begin(), pl->fpi_order.end(),
-                   [&](int x, int y) { return fpi_cost[x] > fpi_cost[y]; });
+  // FPI queue: teams (largest first, units of a team consecutive), then single units
+  std::stable_sort(teams.begin(), teams.end(), [](const UnitC& x, const UnitC& y) { return x.cost > y.cost; });
+  std::stable_sort(singles.begin(), singles.end(), [](const UnitC& x, const UnitC& y) { return x.cost > y.cost; });
+  for (auto& v : teams)
+    for (auto& u : v.units) pl->units.push_back(u);
+  for (auto& v : singles)
+    for (auto& u : v.units) pl->units.push_back(u);
+  auto by_cost = [](const TileC& x, const TileC& y) { return x.cost > y.cost; };
+  for (auto* v : {&t1, &t2, &t3}) std::stable_sort(v->begin(), v->end(), by_cost);
+  for (auto& x : t1) pl->t1.push_back(x.t);
+  for (auto& x : t2) pl->t2.push_back(x.t);
+  for (auto& x : t3) pl->t3.push_back(x.t);
+  if (pl->t1.size() > INT32_MAX || pl->t2.size() > INT32_MAX || pl->t3.size() > INT32_MAX) {
+    delete pl;
+    return einval("too many EFE tiles");
+  }
   pl->sumF = pS;
   pl->sumM = pO;
   pl->hist_total = hist;
   pl->score_total = score;
+  pl->z1_total = z1;
+  pl->z2_total = z2;
+  pl->part_total = part;
   // layout vector (host)
   auto& L = pl->layout;
   L = {params, obs, outf, acts};
@@ -980,24 +1326,6 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
   L.insert(L.end(), G_off.begin(), G_off.end());
   L.insert(L.end(), qpi_off.begin(), qpi_off.end());
   L.insert(L.end(), act_off.begin(), act_off.end());
-
-  // launch grids (workgroups per CU of the 256-CU MI355X; see pymdp_hip_ragged_autotune)
-  const int cus = 256;
-  pl->grid_max = cus * 8;
-  pl->grid1 = (int)std::min<long long>(n_agents, cus * 4);
-  pl->grid2 = (int)std::min<long long>((long long)pl->items.size(), cus * 4);
-  // keep the EFE workspace within 16 GiB
-  const long long ws2_slot = std::max<long long>(pl->ws2_need, 1);
-  const long long cap = (16LL << 30) / 8 / ws2_slot;
-  if (cap < 1) {
-    free_plan(pl);
-    return einval("an EFE work item needs more than 16 GiB of workspace");
-  }
-  pl->grid_max = (int)std::min<long long>(pl->grid_max, std::max<long long>(cap, 1));
-  pl->grid1 = std::min(pl->grid1, pl->grid_max);
-  pl->grid2 = std::max(1, std::min(pl->grid2, pl->grid_max));
-  pl->ws2_need = ws2_slot;
-  pl->ws1_need = std::max<long long>(pl->ws1_need, 1);
   *plan_out = pl;
   return PYMDP_HIP_OK;
 }
@@ -1008,34 +1336,49 @@ static int upload(Plan* pl) {
     set_error(std::string("ragged: ") + what + ": " + hipGetErrorString(e));
     return PYMDP_HIP_EDEVICE;
   };
+  // persistent FPI grid: every workgroup co-resident (the team barriers rely
+  // on it), at least one workgroup more than the largest team
+  int cus = 0, per_cu = 0;
   hipError_t e;
-  if ((e = hipMalloc(&pl->d_ag, sizeof(Agent) * pl->agents.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
-  if ((e = hipMalloc(&pl->d_order, sizeof(int) * pl->fpi_order.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
-  if ((e = hipMalloc(&pl->d_items, sizeof(Item) * std::max<size_t>(1, pl->items.size()))) != hipSuccess)
+  if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device)) != hipSuccess)
+    return fail_dev(e, "hipDeviceGetAttribute");
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rg_fpi_kernel, kNT, 0)) != hipSuccess)
+    return fail_dev(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+  pl->grid_cap = std::max(1, cus * per_cu);
+  if (pl->grid_cap <= pl->team_max) {
+    set_error("ragged: the device holds " + std::to_string(pl->grid_cap) +
+              " FPI workgroups at once, an agent needs a team of " + std::to_string(pl->team_max));
+    return PYMDP_HIP_EDEVICE;
+  }
+  pl->grid1 = std::min<long long>(pl->grid_cap, (long long)pl->units.size());
+  pl->grid1 = std::max(pl->grid1, std::min(pl->grid_cap, pl->team_max + 1));
+  auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 8)); };
+  if ((e = alloc((void**)&pl->d_ag, sizeof(Agent) * pl->agents.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_units, sizeof(Unit) * pl->units.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_t1, sizeof(Tile) * pl->t1.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_t2, sizeof(Tile) * pl->t2.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_t3, sizeof(Tile) * pl->t3.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_iarena, sizeof(int32_t) * pl->iarena.size())) != hipSuccess)
     return fail_dev(e, "hipMalloc");
-  if ((e = hipMalloc(&pl->d_iarena, sizeof(int32_t) * std::max<size_t>(1, pl->iarena.size()))) != hipSuccess)
+  if ((e = alloc((void**)&pl->d_hist, sizeof(double) * pl->hist_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_score, sizeof(double) * pl->score_total)) != hipSuccess)
     return fail_dev(e, "hipMalloc");
-  if ((e = hipMalloc(&pl->d_hist, sizeof(double) * std::max<long long>(1, pl->hist_total))) != hipSuccess)
-    return fail_dev(e, "hipMalloc");
-  if ((e = hipMalloc(&pl->d_score, sizeof(double) * std::max<long long>(1, pl->score_total))) != hipSuccess)
-    return fail_dev(e, "hipMalloc");
-  if ((e = hipMalloc(&pl->d_ws1, sizeof(double) * (size_t)pl->ws1_need * pl->grid_max)) != hipSuccess)
-    return fail_dev(e, "hipMalloc");
-  if ((e = hipMalloc(&pl->d_ws2, sizeof(double) * (size_t)pl->ws2_need * pl->grid_max)) != hipSuccess)
-    return fail_dev(e, "hipMalloc");
-  if ((e = hipMalloc(&pl->d_cnt, sizeof(int) * 4)) != hipSuccess) return fail_dev(e, "hipMalloc");
-  if ((e = hipMemcpy(pl->d_ag, pl->agents.data(), sizeof(Agent) * pl->agents.size(), hipMemcpyHostToDevice)) != hipSuccess)
+  if ((e = alloc((void**)&pl->d_z1, sizeof(double) * pl->z1_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_z2, sizeof(double) * pl->z2_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_part, sizeof(double) * pl->part_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_team, sizeof(int) * pl->n_agents)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_cnt, sizeof(int) * 4)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  auto up = [&](void* d, const void* h, size_t bytes) {
+    return bytes ? hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) : hipSuccess;
+  };
+  if ((e = up(pl->d_ag, pl->agents.data(), sizeof(Agent) * pl->agents.size())) != hipSuccess ||
+      (e = up(pl->d_units, pl->units.data(), sizeof(Unit) * pl->units.size())) != hipSuccess ||
+      (e = up(pl->d_t1, pl->t1.data(), sizeof(Tile) * pl->t1.size())) != hipSuccess ||
+      (e = up(pl->d_t2, pl->t2.data(), sizeof(Tile) * pl->t2.size())) != hipSuccess ||
+      (e = up(pl->d_t3, pl->t3.data(), sizeof(Tile) * pl->t3.size())) != hipSuccess ||
+      (e = up(pl->d_iarena, pl->iarena.data(), sizeof(int32_t) * pl->iarena.size())) != hipSuccess)
     return fail_dev(e, "hipMemcpy");
-  if ((e = hipMemcpy(pl->d_order, pl->fpi_order.data(), sizeof(int) * pl->fpi_order.size(), hipMemcpyHostToDevice)) !=
-      hipSuccess)
-    return fail_dev(e, "hipMemcpy");
-  if (!pl->items.empty() &&
-      (e = hipMemcpy(pl->d_items, pl->items.data(), sizeof(Item) * pl->items.size(), hipMemcpyHostToDevice)) !=
-          hipSuccess)
-    return fail_dev(e, "hipMemcpy");
-  if (!pl->iarena.empty() && (e = hipMemcpy(pl->d_iarena, pl->iarena.data(), sizeof(int32_t) * pl->iarena.size(),
-                                            hipMemcpyHostToDevice)) != hipSuccess)
-    return fail_dev(e, "hipMemcpy");
+  if ((e = hipMemset(pl->d_team, 0, sizeof(int) * pl->n_agents)) != hipSuccess) return fail_dev(e, "hipMemset");
   if ((e = hipMemset(pl->d_cnt, 0, sizeof(int) * 4)) != hipSuccess) return fail_dev(e, "hipMemset");
   if ((e = hipDeviceSynchronize()) != hipSuccess) return fail_dev(e, "hipDeviceSynchronize");
   return PYMDP_HIP_OK;
@@ -1053,23 +1396,29 @@ extern "C" int pymdp_hip_ragged_plan_upload(void* plan) {
   return upload(pl);
 }
 
-extern "C" int pymdp_hip_ragged_tables(void* plan, int64_t* sizes8, void* agents_out, int32_t* order_out,
-                                       int32_t* items_out, int32_t* iarena_out) {
+extern "C" int pymdp_hip_ragged_tables(void* plan, int64_t* sizes16, void* agents_out, void* units_out,
+                                       void* tiles_out, int32_t* iarena_out) {
   Plan* pl = static_cast<Plan*>(plan);
   if (!pl) return einval("plan is NULL");
-  if (sizes8) {
-    sizes8[0] = (int64_t)sizeof(Agent);
-    sizes8[1] = (int64_t)sizeof(Item);
-    sizes8[2] = (int64_t)pl->items.size();
-    sizes8[3] = (int64_t)pl->iarena.size();
-    sizes8[4] = pl->grid_max;
-    sizes8[5] = 0;
-    sizes8[6] = 0;
-    sizes8[7] = 0;
+  if (sizes16) {
+    std::fill(sizes16, sizes16 + 16, 0);
+    sizes16[0] = (int64_t)sizeof(Agent);
+    sizes16[1] = (int64_t)sizeof(Unit);
+    sizes16[2] = (int64_t)pl->units.size();
+    sizes16[3] = (int64_t)sizeof(Tile);
+    sizes16[4] = (int64_t)pl->t1.size();
+    sizes16[5] = (int64_t)pl->t2.size();
+    sizes16[6] = (int64_t)pl->t3.size();
+    sizes16[7] = (int64_t)pl->iarena.size();
   }
   if (agents_out) std::memcpy(agents_out, pl->agents.data(), sizeof(Agent) * pl->agents.size());
-  if (order_out) std::copy(pl->fpi_order.begin(), pl->fpi_order.end(), order_out);
-  if (items_out) std::memcpy(items_out, pl->items.data(), sizeof(Item) * pl->items.size());
+  if (units_out) std::memcpy(units_out, pl->units.data(), sizeof(Unit) * pl->units.size());
+  if (tiles_out) {
+    Tile* t = static_cast<Tile*>(tiles_out);
+    std::copy(pl->t1.begin(), pl->t1.end(), t);
+    std::copy(pl->t2.begin(), pl->t2.end(), t + pl->t1.size());
+    std::copy(pl->t3.begin(), pl->t3.end(), t + pl->t1.size() + pl->t2.size());
+  }
   if (iarena_out) std::copy(pl->iarena.begin(), pl->iarena.end(), iarena_out);
   return PYMDP_HIP_OK;
 }
@@ -1085,26 +1434,36 @@ extern "C" int pymdp_hip_ragged_layout(void* plan, int64_t* out, int64_t n_out) 
   return PYMDP_HIP_OK;
 }
 
-extern "C" int pymdp_hip_ragged_info(void* plan, int64_t* out8) {
+extern "C" int pymdp_hip_ragged_info(void* plan, int64_t* out16) {
   Plan* pl = static_cast<Plan*>(plan);
-  if (!pl || !out8) return einval("plan or out is NULL");
-  out8[0] = pl->n_agents;
-  out8[1] = (int64_t)pl->items.size();
-  out8[2] = pl->grid1;
-  out8[3] = pl->grid2;
-  out8[4] = pl->hist_total;
-  out8[5] = pl->score_total;
-  out8[6] = pl->ws1_need;
-  out8[7] = pl->ws2_need;
+  if (!pl || !out16) return einval("plan or out is NULL");
+  std::fill(out16, out16 + 16, 0);
+  out16[0] = pl->n_agents;
+  out16[1] = (int64_t)pl->units.size();
+  out16[2] = (int64_t)pl->t1.size();
+  out16[3] = (int64_t)pl->t2.size();
+  out16[4] = (int64_t)pl->t3.size();
+  out16[5] = pl->grid1;
+  out16[6] = pl->hist_total;
+  out16[7] = pl->score_total;
+  out16[8] = pl->z1_total;
+  out16[9] = pl->z2_total;
+  out16[10] = pl->part_total;
+  out16[11] = pl->team_max;
+  out16[12] = (int64_t)pl->efe_macs;
+  out16[13] = (int64_t)pl->fpi_macs;
+  out16[14] = pl->grid_cap;
   return PYMDP_HIP_OK;
 }
 
 static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, float* out, int32_t* act, hipStream_t s,
-                         int grid1, int grid2) {
+                         int grid1) {
   Bufs b;
   b.ag = pl->d_ag;
-  b.fpi_order = pl->d_order;
-  b.items = pl->d_items;
+  b.units = pl->d_units;
+  b.t1 = pl->d_t1;
+  b.t2 = pl->d_t2;
+  b.t3 = pl->d_t3;
   b.iarena = pl->d_iarena;
   b.params = params;
   b.obs = obs;
@@ -1112,16 +1471,21 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
   b.act = act;
   b.hist = pl->d_hist;
   b.score = pl->d_score;
-  b.ws1 = pl->d_ws1;
-  b.ws2 = pl->d_ws2;
-  b.ws1_slot = pl->ws1_need;
-  b.ws2_slot = pl->ws2_need;
+  b.z1 = pl->d_z1;
+  b.z2 = pl->d_z2;
+  b.part = pl->d_part;
+  b.team_cnt = pl->d_team;
   b.cnt = pl->d_cnt;
   b.n_agents = (int)pl->n_agents;
-  b.n_items = (int)pl->items.size();
+  b.n_units = (int)pl->units.size();
+  b.n_t1 = (int)pl->t1.size();
+  b.n_t2 = (int)pl->t2.size();
+  b.n_t3 = (int)pl->t3.size();
   b.trace = pl->trace;
   hipLaunchKernelGGL(rg_fpi_kernel, dim3(grid1), dim3(kNT), 0, s, b);
-  hipLaunchKernelGGL(rg_efe_kernel, dim3(grid2), dim3(kNT), 0, s, b);
+  if (b.n_t1) hipLaunchKernelGGL(rg_e1_kernel, dim3(b.n_t1), dim3(kNT), 0, s, b);
+  if (b.n_t2) hipLaunchKernelGGL(rg_e2_kernel, dim3(b.n_t2), dim3(kNT), 0, s, b);
+  if (b.n_t3) hipLaunchKernelGGL(rg_e3_kernel, dim3(b.n_t3), dim3(kNT), 0, s, b);
   hipLaunchKernelGGL(rg_assemble_kernel, dim3((unsigned)pl->n_agents), dim3(kNT), 0, s, b);
   PMH_HIP_CHECK(hipGetLastError(), "ragged: ");
   return 0;
@@ -1131,6 +1495,7 @@ extern "C" int pymdp_hip_ragged_step(void* plan, const float* params, const int3
                                      void* stream) {
   Plan* pl = static_cast<Plan*>(plan);
   if (!pl) return einval("plan is NULL");
+  if (!pl->d_ag) return einval("plan is not uploaded");
   if (!params || !obs || !out || !act) return einval("arena pointer is NULL");
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev != pl->device) return einval("plan was created on another device");
@@ -1138,16 +1503,18 @@ extern "C" int pymdp_hip_ragged_step(void* plan, const float* params, const int3
     set_error("ragged: cannot initialise the log table");
     return PYMDP_HIP_EDEVICE;
   }
-  return ragged_launch(pl, params, obs, out, act, static_cast<hipStream_t>(stream), pl->grid1, pl->grid2);
+  return ragged_launch(pl, params, obs, out, act, static_cast<hipStream_t>(stream), pl->grid1);
 }
 
-// Times every (grid1, grid2) pair in {2, 4, 8} workgroups per CU (3 steps
-// each after one warm-up, hipEvents on `stream`) and keeps the fastest for
-// later steps.  Must not be called while `stream` is being captured.
+// Times the persistent FPI grid at 1, 2, ... workgroups per CU up to the
+// occupancy limit (3 steps each after one warm-up, hipEvents on `stream`)
+// and keeps the fastest for later steps.  Must not be called while `stream`
+// is being captured.
 extern "C" int pymdp_hip_ragged_autotune(void* plan, const float* params, const int32_t* obs, float* out, int32_t* act,
                                          void* stream, float* best_ms_out) {
   Plan* pl = static_cast<Plan*>(plan);
   if (!pl) return einval("plan is NULL");
+  if (!pl->d_ag) return einval("plan is not uploaded");
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
@@ -1156,39 +1523,48 @@ extern "C" int pymdp_hip_ragged_autotune(void* plan, const float* params, const 
     set_error("ragged: cannot initialise the log table");
     return PYMDP_HIP_EDEVICE;
   }
-  int cus = pl->grid_max / 8;
-  if (cus < 1) cus = 1;
+  int cus = 0;
+  PMH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device), "ragged: ");
   hipEvent_t e0, e1;
   PMH_HIP_CHECK(hipEventCreate(&e0), "ragged: ");
   PMH_HIP_CHECK(hipEventCreate(&e1), "ragged: ");
   float best = 1e30f;
-  int b1 = pl->grid1, b2 = pl->grid2;
-  for (int w1 : {2, 4, 8})
-    for (int w2 : {2, 4, 8}) {
-      const int g1 = (int)std::max<long long>(1, std::min<long long>({(long long)cus * w1, pl->n_agents, (long long)pl->grid_max}));
-      const int g2 = (int)std::max<long long>(
-          1, std::min<long long>({(long long)cus * w2, (long long)pl->items.size(), (long long)pl->grid_max}));
-      int rc = ragged_launch(pl, params, obs, out, act, s, g1, g2);
-      if (rc) return rc;
-      PMH_HIP_CHECK(hipEventRecord(e0, s), "ragged: ");
-      for (int r = 0; r < 3; ++r)
-        if ((rc = ragged_launch(pl, params, obs, out, act, s, g1, g2))) return rc;
-      PMH_HIP_CHECK(hipEventRecord(e1, s), "ragged: ");
-      PMH_HIP_CHECK(hipEventSynchronize(e1), "ragged: ");
-      float ms = 0.f;
-      PMH_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1), "ragged: ");
-      if (ms < best) {
-        best = ms;
-        b1 = g1;
-        b2 = g2;
-      }
+  int b1 = pl->grid1;
+  const int lo = std::min(pl->grid_cap, pl->team_max + 1);
+  for (int w = 1; w * cus <= pl->grid_cap + cus - 1; ++w) {
+    const int g1 = std::max(lo, std::min(pl->grid_cap, w * cus));
+    int rc = ragged_launch(pl, params, obs, out, act, s, g1);
+    if (rc) return rc;
+    PMH_HIP_CHECK(hipEventRecord(e0, s), "ragged: ");
+    for (int r = 0; r < 3; ++r)
+      if ((rc = ragged_launch(pl, params, obs, out, act, s, g1))) return rc;
+    PMH_HIP_CHECK(hipEventRecord(e1, s), "ragged: ");
+    PMH_HIP_CHECK(hipEventSynchronize(e1), "ragged: ");
+    float ms = 0.f;
+    PMH_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1), "ragged: ");
+    if (ms < best) {
+      best = ms;
+      b1 = g1;
     }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   pl->grid1 = b1;
-  pl->grid2 = b2;
   pl->tuned = true;
   if (best_ms_out) *best_ms_out = best / 3.0f;
+  return PYMDP_HIP_OK;
+}
+
+// 1 if a team barrier of a previous step gave up waiting (its agents' qs are
+// NaN), else 0.  Synchronises the device.
+extern "C" int pymdp_hip_ragged_status(void* plan, int32_t* flag_out) {
+  Plan* pl = static_cast<Plan*>(plan);
+  if (!pl || !flag_out) return einval("plan or flag_out is NULL");
+  if (!pl->d_cnt) return einval("plan is not uploaded");
+  int h[4] = {0, 0, 0, 0};
+  PMH_HIP_CHECK(hipDeviceSynchronize(), "ragged: ");
+  PMH_HIP_CHECK(hipMemcpy(h, pl->d_cnt, sizeof(h), hipMemcpyDeviceToHost), "ragged: ");
+  *flag_out = h[3];
   return PYMDP_HIP_OK;
 }
 

@@ -13,7 +13,7 @@ drop-in (pymdp_amd.ffi).
 
 Layout: one float32 parameter arena (every agent's A, B, C, D, E at offsets
 the plan chooses; A_m is stored with its dependency axes in the plan's
-contraction order), one int32 observation arena, one float32 output arena
+contraction order, B_f with its control axis first), one int32 observation arena, one float32 output arena
 (qs, G, q_pi per agent) and one int32 action arena.  A step allocates nothing
 and does not synchronise, so it can be captured in a HIP graph
 (RaggedBatch.capture).
@@ -93,9 +93,10 @@ class RaggedPlan:
         self.uploaded = False
 
     def info(self) -> dict:
-        out = np.zeros(8, np.int64)
+        out = np.zeros(16, np.int64)
         _native.check(_native.lib().pymdp_hip_ragged_info(self.handle, out.ctypes.data_as(_native._i64p)))
-        keys = ["agents", "items", "grid_fpi", "grid_efe", "hist_doubles", "score_doubles", "ws_fpi", "ws_efe"]
+        keys = ["agents", "units", "tiles1", "tiles2", "tiles3", "grid_fpi", "hist_doubles", "score_doubles",
+                "z1_doubles", "z2_doubles", "part_doubles", "team_max", "efe_macs", "fpi_macs", "grid_cap"]
         return dict(zip(keys, (int(x) for x in out)))
 
     def upload(self):
@@ -130,7 +131,8 @@ def pack_params(plan: RaggedPlan, agents: Sequence[dict], device) -> torch.Tenso
             arena[o:o + C.numel()] = C.to(device)
             mi += 1
         for f in range(len(ag["S"])):
-            B = ag["B"][f].contiguous().reshape(-1)
+            Bf = ag["B"][f]
+            B = Bf.permute(Bf.dim() - 1, *range(Bf.dim() - 1)).contiguous().reshape(-1)  # u-major
             o = int(lay["B_off"][fi])
             arena[o:o + B.numel()] = B.to(device)
             D = ag["D"][f].reshape(-1)
@@ -192,8 +194,16 @@ class RaggedBatch:
                                                                   ctypes.byref(ms)))
         return float(ms.value)
 
+    def status(self) -> int:
+        """1 if an FPI team barrier of an earlier step timed out (that
+        agent's posterior is NaN); synchronises the device."""
+        flag = ctypes.c_int32(0)
+        with torch.cuda.device(self.device):
+            _native.check(_native.lib().pymdp_hip_ragged_status(self.plan.handle, ctypes.byref(flag)))
+        return int(flag.value)
+
     def capture(self) -> torch.cuda.CUDAGraph:
-        """HIP graph of one step (three kernel nodes)."""
+        """HIP graph of one step (five kernel nodes)."""
         with torch.cuda.device(self.device):
             s = torch.cuda.Stream(self.device)
             s.wait_stream(torch.cuda.current_stream(self.device))

@@ -1,8 +1,8 @@
 """Host emulation of the batched ragged kernels (pymdp_amd/csrc/ragged.hip).
 
-Reads the plan's own device tables (agent records, work items, int arena)
-and replays the three kernels' index arithmetic in NumPy, asserting that
-every access stays inside its buffer (LDS arrays, workspace slots, arenas),
+Reads the plan's own device tables (agent records, FPI units, EFE tiles, int
+arena) and replays the kernels' index arithmetic in NumPy, asserting that
+every access stays inside its buffer (LDS arrays, stage buffers, arenas),
 and computing the same values, so that the plan and the kernels' indexing
 are checked on the CPU and their results compared with the oracle.
 
@@ -10,49 +10,55 @@ Test infrastructure only.
 """
 from __future__ import annotations
 
-import ctypes
-
 import numpy as np
 
 from pymdp_amd import _native
 
 MINVAL = np.float32(2.220446049250313e-16)
 kF, kM, kR, kT, kDim, kNT = 8, 8, 3, 4, 64, 256
-kHC, kFpiScr, kTileFloats, kHaPart = 16, 2048, 4096, 512
-TILE_LDS = kTileFloats + 2 * kDim
+kCap, kRowsCap, kTeamMax = 6144, 1024, 128
+kTile, kQTab, kCols = 4608, 64 * 65, 64
+kE1A, kE1H, kE1Q = 4352, 1024, 16 * 65
 
 AGENT_DT = np.dtype([
     ("F", "i4"), ("M", "i4"), ("T", "i4"), ("P", "i4"), ("G", "i4"), ("iters", "i4"), ("totS", "i4"),
-    ("has_E", "i4"), ("gamma", "f4"), ("scr_lds", "i4"), ("ll_total", "i4"), ("xa_off", "i4"),
-    ("xb_off", "i4"), ("scr_total", "i4"),
+    ("has_E", "i4"), ("team", "i4"), ("gamma", "f4"),
     ("S", "i4", (kF,)), ("S_off", "i4", (kF + 1,)), ("O", "i4", (kM,)), ("mod_group", "i4", (kM,)),
-    ("gR", "i4", (kM,)), ("gfac", "i4", (kM, kR)), ("gdim", "i4", (kM, kR)), ("gll", "i4", (kM,)),
+    ("gR", "i4", (kM,)), ("gfac", "i4", (kM, kR)), ("gdim", "i4", (kM, kR)),
     ("BR", "i4", (kF,)), ("Bdep", "i4", (kF, 3)), ("BK", "i4", (kF,)), ("U", "i4", (kF,)),
     ("C_time", "i4", (kM,)),
     ("H", "i4", (kT, kF)), ("hist_lvl", "i4", (kT, kF)), ("roll_lvl", "i4", (kT, kF)),
     ("score_base", "i4", (kT, kM)), ("score_stride", "i4", (kT, kM, kR)),
+    ("ed", "i4", (kM, 3)), ("ef", "i4", (kM, 3)), ("tstart", "i4", (kM, kT + 1)),
     ("K", "i8", (kM,)), ("A_off", "i8", (kM,)), ("B_off", "i8", (kF,)), ("C_off", "i8", (kM,)),
     ("D_off", "i8", (kF,)), ("E_off", "i8"),
     ("obs_off", "i8"), ("qs_off", "i8"), ("G_off", "i8"), ("qpi_off", "i8"), ("act_off", "i8"),
     ("hist_off", "i8"), ("roll_off", "i8"), ("p2h_off", "i8"), ("pol0_off", "i8"), ("score_off", "i8"),
-    ("logit_off", "i8"),
+    ("logit_off", "i8"), ("part_off", "i8"), ("z1_off", "i8", (kM,)), ("z2_off", "i8", (kM, kT)),
 ], align=True)
+
+UNIT_DT = np.dtype([("agent", "i4"), ("slice", "i4"), ("nseg", "i4"), ("pad", "i4"), ("g", "i4", (kM,)),
+                    ("r0", "i4", (kM,)), ("r1", "i4", (kM,)), ("lds", "i4", (kM,))])
+TILE_DT = np.dtype([("agent", "i4"), ("m", "i4"), ("t", "i4"), ("r0", "i4"), ("r1", "i4"), ("c0", "i4"),
+                    ("c1", "i4"), ("pad", "i4")])
 
 
 def tables(plan):
+    """(agents, units, (tiles1, tiles2, tiles3), iarena) copied from the plan."""
     L = _native.lib()
-    sizes = np.zeros(8, np.int64)
+    sizes = np.zeros(16, np.int64)
     _native.check(L.pymdp_hip_ragged_tables(plan.handle, sizes.ctypes.data_as(_native._i64p), None, None, None,
                                             None))
     assert sizes[0] == AGENT_DT.itemsize, (int(sizes[0]), AGENT_DT.itemsize)
-    assert sizes[1] == 20
+    assert sizes[1] == UNIT_DT.itemsize and sizes[3] == TILE_DT.itemsize
     ag = np.zeros(plan.n, AGENT_DT)
-    order = np.zeros(plan.n, np.int32)
-    items = np.zeros((int(sizes[2]), 5), np.int32)
-    iarena = np.zeros(max(1, int(sizes[3])), np.int32)
-    _native.check(L.pymdp_hip_ragged_tables(plan.handle, None, ag.ctypes.data, order.ctypes.data,
-                                            items.ctypes.data, iarena.ctypes.data))
-    return ag, order, items, iarena[:int(sizes[3])], int(sizes[4])
+    units = np.zeros(int(sizes[2]), UNIT_DT)
+    n1, n2, n3 = int(sizes[4]), int(sizes[5]), int(sizes[6])
+    tiles = np.zeros(n1 + n2 + n3, TILE_DT)
+    iarena = np.zeros(max(1, int(sizes[7])), np.int32)
+    _native.check(L.pymdp_hip_ragged_tables(plan.handle, None, ag.ctypes.data, units.ctypes.data,
+                                            tiles.ctypes.data, iarena.ctypes.data))
+    return ag, units, (tiles[:n1], tiles[n1:n1 + n2], tiles[n1 + n2:]), iarena[:int(sizes[7])]
 
 
 def _chk(idx, n, what):
@@ -70,96 +76,143 @@ def _ceil_log2(n):
     return l
 
 
+def _rows_lg(n):
+    return 0 if n <= 8 else _ceil_log2(n) - 3
+
+
 class Emu:
     """Replays one step.  `params`/`obs` are the packed arenas as NumPy."""
 
     def __init__(self, plan, params: np.ndarray, obs: np.ndarray):
         self.plan = plan
         self.info = plan.info()
-        self.ag, self.order, self.items, self.iarena, self.grid_max = tables(plan)
+        self.ag, self.units, (self.t1, self.t2, self.t3), self.iarena = tables(plan)
         self.params = params.astype(np.float32)
         self.obs = obs.astype(np.int32)
         self.hist = np.full(max(1, self.info["hist_doubles"]), np.nan)
         self.score = np.full(max(1, self.info["score_doubles"]), np.nan)
+        self.z1 = np.full(max(1, self.info["z1_doubles"]), np.nan)
+        self.z2 = np.full(max(1, self.info["z2_doubles"]), np.nan)
         self.out = np.full(max(1, int(plan.sizes[2])), np.nan, np.float32)
         self.act = np.full(max(1, int(plan.sizes[3])), -1, np.int32)
-        assert sorted(self.order.tolist()) == list(range(plan.n))
-        assert self.info["grid_fpi"] <= self.grid_max and self.info["grid_efe"] <= self.grid_max
+        check_units(self.ag, self.units, self.info)
 
     # ---------------------------------------------------------------- FPI
-    def fpi(self, ai: int):
+    def _segment(self, a, g, r0, nr, X, qv, part):
+        """rg_fpi_segment: X is the unit's LDS slice [kCap] at the segment."""
+        S_off = a["S_off"]
+        R = int(a["gR"][g])
+        n = int(a["gdim"][g][R - 1])
+        ld = n | 1
+        fl = int(a["gfac"][g][R - 1])
+        d1 = int(a["gdim"][g][1]) if R == 3 else 1
+        _chk(np.arange(nr), kRowsCap, "s_y")
+        rows = X(np.arange(nr)[:, None] * ld + np.arange(n)[None, :])  # [nr, n]
+        qc = qv[S_off[fl]:S_off[fl] + n]
+        r = r0 + np.arange(nr)
+        if R == 1:
+            w = np.ones(nr)
+        elif R == 2:
+            q0 = qv[S_off[a["gfac"][g][0]]:]
+            w = q0[_chk(r, int(a["gdim"][g][0]), "w q0")]
+        else:
+            q0 = qv[S_off[a["gfac"][g][0]]:]
+            q1 = qv[S_off[a["gfac"][g][1]]:]
+            i = r // d1
+            w = q0[_chk(i, int(a["gdim"][g][0]), "w q0")] * q1[_chk(r - i * d1, d1, "w q1")]
+        nrg = kNT // n
+        assert nrg * n <= kNT
+        part[S_off[fl]:S_off[fl] + n] += w @ rows
+        if R >= 2:
+            lt = _rows_lg(n)
+            assert (1 << lt) <= 64
+            y = rows @ qc
+            if R == 2:
+                f0 = int(a["gfac"][g][0])
+                _chk(r, int(a["S"][f0]), "m0 (R=2)")
+                part[S_off[f0] + r] += y
+            else:
+                i = r // d1
+                j = r - i * d1
+                i_lo, i_hi = r0 // d1, (r0 + nr - 1) // d1
+                assert 64 + d1 <= 128 and 128 + (i_hi - i_lo + 1) <= kNT
+                f0, f1 = int(a["gfac"][g][0]), int(a["gfac"][g][1])
+                np.add.at(part, S_off[f1] + j, q0[i] * y)
+                np.add.at(part, S_off[f0] + i, q1[j] * y)
+
+    def fpi_agent(self, ai, units):
         a = self.ag[ai]
         P = self.params
-        F, M, G = int(a["F"]), int(a["M"]), int(a["G"])
+        F, M = int(a["F"]), int(a["M"])
         totS = int(a["totS"])
         assert totS <= kF * kDim and M <= kM and F <= kF
         S, S_off = a["S"], a["S_off"]
-        scr_n = kFpiScr if a["scr_lds"] else self.info["ws_fpi"]
-        assert int(a["scr_total"]) <= scr_n
-        scr = np.zeros(scr_n)
+        team = int(a["team"])
+        assert len(units) == team and [int(u["slice"]) for u in units] == list(range(team))
         lp = np.zeros(totS)
         for f in range(F):
             d = P[_chk(int(a["D_off"][f]) + np.arange(S[f]), P.size, "D")]
             lp[S_off[f]:S_off[f] + S[f]] = np.log(np.maximum(d, MINVAL).astype(np.float64))
         o_idx = self.obs[_chk(int(a["obs_off"]) + np.arange(M), self.obs.size, "obs")]
-        for g in range(G):
-            R = int(a["gR"][g])
-            Kg = int(np.prod(a["gdim"][g][:R]))
-            e = np.arange(Kg)
-            prod = np.ones(Kg)
-            for m in range(M):
-                if a["mod_group"][m] != g:
-                    continue
-                o = int(o_idx[m])
-                if o < 0 or o >= a["O"][m]:
-                    continue
-                assert int(a["K"][m]) == Kg
-                x = P[_chk(int(a["A_off"][m]) + o * int(a["K"][m]) + e, P.size, "A (ll)")]
-                prod *= np.maximum(x, MINVAL).astype(np.float64)
-            scr[_chk(int(a["gll"][g]) + e, int(a["scr_total"]), "ll")] = np.log(prod)
-        log_q = np.zeros(totS)
-        qv = np.zeros(totS)
+        lds = []
+        for u in units:
+            x = np.full(kCap, np.nan)
+            for sg in range(int(u["nseg"])):
+                g = int(u["g"][sg])
+                R = int(a["gR"][g])
+                n = int(a["gdim"][g][R - 1])
+                ld = n | 1
+                r0, r1 = int(u["r0"][sg]), int(u["r1"][sg])
+                nr = r1 - r0
+                prod = np.ones((nr, n))
+                ej = (r0 + np.arange(nr))[:, None] * n + np.arange(n)[None, :]
+                for m in range(M):
+                    if a["mod_group"][m] != g:
+                        continue
+                    o = int(o_idx[m])
+                    if o < 0 or o >= a["O"][m]:
+                        continue
+                    Km = int(a["K"][m])
+                    assert ej.max() < Km
+                    v = P[_chk(int(a["A_off"][m]) + o * Km + ej, P.size, "A (ll)")]
+                    prod *= np.maximum(v, MINVAL).astype(np.float64)
+                dst = int(u["lds"][sg]) + np.arange(nr)[:, None] * ld + np.arange(n)[None, :]
+                x[_chk(dst, kCap, "s_x (ll)")] = np.log(prod)
+            lds.append(x)
 
         def softmax(src):
             out = np.zeros(totS)
             for f in range(F):
                 assert S[f] <= kDim
-                x = src[S_off[f]:S_off[f] + S[f]]
-                ex = np.exp(x - x.max())
+                xx = src[S_off[f]:S_off[f] + S[f]]
+                ex = np.exp(xx - xx.max())
                 out[S_off[f]:S_off[f] + S[f]] = ex / ex.sum()
             return out
 
-        for _ in range(int(a["iters"])):
+        log_q = np.zeros(totS)
+        for it in range(int(a["iters"])):
             qv = softmax(log_q)
-            log_q = lp.copy()
-            for g in range(G):
-                R = int(a["gR"][g])
-                src_off, src_len = int(a["gll"][g]), int(np.prod(a["gdim"][g][:R]))
-                for k in range(R):
-                    c = R - 1 - k
-                    n = int(a["gdim"][g][c])
-                    f = int(a["gfac"][g][c])
-                    lg = _ceil_log2(n)
-                    assert lg <= 6
-                    rows = int(np.prod(a["gdim"][g][:c])) if c > 0 else 1
-                    assert rows * n == src_len
-                    X = scr[_chk(src_off + np.arange(rows * n), int(a["scr_total"]), "stage src")].reshape(rows, n)
-                    if c == 0:
-                        w = np.ones(1)
-                    elif c == 1:
-                        w = qv[_chk(S_off[a["gfac"][g][0]] + np.arange(rows), totS, "w1")]
-                    else:
-                        d1 = int(a["gdim"][g][1])
-                        r = np.arange(rows)
-                        w = qv[_chk(S_off[a["gfac"][g][0]] + r // d1, totS, "w2a")] * \
-                            qv[_chk(S_off[a["gfac"][g][1]] + r % d1, totS, "w2b")]
-                    log_q[S_off[f]:S_off[f] + n] += w @ X
-                    if c > 0:
-                        dst_off = int(a["xb_off"]) if (k & 1) else int(a["xa_off"])
-                        lim = int(a["scr_total"]) if (k & 1) else int(a["xb_off"])
-                        _chk(dst_off + np.arange(rows), lim, "stage dst")
-                        scr[dst_off:dst_off + rows] = X @ qv[S_off[f]:S_off[f] + n]
-                        src_off, src_len = dst_off, rows
+            parts = []
+            for u, x in zip(units, lds):
+                part = np.zeros(totS)
+                for sg in range(int(u["nseg"])):
+                    base = int(u["lds"][sg])
+
+                    def X(idx, base=base, x=x):
+                        v = x[_chk(base + idx, kCap, "s_x")]
+                        assert np.isfinite(v).all(), "read of an unset LDS log-likelihood entry"
+                        return v
+
+                    self._segment(a, int(u["g"][sg]), int(u["r0"][sg]), int(u["r1"][sg]) - int(u["r0"][sg]),
+                                  X, qv, part)
+                parts.append(part)
+            if team > 1:
+                po = int(a["part_off"]) + (it & 1) * team * totS
+                _chk(po + np.arange(team * totS), self.info["part_doubles"], "team partials")
+            tot = np.zeros(totS)
+            for part in parts:
+                tot += part
+            log_q = lp + tot
         qv = softmax(log_q)
         self.out[_chk(int(a["qs_off"]) + np.arange(totS), self.out.size, "qs out")] = qv.astype(np.float32)
         h0 = int(a["hist_off"])
@@ -168,7 +221,19 @@ class Emu:
             for f in range(F):
                 Sf, rb, KB, Uf = int(S[f]), int(a["BR"][f]), int(a["BK"][f]), int(a["U"][f])
                 nh = int(a["H"][t][f])
-                Bf = P[_chk(int(a["B_off"][f]) + np.arange(Sf * KB * Uf), P.size, "B")].reshape(Sf, KB, Uf)
+                # LDS staging of the rollout: one-dependency parents at the end
+                # of s_x, whole action blocks (floats) before them
+                nv = 0
+                if rb == 1 and t > 0:
+                    d = int(a["Bdep"][f][0])
+                    if int(a["H"][t - 1][d]) * int(S[d]) <= kCap // 2:
+                        nv = int(a["H"][t - 1][d]) * int(S[d])
+                        _chk(kCap - nv + np.arange(nv), kCap, "s_par")
+                blkl = Sf * (KB | 1)
+                per = max(0, min(Uf, (2 * (kCap - nv)) // max(blkl, 1)))
+                if per:
+                    assert per * blkl <= 2 * (kCap - nv), "action blocks overlap the parents"
+                Bf = P[_chk(int(a["B_off"][f]) + np.arange(Sf * KB * Uf), P.size, "B")].reshape(Uf, Sf, KB)
                 for h in range(nh):
                     rec = self.iarena[_chk(int(a["roll_off"]) + int(a["roll_lvl"][t][f]) + h * (1 + rb)
                                            + np.arange(1 + rb), self.iarena.size, "roll rec")]
@@ -186,87 +251,121 @@ class Emu:
                         w = np.kron(w, qp)
                     assert w.size == KB
                     dst = h0 + int(a["hist_lvl"][t][f]) + h * Sf + np.arange(Sf)
-                    self.hist[_chk(dst, self.hist.size, "hist out")] = Bf[:, :, u].astype(np.float64) @ w
+                    self.hist[_chk(dst, self.hist.size, "hist out")] = Bf[u].astype(np.float64) @ w
 
-    # ---------------------------------------------------------------- EFE item
-    def efe_item(self, it):
-        ai, m, t, c0, c1 = (int(x) for x in it)
+    # ---------------------------------------------------------------- EFE stages
+    def _hcount(self, a, m, t, p):
+        f = int(a["ef"][m][p])
+        return 1 if f < 0 else int(a["H"][t][f])
+
+    def _hist_table(self, a, m, t, p, c0, c1):
+        d = int(a["ed"][m][p])
+        f = int(a["ef"][m][p])
+        _chk(np.arange((c1 - c0) * (d | 1)), kQTab, "s_q")
+        if f < 0:
+            assert d == 1
+            return np.ones((c1 - c0, 1))
+        assert d == int(a["S"][f])
+        idx = int(a["hist_off"]) + int(a["hist_lvl"][t][f]) + (c0 + np.arange(c1 - c0))[:, None] * d + \
+            np.arange(d)[None, :]
+        q = self.hist[_chk(idx, self.hist.size, "hist table")]
+        assert np.isfinite(q).all()
+        return q
+
+    def e1(self, tl):
+        ai, m, _, r0, r1, c0, c1 = (int(tl[k]) for k in ("agent", "m", "t", "r0", "r1", "c0", "c1"))
         a = self.ag[ai]
-        P = self.params
-        g = int(a["mod_group"][m])
-        R = int(a["gR"][g])
         O = int(a["O"][m])
         O1 = O + 1
+        d0, d1, d2 = (int(x) for x in a["ed"][m])
+        d01, ld2 = d0 * d1, d2 | 1
+        nr, nc = r1 - r0, c1 - c0
+        assert 1 <= nc <= 16 and 1 <= nr and r1 <= d01 and d2 <= 64
+        lda = ((d2 + 3) & ~3) + 1
+        _chk(np.array([(O * nr - 1) * lda + d2 - 1]), kE1A, "s_a (E1)")
+        _chk(np.array([(nr - 1) * lda + d2 - 1]), kE1H, "s_h (E1)")
+        _chk(np.array([15 * lda + ((d2 + 3) & ~3) - 1]), kE1Q, "s_q (E1)")
+        P = nr * d2
+        G = max(1, min(O, kNT // P))
+        if G > 1:
+            _chk(np.array([P * G - 1]), kNT, "s_hap (E1)")
         K = int(a["K"][m])
-        n = int(a["gdim"][g][R - 1])
-        pre = K // n
-        assert pre * n == K
-        hc = c1 - c0
-        assert 1 <= hc <= kHC and c1 <= a["H"][t][a["gfac"][g][R - 1]]
-        h0 = int(a["hist_off"])
-        fR = int(a["gfac"][g][R - 1])
-        qh_idx = h0 + int(a["hist_lvl"][t][fR]) + c0 * n + np.arange(hc * n)
-        _chk(np.arange(hc * n), kHC * kDim, "s_qh")
-        qh = self.hist[_chk(qh_idx, self.hist.size, "qh")].reshape(hc, n)
-        assert np.isfinite(qh).all()
-        ws_n = self.info["ws_efe"]
-        Z = np.full(ws_n, np.nan)
-        np1 = n + 1
-        pt = kTileFloats // (O * np1)
-        pt = min(pt, kHaPart // np1)
-        pt = max(pt, 1)
-        pt = min(pt, pre)
-        for p0 in range(0, pre, pt):
-            np_ = min(pt, pre - p0)
-            ncol = np_ * n
-            oc = min(max(kNT // ncol, 1), O)
-            _chk(np.array([0, (O * np_ - 1) * np1 + n - 1]), TILE_LDS, "s_tile")
-            _chk(np.array([0, (np_ - 1) * np1 + n - 1]), kHaPart, "s_ha")
-            _chk(np.array([0, oc * ncol - 1]), kHaPart, "s_hap")
-            gidx = int(a["A_off"][m]) + np.arange(O)[:, None] * K + p0 * n + np.arange(np_ * n)[None, :]
-            tile = P[_chk(gidx, P.size, "A tile")].reshape(O, np_, n).astype(np.float64)
-            t32 = tile.astype(np.float32)
-            xl = np.where(t32 == 0, 0.0, t32.astype(np.float64) * np.log(np.maximum(t32, MINVAL).astype(np.float64)))
-            ha = xl.sum(0)  # [np, n]
-            X = np.concatenate([tile, ha[None]], 0)  # [O1, np, n]
-            Z1 = np.einsum("opi,hi->hop", X, qh)  # [hc, O1, np]
-            zi = (np.arange(hc)[:, None, None] * O1 + np.arange(O1)[None, :, None]) * pre + p0 + \
-                np.arange(np_)[None, None, :]
-            Z[_chk(zi, ws_n, "Z1")] = Z1
-        zin_off, Bin, rows_in = 0, hc, pre
-        for j in range(R - 2, -1, -1):
-            nj = int(a["gdim"][g][j])
-            fj = int(a["gfac"][g][j])
-            Hj = int(a["H"][t][fj])
-            qj = self.hist[_chk(h0 + int(a["hist_lvl"][t][fj]) + np.arange(Hj * nj), self.hist.size, "qj")]
-            qj = qj.reshape(Hj, nj)
-            rows_out = rows_in // nj
-            assert rows_out * nj == rows_in
-            rt = min((TILE_LDS * 4 // 8) // (nj + 1), O1 * rows_out)
-            _chk(np.array([0, (rt - 1) * (nj + 1) + nj - 1]), TILE_LDS * 4 // 8, "stage tile")
-            _chk(np.array([0, min(kHC, Hj) * nj - 1]), kHC * kDim, "stage qh")
-            zout_off = zin_off + Bin * O1 * rows_in
-            zin = Z[_chk(zin_off + np.arange(Bin * O1 * rows_in), ws_n, "zin")].reshape(Bin, O1, rows_out, nj)
-            assert np.isfinite(zin).all()
-            zo = np.einsum("borj,hj->bhor", zin, qj)
-            _chk(zout_off + np.arange(Bin * Hj * O1 * rows_out), ws_n, "zout")
-            Z[zout_off:zout_off + zo.size] = zo.reshape(-1)
-            zin_off, Bin, rows_in = zout_off, Bin * Hj, rows_out
-        assert rows_in == 1
-        zf = Z[zin_off:zin_off + Bin * O1].reshape(Bin, O1)
-        assert np.isfinite(zf).all()
+        assert K == d01 * d2
+        idx = int(a["A_off"][m]) + np.arange(O)[:, None, None] * K + (r0 + np.arange(nr))[None, :, None] * d2 + \
+            np.arange(d2)[None, None, :]
+        At = self.params[_chk(idx, self.params.size, "A (E1)")]  # [O, nr, d2]
+        xl = np.where(At == 0, np.float32(0), At * np.log(np.maximum(At, MINVAL)).astype(np.float32))
+        ha = xl.astype(np.float64).sum(0)
+        X = np.concatenate([At.astype(np.float64), ha[None]], 0)  # [O1, nr, d2]
+        T = int(a["T"])
+        f = int(a["ef"][m][2])
+        cols = []
+        for n in range(c0, c1):
+            t = 0
+            while t + 1 < T and n >= a["tstart"][m][t + 1]:
+                t += 1
+            h = n - int(a["tstart"][m][t])
+            assert 0 <= h < a["H"][t][f]
+            cols.append(self._hist_table(a, m, t, 2, h, h + 1)[0])
+        Q = np.stack(cols)  # [nc, d2]
+        Y = np.einsum("ork,ck->cor", X, Q)  # [nc, O1, nr]
+        zi = int(a["z1_off"][m]) + ((c0 + np.arange(nc))[:, None, None] * O1 + np.arange(O1)[None, :, None]) * d01 + \
+            r0 + np.arange(nr)[None, None, :]
+        N1 = int(a["tstart"][m][T])
+        _chk(zi - int(a["z1_off"][m]), N1 * O1 * d01, "Z1 (block)")
+        self.z1[_chk(zi, self.z1.size, "Z1")] = Y
+
+    def e2(self, tl):
+        ai, m, t, r0, r1, c0, c1 = (int(tl[k]) for k in ("agent", "m", "t", "r0", "r1", "c0", "c1"))
+        a = self.ag[ai]
+        O1 = int(a["O"][m]) + 1
+        d0, d1, _ = (int(x) for x in a["ed"][m])
+        assert int(a["gR"][a["mod_group"][m]]) >= 2
+        ld1 = d1 | 1
+        nr, nc = r1 - r0, c1 - c0
+        H1, H2 = self._hcount(a, m, t, 1), self._hcount(a, m, t, 2)
+        assert r1 <= H2 * O1 * d0 and c1 <= H1 and nc <= kCols
+        _chk(np.array([(nr - 1) * ld1 + d1 - 1]), kTile, "s_t (E2)")
+        src = int(a["z1_off"][m]) + (int(a["tstart"][m][t]) * O1 * d0 + r0) * d1 + np.arange(nr * d1)
+        Z = self.z1[_chk(src, self.z1.size, "Z1 (E2 read)")].reshape(nr, d1)
+        assert np.isfinite(Z).all(), "E2 reads an unset stage-1 value"
+        Q = self._hist_table(a, m, t, 1, c0, c1)  # [nc, d1]
+        Y = Z @ Q.T  # [nr, nc]
+        Rr = r0 + np.arange(nr)
+        od = O1 * d0
+        h2 = Rr // od
+        oi = Rr - h2 * od
+        zi = int(a["z2_off"][m][t]) + ((h2[:, None] * H1 + c0 + np.arange(nc)[None, :]) * od + oi[:, None])
+        self.z2[_chk(zi, self.z2.size, "Z2")] = Y
+
+    def e3(self, tl):
+        ai, m, t, r0, r1, c0, c1 = (int(tl[k]) for k in ("agent", "m", "t", "r0", "r1", "c0", "c1"))
+        a = self.ag[ai]
+        O = int(a["O"][m])
+        O1 = O + 1
+        d0 = int(a["ed"][m][0])
+        ld0 = d0 | 1
+        np_, nh = r1 - r0, c1 - c0
+        H0, H1, H2 = (self._hcount(a, m, t, p) for p in range(3))
+        assert r1 <= H2 * H1 and c1 <= H0 and nh <= kCols
+        _chk(np.array([(np_ * O1 - 1) * ld0 + d0 - 1]), kTile, "s_t (E3)")
+        R = int(a["gR"][a["mod_group"][m]])
+        buf = self.z1 if R == 1 else self.z2
+        src = int(a["z2_off"][m][t]) + r0 * O1 * d0 + np.arange(np_ * O1 * d0)
+        Z = buf[_chk(src, buf.size, "Z2 (E3 read)")].reshape(np_, O1, d0)
+        assert np.isfinite(Z).all(), "E3 reads an unset stage-2 value"
+        Q = self._hist_table(a, m, t, 0, c0, c1)  # [nh, d0]
+        qo = np.einsum("poi,hi->pho", Z, Q)  # [np, nh, O1]
         coff = int(a["C_off"][m]) + (t * O if a["C_time"][m] else 0)
-        C = P[_chk(coff + np.arange(O), P.size, "C")].astype(np.float64)
-        q = zf[:, :O]
+        C = self.params[_chk(coff + np.arange(O), self.params.size, "C")].astype(np.float64)
+        q = qo[..., :O]
         qf = q.astype(np.float32)
-        lq = np.log(np.maximum(qf, MINVAL).astype(np.float64))
-        ent = -np.where(qf != 0, q * lq, 0.0).sum(1)
-        v = ent + q @ C + zf[:, O]
-        stride = int(a["score_stride"][t][m][R - 1])
-        base = int(a["score_off"]) + int(a["score_base"][t][m]) + c0 * stride
-        blk = int(a["score_base"][t][m]) + int(np.prod([a["H"][t][a["gfac"][g][j]] for j in range(R)]))
-        _chk(base + np.arange(Bin), int(a["score_off"]) + blk, "score (block)")
-        self.score[_chk(base + np.arange(Bin), self.score.size, "score")] = v
+        lq = np.log(np.maximum(qf, MINVAL)).astype(np.float32).astype(np.float64)
+        v = (q * C).sum(-1) - np.where(qf != 0, q * lq, 0.0).sum(-1) + qo[..., O]
+        base = int(a["score_off"]) + int(a["score_base"][t][m])
+        si = base + (r0 + np.arange(np_))[:, None] * H0 + c0 + np.arange(nh)[None, :]
+        _chk(si - base, H0 * H1 * H2, "score (block)")
+        self.score[_chk(si, self.score.size, "score")] = v
 
     # ---------------------------------------------------------------- assemble
     def assemble(self, ai):
@@ -299,23 +398,75 @@ class Emu:
         self.act[_chk(int(a["act_off"]) + np.arange(F), self.act.size, "act")] = pol0
 
     def run(self):
-        for ai in self.order:
-            self.fpi(int(ai))
-        for it in self.items:
-            self.efe_item(it)
+        by_agent = {}
+        for u in self.units:
+            by_agent.setdefault(int(u["agent"]), []).append(u)
+        for ai, us in by_agent.items():
+            self.fpi_agent(ai, us)
+        for tl in self.t1:
+            self.e1(tl)
+        for tl in self.t2:
+            self.e2(tl)
+        for tl in self.t3:
+            self.e3(tl)
         for ai in range(self.plan.n):
             self.assemble(ai)
         return self
 
 
-def check_bounds_only(plan, total_hist=None):
-    """Index-range checks of every agent and item of a (large) plan without
-    data: the extreme index of each access, from the same formulas."""
+def check_units(ag, units, info):
+    """FPI units: every agent's stage-0 rows covered exactly once by units of
+    its team, segments inside the LDS slice, teams consecutive in the queue,
+    the largest team below the resident grid."""
+    seen = {}
+    last_agent, order = None, []
+    for u in units:
+        ai = int(u["agent"])
+        if ai != last_agent:
+            assert ai not in order, f"units of agent {ai} are not consecutive"
+            order.append(ai)
+            last_agent = ai
+        a = ag[ai]
+        assert 1 <= int(u["nseg"]) <= kM
+        used = []
+        for sg in range(int(u["nseg"])):
+            g = int(u["g"][sg])
+            assert 0 <= g < int(a["G"])
+            R = int(a["gR"][g])
+            n = int(a["gdim"][g][R - 1])
+            ld = n | 1
+            rows = int(np.prod(a["gdim"][g][:R - 1])) if R > 1 else 1
+            r0, r1 = int(u["r0"][sg]), int(u["r1"][sg])
+            assert 0 <= r0 < r1 <= rows and r1 - r0 <= kRowsCap
+            lo, hi = int(u["lds"][sg]), int(u["lds"][sg]) + (r1 - r0) * ld
+            assert 0 <= lo and hi <= kCap
+            used.append((lo, hi))
+            seen.setdefault((ai, g), []).append((r0, r1))
+        used.sort()
+        for (a0, b0), (a1, _) in zip(used, used[1:]):
+            assert b0 <= a1, "overlapping LDS segments"
+    assert len(order) == len(ag)
+    for ai, a in enumerate(ag):
+        assert 1 <= int(a["team"]) <= kTeamMax
+        for g in range(int(a["G"])):
+            R = int(a["gR"][g])
+            rows = int(np.prod(a["gdim"][g][:R - 1])) if R > 1 else 1
+            segs = sorted(seen[(ai, g)])
+            assert segs[0][0] == 0 and segs[-1][1] == rows
+            for (x0, x1), (y0, _) in zip(segs, segs[1:]):
+                assert x1 == y0
+        if int(a["team"]) > 1:
+            assert int(a["part_off"]) + 2 * int(a["team"]) * int(a["totS"]) <= info["part_doubles"]
+
+
+def check_bounds_only(plan):
+    """Index-range checks of every agent, unit and tile of a (large) plan
+    without data: the extreme index of each access, from the same formulas."""
     info = plan.info()
-    ag, order, items, iarena, grid_max = tables(plan)
+    ag, units, (t1, t2, t3), iarena = tables(plan)
+    check_units(ag, units, info)
     params_n = int(plan.sizes[0])
     for a in ag:
-        assert int(a["scr_total"]) <= (kFpiScr if a["scr_lds"] else info["ws_fpi"])
         assert int(a["totS"]) <= kF * kDim
         for m in range(int(a["M"])):
             assert int(a["A_off"][m]) + int(a["O"][m]) * int(a["K"][m]) <= params_n
@@ -327,29 +478,48 @@ def check_bounds_only(plan, total_hist=None):
         assert int(a["logit_off"]) + int(a["P"]) <= info["score_doubles"]
         assert int(a["p2h_off"]) + int(a["P"]) * int(a["T"]) * int(a["F"]) <= iarena.size
         assert int(a["pol0_off"]) + int(a["P"]) * int(a["F"]) <= iarena.size
-    for it in items:
-        ai, m, t, c0, c1 = (int(x) for x in it)
-        a = ag[ai]
-        g = int(a["mod_group"][m])
-        R = int(a["gR"][g])
+        T = int(a["T"])
+        for m in range(int(a["M"])):
+            d0, d1, d2 = (int(x) for x in a["ed"][m])
+            O1 = int(a["O"][m]) + 1
+            assert d0 * d1 * d2 == int(a["K"][m])
+            N1 = int(a["tstart"][m][T])
+            assert int(a["z1_off"][m]) + N1 * O1 * d0 * d1 <= info["z1_doubles"]
+            R = int(a["gR"][a["mod_group"][m]])
+            for t in range(T):
+                H = [1 if a["ef"][m][p] < 0 else int(a["H"][t][a["ef"][m][p]]) for p in range(3)]
+                if R == 1:
+                    assert int(a["z2_off"][m][t]) + H[2] * O1 <= info["z1_doubles"]
+                else:
+                    assert int(a["z2_off"][m][t]) + H[2] * H[1] * O1 * d0 <= info["z2_doubles"]
+                blk = int(a["score_base"][t][m]) + H[0] * H[1] * H[2]
+                assert int(a["score_off"]) + blk <= info["score_doubles"]
+    for tl in t1:
+        a = ag[int(tl["agent"])]
+        m = int(tl["m"])
         O1 = int(a["O"][m]) + 1
-        K = int(a["K"][m])
-        n = int(a["gdim"][g][R - 1])
-        hc = c1 - c0
-        assert 1 <= hc <= kHC
-        need = hc * O1 * (K // n)
-        Bin, rows = hc, K // n
-        for j in range(R - 2, -1, -1):
-            Hj = int(a["H"][t][a["gfac"][g][j]])
-            rows //= int(a["gdim"][g][j])
-            Bin *= Hj
-            need += Bin * O1 * rows
-        assert need <= info["ws_efe"], (need, info["ws_efe"])
-        np1 = n + 1
-        pt = max(1, min(kTileFloats // ((O1 - 1) * np1), kHaPart // np1))
-        pt = min(pt, K // n)
-        assert ((O1 - 1) * pt - 1) * np1 + n - 1 < TILE_LDS
-        assert (pt - 1) * np1 + n - 1 < kHaPart
-        ncol = pt * n
-        assert min(max(kNT // ncol, 1), O1 - 1) * ncol <= kHaPart
+        d0, d1, d2 = (int(x) for x in a["ed"][m])
+        nr, nc = int(tl["r1"]) - int(tl["r0"]), int(tl["c1"]) - int(tl["c0"])
+        assert 1 <= nr and int(tl["r1"]) <= d0 * d1 and 1 <= nc <= kCols
+        assert int(tl["c1"]) <= int(a["tstart"][m][int(a["T"])])
+        lda = ((d2 + 3) & ~3) + 1
+        assert nc <= 16 and int(a["O"][m]) * nr * lda <= kE1A and nr * lda <= kE1H
+    for tl in t2:
+        a = ag[int(tl["agent"])]
+        m, t = int(tl["m"]), int(tl["t"])
+        O1 = int(a["O"][m]) + 1
+        d0, d1, _ = (int(x) for x in a["ed"][m])
+        H = [1 if a["ef"][m][p] < 0 else int(a["H"][t][a["ef"][m][p]]) for p in range(3)]
+        nr, nc = int(tl["r1"]) - int(tl["r0"]), int(tl["c1"]) - int(tl["c0"])
+        assert int(tl["r1"]) <= H[2] * O1 * d0 and int(tl["c1"]) <= H[1] and 1 <= nc <= kCols
+        assert nr * (d1 | 1) <= kTile and nc * (d1 | 1) <= kQTab
+    for tl in t3:
+        a = ag[int(tl["agent"])]
+        m, t = int(tl["m"]), int(tl["t"])
+        O1 = int(a["O"][m]) + 1
+        d0 = int(a["ed"][m][0])
+        H = [1 if a["ef"][m][p] < 0 else int(a["H"][t][a["ef"][m][p]]) for p in range(3)]
+        np_, nh = int(tl["r1"]) - int(tl["r0"]), int(tl["c1"]) - int(tl["c0"])
+        assert int(tl["r1"]) <= H[2] * H[1] and int(tl["c1"]) <= H[0] and 1 <= nh <= kCols
+        assert np_ * O1 * (d0 | 1) <= kTile and nh * (d0 | 1) <= kQTab
     return info

@@ -92,6 +92,22 @@ def test_ragged_wide_shapes_parity(dev):
         _check_agent(rb, i, use_dedup=True)
 
 
+def test_ragged_team_agents_parity(dev):
+    """Agents whose log-likelihood spans several FPI units: the units of one
+    agent exchange partial marginals every sweep (team barrier)."""
+    w = workloads.ragged(seed=0, device=dev, n_agents=5, state_range=(20, 40), obs_range=(2, 6), max_policies=64,
+                         max_A_bytes=1 << 22, max_modalities=3)
+    print("ragged teams: 5 agents", flush=True)
+    rb = ragged.RaggedBatch(w["agents"], num_iter=16, device=dev)
+    assert rb.plan.info()["team_max"] > 2
+    for _ in range(2):
+        rb.step()
+    torch.cuda.synchronize()
+    assert rb.status() == 0
+    for i in range(5):
+        _check_agent(rb, i, use_dedup=True)
+
+
 def test_ragged_full_batch(dev):
     w = workloads.ragged(seed=0, device=dev, n_agents=4096)
     print("ragged full: 4096 agents", flush=True)
@@ -100,6 +116,8 @@ def test_ragged_full_batch(dev):
     torch.cuda.synchronize()
     out1 = rb.out.clone()
     act1 = rb.act.clone()
+    assert rb.status() == 0
+    assert rb.plan.info()["team_max"] > 1
     assert torch.isfinite(out1).all()
     for i in range(0, 4096, 7):
         qs, G, q_pi, act = rb.agent_outputs(i)

@@ -91,14 +91,23 @@ def test_full_plan_bounds():
     w = workloads.ragged(seed=0, n_agents=4096, tensors=False)
     plan = ragged.RaggedPlan(w["agents"], 16)
     info = E.check_bounds_only(plan)
-    assert info["agents"] == 4096 and info["items"] > 4096
-    ag, order, items, iarena, _ = E.tables(plan)
-    # work queues are ordered by decreasing cost and cover everything once
-    assert sorted(order.tolist()) == list(range(4096))
-    per = np.zeros(4096, np.int64)
-    for it in items:
-        per[it[0]] += 1
-    assert (per >= 1).all()
+    assert info["agents"] == 4096 and info["units"] >= 4096 and info["team_max"] > 1
+    ag, units, (t1, t2, t3), iarena = E.tables(plan)
+    # FPI units: teams first (largest first), every agent present; the EFE
+    # stage tiles cover every modality of every agent at every level
+    teams = [int(u["agent"]) for u in units if ag[int(u["agent"])]["team"] > 1]
+    assert teams == [int(u["agent"]) for u in units[:len(teams)]]
+    assert sorted(set(int(u["agent"]) for u in units)) == list(range(4096))
+    per = np.zeros((4096, 8), np.int64)
+    for tl in t1:
+        per[tl["agent"], tl["m"]] += 1
+    for i, a in enumerate(w["agents"]):
+        assert (per[i, :len(a["O"])] >= 1).all() and (per[i, len(a["O"]):] == 0).all()
+    lv = np.zeros((4096, 8, 4), np.int64)
+    for tl in t3:
+        lv[tl["agent"], tl["m"], tl["t"]] += 1
+    for i, a in enumerate(w["agents"]):
+        assert (lv[i, :len(a["O"]), :a["T"]] >= 1).all()
 
 
 def _one_agent(**over):
@@ -141,3 +150,31 @@ def test_plan_layout_is_disjoint():
     for (o0, n0), (o1, _) in zip(spans, spans[1:]):
         assert o0 + n0 <= o1
     assert spans[-1][0] + spans[-1][1] <= int(plan.sizes[0])
+
+
+def test_emulated_teams_match_oracle():
+    """Agents whose log-likelihood spans several FPI units (a team exchanging
+    partial marginals every sweep), with rank-1, -2 and -3 modalities."""
+    w = workloads.ragged(seed=0, n_agents=5, state_range=(20, 40), obs_range=(2, 6), max_policies=64,
+                         max_A_bytes=1 << 22, max_modalities=3)
+    ags = w["agents"]
+    plan, em = _emulate(ags)
+    teams = [int(a["team"]) for a in em.ag]
+    assert max(teams) > 2, teams
+    ranks = {len(d) for a in ags for d in a["A_dependencies"]}
+    assert {1, 2, 3} <= ranks, ranks
+    lay = plan.layout
+    for i, a in enumerate(ags):
+        A = [x.numpy() for x in a["A"]]
+        ob = [np.eye(o, dtype=np.float32)[x] for o, x in zip(a["O"], a["obs"])]
+        q64 = R.run_factorized_fpi(A, ob, [x.numpy() for x in a["D"]], a["A_dependencies"], num_iter=16,
+                                   dt=np.float64)
+        S = a["S"]
+        qs = np.split(em.out[lay["qs_off"][i]:lay["qs_off"][i] + sum(S)], np.cumsum(S)[:-1])
+        for x, y in zip(qs, q64):
+            assert np.abs(x - y).max() <= 1e-6
+        pm = a["policies"]
+        G64 = DR.neg_efe_dedup(pm, [x.astype(np.float64) for x in qs], A, [x.numpy() for x in a["B"]],
+                               [x.numpy() for x in a["C"]], a["A_dependencies"], a["B_dependencies"], dt=np.float64)
+        G = em.out[lay["G_off"][i]:lay["G_off"][i] + pm.shape[0]]
+        assert np.abs(G - G64).max() <= 2e-6
