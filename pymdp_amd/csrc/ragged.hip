@@ -60,9 +60,10 @@ constexpr int kNW = kNT / kWave;
 constexpr int kCap = 6144;        // log-likelihood doubles one FPI unit keeps in LDS
 constexpr int kRowsCap = 1024;    // rows of one FPI segment
 constexpr int kTeamMax = 128;     // FPI units of one agent
-constexpr int kTile = 4608;       // doubles of an EFE operand tile in LDS
-constexpr int kQTab = 64 * 65;    // doubles of an EFE history table in LDS
-constexpr int kCols = 64;         // histories per EFE tile
+constexpr int kTile = 4608;       // doubles of a stage-3 operand tile in LDS
+constexpr int kTile2 = 2304;      // doubles of a stage-2 operand tile in LDS
+constexpr int kCols = 32;         // histories per stage-2 / stage-3 tile
+constexpr int kQTab = kCols * 65;  // doubles of a stage-2 / stage-3 history table in LDS
 constexpr int kE3Combos = 128;    // history combinations scored per E3 tile
 constexpr long long kSpinMax = 1LL << 22;
 
@@ -123,22 +124,29 @@ struct Bufs {
   int n_agents, n_units, n_t1, n_t2, n_t3;
 };
 
-__device__ __forceinline__ float xlogx_f(float x) { return x == 0.0f ? 0.0f : x * logf(fmaxf(x, kMinVal)); }
+// ln of a float clipped at MINVAL (always a normal float) from the hardware
+// log2 (v_log_f32, about 1 ulp) times ln 2: a few instructions instead of the
+// library logf; the outcome entropies and sum_o A ln A it feeds are summed
+// in FP64.
+__device__ __forceinline__ float ln_clip_f(float x) {
+  return __builtin_amdgcn_logf(fmaxf(x, kMinVal)) * 0.693147180559945309f;
+}
+__device__ __forceinline__ float xlogx_f(float x) { return x == 0.0f ? 0.0f : x * ln_clip_f(x); }
 
-// dst(e) = src(e) for e in [0, n): every thread issues eight global loads
-// before it stores any of them (one memory round trip per eight elements
-// instead of one per element).
-template <typename T, typename Ld, typename St>
+// dst(e) = src(e) for e in [0, n): every thread issues NB global loads
+// before it stores any of them (one memory round trip per NB elements per
+// thread instead of one per element).
+template <typename T, int NB = 8, typename Ld, typename St>
 __device__ __forceinline__ void rg_stage(long long n, int tid, Ld ld, St st) {
-  for (long long e0 = tid; e0 < n; e0 += 8 * kNT) {
-    T v[8];
+  for (long long e0 = tid; e0 < n; e0 += NB * kNT) {
+    T v[NB];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < NB; ++q) {
       const long long e = e0 + (long long)q * kNT;
       if (e < n) v[q] = ld(e);
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < NB; ++q) {
       const long long e = e0 + (long long)q * kNT;
       if (e < n) st(e, v[q]);
     }
@@ -179,14 +187,20 @@ __device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const doub
   const double* q0 = R >= 2 ? s_qv + a.S_off[a.gfac[g][0]] : nullptr;
   const double* q1 = R == 3 ? s_qv + a.S_off[a.gfac[g][1]] : nullptr;
   const int d1 = R == 3 ? a.gdim[g][1] : 1;
-  // row dot products (R >= 2)
+  // row dot products (R >= 2): 2^lt lanes per row, two accumulators each
   if (R >= 2) {
     const int lt = rows_lg(n), tpr = 1 << lt;
     for (int e = tid; e < nr * tpr; e += kNT) {
       const int rl = e >> lt, sub = e & (tpr - 1);
-      double v = 0.0;
-      for (int k = sub; k < n; k += tpr) v = fma(X[rl * ld + k], qc[k], v);
-      v = group_sum_low(v, lt);
+      const double* x = X + rl * ld;
+      double v0 = 0.0, v1 = 0.0;
+      int k = sub;
+      for (; k + tpr < n; k += 2 * tpr) {
+        v0 = fma(x[k], qc[k], v0);
+        v1 = fma(x[k + tpr], qc[k + tpr], v1);
+      }
+      if (k < n) v0 = fma(x[k], qc[k], v0);
+      double v = group_sum_low(v0 + v1, lt);
       if (sub == 0) {
         if (R == 2)
           s_part[a.S_off[a.gfac[g][0]] + r0 + rl] += v;
@@ -195,57 +209,77 @@ __device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const doub
       }
     }
   }
-  // weighted column sums
+  // weighted column sums: thread (col, rg) over rows rg, rg + nrg, ...;
+  // the row's (i, j) advances incrementally (no division per row)
   const int nrg = kNT / n;
   const int col = tid % n, rg = tid / n;
   if (rg < nrg) {
-    double acc = 0.0;
-    for (int rl = rg; rl < nr; rl += nrg) {
-      const int r = r0 + rl;
-      double w = 1.0;
-      if (R == 2) {
-        w = q0[r];
-      } else if (R == 3) {
-        const int i = r / d1;
-        w = q0[i] * q1[r - i * d1];
+    double acc0 = 0.0, acc1 = 0.0;
+    if (R == 1) {
+      if (rg == 0) acc0 = X[col];
+    } else if (R == 2) {
+      int rl = rg;
+      for (; rl + nrg < nr; rl += 2 * nrg) {
+        acc0 = fma(q0[r0 + rl], X[rl * ld + col], acc0);
+        acc1 = fma(q0[r0 + rl + nrg], X[(rl + nrg) * ld + col], acc1);
       }
-      acc = fma(w, X[rl * ld + col], acc);
+      if (rl < nr) acc0 = fma(q0[r0 + rl], X[rl * ld + col], acc0);
+    } else {
+      const int di = nrg / d1, dj = nrg - di * d1;
+      int i = (r0 + rg) / d1, j = (r0 + rg) - i * d1;
+      for (int rl = rg; rl < nr; rl += nrg) {
+        acc0 = fma(q0[i] * q1[j], X[rl * ld + col], acc0);
+        i += di;
+        j += dj;
+        if (j >= d1) {
+          j -= d1;
+          ++i;
+        }
+      }
     }
-    s_red[tid] = acc;
+    s_red[tid] = acc0 + acc1;
   }
   __syncthreads();
   if (tid < n) {
-    double s = 0.0;
-    for (int k = 0; k < nrg; ++k) s += s_red[k * n + tid];
-    s_part[a.S_off[fl] + tid] += s;
+    double sacc = 0.0;
+    for (int k = 0; k < nrg; ++k) sacc += s_red[k * n + tid];
+    s_part[a.S_off[fl] + tid] += sacc;
   }
   if (R == 3) {
-    const int i_lo = r0 / d1, i_hi = (r0 + nr - 1) / d1;
-    if (tid >= 64 && tid < 64 + d1) {
-      const int j = tid - 64;
-      double s = 0.0;
-      for (int i = i_lo; i <= i_hi; ++i) {
-        const int r = i * d1 + j;
-        if (r >= r0 && r < r0 + nr) s = fma(q0[i], s_y[r - r0], s);
+    // m_0[i] = sum_j q_1[j] y_ij: one group of 2^lg0 >= d1 lanes per i
+    const int i_lo = r0 / d1, i_hi = (r0 + nr - 1) / d1, ni = i_hi - i_lo + 1;
+    const int lg0 = ceil_log2(d1);
+    const double* q0p = q0;
+    for (int e = tid; e < (ni << lg0); e += kNT) {
+      const int ii = e >> lg0, j = e & ((1 << lg0) - 1);
+      const int r = (i_lo + ii) * d1 + j;
+      double v = (j < d1 && r >= r0 && r < r0 + nr) ? q1[j] * s_y[r - r0] : 0.0;
+      v = group_sum_low(v, lg0);
+      if (j == 0) s_part[a.S_off[a.gfac[g][0]] + i_lo + ii] += v;
+    }
+    // m_1[j] = sum_i q_0[i] y_ij: one group of 2^lg1 lanes per j, each lane
+    // over every 2^lg1-th i
+    const int lg1 = std::min(6, ceil_log2(ni));
+    for (int e = tid; e < (d1 << lg1); e += kNT) {
+      const int j = e >> lg1, sub = e & ((1 << lg1) - 1);
+      double v = 0.0;
+      for (int ii = sub; ii < ni; ii += 1 << lg1) {
+        const int r = (i_lo + ii) * d1 + j;
+        if (r >= r0 && r < r0 + nr) v = fma(q0p[i_lo + ii], s_y[r - r0], v);
       }
-      s_part[a.S_off[a.gfac[g][1]] + j] += s;
-    } else if (tid >= 128 && tid < 128 + (i_hi - i_lo + 1)) {
-      const int i = i_lo + (tid - 128);
-      double s = 0.0;
-      for (int j = 0; j < d1; ++j) {
-        const int r = i * d1 + j;
-        if (r >= r0 && r < r0 + nr) s = fma(q1[j], s_y[r - r0], s);
-      }
-      s_part[a.S_off[a.gfac[g][0]] + i] += s;
+      v = group_sum_low(v, lg1);
+      if (sub == 0) s_part[a.S_off[a.gfac[g][1]] + j] += v;
     }
   }
   __syncthreads();
 }
 
+// kV: capacity of the per-factor vectors (sum of an agent's state counts)
+template <int kV>
 __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
   __shared__ double s_x[kCap];
   __shared__ double s_y[kRowsCap];
-  __shared__ double s_logq[kF * kDim], s_qv[kF * kDim], s_lp[kF * kDim], s_part[kF * kDim];
+  __shared__ double s_logq[kV], s_qv[kV], s_lp[kV], s_part[kV];
   __shared__ double s_red[kNT];
   __shared__ double s_ltab[256];
   __shared__ int s_obs[kM];
@@ -291,12 +325,15 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
           if (a.mod_group[m] != g) continue;
           const int o = s_obs[m];
           if (o < 0 || o >= a.O[m]) continue;
-          const float* Ar = b.params + a.A_off[m] + (long long)o * a.K[m] + (long long)r0 * n;
+          // A_m is position-major: row (position r, outcome o) at (r O + o) n
+          const float* Ar = b.params + a.A_off[m] + ((long long)r0 * a.O[m] + o) * n;
+          const long long rs = (long long)a.O[m] * n;
           float v[4];
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int e = e0 + q * kNT;
-            v[q] = e < nr * n ? Ar[e] : 1.0f;
+            const int rl = e / n;
+            v[q] = e < nr * n ? Ar[rl * rs + (e - rl * n)] : 1.0f;
           }
 #pragma unroll
           for (int q = 0; q < 4; ++q) prod[q] *= (double)fmaxf(v[q], kMinVal);
@@ -543,6 +580,36 @@ __device__ __forceinline__ void rg_gemm_lds(const double* X, int rows, const dou
     rg_gemm_blk<16, 1>(X, rows, Q, cols, nk, tid, store);
 }
 
+// Copies a row-major block [rows x w] (contiguous in global memory) into LDS
+// with row stride ld: every thread issues NB loads before storing, and the
+// (row, column) of its elements advance by the thread count with one carry
+// compare each (no per-element division).
+template <typename T, int NB, typename Src>
+__device__ __forceinline__ void rg_stage_rows(const Src* src, int rows, int w, int ld, T* dst, int tid) {
+  const long long n = (long long)rows * w;
+  const int dr = kNT / w, dc = kNT - dr * w;
+  int r = tid / w, c = tid - r * w;
+  for (long long e0 = tid; e0 < n; e0 += (long long)NB * kNT) {
+    T v[NB];
+    int off[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const long long e = e0 + (long long)q * kNT;
+      off[q] = r * ld + c;
+      if (e < n) v[q] = (T)src[e];
+      c += dc;
+      r += dr;
+      if (c >= w) {
+        c -= w;
+        ++r;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+      if (e0 + (long long)q * kNT < n) dst[off[q]] = v[q];
+  }
+}
+
 __device__ __forceinline__ int rg_hcount(const Agent& a, int m, int t, int p) {
   const int f = a.ef[m][p];
   return f < 0 ? 1 : a.H[t][f];
@@ -558,12 +625,7 @@ __device__ __forceinline__ void rg_load_hist(const Agent& a, const double* hist,
     return;
   }
   const double* src = hist + a.hist_lvl[t][f] + (long long)c0 * a.S[f];
-  rg_stage<double>(
-      (long long)(c1 - c0) * d, tid, [&](long long e) { return src[e]; },
-      [&](long long e, double v) {
-        const int c = (int)e / d, k = (int)e - c * d;
-        Q[c * ld + k] = v;
-      });
+  rg_stage_rows<double, (kCols * 64 + kNT - 1) / kNT>(src, c1 - c0, d, ld, Q, tid);
 }
 
 // Stage 1: for positions ij in [r0, r1) (ij = i * d1 + j) and concatenated
@@ -577,7 +639,7 @@ __device__ __forceinline__ void rg_load_hist(const Agent& a, const double* hist,
 // of lane l is row 4 v + l / 16, column l % 16).
 typedef double rg_d4 __attribute__((ext_vector_type(4)));
 constexpr int kE1A = 4352;    // floats of the A block
-constexpr int kE1H = 1024;    // doubles of the sum_o A ln A rows
+constexpr int kE1H = 512;     // doubles of the sum_o A ln A rows
 constexpr int kE1Q = 16 * 65;  // doubles of the history block
 
 __host__ __device__ __forceinline__ int rg_e1_rows(int O, int d2, int d01) {
@@ -605,22 +667,41 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
   const int nr = tl.r1 - tl.r0, nc = tl.c1 - tl.c0;
   const long long K = a.K[m];
   {
-    const float* Am = b.params + a.A_off[m] + (long long)tl.r0 * d2;
-    const int per = nr * d2;
-    rg_stage<float>(
-        (long long)O * per, tid,
-        [&](long long e) {
-          const int o = (int)(e / per);
-          return __builtin_nontemporal_load(Am + (long long)o * K + (e - (long long)o * per));
-        },
-        [&](long long e, float v) {
-          const int o = (int)(e / per), rem = (int)(e - (long long)o * per);
-          const int rl = rem / d2, k = rem - rl * d2;
-          s_a[(o * nr + rl) * lda + k] = v;
-        });
+    // A_m is position-major, so the tile's rows (position rl, outcome o) are
+    // one contiguous chunk: element e = (rl, o, k) at Am[e], LDS row o nr + rl
+    const float* Am = b.params + a.A_off[m] + (long long)tl.r0 * O * d2;
+    constexpr int NB = (kE1A + kNT - 1) / kNT;
+    const int per = O * d2;  // floats per position
+    const int tot = nr * per;
+    const int drl = kNT / per, dm = kNT - drl * per;  // per-step advance of (rl, rem)
+    const int dq = dm / d2, dk = dm - dq * d2;
+    int rl = tid / per, rem = tid - rl * per;
+    int o = rem / d2, k = rem - o * d2;
+    float v[NB];
+    int off[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int e = tid + q * kNT;
+      off[q] = (o * nr + rl) * lda + k;
+      if (e < tot) v[q] = __builtin_nontemporal_load(Am + e);
+      k += dk;
+      o += dq;
+      if (k >= d2) {
+        k -= d2;
+        ++o;
+      }
+      rl += drl;
+      if (o >= O) {
+        o -= O;
+        ++rl;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+      if (tid + q * kNT < tot) s_a[off[q]] = v[q];
     const int f = a.ef[m][2];
     const double* hist = b.hist + a.hist_off;
-    rg_stage<double>(
+    rg_stage<double, (16 * 64 + kNT - 1) / kNT>(
         16 * d2p, tid,
         [&](long long e) {
           const int c = (int)e / d2p, k = (int)e - c * d2p;
@@ -708,7 +789,7 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
 // Stage 2 at level t: rows R = (h2, o, i) in [r0, r1) of Z1 (length d1),
 // columns = middle-dim histories [c0, c1).  Z2[((h2 * H1 + h1) * (O+1) + o) * d0 + i].
 __global__ __launch_bounds__(kNT) void rg_e2_kernel(Bufs b) {
-  __shared__ double s_t[kTile];
+  __shared__ double s_t[kTile2];
   __shared__ double s_q[kQTab];
   const int tid = threadIdx.x;
   const Tile tl = b.t2[blockIdx.x];
@@ -720,12 +801,7 @@ __global__ __launch_bounds__(kNT) void rg_e2_kernel(Bufs b) {
   const int nr = tl.r1 - tl.r0, nc = tl.c1 - tl.c0;
   const int H1 = rg_hcount(a, m, t, 1);
   const double* src = b.z1 + a.z1_off[m] + ((long long)a.tstart[m][t] * O1 * d0 + tl.r0) * d1;
-  rg_stage<double>(
-      (long long)nr * d1, tid, [&](long long e) { return src[e]; },
-      [&](long long e, double v) {
-        const int rl = (int)e / d1, j = (int)e - rl * d1;
-        s_t[rl * ld1 + j] = v;
-      });
+  rg_stage_rows<double, (kTile2 + kNT - 1) / kNT>(src, nr, d1, ld1, s_t, tid);
   rg_load_hist(a, b.hist + a.hist_off, m, t, 1, tl.c0, tl.c1, s_q, tid);
   __syncthreads();
   double* Z = b.z2 + a.z2_off[m][t];
@@ -756,12 +832,7 @@ __global__ __launch_bounds__(kNT) void rg_e3_kernel(Bufs b) {
   // a rank-1 modality's stage-1 output already has the stage-2 layout
   const double* zb = a.gR[a.mod_group[m]] == 1 ? b.z1 : b.z2;
   const double* src = zb + a.z2_off[m][t] + (long long)tl.r0 * O1 * d0;
-  rg_stage<double>(
-      (long long)np * O1 * d0, tid, [&](long long e) { return src[e]; },
-      [&](long long e, double v) {
-        const int row = (int)e / d0, i = (int)e - row * d0;
-        s_t[row * ld0 + i] = v;
-      });
+  rg_stage_rows<double, (kTile + kNT - 1) / kNT>(src, np * O1, d0, ld0, s_t, tid);
   rg_load_hist(a, b.hist + a.hist_off, m, t, 0, tl.c0, tl.c1, s_q, tid);
   const float* Cm = b.params + a.C_off[m] + (a.C_time[m] ? (long long)t * O : 0);
   const float c_lo = lane < O ? Cm[lane] : 0.0f;
@@ -782,7 +853,7 @@ __global__ __launch_bounds__(kNT) void rg_e3_kernel(Bufs b) {
       if (o < O) {
         const float qf = (float)qo;
         double w = qo * (double)(half ? c_hi : c_lo);
-        if (qf != 0.0f) w -= qo * (double)logf(fmaxf(qf, kMinVal));
+        if (qf != 0.0f) w -= qo * (double)ln_clip_f(qf);
         v += w;
       } else {
         v += qo;
@@ -891,7 +962,7 @@ struct Plan {
   std::vector<int64_t> layout;  // see pymdp_hip_ragged_layout
   long long hist_total = 0, score_total = 0, z1_total = 0, z2_total = 0, part_total = 0;
   double efe_macs = 0.0, fpi_macs = 0.0;
-  int team_max = 1;
+  int team_max = 1, totS_max = 1;
   int grid1 = 0, grid_cap = 0;
   bool tuned = false;
   long long* trace = nullptr;  // caller-owned device buffer, see pymdp_hip_ragged_set_trace
@@ -1263,7 +1334,7 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
           a.z2_off[m][t] = z2;
           z2 += (long long)H2 * H1 * O1 * d0;
           const int rows = H2 * O1 * d0, ld1 = d1 | 1;
-          const int nr = std::max(1, std::min(std::min(rows, kTile / ld1), 1024));
+          const int nr = std::max(1, std::min(rows, kTile2 / ld1));
           for (int r0 = 0; r0 < rows; r0 += nr)
             for (int c0 = 0; c0 < H1; c0 += kCols) {
               const int r1 = std::min(rows, r0 + nr), c1 = std::min(H1, c0 + kCols);
@@ -1284,6 +1355,7 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
         pl->efe_macs += (double)pairs * H0 * O1 * d0;
       }
     }
+    pl->totS_max = std::max(pl->totS_max, a.totS);
     pl->agents.push_back(a);
     pS += F;
     pO += M;
@@ -1342,7 +1414,10 @@ static int upload(Plan* pl) {
   hipError_t e;
   if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device)) != hipSuccess)
     return fail_dev(e, "hipDeviceGetAttribute");
-  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rg_fpi_kernel, kNT, 0)) != hipSuccess)
+  if ((e = pl->totS_max <= 256
+               ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rg_fpi_kernel<256>, kNT, 0)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rg_fpi_kernel<kF * kDim>, kNT, 0)) !=
+      hipSuccess)
     return fail_dev(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor");
   pl->grid_cap = std::max(1, cus * per_cu);
   if (pl->grid_cap <= pl->team_max) {
@@ -1482,7 +1557,10 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
   b.n_t2 = (int)pl->t2.size();
   b.n_t3 = (int)pl->t3.size();
   b.trace = pl->trace;
-  hipLaunchKernelGGL(rg_fpi_kernel, dim3(grid1), dim3(kNT), 0, s, b);
+  if (pl->totS_max <= 256)
+    hipLaunchKernelGGL(rg_fpi_kernel<256>, dim3(grid1), dim3(kNT), 0, s, b);
+  else
+    hipLaunchKernelGGL(rg_fpi_kernel<kF * kDim>, dim3(grid1), dim3(kNT), 0, s, b);
   if (b.n_t1) hipLaunchKernelGGL(rg_e1_kernel, dim3(b.n_t1), dim3(kNT), 0, s, b);
   if (b.n_t2) hipLaunchKernelGGL(rg_e2_kernel, dim3(b.n_t2), dim3(kNT), 0, s, b);
   if (b.n_t3) hipLaunchKernelGGL(rg_e3_kernel, dim3(b.n_t3), dim3(kNT), 0, s, b);

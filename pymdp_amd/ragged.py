@@ -13,7 +13,9 @@ drop-in (pymdp_amd.ffi).
 
 Layout: one float32 parameter arena (every agent's A, B, C, D, E at offsets
 the plan chooses; A_m is stored with its dependency axes in the plan's
-contraction order, B_f with its control axis first), one int32 observation arena, one float32 output arena
+contraction order and position-major: [positions of the leading axes][O_m]
+[last axis], so that every outcome of a block of positions is contiguous;
+B_f with its control axis first), one int32 observation arena, one float32 output arena
 (qs, G, q_pi per agent) and one int32 action arena.  A step allocates nothing
 and does not synchronise, so it can be captured in a HIP graph
 (RaggedBatch.capture).
@@ -123,7 +125,11 @@ def pack_params(plan: RaggedPlan, agents: Sequence[dict], device) -> torch.Tenso
     for ai, ag in enumerate(agents):
         for m, A in enumerate(ag["A"]):
             perm = [int(p) for p in lay["A_perm"][mi] if p >= 0]
-            At = A.permute(0, *[1 + p for p in perm]).contiguous().reshape(-1)
+            # storage axes in the plan's order, then position-major:
+            # [positions of all but the last storage axis][O][last axis]
+            As = A.permute(0, *[1 + p for p in perm])
+            O_m, last = int(As.shape[0]), int(As.shape[-1])
+            At = As.reshape(O_m, -1, last).transpose(0, 1).contiguous().reshape(-1)
             o = int(lay["A_off"][mi])
             arena[o:o + At.numel()] = At.to(device)
             C = ag["C"][m].reshape(-1)

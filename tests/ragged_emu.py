@@ -17,8 +17,9 @@ from pymdp_amd import _native
 MINVAL = np.float32(2.220446049250313e-16)
 kF, kM, kR, kT, kDim, kNT = 8, 8, 3, 4, 64, 256
 kCap, kRowsCap, kTeamMax = 6144, 1024, 128
-kTile, kQTab, kCols = 4608, 64 * 65, 64
-kE1A, kE1H, kE1Q = 4352, 1024, 16 * 65
+kTile, kTile2, kCols = 4608, 2304, 32
+kQTab = kCols * 65
+kE1A, kE1H, kE1Q = 4352, 512, 16 * 65
 
 AGENT_DT = np.dtype([
     ("F", "i4"), ("M", "i4"), ("T", "i4"), ("P", "i4"), ("G", "i4"), ("iters", "i4"), ("totS", "i4"),
@@ -135,7 +136,8 @@ class Emu:
                 i = r // d1
                 j = r - i * d1
                 i_lo, i_hi = r0 // d1, (r0 + nr - 1) // d1
-                assert 64 + d1 <= 128 and 128 + (i_hi - i_lo + 1) <= kNT
+                # lane groups of the kernel's segmented reductions fit a wave
+                assert _ceil_log2(d1) <= 6 and min(6, _ceil_log2(i_hi - i_lo + 1)) <= 6
                 f0, f1 = int(a["gfac"][g][0]), int(a["gfac"][g][1])
                 np.add.at(part, S_off[f1] + j, q0[i] * y)
                 np.add.at(part, S_off[f0] + i, q1[j] * y)
@@ -174,7 +176,10 @@ class Emu:
                         continue
                     Km = int(a["K"][m])
                     assert ej.max() < Km
-                    v = P[_chk(int(a["A_off"][m]) + o * Km + ej, P.size, "A (ll)")]
+                    # position-major A_m: (position, o, k) at (position O + o) n + k
+                    pos = (r0 + np.arange(nr))[:, None]
+                    addr = (pos * int(a["O"][m]) + o) * n + np.arange(n)[None, :]
+                    v = P[_chk(int(a["A_off"][m]) + addr, P.size, "A (ll)")]
                     prod *= np.maximum(v, MINVAL).astype(np.float64)
                 dst = int(u["lds"][sg]) + np.arange(nr)[:, None] * ld + np.arange(n)[None, :]
                 x[_chk(dst, kCap, "s_x (ll)")] = np.log(prod)
@@ -291,7 +296,8 @@ class Emu:
             _chk(np.array([P * G - 1]), kNT, "s_hap (E1)")
         K = int(a["K"][m])
         assert K == d01 * d2
-        idx = int(a["A_off"][m]) + np.arange(O)[:, None, None] * K + (r0 + np.arange(nr))[None, :, None] * d2 + \
+        # position-major A_m: (position, o, k) at (position O + o) d2 + k
+        idx = int(a["A_off"][m]) + ((r0 + np.arange(nr))[None, :, None] * O + np.arange(O)[:, None, None]) * d2 + \
             np.arange(d2)[None, None, :]
         At = self.params[_chk(idx, self.params.size, "A (E1)")]  # [O, nr, d2]
         xl = np.where(At == 0, np.float32(0), At * np.log(np.maximum(At, MINVAL)).astype(np.float32))
@@ -325,7 +331,7 @@ class Emu:
         nr, nc = r1 - r0, c1 - c0
         H1, H2 = self._hcount(a, m, t, 1), self._hcount(a, m, t, 2)
         assert r1 <= H2 * O1 * d0 and c1 <= H1 and nc <= kCols
-        _chk(np.array([(nr - 1) * ld1 + d1 - 1]), kTile, "s_t (E2)")
+        _chk(np.array([(nr - 1) * ld1 + d1 - 1]), kTile2, "s_t (E2)")
         src = int(a["z1_off"][m]) + (int(a["tstart"][m][t]) * O1 * d0 + r0) * d1 + np.arange(nr * d1)
         Z = self.z1[_chk(src, self.z1.size, "Z1 (E2 read)")].reshape(nr, d1)
         assert np.isfinite(Z).all(), "E2 reads an unset stage-1 value"
@@ -512,7 +518,7 @@ def check_bounds_only(plan):
         H = [1 if a["ef"][m][p] < 0 else int(a["H"][t][a["ef"][m][p]]) for p in range(3)]
         nr, nc = int(tl["r1"]) - int(tl["r0"]), int(tl["c1"]) - int(tl["c0"])
         assert int(tl["r1"]) <= H[2] * O1 * d0 and int(tl["c1"]) <= H[1] and 1 <= nc <= kCols
-        assert nr * (d1 | 1) <= kTile and nc * (d1 | 1) <= kQTab
+        assert nr * (d1 | 1) <= kTile2 and nc * (d1 | 1) <= kQTab
     for tl in t3:
         a = ag[int(tl["agent"])]
         m, t = int(tl["m"]), int(tl["t"])

@@ -178,3 +178,53 @@ def test_emulated_teams_match_oracle():
                                [x.numpy() for x in a["C"]], a["A_dependencies"], a["B_dependencies"], dt=np.float64)
         G = em.out[lay["G_off"][i]:lay["G_off"][i] + pm.shape[0]]
         assert np.abs(G - G64).max() <= 2e-6
+
+
+def _cursor(tid, per, d2, nr, nb, nt=256):
+    """The stage-1 kernel's per-thread cursor over e = tid + q * nt of a
+    [outer][mid < nr][inner < d2] block (per = nr * d2): the kernel's
+    (position, outcome, k) with nr = the outcome count."""
+    dq, dm = nt // per, nt % per
+    drl, dk = dm // d2, dm % d2
+    o, rem = tid // per, tid % per
+    rl, k = rem // d2, rem % d2
+    out = []
+    for _ in range(nb):
+        out.append((o, rl, k))
+        k += dk
+        rl += drl
+        if k >= d2:
+            k -= d2
+            rl += 1
+        o += dq
+        if rl >= nr:
+            rl -= nr
+            o += 1
+    return out
+
+
+def _rowcol(tid, w, nb, nt=256):
+    """rg_stage_rows' (row, column) cursor over e = tid + q * nt."""
+    dr, dc = nt // w, nt % w
+    r, c = tid // w, tid % w
+    out = []
+    for _ in range(nb):
+        out.append((r, c))
+        c += dc
+        r += dr
+        if c >= w:
+            c -= w
+            r += 1
+    return out
+
+
+@pytest.mark.parametrize("d2,nr", [(1, 64), (2, 7), (7, 3), (33, 3), (64, 1), (5, 200), (63, 2)])
+def test_stage_cursors_match_division(d2, nr):
+    per = nr * d2
+    for tid in (0, 1, 17, 100, 255):
+        for q, (o, rl, k) in enumerate(_cursor(tid, per, d2, nr, 18)):
+            e = tid + q * 256
+            assert (o, rl, k) == (e // per, (e % per) // d2, e % d2)
+        for q, (r, c) in enumerate(_rowcol(tid, d2, 18)):
+            e = tid + q * 256
+            assert (r, c) == (e // d2, e % d2)
