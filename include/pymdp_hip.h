@@ -130,6 +130,13 @@ int pymdp_hip_infer_states(const float* const* A_ptrs, const int64_t* A_batch_st
  *   B_ptrs[F] / B_batch_stride[F]     B[f] [batch?, S_f, S_Bdeps..., U_f]
  *   C_ptrs[M] / C_batch_stride[M]     C[m] [batch?, O_m] (C_has_time[m]=0)
  *                                     or [batch?, T, O_m] (C_has_time[m]=1)
+ *   pA_ptrs[M] / pA_batch_stride[M]   Dirichlet counts of A (shape of A[m])
+ *                                     or pA_ptrs NULL; pB_ptrs[F] /
+ *                                     pB_batch_stride[F] likewise for B.  Read
+ *                                     only with PYMDP_HIP_FLAG_USE_PARAM_INFO_GAIN
+ *                                     (then at least one must be given): the
+ *                                     expected information gain about them,
+ *                                     pymdp/control.py:511-602, is added to G
  *   E [batch, P] or NULL (uniform 1/P), E_batch_stride
  *   gamma [batch] device or NULL (then gamma_scalar)
  *   I_ptrs[F], I_batch_stride[F], I_depths[F]   inductive matrices
@@ -148,6 +155,8 @@ int pymdp_hip_neg_efe(const int32_t* policy_host, int64_t P, int64_t T,
                       const float* const* B_ptrs, const int64_t* B_batch_stride,
                       const float* const* C_ptrs, const int64_t* C_batch_stride,
                       const int64_t* C_has_time,
+                      const float* const* pA_ptrs, const int64_t* pA_batch_stride,
+                      const float* const* pB_ptrs, const int64_t* pB_batch_stride,
                       const float* E, int64_t E_batch_stride,
                       const float* gamma, float gamma_scalar,
                       const float* const* I_ptrs, const int64_t* I_batch_stride,
@@ -169,11 +178,15 @@ int pymdp_hip_neg_efe(const int32_t* policy_host, int64_t P, int64_t T,
  * [batch, per-agent size]; C is the (T, O_m)-broadcast form; the output is
  * neg_efe [batch, P] (the caller applies gamma, ln E and the softmax, as in
  * the reference).  policy_host is the [P, T, F] int32 policy matrix in host
- * memory (identical for every agent).
+ * memory (identical for every agent).  pA_flat / pB_flat share the offsets
+ * of A / B; pA_count / pB_count = 0 means absent (as the reference's
+ * zero-element buffers, pymdp/ffi/_efe.py:602-613).
  */
 int pymdp_hip_neg_efe_all_policies(const int32_t* policy_host, const float* qs_flat,
                                    const float* A_flat, const float* B_flat,
                                    const float* C_flat, const float* I_flat,
+                                   const float* pA_flat, int64_t pA_count,
+                                   const float* pB_flat, int64_t pB_count,
                                    const float* inductive_epsilon, int64_t epsilon_count,
                                    float* neg_efe_out, int64_t batch,
                                    int64_t P, int64_t T,

@@ -52,6 +52,7 @@ SIGNATURES: dict[str, tuple] = {
         _vp,                                     # qs_flat
         _vp, _i64p, _vp, _i64p,                  # A ptrs/stride, B ptrs/stride
         _vp, _i64p, _i64p,                       # C ptrs/stride, C_has_time
+        _vp, _i64p, _vp, _i64p,                  # pA ptrs/stride, pB ptrs/stride (or NULL)
         _vp, ctypes.c_int64,                     # E, E stride
         _vp, ctypes.c_float,                     # gamma, gamma_scalar
         _vp, _i64p, _i64p,                       # I ptrs/stride/depths
@@ -63,6 +64,7 @@ SIGNATURES: dict[str, tuple] = {
         ctypes.c_int32, _vp]),
     "pymdp_hip_neg_efe_all_policies": (ctypes.c_int, [
         _vp, _vp, _vp, _vp, _vp, _vp,            # policy, qs, A, B, C, I
+        _vp, ctypes.c_int64, _vp, ctypes.c_int64,  # pA, pA count, pB, pB count
         _vp, ctypes.c_int64,                     # eps, eps count
         _vp, ctypes.c_int64,                     # out, batch
         ctypes.c_int64, ctypes.c_int64,          # P, T

@@ -9,8 +9,9 @@ is made: the kernels read it with batch stride 0).
 
 Not on the hot path and therefore not provided: parameter learning
 (`infer_parameters`), sequence inference methods (mmp / vmp / ovf),
-`return_info` VFE diagnostics, multi-action flattening
-(`B_action_dependencies`) and parameter information gain.
+`return_info` VFE diagnostics and multi-action flattening
+(`B_action_dependencies`).  Parameter information gain
+(`use_param_info_gain` with `pA` / `pB`) is computed by the neg-EFE kernels.
 """
 from __future__ import annotations
 
@@ -51,8 +52,8 @@ class Agent:
                  learn_A=False, learn_B=False, learn_C=False, learn_D=False, learn_E=False, device=None):
         if B_action_dependencies is not None:
             raise NotImplementedError("B_action_dependencies (multi-action flattening) is not supported")
-        if use_param_info_gain:
-            raise NotImplementedError("use_param_info_gain is not supported by the HIP neg-EFE kernel")
+        if use_param_info_gain and pA is None and pB is None:
+            raise ValueError("use_param_info_gain=True requires at least one of pA or pB.")
         if learn_A or learn_B or learn_C or learn_D or learn_E:
             raise NotImplementedError("parameter learning is not part of the HIP hot path")
         dev = torch.device(device) if device is not None else A[0].device
@@ -131,7 +132,9 @@ class Agent:
         if E.dim() == 1 or E.shape[0] == 1:
             E = E.reshape(1, -1).expand(bs, -1).contiguous()
         self.A, self.B, self.C, self.D, self.E = A, B, C, D, E
-        self.pA, self.pB, self.H = pA, pB, H
+        self.pA = [_dev(x) for x in pA] if pA is not None else None
+        self.pB = [_dev(x) for x in pB] if pB is not None else None
+        self.H = H
         self.gamma = torch.as_tensor(gamma, dtype=torch.float32, device=dev).expand(bs).contiguous()
         self.alpha = torch.as_tensor(alpha, dtype=torch.float32, device=dev).expand(bs).contiguous()
         self.inductive_threshold = torch.as_tensor(inductive_threshold, dtype=torch.float32,
@@ -213,7 +216,7 @@ class Agent:
             self.B_dependencies, gamma=self.gamma, I=self.I, inductive_epsilon=self.inductive_epsilon,
             use_utility=self.use_utility, use_states_info_gain=self.use_states_info_gain,
             use_param_info_gain=self.use_param_info_gain, use_inductive=self.use_inductive,
-            ffi_cache_params_static=True)
+            ffi_cache_params_static=True, pA=self.pA, pB=self.pB)
 
     def multiaction_probabilities(self, q_pi):
         """pymdp/agent.py:908-937."""

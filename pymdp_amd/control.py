@@ -61,7 +61,7 @@ def update_posterior_policies(policy_matrix, qs_init, A, B, C, E, pA, pB, A_depe
         policy_matrix, qs_init, A, B, C, E, A_dependencies, B_dependencies, gamma=gamma,
         use_utility=use_utility, use_states_info_gain=use_states_info_gain,
         use_param_info_gain=use_param_info_gain, use_inductive=False,
-        ffi_cache_params_static=ffi_cache_params_static)
+        ffi_cache_params_static=ffi_cache_params_static, pA=pA, pB=pB)
 
 
 def update_posterior_policies_inductive(policy_matrix, qs_init, A, B, C, E, pA, pB, A_dependencies, B_dependencies,
@@ -75,7 +75,7 @@ def update_posterior_policies_inductive(policy_matrix, qs_init, A, B, C, E, pA, 
         policy_matrix, qs_init, A, B, C, E, A_dependencies, B_dependencies, gamma=gamma, I=I,
         inductive_epsilon=inductive_epsilon, use_utility=use_utility, use_states_info_gain=use_states_info_gain,
         use_param_info_gain=use_param_info_gain, use_inductive=use_inductive,
-        ffi_cache_params_static=ffi_cache_params_static)
+        ffi_cache_params_static=ffi_cache_params_static, pA=pA, pB=pB)
 
 
 def compute_expected_state(qs_prior: Sequence[torch.Tensor], B: Sequence[torch.Tensor], u_t: torch.Tensor,

@@ -73,6 +73,15 @@ struct Desc {
   long long sc_off[kMaxT][kMaxM];
   long long fsc_off[kMaxT][kMaxF];  // factor (inductive) scores [n]
   long long ha_off[kMaxM];
+  // parameter information gain (plans built with pA / pB): W_A per modality
+  // [O][S0][Krest] and W_B per factor [S][Kb][U] (FP64), the per-history pB
+  // terms, and the row stride of W[n] ([S0][Ox]: O outcome rows, the
+  // sum_o A ln A row, then O rows of W_A when pA is given)
+  int has_pA, has_pB;
+  int Ox[kMaxM];
+  long long wa_off[kMaxM];
+  long long wb_off[kMaxF];
+  long long pbs_off[kMaxT][kMaxF];
   int g_ntot[kMaxG];  // inner tuples of all levels
   long long qs0_off;  // qs_init in FP64
   long long g_off;    // FP64 logits / exp of the policy posterior [P]
@@ -98,6 +107,10 @@ struct Args {
   const float* C[kMaxM];
   long long C_stride[kMaxM];
   int C_time[kMaxM];
+  const float* pA[kMaxM];  // Dirichlet counts of A / B (param info gain), or null
+  long long pA_stride[kMaxM];
+  const float* pB[kMaxF];
+  long long pB_stride[kMaxF];
   const float* E;  // [batch, P] or null (uniform)
   long long E_stride;
   const float* gamma;  // [batch] or null
@@ -136,6 +149,68 @@ __global__ __launch_bounds__(kNT) void efe_ha_kernel(const Desc* __restrict__ dp
   double acc = 0.0;
   for (int o = 0; o < O; ++o) acc += xlogx_f64(Am[(long long)o * K + k]);
   a.ws[(long long)b * d.ws_stride + d.ha_off[m] + k] = acc;
+}
+
+// digamma of a positive double: upward recurrence psi(x) = psi(x + 1) - 1/x
+// to x >= 6, then the asymptotic series psi(x) ~ ln x - 1/(2x) - 1/(12x^2)
+// + 1/(120x^4) - 1/(252x^6) + 1/(240x^8) - 1/(132x^10) (Abramowitz & Stegun
+// 6.3.5 and 6.3.18; truncation below 1e-13 for x >= 6).
+__device__ __forceinline__ double digamma_f64(double x) {
+  double r = 0.0;
+  while (x < 6.0) {
+    r -= 1.0 / x;
+    x += 1.0;
+  }
+  const double f = 1.0 / (x * x);
+  const double t = f * (-1.0 / 12.0 + f * (1.0 / 120.0 + f * (-1.0 / 252.0 + f * (1.0 / 240.0 + f * (-1.0 / 132.0)))));
+  return r + log(x) - 0.5 / x + t;
+}
+
+// Parameter information-gain weight of pymdp/maths.py:783-809 times the
+// (a > 0) mask of pymdp/control.py:546 / :594: with a' = max(a, MINVAL) and
+// s = max(sum over the leading axis of a', MINVAL),
+//   w = -(ln s - ln a' + 1/a' - 1/s + psi(a') - psi(s))  (0 where a <= 0).
+__device__ __forceinline__ double wnorm_f64(float a, double s) {
+  if (!(a > 0.0f)) return 0.0;
+  const double sa = (double)fmaxf(a, kMinVal);
+  return -(log(s) - log(sa) + 1.0 / sa - 1.0 / s + digamma_f64(sa) - digamma_f64(s));
+}
+
+// 1b. W_A[m][o, k] from pA (column sums over the outcome axis)
+__global__ __launch_bounds__(kNT) void efe_wa_kernel(const Desc* __restrict__ dp, const Args a) {
+  const Desc& d = *dp;
+  const int m = blockIdx.y;
+  const int b = blockIdx.z;
+  if (!a.pA[m]) return;
+  const int O = d.O[m];
+  const int g = d.mod_group[m];
+  const long long K = (long long)d.gS0[g] * d.gKrest[g];
+  const long long k = (long long)blockIdx.x * kNT + threadIdx.x;
+  if (k >= K) return;
+  const float* pa = a.pA[m] + (long long)b * a.pA_stride[m];
+  double s = 0.0;
+  for (int o = 0; o < O; ++o) s += (double)fmaxf(pa[(long long)o * K + k], kMinVal);
+  s = fmax(s, (double)kMinVal);
+  double* w = a.ws + (long long)b * d.ws_stride + d.wa_off[m];
+  for (int o = 0; o < O; ++o) w[(long long)o * K + k] = wnorm_f64(pa[(long long)o * K + k], s);
+}
+
+// 1c. W_B[f][s, k, u] from pB (column sums over the next-state axis)
+__global__ __launch_bounds__(kNT) void efe_wb_kernel(const Desc* __restrict__ dp, const Args a) {
+  const Desc& d = *dp;
+  const int f = blockIdx.y;
+  const int b = blockIdx.z;
+  if (!a.pB[f]) return;
+  const int S = d.S[f];
+  const long long KU = (long long)d.Kb[f] * d.U[f];
+  const long long c = (long long)blockIdx.x * kNT + threadIdx.x;  // (k, u) column
+  if (c >= KU) return;
+  const float* pb = a.pB[f] + (long long)b * a.pB_stride[f];
+  double s = 0.0;
+  for (int i = 0; i < S; ++i) s += (double)fmaxf(pb[(long long)i * KU + c], kMinVal);
+  s = fmax(s, (double)kMinVal);
+  double* w = a.ws + (long long)b * d.ws_stride + d.wb_off[f];
+  for (int i = 0; i < S; ++i) w[(long long)i * KU + c] = wnorm_f64(pb[(long long)i * KU + c], s);
 }
 
 // 2. Rollout: qs[t][f][h] = B_f[:, parents..., u_h] . (x parent marginals)
@@ -198,6 +273,52 @@ __global__ __launch_bounds__(kNT) void efe_rollout_kernel(const Desc* __restrict
   }
 }
 
+// 2b. pB term of every history (pymdp/control.py:554-602):
+//     pbs[t][f][h] = qs[t][f][h] . (W_B f[:, :, u_h] . (x parent marginals at t-1))
+__global__ __launch_bounds__(kNT) void efe_pb_kernel(const Desc* __restrict__ dp, const int* __restrict__ tab,
+                                                     const Args a) {
+  const Desc& d = *dp;
+  const int b = blockIdx.x;
+  const int f = blockIdx.y;
+  if (!a.pB[f]) return;
+  double* ws = a.ws + (long long)b * d.ws_stride;
+  const double* qs0 = ws + d.qs0_off;
+  const int S = d.S[f], U = d.U[f], rank = d.Brank[f], Kb = d.Kb[f];
+  const double* W = ws + d.wb_off[f];
+  for (int t = 0; t < d.T; ++t) {
+    const int n = d.fn[t][f];
+    const int* par = tab + d.f_par_off[t][f];
+    const int* act = tab + d.f_act_off[t][f];
+    const double* qn = ws + d.qs_off[t][f];
+    for (int h = threadIdx.x; h < n; h += kNT) {
+      const int u = act[h];
+      const double* qp[8];
+      for (int r = 0; r < rank; ++r) {
+        const int pf = d.Bdeps[f][r];
+        qp[r] = (t == 0) ? qs0 + a.qs_off[pf] : ws + d.qs_off[t - 1][pf] + (long long)par[h * rank + r] * d.S[pf];
+      }
+      double term = 0.0;
+      for (int i = 0; i < S; ++i) {
+        const double* row = W + (long long)i * Kb * U + u;
+        double acc = 0.0;
+        for (int k = 0; k < Kb; ++k) {
+          int rem = k;
+          double w = 1.0;
+          for (int r = rank - 1; r >= 0; --r) {
+            const int sz = d.S[d.Bdeps[f][r]];
+            const int c = rem % sz;
+            rem /= sz;
+            w *= qp[r][c];
+          }
+          acc = fma(row[(long long)k * U], w, acc);
+        }
+        term = fma(qn[(long long)h * S + i], acc, term);
+      }
+      ws[d.pbs_off[t][f] + h] = term;
+    }
+  }
+}
+
 // 3. Kronecker vector of each realised inner tuple:
 //    Qin[t][g][n][k] = prod_{i>=1} qs[t][deps_i][h_i(n)][s_i(k)]
 __global__ __launch_bounds__(kNT) void efe_kron_kernel(const Desc* __restrict__ dp, const int* __restrict__ tab,
@@ -244,7 +365,9 @@ __global__ __launch_bounds__(256) void efe_gemm1_kernel(const Desc* __restrict__
   const int Kr = d.gKrest[g];
   const int R = d.gR[g];
   const bool sig = (a.flags & kFlagUseStatesInfoGain) != 0;
-  const int rows = (O + 1) * S0;  // A rows j = o' * S0 + s0
+  const int Ox = d.Ox[m];
+  const int rows = Ox * S0;  // rows j = o' * S0 + s0: A, sum_o A ln A, W_A
+  const double* wa = d.has_pA && a.pA[m] ? a.ws + (long long)b * d.ws_stride + d.wa_off[m] : nullptr;
   const int ntot = d.g_ntot[g];
   const int tiles_m = (rows + G1_BM - 1) / G1_BM;
   const int tiles_n = (ntot + G1_BN - 1) / G1_BN;
@@ -255,7 +378,7 @@ __global__ __launch_bounds__(256) void efe_gemm1_kernel(const Desc* __restrict__
   const float* Am = a.A[m] + (long long)b * a.A_stride[m];
   const double* ha = ws + d.ha_off[m];
   const double* Q = ws + d.qin_off[0][g];  // all levels, [ntot, Kr]
-  double* W = ws + d.w_off[0][m];           // all levels, [ntot, S0, O + 1]
+  double* W = ws + d.w_off[0][m];           // all levels, [ntot, S0, Ox]
   const int tid = threadIdx.x;
   if (R < 2) {
     // W[nn][s0][o'] = Arow(o', s0) for every level
@@ -264,8 +387,11 @@ __global__ __launch_bounds__(256) void efe_gemm1_kernel(const Desc* __restrict__
       const int nn = col0 + e / G1_BM;
       if (r >= rows || nn >= ntot) continue;
       const int o = r / S0, s0 = r - o * S0;
-      const double v = (o < O) ? (double)Am[r] : (sig ? ha[s0] : 0.0);
-      W[((long long)nn * S0 + s0) * (O + 1) + o] = v;
+      double v;
+      if (o < O) v = (double)Am[r];
+      else if (o == O) v = sig ? ha[s0] : 0.0;
+      else v = wa ? wa[r - (O + 1) * S0] : 0.0;
+      W[((long long)nn * S0 + s0) * Ox + o] = v;
     }
     return;
   }
@@ -287,7 +413,8 @@ __global__ __launch_bounds__(256) void efe_gemm1_kernel(const Desc* __restrict__
       double v = 0.0;
       if (r < rows && k < Kr) {
         if (r < O * S0) v = (double)Am[(long long)r * Kr + k];
-        else if (sig) v = ha[(long long)(r - O * S0) * Kr + k];
+        else if (r < (O + 1) * S0) v = sig ? ha[(long long)(r - O * S0) * Kr + k] : 0.0;
+        else if (wa) v = wa[(long long)(r - (O + 1) * S0) * Kr + k];
       }
       As[kk][rr] = v;
     }
@@ -321,7 +448,7 @@ __global__ __launch_bounds__(256) void efe_gemm1_kernel(const Desc* __restrict__
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int nn = col0 + tc * 4 + j;
-      if (nn < ntot) W[((long long)nn * S0 + s0) * (O + 1) + o] = acc[i][j];
+      if (nn < ntot) W[((long long)nn * S0 + s0) * Ox + o] = acc[i][j];
     }
   }
 }
@@ -343,12 +470,13 @@ __global__ __launch_bounds__(kNT) void efe_score_kernel(const Desc* __restrict__
   const int t = it.t, m = it.m, n = it.n;
   const int g = d.mod_group[m];
   const int O = d.O[m];
-  const int O1 = O + 1;
+  const int O1 = d.Ox[m];  // row stride of W[n]
   const int S0 = d.gS0[g];
   const int f0 = d.gdeps[g][0];
   const int H0 = d.g_H0[t][g];
   const bool sig = (a.flags & kFlagUseStatesInfoGain) != 0;
   const bool util = (a.flags & kFlagUseUtility) != 0;
+  const bool pa = d.has_pA && a.pA[m] != nullptr;
   double* ws = a.ws + (long long)b * d.ws_stride;
   const double* Wg = ws + d.w_off[t][m] + (long long)n * S0 * O1;
   const double* W = Wg;
@@ -364,7 +492,7 @@ __global__ __launch_bounds__(kNT) void efe_score_kernel(const Desc* __restrict__
   const bool has_b = hb < H0;
   const double* qa = ws + d.qs_off[t][f0] + (long long)ha * S0;
   const double* qb = ws + d.qs_off[t][f0] + (long long)(has_b ? hb : ha) * S0;
-  double ent_a = 0.0, ent_b = 0.0, ut_a = 0.0, ut_b = 0.0;
+  double ent_a = 0.0, ent_b = 0.0, ut_a = 0.0, ut_b = 0.0, pg_a = 0.0, pg_b = 0.0;
   for (int o0 = 0; o0 < O; o0 += CH) {
     double acc_a[CH], acc_b[CH];
 #pragma unroll
@@ -391,6 +519,28 @@ __global__ __launch_bounds__(kNT) void efe_score_kernel(const Desc* __restrict__
         ut_b = fma(acc_b[i], c, ut_b);
       }
     }
+    if (pa) {
+      // pA term: qo . (W_A x qs) over this chunk of outcomes (pymdp/control.py:511-552)
+      double wq_a[CH], wq_b[CH];
+#pragma unroll
+      for (int i = 0; i < CH; ++i) wq_a[i] = wq_b[i] = 0.0;
+      for (int s = 0; s < S0; ++s) {
+        const double xa = qa[s];
+        const double xb = qb[s];
+        const double* wr = W + s * O1 + O + 1 + o0;
+#pragma unroll
+        for (int i = 0; i < CH; ++i) {
+          const double w = (o0 + i < O) ? wr[i] : 0.0;
+          wq_a[i] = fma(w, xa, wq_a[i]);
+          wq_b[i] = fma(w, xb, wq_b[i]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < CH; ++i) {
+        pg_a = fma(acc_a[i], wq_a[i], pg_a);
+        pg_b = fma(acc_b[i], wq_b[i], pg_b);
+      }
+    }
   }
   double hq_a = 0.0, hq_b = 0.0;
   if (sig) {
@@ -401,8 +551,8 @@ __global__ __launch_bounds__(kNT) void efe_score_kernel(const Desc* __restrict__
     }
   }
   double* out = ws + d.sc_off[t][m] + (long long)n * H0;
-  out[ha] = (sig ? -ent_a + hq_a : 0.0) + (util ? ut_a : 0.0);
-  if (has_b) out[hb] = (sig ? -ent_b + hq_b : 0.0) + (util ? ut_b : 0.0);
+  out[ha] = (sig ? -ent_a + hq_a : 0.0) + (util ? ut_a : 0.0) - pg_a;
+  if (has_b) out[hb] = (sig ? -ent_b + hq_b : 0.0) + (util ? ut_b : 0.0) - pg_b;
 }
 
 // Inductive value per factor history (pymdp/control.py:1021-1061):
@@ -475,6 +625,10 @@ __global__ __launch_bounds__(1024) void efe_scatter_kernel(const Desc* __restric
       }
       if (ind) {
         for (int f = 0; f < d.F; ++f) st += ws[d.fsc_off[t][f] + tab[d.f_p2h_off[t][f] + p]];
+      }
+      if (d.has_pB) {
+        for (int f = 0; f < d.F; ++f)
+          if (a.pB[f]) st -= ws[d.pbs_off[t][f] + tab[d.f_p2h_off[t][f] + p]];
       }
       sum += st;
     }
@@ -554,9 +708,9 @@ static int efe_err(const std::string& msg) {
     }                                                                                 \
   } while (0)
 
-static int make_dev_plan(EfeHostPlan&& hp, int device, DevPlan** out) {
+static int make_dev_plan(EfeHostPlan&& hp, int device, bool has_pA, bool has_pB, uint64_t key, DevPlan** out) {
   auto dp = std::make_unique<DevPlan>();
-  dp->key = hp.key;
+  dp->key = key;
   dp->device = device;
   dp->host = std::move(hp);
   EfeHostPlan& h = dp->host;
@@ -582,9 +736,12 @@ static int make_dev_plan(EfeHostPlan&& hp, int device, DevPlan** out) {
     }
     d.Kb[f] = (int)kb;
   }
+  d.has_pA = has_pA ? 1 : 0;
+  d.has_pB = has_pB ? 1 : 0;
   for (int m = 0; m < d.M; ++m) {
     d.O[m] = (int)h.O[m];
     d.mod_group[m] = h.mod_group[m];
+    d.Ox[m] = d.O[m] + 1 + (has_pA ? d.O[m] : 0);
   }
   for (int g = 0; g < d.G; ++g) {
     const auto& deps = h.groups[g].deps;
@@ -608,7 +765,10 @@ static int make_dev_plan(EfeHostPlan&& hp, int device, DevPlan** out) {
   for (int m = 0; m < d.M; ++m) {
     const int g = d.mod_group[m];
     d.ha_off[m] = walloc((long long)d.gS0[g] * d.gKrest[g]);
+    if (has_pA) d.wa_off[m] = walloc((long long)d.O[m] * d.gS0[g] * d.gKrest[g]);
   }
+  if (has_pB)
+    for (int f = 0; f < d.F; ++f) d.wb_off[f] = walloc((long long)d.S[f] * d.Kb[f] * d.U[f]);
   {
     long long totS = 0;
     for (int f = 0; f < d.F; ++f) totS += d.S[f];
@@ -629,6 +789,7 @@ static int make_dev_plan(EfeHostPlan&& hp, int device, DevPlan** out) {
       tab.insert(tab.end(), lv.p2h.begin(), lv.p2h.end());
       d.qs_off[t][f] = walloc((long long)lv.n * d.S[f]);
       d.fsc_off[t][f] = walloc(lv.n);
+      if (has_pB) d.pbs_off[t][f] = walloc(lv.n);
     }
     for (int g = 0; g < d.G; ++g) {
       const GroupLevelPlan& gl = h.grp[t][g];
@@ -663,7 +824,7 @@ static int make_dev_plan(EfeHostPlan&& hp, int device, DevPlan** out) {
   int g1_blocks = 1;
   for (int m = 0; m < d.M; ++m) {
     const int g = d.mod_group[m];
-    const long long per = (long long)d.gS0[g] * (d.O[m] + 1);
+    const long long per = (long long)d.gS0[g] * d.Ox[m];
     const long long base = walloc((long long)d.g_ntot[g] * per);
     long long cum = 0;
     for (int t = 0; t < d.T; ++t) {
@@ -697,7 +858,7 @@ static int make_dev_plan(EfeHostPlan&& hp, int device, DevPlan** out) {
 // Finds or builds the device plan for this policy set (content-keyed).
 static int get_plan(const int32_t* policy, int64_t P, int64_t T, int64_t F, const int64_t* S, const int64_t* U,
                     const int64_t* O, int64_t M, const int64_t* Adf, const int64_t* Ado, const int64_t* Bdf,
-                    const int64_t* Bdo, DevPlan** out) {
+                    const int64_t* Bdo, bool has_pA, bool has_pB, DevPlan** out) {
   int device = 0;
   EFE_HIP(hipGetDevice(&device));
   // content key first (cheap relative to the tree build)
@@ -710,7 +871,9 @@ static int get_plan(const int32_t* policy, int64_t P, int64_t T, int64_t F, cons
   h = fnv1a64(Bdo, sizeof(int64_t) * (F + 1), h);
   h = fnv1a64(Bdf, sizeof(int64_t) * Bdo[F], h);
   const int64_t dims[4] = {P, T, F, M};
-  const uint64_t key = fnv1a64(dims, sizeof(dims), h);
+  const uint64_t hkey = fnv1a64(dims, sizeof(dims), h);  // the history plan's own key
+  const int64_t pflags = (has_pA ? 1 : 0) | (has_pB ? 2 : 0);
+  const uint64_t key = fnv1a64(&pflags, sizeof(pflags), hkey);
   std::lock_guard<std::mutex> lk(g_mu);
   for (auto it = g_plans.begin(); it != g_plans.end(); ++it) {
     if ((*it)->key == key && (*it)->device == device) {
@@ -722,9 +885,9 @@ static int get_plan(const int32_t* policy, int64_t P, int64_t T, int64_t F, cons
   EfeHostPlan hp;
   std::string err = build_efe_host_plan(policy, P, T, F, S, U, O, M, Adf, Ado, Bdf, Bdo, &hp);
   if (!err.empty()) return efe_err(err);
-  if (hp.key != key) return efe_err("internal: plan key mismatch");
+  if (hp.key != hkey) return efe_err("internal: plan key mismatch");
   DevPlan* dp = nullptr;
-  int rc = make_dev_plan(std::move(hp), device, &dp);
+  int rc = make_dev_plan(std::move(hp), device, has_pA, has_pB, key, &dp);
   if (rc) return rc;
   g_plans.emplace_front(dp);
   while (g_plans.size() > kMaxPlans) g_plans.pop_back();
@@ -759,6 +922,10 @@ struct CallIn {
   const float* const* C;
   const int64_t* C_stride;
   const int64_t* C_time;
+  const float* const* pA;  // null: no pA
+  const int64_t* pA_stride;
+  const float* const* pB;  // null: no pB
+  const int64_t* pB_stride;
   const float* E;
   int64_t E_stride;
   const float* gamma;
@@ -783,14 +950,18 @@ struct CallIn {
 };
 
 static int run_neg_efe(const CallIn& c) {
-  if (c.flags & kFlagUseParamInfoGain)
-    return efe_err("use_param_info_gain is not supported by the HIP neg-EFE kernel");
   if (c.batch < 0) return efe_err("batch must be non-negative");
   if (c.F <= 0 || c.M <= 0) return efe_err("neg-EFE FFI requires at least one factor and one modality");
   if (c.F > kMaxF || c.M > kMaxM) return efe_err("too many factors or modalities");
   if (c.P <= 0 || c.T <= 0) return efe_err("policy_matrix shape is inconsistent with the model");
+  // parameter information gain: pA / pB used only with the flag, as in
+  // pymdp/control.py:677-680 (each may be absent)
+  const bool pig = (c.flags & kFlagUseParamInfoGain) != 0;
+  const bool has_pA = pig && c.pA != nullptr;
+  const bool has_pB = pig && c.pB != nullptr;
+  if (pig && !has_pA && !has_pB) return efe_err("use_param_info_gain=True requires at least one of pA or pB");
   DevPlan* plan = nullptr;
-  int rc = get_plan(c.policy, c.P, c.T, c.F, c.S, c.U, c.O, c.M, c.Adf, c.Ado, c.Bdf, c.Bdo, &plan);
+  int rc = get_plan(c.policy, c.P, c.T, c.F, c.S, c.U, c.O, c.M, c.Adf, c.Ado, c.Bdf, c.Bdo, has_pA, has_pB, &plan);
   if (rc) return rc;
   if (c.batch == 0) return 0;
   if (ensure_log_table()) return efe_err("cannot initialise the log table");
@@ -812,6 +983,14 @@ static int run_neg_efe(const CallIn& c) {
     if (!c.B[f]) return efe_err("null B pointer");
     a.B[f] = c.B[f];
     a.B_stride[f] = c.B_stride[f];
+  }
+  for (int m = 0; m < d.M; ++m) {
+    a.pA[m] = has_pA ? c.pA[m] : nullptr;
+    a.pA_stride[m] = has_pA ? c.pA_stride[m] : 0;
+  }
+  for (int f = 0; f < d.F; ++f) {
+    a.pB[f] = has_pB ? c.pB[f] : nullptr;
+    a.pB_stride[f] = has_pB ? c.pB_stride[f] : 0;
   }
   const bool ind = (c.flags & kFlagUseInductive) != 0;
   if (ind) {
@@ -847,13 +1026,23 @@ static int run_neg_efe(const CallIn& c) {
     hipLaunchKernelGGL(efe_ha_kernel, dim3((unsigned)((maxK + kNT - 1) / kNT), d.M, B), dim3(kNT), 0, st,
                        plan->d_desc, a);
   }
+  long long maxK = 1, maxKU = 1;
+  for (int g = 0; g < d.G; ++g) maxK = std::max(maxK, (long long)d.gS0[g] * d.gKrest[g]);
+  for (int f = 0; f < d.F; ++f) maxKU = std::max(maxKU, (long long)d.Kb[f] * d.U[f]);
+  if (has_pA)
+    hipLaunchKernelGGL(efe_wa_kernel, dim3((unsigned)((maxK + kNT - 1) / kNT), d.M, B), dim3(kNT), 0, st,
+                       plan->d_desc, a);
+  if (has_pB)
+    hipLaunchKernelGGL(efe_wb_kernel, dim3((unsigned)((maxKU + kNT - 1) / kNT), d.F, B), dim3(kNT), 0, st,
+                       plan->d_desc, a);
   hipLaunchKernelGGL(efe_rollout_kernel, dim3(B), dim3(kNT), 0, st, plan->d_desc, plan->d_tab, a);
+  if (has_pB) hipLaunchKernelGGL(efe_pb_kernel, dim3(B, d.F), dim3(kNT), 0, st, plan->d_desc, plan->d_tab, a);
   hipLaunchKernelGGL(efe_kron_kernel, dim3(plan->kron_blocks, d.T * d.G, B), dim3(kNT), 0, st, plan->d_desc,
                      plan->d_tab, a);
   hipLaunchKernelGGL(efe_gemm1_kernel, dim3(plan->g1_blocks, d.M, B), dim3(256), 0, st, plan->d_desc, a);
   {
     size_t wmax = 0;
-    for (int m = 0; m < d.M; ++m) wmax = std::max(wmax, (size_t)d.gS0[d.mod_group[m]] * (d.O[m] + 1));
+    for (int m = 0; m < d.M; ++m) wmax = std::max(wmax, (size_t)d.gS0[d.mod_group[m]] * d.Ox[m]);
     if (wmax * sizeof(double) <= 48 * 1024) {
       hipLaunchKernelGGL(efe_score_kernel<true>, dim3(plan->n_items2, B), dim3(kNT), wmax * sizeof(double), st,
                          plan->d_desc, plan->d_items2, a);
@@ -877,7 +1066,9 @@ extern "C" int pymdp_hip_neg_efe(const int32_t* policy_host, int64_t P, int64_t 
                                  const float* const* A_ptrs, const int64_t* A_batch_stride,
                                  const float* const* B_ptrs, const int64_t* B_batch_stride,
                                  const float* const* C_ptrs, const int64_t* C_batch_stride,
-                                 const int64_t* C_has_time, const float* E, int64_t E_batch_stride,
+                                 const int64_t* C_has_time, const float* const* pA_ptrs,
+                                 const int64_t* pA_batch_stride, const float* const* pB_ptrs,
+                                 const int64_t* pB_batch_stride, const float* E, int64_t E_batch_stride,
                                  const float* gamma, float gamma_scalar, const float* const* I_ptrs,
                                  const int64_t* I_batch_stride, const int64_t* I_depths,
                                  const float* inductive_epsilon, int64_t epsilon_count, float epsilon_scalar,
@@ -899,6 +1090,10 @@ extern "C" int pymdp_hip_neg_efe(const int32_t* policy_host, int64_t P, int64_t 
   c.C = C_ptrs;
   c.C_stride = C_batch_stride;
   c.C_time = C_has_time;
+  c.pA = pA_ptrs;
+  c.pA_stride = pA_batch_stride;
+  c.pB = pB_ptrs;
+  c.pB_stride = pB_batch_stride;
   c.E = E;
   c.E_stride = E_batch_stride;
   c.gamma = gamma;
@@ -928,7 +1123,8 @@ extern "C" int pymdp_hip_neg_efe(const int32_t* policy_host, int64_t P, int64_t 
 
 extern "C" int pymdp_hip_neg_efe_all_policies(const int32_t* policy_host, const float* qs_flat, const float* A_flat,
                                               const float* B_flat, const float* C_flat, const float* I_flat,
-                                              const float* inductive_epsilon, int64_t epsilon_count,
+                                              const float* pA_flat, int64_t pA_count, const float* pB_flat,
+                                              int64_t pB_count, const float* inductive_epsilon, int64_t epsilon_count,
                                               float* neg_efe_out, int64_t batch,
                                               int64_t P, int64_t T, const int64_t* S, int64_t F, const int64_t* O,
                                               int64_t M, const int64_t* U, const int64_t* qs_offsets,
@@ -960,14 +1156,22 @@ extern "C" int pymdp_hip_neg_efe_all_policies(const int32_t* policy_host, const 
     if (A_offsets[m + 1] - A_offsets[m] != O[m] * k) return efe::efe_err("A_offsets not consistent");
     if (C_offsets[m + 1] - C_offsets[m] != T * O[m]) return efe::efe_err("C_offsets not consistent with (T, O_m)");
   }
+  // pA / pB share the offsets of A / B; an empty buffer means absent
+  // (pymdp/ffi/_efe.py:602-613)
+  const bool hasA = pA_flat != nullptr && pA_count > 0, hasB = pB_flat != nullptr && pB_count > 0;
+  if (hasA && pA_count != batch * A_offsets[M]) return efe::efe_err("pA must have the shape of A");
+  if (hasB && pB_count != batch * B_offsets[F]) return efe::efe_err("pB must have the shape of B");
   const float* Ap[efe::kMaxM];
   const float* Cp[efe::kMaxM];
+  const float* pAp[efe::kMaxM];
   const float* Bp[efe::kMaxF];
+  const float* pBp[efe::kMaxF];
   const float* Ip[efe::kMaxF];
   int64_t As[efe::kMaxM], Cs[efe::kMaxM], Ct[efe::kMaxM], Bs[efe::kMaxF], Is[efe::kMaxF];
   for (int64_t m = 0; m < M; ++m) {
     Ap[m] = A_flat + A_offsets[m];
     As[m] = A_offsets[M];
+    pAp[m] = hasA ? pA_flat + A_offsets[m] : nullptr;
     Cp[m] = C_flat + C_offsets[m];
     Cs[m] = C_offsets[M];
     Ct[m] = 1;
@@ -975,10 +1179,12 @@ extern "C" int pymdp_hip_neg_efe_all_policies(const int32_t* policy_host, const 
   for (int64_t f = 0; f < F; ++f) {
     Bp[f] = B_flat + B_offsets[f];
     Bs[f] = B_offsets[F];
+    pBp[f] = hasB ? pB_flat + B_offsets[f] : nullptr;
     Ip[f] = I_flat ? I_flat + I_offsets[f] : nullptr;
     Is[f] = I_offsets[F];
   }
-  return pymdp_hip_neg_efe(policy_host, P, T, qs_flat, Ap, As, Bp, Bs, Cp, Cs, Ct, nullptr, 0, nullptr, 1.0f, Ip, Is,
+  return pymdp_hip_neg_efe(policy_host, P, T, qs_flat, Ap, As, Bp, Bs, Cp, Cs, Ct, hasA ? pAp : nullptr, As,
+                           hasB ? pBp : nullptr, Bs, nullptr, 0, nullptr, 1.0f, Ip, Is,
                            I_depths, inductive_epsilon, epsilon_count, 0.0f, neg_efe_out, nullptr, batch, S, F, O, M, U, A_dep_flat,
                            A_dep_offsets, B_dep_flat, B_dep_offsets, flags, stream);
 }

@@ -30,11 +30,9 @@ MAX_HORIZON = 8
 
 def can_handle(A_dependencies, B_dependencies, use_param_info_gain: bool, *, num_factors=None,
                num_modalities=None, ffi_beneficial=None) -> bool:
-    """pymdp/ffi/_efe.py:296-358.  The HIP pipeline has no parameter
-    information-gain term, so use_param_info_gain=True declines."""
+    """pymdp/ffi/_efe.py:296-358 (the HIP pipeline also handles the
+    parameter information-gain term)."""
     if not is_available():
-        return False
-    if use_param_info_gain:
         return False
     if ffi_beneficial is False:
         return False
@@ -88,14 +86,15 @@ def update_posterior_policies_ffi(policy_matrix, qs_init: Sequence[torch.Tensor]
                                   A_dependencies, B_dependencies, *, gamma=16.0, I=None,
                                   inductive_epsilon=1e-3, use_utility=True, use_states_info_gain=True,
                                   use_param_info_gain=False, use_inductive=False, ffi_cache_params_static=False,
-                                  return_q_pi=True, C_time: Sequence[bool] | None = None):
+                                  return_q_pi=True, C_time: Sequence[bool] | None = None, pA=None, pB=None):
     """Batched (q_pi, neg_efe) for every agent; shapes [batch, P].
 
     qs_init[f] [batch, S_f]; A[m] [batch?, O_m, S_deps...]; B[f]
     [batch?, S_f, S_Bdeps..., U_f]; C[m] [batch?, O_m] or [batch?, T, O_m];
     E [batch, P] or None (uniform); gamma float or [batch] tensor;
     I[f] [batch?, depth_f, S_f] (inductive only); inductive_epsilon float or
-    [batch] tensor.
+    [batch] tensor; pA / pB Dirichlet counts shaped like A / B (read only
+    with use_param_info_gain, pymdp/control.py:511-602).
     """
     pm = _policy_host(policy_matrix)
     P, T, Fp = pm.shape
@@ -109,8 +108,8 @@ def update_posterior_policies_ffi(policy_matrix, qs_init: Sequence[torch.Tensor]
         raise ValueError("C must have one entry per observation modality")
     if Fp != F:
         raise ValueError(f"policy_matrix shape {pm.shape} is inconsistent with {F} factors")
-    if use_param_info_gain:
-        raise ValueError("efe_ffi: use_param_info_gain is not supported by the HIP neg-EFE kernel")
+    if use_param_info_gain and pA is None and pB is None:
+        raise ValueError("use_param_info_gain=True requires at least one of pA or pB.")
     if not can_handle(A_dependencies, B_dependencies, False, num_factors=F, num_modalities=M):
         raise ValueError("unsupported or inconsistent A/B dependency metadata")
     if T > MAX_HORIZON:
@@ -142,6 +141,20 @@ def update_posterior_policies_ffi(policy_matrix, qs_init: Sequence[torch.Tensor]
         exp = (int(S[f]),) + tuple(int(S[x]) for x in d)
         if tuple(b.shape[-len(d) - 2:-1]) != exp:
             raise ValueError(f"B[{f}] shape {tuple(b.shape)} does not match expected {exp} + (U,)")
+    pA_p = pA_s = pB_p = pB_s = None
+    if use_param_info_gain and pA is not None:
+        if len(pA) != M:
+            raise ValueError("pA must have one entry per observation modality")
+        pA = [x.to(torch.float32).contiguous() for x in pA]
+        for m, (x, a) in enumerate(zip(pA, A)):
+            if tuple(x.shape[-a.dim():]) != tuple(a.shape[-a.dim():]) and tuple(x.shape) != tuple(a.shape[1:]):
+                raise ValueError(f"pA[{m}] must have the shape of A[{m}]")
+        pA_p, pA_s = zip(*[_ptr_stride(x, len(d) + 1, f"pA[{m}]") for m, (x, d) in enumerate(zip(pA, A_dependencies))])
+    if use_param_info_gain and pB is not None:
+        if len(pB) != F:
+            raise ValueError("pB must have one entry per hidden-state factor")
+        pB = [x.to(torch.float32).contiguous() for x in pB]
+        pB_p, pB_s = zip(*[_ptr_stride(x, len(d) + 2, f"pB[{f}]") for f, (x, d) in enumerate(zip(pB, B_dependencies))])
     C_p, C_s, C_t = [], [], []
     for m, c in enumerate(C):
         if C_time is not None:
@@ -197,12 +210,16 @@ def update_posterior_policies_ffi(policy_matrix, qs_init: Sequence[torch.Tensor]
     Adf, Ado = flatten_dep_lists(A_dependencies)
     Bdf, Bdo = flatten_dep_lists(B_dependencies)
     i64 = {k: _native.I64(v) for k, v in dict(As=A_s, Bs=B_s, Cs=C_s, Ct=C_t, Is=I_s, Id=I_d, S=S, O=O, U=U,
-                                               Adf=Adf, Ado=Ado, Bdf=Bdf, Bdo=Bdo).items()}
-    flags = _flags(use_utility, use_states_info_gain, False, use_inductive, ffi_cache_params_static)
+                                               Adf=Adf, Ado=Ado, Bdf=Bdf, Bdo=Bdo, pAs=pA_s or [0],
+                                               pBs=pB_s or [0]).items()}
+    flags = _flags(use_utility, use_states_info_gain, use_param_info_gain, use_inductive, ffi_cache_params_static)
+    pA_arr = _native.ptr_array(pA_p) if pA_p is not None else None
+    pB_arr = _native.ptr_array(pB_p) if pB_p is not None else None
     rc = _native.lib().pymdp_hip_neg_efe(
         pm.ctypes.data, P, T, qs_flat.data_ptr(),
         _native.ptr_array(A_p), i64["As"].ptr, _native.ptr_array(B_p), i64["Bs"].ptr,
         _native.ptr_array(C_p), i64["Cs"].ptr, i64["Ct"].ptr,
+        pA_arr, i64["pAs"].ptr, pB_arr, i64["pBs"].ptr,
         E_ptr, E_stride, g_ptr, g_sc,
         _native.ptr_array(I_p), i64["Is"].ptr, i64["Id"].ptr,
         eps_ptr, eps_cnt, eps_sc,
@@ -229,17 +246,20 @@ def neg_efe_all_policies_ffi(policy_matrix, qs_init, A, B, C, I, A_dependencies,
         policy_matrix, qs_init, A, B, C, None, A_dependencies, B_dependencies, gamma=1.0, I=I,
         inductive_epsilon=inductive_epsilon, use_utility=use_utility, use_states_info_gain=use_states_info_gain,
         use_param_info_gain=use_param_info_gain, use_inductive=use_inductive,
-        ffi_cache_params_static=ffi_cache_params_static, return_q_pi=False)
+        ffi_cache_params_static=ffi_cache_params_static, return_q_pi=False, pA=pA, pB=pB)
     return G[0] if unbatched else G
 
 
 def neg_efe_all_policies_packed(policy_matrix, qs_flat: torch.Tensor, A_flat: torch.Tensor, B_flat: torch.Tensor,
                                 C_flat: torch.Tensor, I_flat: torch.Tensor, inductive_epsilon: torch.Tensor, *,
                                 S, O, U, qs_offsets, A_offsets, B_offsets, C_offsets, I_offsets, I_depths,
-                                A_dep_flat, A_dep_offsets, B_dep_flat, B_dep_offsets, flags) -> torch.Tensor:
+                                A_dep_flat, A_dep_offsets, B_dep_flat, B_dep_offsets, flags,
+                                pA_flat: torch.Tensor | None = None, pB_flat: torch.Tensor | None = None
+                                ) -> torch.Tensor:
     """The reference's packed ABI verbatim (`pymdp_neg_efe_all_policies`,
     pymdp/ffi/registration/xla_register.cc:30-64): agent-major flat buffers
-    plus the 13 int64 attribute spans and the flags word."""
+    (pA / pB share A's / B's offsets, absent = empty) plus the 13 int64
+    attribute spans and the flags word."""
     pm = _policy_host(policy_matrix)
     P, T, F = pm.shape
     batch = qs_flat.shape[0] if qs_flat.dim() == 2 else 1
@@ -247,9 +267,13 @@ def neg_efe_all_policies_packed(policy_matrix, qs_flat: torch.Tensor, A_flat: to
     a = [_native.I64(x) for x in (S, O, U, qs_offsets, A_offsets, B_offsets, C_offsets, I_offsets, I_depths,
                                   A_dep_flat, A_dep_offsets, B_dep_flat, B_dep_offsets)]
     eps = inductive_epsilon.to(torch.float32).reshape(-1).contiguous()
+    pa = pA_flat.to(torch.float32).contiguous() if pA_flat is not None else None
+    pb = pB_flat.to(torch.float32).contiguous() if pB_flat is not None else None
     rc = _native.lib().pymdp_hip_neg_efe_all_policies(
         pm.ctypes.data, qs_flat.data_ptr(), A_flat.data_ptr(), B_flat.data_ptr(), C_flat.data_ptr(),
-        I_flat.data_ptr(), eps.data_ptr(), eps.numel(), out.data_ptr(), batch, P, T,
+        I_flat.data_ptr(), pa.data_ptr() if pa is not None and pa.numel() else None,
+        pa.numel() if pa is not None else 0, pb.data_ptr() if pb is not None and pb.numel() else None,
+        pb.numel() if pb is not None else 0, eps.data_ptr(), eps.numel(), out.data_ptr(), batch, P, T,
         a[0].ptr, len(S), a[1].ptr, len(O), a[2].ptr, a[3].ptr, a[4].ptr, a[5].ptr, a[6].ptr, a[7].ptr,
         a[8].ptr, a[9].ptr, a[10].ptr, a[11].ptr, a[12].ptr, int(flags), _native.stream_handle(qs_flat.device))
     _native.check(rc)

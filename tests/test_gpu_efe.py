@@ -120,6 +120,80 @@ def test_term_flags_and_reference_packed_abi(dev):
     assert torch.allclose(Gp, G, atol=1e-6)
 
 
+def _counts(x, g, zero_frac=0.15):
+    """Dirichlet-count-like tensor shaped like x (some exact zeros: the
+    (p > 0) mask of pymdp/control.py:546 / :594)."""
+    p = x * (1.0 + 19.0 * torch.rand((), generator=g, device=x.device)) + 2.0 * torch.rand(x.shape, generator=g,
+                                                                                            device=x.device)
+    return torch.where(torch.rand(x.shape, generator=g, device=x.device) < zero_frac, torch.zeros_like(p), p)
+
+
+@pytest.mark.parametrize("case", [1, 2, 4])
+@pytest.mark.parametrize("which", ["pA", "pB", "both"])
+def test_param_info_gain_parity(dev, case, which):
+    """use_param_info_gain with pA and / or pB (pymdp/control.py:511-602,
+    :677-682) against the oracle."""
+    S, U, O, Ad, Bd, T, A, B, C, qs, pm = _random_case(dev, case)
+    batch = qs[0].shape[0]
+    g = torch.Generator(device=dev).manual_seed(90 + case)
+    pA = [_counts(a, g) for a in A] if which in ("pA", "both") else None
+    pB = [_counts(b, g) for b in B] if which in ("pB", "both") else None
+    print("param info gain", case, which, flush=True)
+    q_pi, G = _efe.update_posterior_policies_ffi(pm, qs, A, B, C, None, Ad, Bd, gamma=16.0, pA=pA, pB=pB,
+                                                 use_param_info_gain=True)
+    torch.cuda.synchronize()
+    for b in range(batch):
+        Ab = _agent(A, b, [len(d) + 1 for d in Ad])
+        Bb = _agent(B, b, [len(d) + 2 for d in Bd])
+        kw = dict(use_param_info_gain=True, pA=[_np(x[b]) for x in pA] if pA else None,
+                  pB=[_np(x[b]) for x in pB] if pB else None)
+        E = np.full(pm.shape[0], 1.0 / pm.shape[0])
+        qb, Cb = [_np(q[b]) for q in qs], [_np(c[b]) for c in C]
+        q32, G32 = R.update_posterior_policies(pm, qb, Ab, Bb, Cb, E, Ad, Bd, gamma=16.0, dt=np.float32, **kw)
+        q64, G64 = R.update_posterior_policies(pm, qb, Ab, Bb, Cb, E, Ad, Bd, gamma=16.0, dt=np.float64, **kw)
+        _bar(_np(G[b]), G32, G64)
+        _bar(_np(q_pi[b]), q32, q64)
+        assert int(torch.argmax(q_pi[b])) == int(np.argmax(q64))
+
+
+def test_param_info_gain_packed_abi_and_errors(dev):
+    S, U, O, Ad, Bd, T, A, B, C, qs, pm = _random_case(dev, 2, timeC=False)
+    batch = qs[0].shape[0]
+    g = torch.Generator(device=dev).manual_seed(5)
+    pA = [_counts(a, g) for a in A]
+    pB = [_counts(b, g) for b in B]
+    from pymdp_amd.ffi import flatten_dep_lists, flatten_ragged_f32
+    qs_flat, qs_off = flatten_ragged_f32(qs, batch)
+    A_flat, A_off = flatten_ragged_f32(A, batch)
+    B_flat, B_off = flatten_ragged_f32(B, batch)
+    pA_flat, _ = flatten_ragged_f32(pA, batch)
+    pB_flat, _ = flatten_ragged_f32(pB, batch)
+    C_T = [c[:, None, :].expand(batch, T, c.shape[-1]) for c in C]
+    C_flat, C_off = flatten_ragged_f32(C_T, batch)
+    I = [torch.zeros((batch, 1, s), device=dev) for s in S]
+    I_flat, I_off = flatten_ragged_f32(I, batch)
+    Adf, Ado = flatten_dep_lists(Ad)
+    Bdf, Bdo = flatten_dep_lists(Bd)
+    flags = _efe.FLAG_USE_UTILITY | _efe.FLAG_USE_STATES_INFO_GAIN | _efe.FLAG_USE_PARAM_INFO_GAIN
+    Gp = _efe.neg_efe_all_policies_packed(
+        pm, qs_flat, A_flat, B_flat, C_flat, I_flat, torch.zeros(1, device=dev), S=S, O=O, U=U,
+        qs_offsets=qs_off, A_offsets=A_off, B_offsets=B_off, C_offsets=C_off, I_offsets=I_off,
+        I_depths=[1] * len(S), A_dep_flat=Adf, A_dep_offsets=Ado, B_dep_flat=Bdf, B_dep_offsets=Bdo, flags=flags,
+        pA_flat=pA_flat, pB_flat=pB_flat)
+    _, G = _efe.update_posterior_policies_ffi(pm, qs, A, B, C, None, Ad, Bd, return_q_pi=False, pA=pA, pB=pB,
+                                              use_param_info_gain=True)
+    assert torch.allclose(Gp, G, atol=1e-6)
+    # the flag with neither pA nor pB is an error (pymdp/control.py:657-660)
+    with pytest.raises(ValueError, match="pA or pB"):
+        _efe.update_posterior_policies_ffi(pm, qs, A, B, C, None, Ad, Bd, use_param_info_gain=True)
+    with pytest.raises(ValueError, match="efe_ffi: .*pA or pB"):
+        _efe.neg_efe_all_policies_packed(
+            pm, qs_flat, A_flat, B_flat, C_flat, I_flat, torch.zeros(1, device=dev), S=S, O=O, U=U,
+            qs_offsets=qs_off, A_offsets=A_off, B_offsets=B_off, C_offsets=C_off, I_offsets=I_off,
+            I_depths=[1] * len(S), A_dep_flat=Adf, A_dep_offsets=Ado, B_dep_flat=Bdf, B_dep_offsets=Bdo,
+            flags=flags)
+
+
 def test_inductive_parity(dev):
     S, U, O, Ad, Bd, T, A, B, C, qs, pm = _random_case(dev, 1, timeC=False)
     batch = qs[0].shape[0]

@@ -11,6 +11,10 @@ Pinned by
   * q_pi = softmax(gamma G + ln E) sums to 1              test/test_agent_jax.py:452-478
   * inductive value KATs 0 / ln eps / 0.25 ln eps          test/test_inductive_inference_jax.py:114-133
   * exact_wnorm closed form                                test/test_param_info_gain_jax.py:30-55
+  * parameter information gain (pA / pB) against explicit loops with an
+    independent digamma (known values psi(1), psi(1/2), recurrence), the
+    precision ordering of test/test_param_info_gain_jax.py:60-114 and the
+    large-count asymptote of the weight
   * T-maze ordering cue > arm > start                      test/test_control_jax.py:76-127
   * a planted mistake (dropped H_A term) is caught by the brute force.
 """
@@ -185,3 +189,64 @@ def test_bruteforce_catches_dropped_HA_term():
         bad.append(g)
     good = [brute.neg_efe_policy(qs, A, B, C, Ad, Bd, pm[p]) for p in range(pm.shape[0])]
     assert np.abs(np.array(bad) - np.array(good)).max() > 1e-3
+
+
+# ---- parameter information gain (pymdp/control.py:511-602) ------------------
+
+def _counts(rng, like, zero_frac=0.15):
+    p = like * rng.uniform(1.0, 20.0) + rng.uniform(0.0, 2.0, size=like.shape)
+    p[rng.uniform(size=like.shape) < zero_frac] = 0.0
+    return p
+
+
+@pytest.mark.parametrize("case", [1, 2])
+def test_param_info_gain_oracle_matches_brute_force(case):
+    """neg_efe_all_policies with pA / pB against explicit loops (its own
+    digamma) on every policy of a small model."""
+    rng = np.random.default_rng(30 + case)
+    S, U, O, Ad, Bd, T = EFE_CASES[case]
+    A, B, C, qs = _model(rng, S, U, O, Ad, Bd, T=T)
+    pA = [_counts(rng, a) for a in A]
+    pB = [_counts(rng, b) for b in B]
+    pm = R.construct_policies(S, U, T)
+    for kw in (dict(pA=pA), dict(pB=pB), dict(pA=pA, pB=pB)):
+        G = R.neg_efe_all_policies(pm, qs, A, B, C, Ad, Bd, use_param_info_gain=True, dt=np.float64, **kw)
+        for p in range(pm.shape[0]):
+            ref = brute.neg_efe_policy(qs, A, B, C, Ad, Bd, pm[p], **kw)
+            assert abs(G[p] - ref) <= 1e-9 * max(1.0, abs(ref)), (kw.keys(), p, G[p], ref)
+
+
+def test_brute_digamma_known_values():
+    """psi(1) = -Euler gamma, psi(1/2) = -gamma - 2 ln 2, psi(x + 1) = psi(x) + 1/x."""
+    g = 0.5772156649015329
+    assert abs(brute.digamma(1.0) + g) < 1e-12
+    assert abs(brute.digamma(0.5) + g + 2 * np.log(2.0)) < 1e-12
+    for x in (1e-6, 0.3, 2.5, 17.0):
+        assert abs(brute.digamma(x + 1) - brute.digamma(x) - 1 / x) < 1e-9 * max(1.0, 1 / x)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3, 42])
+def test_pA_info_gain_precision_ordering(seed):
+    """Scaling Dirichlet counts up makes the (negative) information gain less
+    negative (test/test_param_info_gain_jax.py:60-114)."""
+    rng = np.random.default_rng(seed)
+    shapes = [(5, 5), (3, 5, 2), (3, 5, 2)]
+    deps = [[0], [0, 1], [0, 1]]
+    qs = [rng.uniform(size=n) + 1e-16 for n in (5, 2)]
+    qs = [q / q.sum() for q in qs]
+    qo = [rng.uniform(size=n) + 1e-16 for n in (5, 3, 3)]
+    qo = [q / q.sum() for q in qo]
+    pA = [rng.uniform(size=s) * 2.0 + 1e-16 for s in shapes]
+    scalar = 1.0 + rng.uniform() * 0.1
+    lo = R.calc_negative_pA_info_gain(pA, qo, qs, deps, np.float64)
+    hi = R.calc_negative_pA_info_gain([p * scalar for p in pA], qo, qs, deps, np.float64)
+    assert hi > lo
+
+
+def test_wnorm_large_count_asymptote():
+    """For large counts the weight tends to -(1/(2a) - 1/(2s)) (psi(x) ~ ln x - 1/(2x))."""
+    p = np.array([[1.0, 3.0], [2.0, 5.0], [4.0, 1.0]]) * 1e4
+    w = R.exact_wnorm(p, np.float64)
+    s = p.sum(0)
+    approx = -(0.5 / p - 0.5 / s)
+    np.testing.assert_allclose(w, approx, rtol=1e-3)
