@@ -252,11 +252,13 @@ int pymdp_hip_neg_efe_adjoint(const int32_t* policy_dev, int64_t P, int64_t T,
 
 /*
  * Batched ragged agents: one infer_states -> infer_policies -> sample_action
- * step for many independent agents of different shapes.  No counterpart in
- * the reference (its FFI handles one agent shape per call); it composes the
- * reference semantics of pymdp/algos.py:88-155 (FPI),
- * pymdp/control.py:604-793 (neg-EFE per policy), :963 (q_pi) and :115-150
- * (deterministic action selection) per agent.
+ * -> update_empirical_prior step for many independent agents of different
+ * shapes.  No counterpart in the reference (its FFI handles one agent shape
+ * per call); it composes the reference semantics of pymdp/algos.py:88-155
+ * (FPI), pymdp/control.py:604-793 (neg-EFE per policy), :963 (q_pi),
+ * :115-150 (deterministic action selection, sampling_mode "full") and
+ * pymdp/agent.py:825-851 (predicted prior B . q of the chosen action) per
+ * agent.
  *
  * Per agent a (in order), hdr[4a..4a+3] = (F, M, T, P).  Concatenated over
  * agents: S, U [F each], O [M each], A_dep_rank [M each] with A_dep_flat,
@@ -290,6 +292,9 @@ int pymdp_hip_ragged_plan_upload(void* plan);
  *   obs_off [n]              M int32 observation indices in the obs arena
  *   qs_off, G_off, qpi_off [n each]   outputs in the float out arena
  *   act_off [n]              F int32 actions in the action arena
+ *   pred_off [n]             predicted prior [sum_f S_f] for the next step in
+ *                            the out arena (B_f[:, deps, u_f] . q, u = the
+ *                            chosen action)
  * With out == NULL returns the number of values. */
 int pymdp_hip_ragged_layout(void* plan, int64_t* out, int64_t n_out);
 /* out16 = (agents, FPI units, stage-1 tiles, stage-2 tiles, stage-3 tiles,
@@ -317,6 +322,13 @@ int pymdp_hip_ragged_autotune(void* plan, const float* params, const int32_t* ob
 /* *flag_out = 1 if a team barrier of an earlier step gave up waiting (the
  * posteriors of that agent are NaN), else 0.  Synchronises the device. */
 int pymdp_hip_ragged_status(void* plan, int32_t* flag_out);
+/* Prior of later steps' FPI: carry = 0 reads D from the params arena (the
+ * default), carry = 1 reads the predicted prior the previous step wrote at
+ * pred_off in the out arena, as the reference's rollout passes
+ * update_empirical_prior's result to the next infer_states
+ * (pymdp/envs/rollout.py:65-82).  Applies to steps launched (or graphs
+ * captured) afterwards. */
+int pymdp_hip_ragged_set_prior_source(void* plan, int32_t carry);
 /* Optional timing trace: a device buffer of 2 * (units + stage-1 tiles +
  * stage-2 tiles + stage-3 tiles) int64 values that later steps fill with
  * wall-clock ticks (100 MHz): [2u, 2u+1] = start / end of FPI unit u, then

@@ -86,6 +86,7 @@ struct Agent {
   long long obs_off, qs_off, G_off, qpi_off, act_off;
   long long hist_off, roll_off, p2h_off, pol0_off, score_off, logit_off, part_off;
   long long z1_off[kM], z2_off[kM][kT];
+  long long pred_off;  // predicted prior B . q for the chosen action, [totS] in `out`
 };
 
 // One FPI unit: up to kM segments, each a row range [r0, r1) of the stage-0
@@ -122,6 +123,7 @@ struct Bufs {
   int* cnt;          // [0] FPI queue, [3] team-barrier timeout flag
   long long* trace;  // optional: [unit or tile][start, end] wall clock ticks
   int n_agents, n_units, n_t1, n_t2, n_t3;
+  int carry;  // 1: the FPI prior is the previous step's predicted prior (out + pred_off), 0: D
 };
 
 // ln of a float clipped at MINVAL (always a normal float) from the hardware
@@ -300,8 +302,10 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
     const int totS = a.totS;
     if (b.trace && tid == 0) b.trace[2 * idx] = wall_clock64();
 
-    for (int f = 0; f < a.F; ++f)
-      for (int i = tid; i < a.S[f]; i += kNT) s_lp[a.S_off[f] + i] = log_stable_f64(b.params[a.D_off[f] + i]);
+    for (int f = 0; f < a.F; ++f) {
+      const float* pr = b.carry ? b.out + a.pred_off + a.S_off[f] : b.params + a.D_off[f];
+      for (int i = tid; i < a.S[f]; i += kNT) s_lp[a.S_off[f] + i] = log_stable_f64(pr[i]);
+    }
     for (int i = tid; i < totS; i += kNT) {
       s_logq[i] = 0.0;
       s_part[i] = 0.0;
@@ -872,6 +876,7 @@ __global__ __launch_bounds__(kNT) void rg_assemble_kernel(Bufs b) {
   __shared__ double s_red[kNW];
   __shared__ float s_redf[kNW];
   __shared__ int s_redi[kNW];
+  __shared__ int s_best;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
   const Agent& a = b.ag[blockIdx.x];
   const int P = a.P, T = a.T, F = a.F;
@@ -942,8 +947,20 @@ __global__ __launch_bounds__(kNT) void rg_assemble_kernel(Bufs b) {
       }
     const int* pol0 = b.iarena + a.pol0_off;
     for (int f = 0; f < F; ++f) b.act[a.act_off + f] = pol0[(long long)bx * F + f];
+    s_best = bx;
     // this agent's team barrier starts from zero at the next step
     b.team_cnt[blockIdx.x] = 0;
+  }
+  __syncthreads();
+  // empirical prior for the next step (pymdp/agent.py:825-851): B_f[:, deps,
+  // u_f] . (x)q of the chosen first action u, which is the level-0 rollout
+  // vector of that policy's first-step history
+  {
+    const long long base = (long long)s_best * T * F;
+    for (int f = 0; f < F; ++f) {
+      const double* h = b.hist + a.hist_off + a.hist_lvl[0][f] + (long long)p2h[base + f] * a.S[f];
+      for (int i = tid; i < a.S[f]; i += kNT) b.out[a.pred_off + a.S_off[f] + i] = (float)h[i];
+    }
   }
   // the FPI queue starts from zero at the next step
   if (blockIdx.x == 0 && tid == 0) b.cnt[0] = 0;
@@ -965,6 +982,7 @@ struct Plan {
   int team_max = 1, totS_max = 1;
   int grid1 = 0, grid_cap = 0;
   bool tuned = false;
+  int carry = 0;               // see pymdp_hip_ragged_set_prior_source
   long long* trace = nullptr;  // caller-owned device buffer, see pymdp_hip_ragged_set_trace
   // device
   Agent* d_ag = nullptr;
@@ -1028,7 +1046,7 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
   long long pA = 0, pO = 0, pS = 0;  // running offsets: A_dep_flat, modality, factor
   long long pB = 0, pPol = 0;
   long long params = 0, obs = 0, outf = 0, acts = 0, hist = 0, score = 0, z1 = 0, z2 = 0, part = 0;
-  std::vector<int64_t> A_off, A_perm, B_off, C_off, D_off, E_off, obs_off, qs_off, G_off, qpi_off, act_off;
+  std::vector<int64_t> A_off, A_perm, B_off, C_off, D_off, E_off, obs_off, qs_off, G_off, qpi_off, act_off, pred_off;
   struct UnitC {
     std::vector<Unit> units;
     double cost;
@@ -1247,6 +1265,9 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
     a.qpi_off = outf;
     qpi_off.push_back(outf);
     outf += P;
+    a.pred_off = outf;
+    pred_off.push_back(outf);
+    outf += a.totS;
     a.act_off = acts;
     act_off.push_back(acts);
     acts += F;
@@ -1398,6 +1419,7 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
   L.insert(L.end(), G_off.begin(), G_off.end());
   L.insert(L.end(), qpi_off.begin(), qpi_off.end());
   L.insert(L.end(), act_off.begin(), act_off.end());
+  L.insert(L.end(), pred_off.begin(), pred_off.end());
   *plan_out = pl;
   return PYMDP_HIP_OK;
 }
@@ -1557,6 +1579,7 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
   b.n_t2 = (int)pl->t2.size();
   b.n_t3 = (int)pl->t3.size();
   b.trace = pl->trace;
+  b.carry = pl->carry;
   if (pl->totS_max <= 256)
     hipLaunchKernelGGL(rg_fpi_kernel<256>, dim3(grid1), dim3(kNT), 0, s, b);
   else
@@ -1643,6 +1666,14 @@ extern "C" int pymdp_hip_ragged_status(void* plan, int32_t* flag_out) {
   PMH_HIP_CHECK(hipDeviceSynchronize(), "ragged: ");
   PMH_HIP_CHECK(hipMemcpy(h, pl->d_cnt, sizeof(h), hipMemcpyDeviceToHost), "ragged: ");
   *flag_out = h[3];
+  return PYMDP_HIP_OK;
+}
+
+extern "C" int pymdp_hip_ragged_set_prior_source(void* plan, int32_t carry) {
+  Plan* pl = static_cast<Plan*>(plan);
+  if (!pl) return einval("plan is NULL");
+  if (carry != 0 && carry != 1) return einval("prior source must be 0 (D) or 1 (predicted prior)");
+  pl->carry = carry;
   return PYMDP_HIP_OK;
 }
 

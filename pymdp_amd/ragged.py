@@ -1,13 +1,17 @@
 """Batched ragged agents: many independent agents of different shapes stepped
-together (infer_states -> infer_policies -> deterministic sample_action) by
-the persistent HIP kernels of libpymdp_hip (pymdp_hip_ragged_*).
+together (infer_states -> infer_policies -> deterministic sample_action ->
+update_empirical_prior) by the persistent HIP kernels of libpymdp_hip
+(pymdp_hip_ragged_*).
 
 Per agent the step computes exactly what the reference's one-agent path
 computes: the FPI posterior of pymdp/algos.py:88-155 from the agent's
 observation indices and prior D, the neg-EFE of every policy
 (pymdp/control.py:604-793 with utility and state information gain), q_pi =
 softmax(gamma G + ln E) (pymdp/control.py:963) and the first action of the
-first-index argmax policy (pymdp/control.py:115-150).  The batched entry has
+first-index argmax policy (pymdp/control.py:115-150), and the predicted prior
+B . q of that action for the next step (pymdp/agent.py:825-851); in carry mode
+the next step's FPI starts from that prior instead of D, as the reference's
+rollout does (pymdp/envs/rollout.py:65-82).  The batched entry has
 no counterpart in the reference; the plain per-agent entry points stay the
 drop-in (pymdp_amd.ffi).
 
@@ -16,7 +20,7 @@ the plan chooses; A_m is stored with its dependency axes in the plan's
 contraction order and position-major: [positions of the leading axes][O_m]
 [last axis], so that every outcome of a block of positions is contiguous;
 B_f with its control axis first), one int32 observation arena, one float32 output arena
-(qs, G, q_pi per agent) and one int32 action arena.  A step allocates nothing
+(qs, G, q_pi, predicted prior per agent) and one int32 action arena.  A step allocates nothing
 and does not synchronise, so it can be captured in a HIP graph
 (RaggedBatch.capture).
 """
@@ -84,7 +88,7 @@ class RaggedPlan:
         self.sizes = lay[:4].copy()
         pos = 4
         fields = [("A_off", sM), ("A_perm", 3 * sM), ("B_off", sF), ("C_off", sM), ("D_off", sF), ("E_off", n),
-                  ("obs_off", n), ("qs_off", n), ("G_off", n), ("qpi_off", n), ("act_off", n)]
+                  ("obs_off", n), ("qs_off", n), ("G_off", n), ("qpi_off", n), ("act_off", n), ("pred_off", n)]
         self.layout = {}
         for name, cnt in fields:
             self.layout[name] = lay[pos:pos + cnt].copy()
@@ -168,6 +172,8 @@ class RaggedBatch:
         rb.autotune()             # optional: pick launch grids by timing
         g = rb.capture()          # HIP graph of one step; g.replay()
         qs, G, q_pi, action = rb.agent_outputs(i)
+        rb.set_prior_source(carry=True)   # later steps start from B . q
+        rb.reset_prior()                  # ... which for the next step is D
     """
 
     def __init__(self, agents: Sequence[dict], num_iter: int = 16, device="cuda:0"):
@@ -181,6 +187,32 @@ class RaggedBatch:
         self.out = torch.zeros(max(1, int(self.plan.sizes[2])), dtype=torch.float32, device=self.device)
         self.act = torch.zeros(max(1, int(self.plan.sizes[3])), dtype=torch.int32, device=self.device)
         self.graph = None
+        self.carry = False
+        # gather / scatter indices D -> predicted-prior slots (reset_prior)
+        d_idx, p_idx = [], []
+        fi = 0
+        for i, ag in enumerate(agents):
+            po = int(self.plan.layout["pred_off"][i])
+            for s in ag["S"]:
+                d_idx.append(np.arange(s) + int(self.plan.layout["D_off"][fi]))
+                p_idx.append(np.arange(s) + po)
+                po += s
+                fi += 1
+        self._d_idx = torch.as_tensor(np.concatenate(d_idx), device=self.device)
+        self._p_idx = torch.as_tensor(np.concatenate(p_idx), device=self.device)
+
+    def set_prior_source(self, carry: bool):
+        """carry=False: every step's FPI prior is D.  carry=True: it is the
+        predicted prior the previous step wrote (graphs captured before this
+        call keep the old source)."""
+        _native.check(_native.lib().pymdp_hip_ragged_set_prior_source(self.plan.handle, 1 if carry else 0))
+        self.carry = bool(carry)
+
+    def reset_prior(self):
+        """Predicted-prior slots := D (stream-ordered device copy), so the next
+        carry-mode step starts from D as the reference's first rollout step."""
+        with torch.cuda.device(self.device):
+            self.out[self._p_idx] = self.params[self._d_idx]
 
     def set_obs(self, obs: Sequence[Sequence[int]]):
         self.obs.copy_(pack_obs(self.plan, obs, self.device))
@@ -232,17 +264,24 @@ class RaggedBatch:
         qs = list(torch.split(qs_flat, S))
         return qs, self.out[G_o:G_o + P], self.out[q_o:q_o + P], self.act[a_o:a_o + len(S)]
 
+    def agent_prior(self, i: int):
+        """Predicted prior per factor that agent i's last step wrote."""
+        o = int(self.plan.layout["pred_off"][i])
+        S = self.agents[i]["S"]
+        return list(torch.split(self.out[o:o + sum(S)], S))
+
 
 def algorithmic_bytes(agents: Sequence[dict]) -> int:
     """HBM bytes one step must move: A read in full once (the EFE needs every
     row; the FPI's observed rows come from the same lines), B, C, D, E and
-    the observation indices read, qs, G, q_pi and actions written."""
+    the observation indices read, qs, G, q_pi, actions and the predicted
+    prior written."""
     tot = 0
     for ag in agents:
         P = int(np.asarray(ag["policies"]).shape[0])
         tot += sum(4 * a.numel() for a in ag["A"]) + sum(4 * b.numel() for b in ag["B"])
         tot += sum(4 * c.numel() for c in ag["C"]) + sum(4 * d.numel() for d in ag["D"])
-        tot += 4 * len(ag["O"]) + 4 * sum(ag["S"]) + 8 * P + 4 * len(ag["S"])
+        tot += 4 * len(ag["O"]) + 8 * sum(ag["S"]) + 8 * P + 4 * len(ag["S"])
         if ag.get("E") is not None:
             tot += 4 * P
     return tot

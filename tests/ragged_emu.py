@@ -36,6 +36,7 @@ AGENT_DT = np.dtype([
     ("obs_off", "i8"), ("qs_off", "i8"), ("G_off", "i8"), ("qpi_off", "i8"), ("act_off", "i8"),
     ("hist_off", "i8"), ("roll_off", "i8"), ("p2h_off", "i8"), ("pol0_off", "i8"), ("score_off", "i8"),
     ("logit_off", "i8"), ("part_off", "i8"), ("z1_off", "i8", (kM,)), ("z2_off", "i8", (kM, kT)),
+    ("pred_off", "i8"),
 ], align=True)
 
 UNIT_DT = np.dtype([("agent", "i4"), ("slice", "i4"), ("nseg", "i4"), ("pad", "i4"), ("g", "i4", (kM,)),
@@ -84,8 +85,9 @@ def _rows_lg(n):
 class Emu:
     """Replays one step.  `params`/`obs` are the packed arenas as NumPy."""
 
-    def __init__(self, plan, params: np.ndarray, obs: np.ndarray):
+    def __init__(self, plan, params: np.ndarray, obs: np.ndarray, carry: bool = False, out=None):
         self.plan = plan
+        self.carry = carry
         self.info = plan.info()
         self.ag, self.units, (self.t1, self.t2, self.t3), self.iarena = tables(plan)
         self.params = params.astype(np.float32)
@@ -94,7 +96,7 @@ class Emu:
         self.score = np.full(max(1, self.info["score_doubles"]), np.nan)
         self.z1 = np.full(max(1, self.info["z1_doubles"]), np.nan)
         self.z2 = np.full(max(1, self.info["z2_doubles"]), np.nan)
-        self.out = np.full(max(1, int(plan.sizes[2])), np.nan, np.float32)
+        self.out = np.full(max(1, int(plan.sizes[2])), np.nan, np.float32) if out is None else out.copy()
         self.act = np.full(max(1, int(plan.sizes[3])), -1, np.int32)
         check_units(self.ag, self.units, self.info)
 
@@ -153,7 +155,10 @@ class Emu:
         assert len(units) == team and [int(u["slice"]) for u in units] == list(range(team))
         lp = np.zeros(totS)
         for f in range(F):
-            d = P[_chk(int(a["D_off"][f]) + np.arange(S[f]), P.size, "D")]
+            if self.carry:
+                d = self.out[_chk(int(a["pred_off"]) + S_off[f] + np.arange(S[f]), self.out.size, "prior")]
+            else:
+                d = P[_chk(int(a["D_off"][f]) + np.arange(S[f]), P.size, "D")]
             lp[S_off[f]:S_off[f] + S[f]] = np.log(np.maximum(d, MINVAL).astype(np.float64))
         o_idx = self.obs[_chk(int(a["obs_off"]) + np.arange(M), self.obs.size, "obs")]
         lds = []
@@ -402,6 +407,13 @@ class Emu:
         bx = int(np.argmax(q))
         pol0 = self.iarena[_chk(int(a["pol0_off"]) + bx * F + np.arange(F), self.iarena.size, "pol0")]
         self.act[_chk(int(a["act_off"]) + np.arange(F), self.act.size, "act")] = pol0
+        S, S_off = a["S"], a["S_off"]
+        for f in range(F):
+            h = int(p2h[bx, 0, f])
+            assert h < a["H"][0][f]
+            src = int(a["hist_off"]) + int(a["hist_lvl"][0][f]) + h * int(S[f]) + np.arange(S[f])
+            dst = int(a["pred_off"]) + int(S_off[f]) + np.arange(S[f])
+            self.out[_chk(dst, self.out.size, "pred out")] = self.hist[_chk(src, self.hist.size, "pred hist")]
 
     def run(self):
         by_agent = {}

@@ -11,6 +11,7 @@ import torch
 from oracle import dedup_ref as DR
 from oracle import numpy_ref as R
 from pymdp_amd import ragged, workloads
+from tests.test_ragged_host import _oracle_rollout
 
 pytestmark = pytest.mark.gpu
 
@@ -169,3 +170,48 @@ def test_ragged_graph_replay_and_autotune(dev):
     torch.cuda.synchronize()
     for i in (0, 100, 511):
         _check_agent(rb, i, use_dedup=True)
+
+
+def test_ragged_rollout_carries_prior(dev):
+    """Carry mode: three steps with fresh observations, each FPI starting from
+    the previous step's predicted prior B . q of the chosen action (the first
+    from D via reset_prior), eagerly and through a captured graph."""
+    w = workloads.ragged(seed=3, device=dev, n_agents=48, state_range=(2, 12), obs_range=(2, 9), max_policies=128,
+                         max_A_bytes=1 << 16)
+    ags = w["agents"]
+    print("ragged rollout: 48 agents, 3 steps", flush=True)
+    rb = ragged.RaggedBatch(ags, num_iter=16, device=dev)
+    rb.set_prior_source(True)
+    rng = np.random.default_rng(7)
+    obs_seq = [[[int(rng.integers(0, o)) for o in a["O"]] for a in ags] for _ in range(3)]
+    rb.reset_prior()
+    eager = []
+    for t in range(3):
+        rb.set_obs(obs_seq[t])
+        rb.step()
+        torch.cuda.synchronize()
+        eager.append((rb.out.clone(), rb.act.clone()))
+    assert rb.status() == 0
+    worst = 0.0
+    for i, a in enumerate(ags):
+        ref = _oracle_rollout(a, [obs_seq[t][i] for t in range(3)])
+        for t, (q64, act, pr) in enumerate(ref):
+            rb.out.copy_(eager[t][0])
+            rb.act.copy_(eager[t][1])
+            qs, _, _, ac = rb.agent_outputs(i)
+            for x, y in zip(qs, q64):
+                worst = max(worst, float(np.abs(_np(x) - y).max()))
+            np.testing.assert_array_equal(ac.cpu().numpy(), act)
+            for x, y in zip(rb.agent_prior(i), pr):
+                worst = max(worst, float(np.abs(_np(x) - y).max()))
+    assert worst <= TOL, worst
+    print(f"ragged rollout max|x-x64|={worst:.3e}")
+    # the same rollout through one captured graph, replayed per step
+    g = rb.capture()
+    torch.cuda.synchronize()
+    rb.reset_prior()
+    for t in range(3):
+        rb.set_obs(obs_seq[t])
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(eager[t][0], rb.out) and torch.equal(eager[t][1], rb.act), t

@@ -180,6 +180,70 @@ def test_emulated_teams_match_oracle():
         assert np.abs(G - G64).max() <= 2e-6
 
 
+def _oracle_rollout(a, obs_seq):
+    """FP64 oracle of a carry-mode rollout (pymdp/envs/rollout.py:65-82):
+    prior_0 = D, qs_t = FPI(obs_t, prior_t), action_t = first action of the
+    argmax policy, prior_{t+1} = B . qs_t for action_t."""
+    def npy(x):
+        return x.detach().cpu().numpy()
+    A = [npy(x) for x in a["A"]]
+    B = [npy(x).astype(np.float64) for x in a["B"]]
+    C = [npy(x) for x in a["C"]]
+    prior = [npy(x) for x in a["D"]]
+    out = []
+    for obs in obs_seq:
+        ob = [np.eye(o, dtype=np.float32)[x] for o, x in zip(a["O"], obs)]
+        q64 = R.run_factorized_fpi(A, ob, prior, a["A_dependencies"], num_iter=16, dt=np.float64)
+        G64 = DR.neg_efe_dedup(a["policies"], q64, A, B, C, a["A_dependencies"], a["B_dependencies"],
+                               dt=np.float64)
+        act = R.sample_policy(a["policies"], DR.q_pi_from_G(G64, a["gamma"]))
+        prior = R.compute_expected_state(q64, B, act, a["B_dependencies"])
+        out.append((q64, act, prior))
+    return out
+
+
+def test_emulated_rollout_carries_prior():
+    """Three carry-mode steps with fresh observations: each step's FPI starts
+    from the previous step's predicted prior B . q (the first from D, copied
+    into the prior slots as RaggedBatch.reset_prior does), and the step writes
+    the next one."""
+    w = workloads.ragged(seed=3, n_agents=12, state_range=(2, 12), obs_range=(2, 9), max_policies=128,
+                         max_A_bytes=1 << 16)
+    ags = w["agents"]
+    plan = ragged.RaggedPlan(ags, 16)
+    lay = plan.layout
+    params = ragged.pack_params(plan, ags, "cpu").numpy()
+    rng = np.random.default_rng(7)
+    obs_seq = [[[int(rng.integers(0, o)) for o in a["O"]] for a in ags] for _ in range(3)]
+    out = np.zeros(int(plan.sizes[2]), np.float32)
+    fi = 0
+    for i, a in enumerate(ags):
+        po = int(lay["pred_off"][i])
+        for f, s in enumerate(a["S"]):
+            d = int(lay["D_off"][fi])
+            out[po:po + s] = params[d:d + s]
+            po += s
+            fi += 1
+    ems = []
+    for t in range(3):
+        obs = ragged.pack_obs(plan, obs_seq[t], "cpu").numpy()
+        em = E.Emu(plan, params, obs, carry=True, out=out).run()
+        out = em.out
+        ems.append(em)
+    for i, a in enumerate(ags):
+        ref = _oracle_rollout(a, [obs_seq[t][i] for t in range(3)])
+        S = a["S"]
+        for t, (q64, act, pr) in enumerate(ref):
+            em = ems[t]
+            qo = int(lay["qs_off"][i])
+            for x, y in zip(np.split(em.out[qo:qo + sum(S)], np.cumsum(S)[:-1]), q64):
+                assert np.abs(x - y).max() <= 1e-6, (i, t)
+            np.testing.assert_array_equal(em.act[lay["act_off"][i]:lay["act_off"][i] + len(S)], act)
+            po = int(lay["pred_off"][i])
+            for x, y in zip(np.split(em.out[po:po + sum(S)], np.cumsum(S)[:-1]), pr):
+                assert np.abs(x - y).max() <= 1e-6, (i, t)
+
+
 def _cursor(tid, per, d2, nr, nb, nt=256):
     """The stage-1 kernel's per-thread cursor over e = tid + q * nt of a
     [outer][mid < nr][inner < d2] block (per = nr * d2): the kernel's
