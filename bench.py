@@ -18,6 +18,10 @@ Workloads (seeded synthetic data, generated on the device):
                              gradient of sum_t ln q_pi(a_t) w.r.t. the logits of
                              A, B, C, D through the HIP adjoints, compared with
                              torch autograd through the pure-torch forward.
+  learning                   1024 agents, Dirichlet learning of A on the
+                             plain_fpi_large model (6 x [32,16,16,8,8] counts
+                             per agent, one time step): one step = the fused
+                             pA -> (qA, E[qA]) update of every modality.
   ragged                     4096 agents of different shapes (1-4 factors of
                              2-64 states, 1-8 controls, 1-5 modalities of 2-64
                              outcomes, horizon 1-2): one step = infer_states ->
@@ -62,7 +66,7 @@ def _args():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "plain_fpi_large", "plain_efe", "ragged", "grad"])
+                    choices=["headline", "plain_fpi_large", "plain_efe", "ragged", "grad", "learning"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -235,6 +239,88 @@ def bench_plain_fpi_large(a, torch):
         "cpu_baseline": cpu,
     }
     return res
+
+
+def bench_learning(a, torch):
+    """Dirichlet learning of A (pymdp/learning.py:57-117): HBM-bound, 12 bytes
+    per count (pA read, qA and E[qA] written)."""
+    import numpy as np
+    from pymdp_amd import learning, workloads
+
+    dev = torch.device("cuda:0")
+    w = workloads.learning_large(seed=0, device=dev)
+    B, S, O, deps = w["batch"], w["S"], w["O"], w["A_dependencies"]
+    torch.cuda.synchronize()
+    out = {}
+
+    def step():
+        qA, EA = learning.update_obs_likelihood_dirichlet(
+            w["pA"], [None] * len(O), w["obs"], w["qs"], A_dependencies=deps, categorical_obs=False, num_obs=O,
+            lr=1.0)
+        out["r"] = (qA, EA)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(a.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_events = e0.elapsed_time(e1) / a.steps
+    ms_step = wall * 1e3 / a.steps
+    counts = sum(int(p.numel()) for p in w["pA"])
+    alg_bytes = 12 * counts  # per step: six launches, one per modality
+    achieved = alg_bytes / (ms_events * 1e-3) / 1e9
+    # parity: two agents of every modality against the FP64 oracle
+    from oracle import numpy_ref as R
+    qA, EA = out["r"]
+    eq = ee = 0.0
+    for b in (0, B - 1):
+        qs_b = [q[b].cpu().numpy() for q in w["qs"]]
+        for m in range(len(O)):
+            ob = np.eye(O[m])[w["obs"][m][b].cpu().numpy()]
+            q64, e64 = R.update_obs_likelihood_dirichlet_m(w["pA"][m][b].cpu().numpy(), ob, qs_b, deps[m], 1.0,
+                                                           np.float64)
+            eq = max(eq, float(np.max(np.abs(qA[m][b].cpu().numpy() - q64) / np.maximum(1.0, np.abs(q64)))))
+            ee = max(ee, float(np.abs(EA[m][b].cpu().numpy() - e64).max()))
+    cpu = None
+    if not a.no_cpu_baseline:
+        # NumPy FP32 restatement on a bounded sample of agents (single thread)
+        n, t0 = 0, time.perf_counter()
+        pA_h = [p[:8].cpu().numpy() for p in w["pA"]]
+        qs_h = [q[:8].cpu().numpy() for q in w["qs"]]
+        obs_h = [o[:8].cpu().numpy() for o in w["obs"]]
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < min(a.cpu_seconds, 10.0):
+            b = n % 8
+            R.update_obs_likelihood_dirichlet([p[b] for p in pA_h], [None] * len(O), [o[b] for o in obs_h],
+                                              [q[b] for q in qs_h], deps, False, O, 1.0, np.float32)
+            n += 1
+        el = time.perf_counter() - t0
+        cpu = {"value": n / el, "unit": "agent-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{n} agent updates over the first 8 agents, {el:.1f} s, NumPy FP32 restatement "
+                         f"of pymdp/learning.py (oracle), single thread"}
+    return {
+        "metric": "agent-updates/s (Dirichlet learning of A, 1024 agents, 6 x [32,16,16,8,8] counts)",
+        "value": B / (ms_step * 1e-3),
+        "unit": "agent-updates/s",
+        "ms_per_step": ms_step,
+        "steps": a.steps,
+        "config": {"workload": "learning_large", "agents": B, "num_states": S, "num_obs": O, "time_steps": 1,
+                   "counts_per_step": counts},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("learning pA"),
+                     "kernel": "dirichlet_kernel (6 launches)", "kernel_ms": ms_events,
+                     "alg_bytes_per_launch": alg_bytes // len(O)},
+        "parity": {"agents_checked": 2, "max_rel_err_counts_vs_fp64": eq, "max_abs_err_expected_vs_fp64": ee},
+        "cpu_baseline": cpu,
+    }
 
 
 def cpu_baseline_fpi(w, seconds):
@@ -521,6 +607,10 @@ def bench_headline(a, torch):
     g = bench_grad(b, torch)
     res["grad"] = {k: g[k] for k in keep if k in g}
     res["grads_per_s"] = g["grads_per_s"]
+    b = copy.copy(a)
+    b.steps, b.warmup = 10, 2
+    lrn = bench_learning(b, torch)
+    res["learning"] = {k: lrn[k] for k in keep if k in lrn}
     return res
 
 
@@ -547,6 +637,8 @@ def main():
         res = bench_ragged(a, torch)
     elif a.workload == "grad":
         res = bench_grad(a, torch)
+    elif a.workload == "learning":
+        res = bench_learning(a, torch)
     else:
         res = bench_plain_efe(a, torch)
     print(json.dumps(res))

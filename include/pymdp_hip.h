@@ -113,8 +113,8 @@ int pymdp_hip_infer_states(const float* const* A_ptrs, const int64_t* A_batch_st
 #define PYMDP_HIP_FLAG_USE_UTILITY (1 << 0)
 #define PYMDP_HIP_FLAG_USE_STATES_INFO_GAIN (1 << 1)
 #define PYMDP_HIP_FLAG_USE_INDUCTIVE (1 << 2)
-#define PYMDP_HIP_FLAG_USE_PARAM_INFO_GAIN (1 << 3) /* rejected: not implemented */
-#define PYMDP_HIP_FLAG_MODEL_PARAMS_STATIC (1 << 4) /* accepted, no effect */
+#define PYMDP_HIP_FLAG_USE_PARAM_INFO_GAIN (1 << 3) /* needs pA and / or pB */
+#define PYMDP_HIP_FLAG_MODEL_PARAMS_STATIC (1 << 4) /* accepted, no effect: A-derived tables are rebuilt every call */
 
 /*
  * Negative expected free energy of every policy, optionally with the policy
@@ -249,6 +249,46 @@ int pymdp_hip_neg_efe_adjoint(const int32_t* policy_dev, int64_t P, int64_t T,
                               const int64_t* A_dep_flat, const int64_t* A_dep_offsets,
                               const int64_t* B_dep_flat, const int64_t* B_dep_offsets,
                               int32_t flags, void* stream);
+
+/*
+ * Dirichlet-parameter learning (pymdp/learning.py:11-234, as driven by
+ * Agent.infer_parameters, pymdp/agent.py:416-620), batched over agents; one
+ * fused pass over the count tensor (p read once, q and E[q] written once).
+ *
+ * pymdp_hip_update_pA replaces update_obs_likelihood_dirichlet_m
+ * (pymdp/learning.py:11-55) for one modality:
+ *   qA[b] = pA[b] + lr[b] * sum_t obs[b, t] (x) qs_dep0[b, t] (x) ... (x) qs_dep{n-1}[b, t]
+ *   EA[b] = max(qA[b], MINVAL) normalised over the outcome axis
+ *           (dirichlet_expected_value, pymdp/maths.py:843-868)
+ * pA [batch?, O, S_dep0, ..., S_dep{n-1}] with batch stride pA_batch_stride
+ * (0 = one tensor shared by every agent); obs [batch, T, O] distributional
+ * (one-hot rows for index observations); qs_deps[j] [batch, T, S_depj]
+ * (batch stride qs_batch_stride[j]); lr [batch] on the device or NULL (then
+ * lr_scalar); qA_out, EA_out [batch, O, S_dep0, ...] contiguous.
+ */
+int pymdp_hip_update_pA(const float* pA, int64_t pA_batch_stride, float* qA_out, float* EA_out,
+                        int64_t batch, int64_t T, int64_t O, int32_t n_deps, const int64_t* dep_sizes,
+                        const float* obs, int64_t obs_batch_stride, const float* const* qs_deps,
+                        const int64_t* qs_batch_stride, const float* lr, float lr_scalar, void* stream);
+/*
+ * pymdp_hip_update_pB replaces update_state_transition_dirichlet_f
+ * (pymdp/learning.py:119-157) for factor `factor`:
+ *   qB[b] = pB[b] + lr[b] * sum_{t < T1} qs_next[b, t] (x) qs_dep0[b, t] (x) ... (x) onehot(actions[b, t, factor])
+ *   EB[b] = max(qB[b], MINVAL) normalised over the next-state axis
+ * pB [batch?, S, S_dep0, ..., U] (reference layout, control axis last);
+ * qs_next [batch, T1, S] (the beliefs at t + 1) and qs_deps[j] [batch, T1,
+ * S_depj] (at t) with their batch and time strides; actions [batch, T1,
+ * num_factors] int32.  An action outside [0, U) contributes nothing
+ * (jax.nn.one_hot); mask_all = 1 also drops every t at which any factor's
+ * action is negative (the transition mask of pymdp/agent.py:478-520).
+ */
+int pymdp_hip_update_pB(const float* pB, int64_t pB_batch_stride, float* qB_out, float* EB_out,
+                        int64_t batch, int64_t T1, int64_t S, int32_t n_deps, const int64_t* dep_sizes,
+                        int64_t U, const float* qs_next, int64_t qs_next_batch_stride,
+                        int64_t qs_next_t_stride, const float* const* qs_deps,
+                        const int64_t* qs_deps_batch_stride, const int64_t* qs_deps_t_stride,
+                        const int32_t* actions, int64_t actions_batch_stride, int32_t num_factors,
+                        int32_t factor, int32_t mask_all, const float* lr, float lr_scalar, void* stream);
 
 /*
  * Batched ragged agents: one infer_states -> infer_policies -> sample_action

@@ -424,3 +424,110 @@ def get_marginals(q_pi, policies, num_controls):
 def sample_action(policies, num_controls, q_pi):
     """pymdp/control.py:69-113, deterministic mode."""
     return np.array([int(np.argmax(m)) for m in get_marginals(q_pi, policies, num_controls)])
+
+
+# ---------------------------------------------------------------------------
+# Dirichlet-parameter learning (one agent; leading time axis on obs / beliefs)
+# ---------------------------------------------------------------------------
+def multidimensional_outer(arrs):
+    """pymdp/maths.py:763-781: outer product by iterative expansion."""
+    x = arrs[0]
+    for q in arrs[1:]:
+        x = np.expand_dims(x, -1) * q
+    return x
+
+
+def dirichlet_expected_value(dir_arr, dt=np.float32):
+    """pymdp/maths.py:843-868: clip at MINVAL, normalise over axis 0."""
+    d = np.maximum(_a(dir_arr, dt), dt(MINVAL))
+    return (d / d.sum(axis=0, keepdims=True)).astype(dt)
+
+
+def update_obs_likelihood_dirichlet_m(pA_m, obs_m, qs, dependencies_m, lr=1.0, dt=np.float32):
+    """pymdp/learning.py:11-55: obs_m [T, O] (distributional), qs[f] [T, S_f];
+    returns (pA_m + lr * sum_t obs_t (x) qs_deps_t, its expected value)."""
+    T = obs_m.shape[0]
+    dfda = sum(multidimensional_outer([_a(obs_m[t], dt)] + [_a(qs[f][t], dt) for f in dependencies_m])
+               for t in range(T)) if T else np.zeros_like(_a(pA_m, dt))
+    new = (_a(pA_m, dt) + dt(lr) * dfda).astype(dt)
+    return new, dirichlet_expected_value(new, dt)
+
+
+def update_obs_likelihood_dirichlet(pA, A, obs, qs, A_dependencies, categorical_obs, num_obs, lr, dt=np.float32):
+    """pymdp/learning.py:57-117 (one agent): obs[m] is [T] indices or [T, O_m]."""
+    qA, EA = [], []
+    for m, pA_m in enumerate(pA):
+        if pA_m is None:
+            qA.append(None)
+            EA.append(A[m])
+            continue
+        o = np.asarray(obs[m])
+        o_m = o.astype(dt) if categorical_obs else np.eye(num_obs[m], dtype=dt)[o.astype(np.int64)]
+        a, e = update_obs_likelihood_dirichlet_m(pA_m, o_m, qs, A_dependencies[m], lr, dt)
+        qA.append(a)
+        EA.append(e)
+    return qA, EA
+
+
+def _one_hot(idx, n, dt):
+    """jax.nn.one_hot: all-zero rows for indices outside [0, n)."""
+    idx = np.asarray(idx, dtype=np.int64)
+    out = np.zeros(idx.shape + (n,), dt)
+    ok = (idx >= 0) & (idx < n)
+    out[ok, idx[ok]] = 1
+    return out
+
+
+def update_state_transition_dirichlet_f(pB_f, actions_f, joint_qs_f, lr=1.0, dt=np.float32):
+    """pymdp/learning.py:119-157: joint_qs_f = [qs_f[t+1], qs_dep[t], ...]
+    each [T1, .], actions_f [T1, U_f] one-hot."""
+    T1 = actions_f.shape[0]
+    dfdb = sum(multidimensional_outer([_a(x[t], dt) for x in joint_qs_f] + [_a(actions_f[t], dt)])
+               for t in range(T1)) if T1 else np.zeros_like(_a(pB_f, dt))
+    new = (_a(pB_f, dt) + dt(lr) * dfdb).astype(dt)
+    return new, dirichlet_expected_value(new, dt)
+
+
+def update_state_transition_dirichlet(pB, B, joint_beliefs, actions, num_controls, lr, factors_to_update="all",
+                                      dt=np.float32):
+    """pymdp/learning.py:159-234 (one agent): actions [T1, F] int."""
+    F = len(pB)
+    upd = list(range(F)) if factors_to_update == "all" else sorted(factors_to_update)
+    qB, EB = [], []
+    for f in range(F):
+        if f in upd:
+            b, e = update_state_transition_dirichlet_f(pB[f], _one_hot(np.asarray(actions)[:, f], num_controls[f], dt),
+                                                       joint_beliefs[f], lr, dt)
+        else:
+            b, e = _a(pB[f], dt), dirichlet_expected_value(pB[f], dt)
+        qB.append(b)
+        EB.append(e)
+    return qB, EB
+
+
+def infer_parameters(pA, A, pB, B, beliefs_A, observations, actions, B_dependencies, A_dependencies, num_obs,
+                     num_controls, learn_A, learn_B, categorical_obs=False, beliefs_B=None, lr_pA=1.0, lr_pB=1.0,
+                     any_valid=None, dt=np.float32):
+    """pymdp/agent.py:416-620 for inference_algo "fpi" (no smoothing), one
+    agent: returns (pA, A, pB, B) after the update.  beliefs [T, S_f];
+    actions [T, F] or [F].  any_valid: the batch-wide "some transition is
+    valid" predicate (default: this agent's)."""
+    seq = beliefs_A if beliefs_B is None else beliefs_B
+    T = seq[0].shape[0]
+    act = None if actions is None else np.asarray(actions)
+    if act is not None and act.ndim == 1:
+        act = act[None]
+    act1 = act[:max(T - 1, 0)] if act is not None else None
+    valid = None if act1 is None else np.all(act1 >= 0, axis=-1)
+    if any_valid is None:
+        any_valid = valid is not None and bool(np.any(valid))
+    can_B = learn_B and act1 is not None and T > 1 and act1.shape[0] == T - 1 and any_valid
+    if learn_A:
+        pA, A = update_obs_likelihood_dirichlet(pA, A, observations, beliefs_A, A_dependencies, categorical_obs,
+                                                num_obs, lr_pA, dt)
+    if learn_B and can_B:
+        mask = valid.astype(dt)
+        joints = [[_a(seq[f][1:], dt) * mask[:, None]] + [_a(seq[d][:-1], dt) * mask[:, None]
+                                                         for d in B_dependencies[f]] for f in range(len(pB))]
+        pB, B = update_state_transition_dirichlet(pB, B, joints, act1, num_controls, lr_pB, dt=dt)
+    return pA, A, pB, B

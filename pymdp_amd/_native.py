@@ -85,6 +85,23 @@ SIGNATURES: dict[str, tuple] = {
         ctypes.c_int64, _i64p, ctypes.c_int64, _i64p, ctypes.c_int64, _i64p,  # batch, S, F, O, M, U
         _i64p, _i64p, _i64p, _i64p,              # A deps, B deps
         ctypes.c_int32, _vp]),
+    # Dirichlet learning (pymdp/learning.py)
+    "pymdp_hip_update_pA": (ctypes.c_int, [
+        _vp, ctypes.c_int64, _vp, _vp,           # pA, stride, qA, EA
+        ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,  # batch, T, O
+        ctypes.c_int32, _i64p,                   # n_deps, dep_sizes
+        _vp, ctypes.c_int64,                     # obs, obs batch stride
+        _vp, _i64p,                              # qs_deps ptrs, batch strides
+        _vp, ctypes.c_float, _vp]),              # lr, lr_scalar, stream
+    "pymdp_hip_update_pB": (ctypes.c_int, [
+        _vp, ctypes.c_int64, _vp, _vp,           # pB, stride, qB, EB
+        ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,  # batch, T1, S
+        ctypes.c_int32, _i64p, ctypes.c_int64,   # n_deps, dep_sizes, U
+        _vp, ctypes.c_int64, ctypes.c_int64,     # qs_next, batch / t strides
+        _vp, _i64p, _i64p,                       # qs_deps ptrs, batch / t strides
+        _vp, ctypes.c_int64, ctypes.c_int32,     # actions, batch stride, num_factors
+        ctypes.c_int32, ctypes.c_int32,          # factor, mask_all
+        _vp, ctypes.c_float, _vp]),              # lr, lr_scalar, stream
     # batched ragged agents (no counterpart in the reference)
     "pymdp_hip_ragged_plan_create": (ctypes.c_int, [
         ctypes.c_int64, _i64p,                   # n_agents, hdr [n, 4]

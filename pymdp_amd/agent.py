@@ -1,27 +1,29 @@
 """Agent (mirror of pymdp/agent.py, the hot-path methods).
 
 Keeps the reference's constructor arguments and the `infer_states`,
-`infer_policies`, `sample_action`, `update_empirical_prior` and
-`multiaction_probabilities` methods with their argument meaning and return
-shapes.  Model tensors are torch tensors on a HIP device; a tensor given
+`infer_policies`, `sample_action`, `update_empirical_prior`,
+`multiaction_probabilities` and `infer_parameters` methods with their
+argument meaning and return shapes.  Model tensors are torch tensors on a HIP device; a tensor given
 without the leading batch axis is shared by every agent (no broadcast copy
 is made: the kernels read it with batch stride 0).
 
-Not on the hot path and therefore not provided: parameter learning
-(`infer_parameters`), sequence inference methods (mmp / vmp / ovf),
-`return_info` VFE diagnostics and multi-action flattening
-(`B_action_dependencies`).  Parameter information gain
+Not on the hot path and therefore not provided: sequence inference methods
+(mmp / vmp / ovf) and the smoothing-based learning they enable, `return_info`
+VFE diagnostics and multi-action flattening (`B_action_dependencies`).
+`infer_parameters` (Dirichlet learning of A and B, pymdp/agent.py:416-620)
+runs the fused learning kernels of `pymdp_amd.learning`.  Parameter information gain
 (`use_param_info_gain` with `pA` / `pB`) is computed by the neg-EFE kernels.
 """
 from __future__ import annotations
 
+import copy
 import math
 from typing import Sequence
 
 import numpy as np
 import torch
 
-from . import control
+from . import control, learning
 from .control import Policies, construct_policies
 from .ffi._efe import update_posterior_policies_ffi
 from .ffi._fpi import infer_states_ffi
@@ -54,8 +56,10 @@ class Agent:
             raise NotImplementedError("B_action_dependencies (multi-action flattening) is not supported")
         if use_param_info_gain and pA is None and pB is None:
             raise ValueError("use_param_info_gain=True requires at least one of pA or pB.")
-        if learn_A or learn_B or learn_C or learn_D or learn_E:
-            raise NotImplementedError("parameter learning is not part of the HIP hot path")
+        if learn_A and pA is None:
+            raise ValueError("pA is required for A learning")
+        if learn_B and pB is None:
+            raise ValueError("pB is required for B learning")
         dev = torch.device(device) if device is not None else A[0].device
         self.device = dev
         self.num_modalities = len(A)
@@ -135,6 +139,9 @@ class Agent:
         self.pA = [_dev(x) for x in pA] if pA is not None else None
         self.pB = [_dev(x) for x in pB] if pB is not None else None
         self.H = H
+        self.learning_mode = learning_mode
+        self.learn_A, self.learn_B = bool(learn_A), bool(learn_B)
+        self.learn_C, self.learn_D, self.learn_E = bool(learn_C), bool(learn_D), bool(learn_E)
         self.gamma = torch.as_tensor(gamma, dtype=torch.float32, device=dev).expand(bs).contiguous()
         self.alpha = torch.as_tensor(alpha, dtype=torch.float32, device=dev).expand(bs).contiguous()
         self.inductive_threshold = torch.as_tensor(inductive_threshold, dtype=torch.float32,
@@ -216,7 +223,51 @@ class Agent:
             self.B_dependencies, gamma=self.gamma, I=self.I, inductive_epsilon=self.inductive_epsilon,
             use_utility=self.use_utility, use_states_info_gain=self.use_states_info_gain,
             use_param_info_gain=self.use_param_info_gain, use_inductive=self.use_inductive,
-            ffi_cache_params_static=True, pA=self.pA, pB=self.pB)
+            ffi_cache_params_static=not (self.learn_A or self.learn_B), pA=self.pA, pB=self.pB)
+
+    def infer_parameters(self, beliefs_A, observations, actions, beliefs_B=None, lr_pA=1.0, lr_pB=1.0,
+                         **kwargs):
+        """pymdp/agent.py:416-620 for the filtering algorithms ("fpi",
+        "exact"): returns a new Agent whose pA, A (learn_A) and pB, B
+        (learn_B) are the Dirichlet posteriors and their expected values.
+        beliefs [batch, T, S_f]; observations [batch, T] per modality
+        (or [batch, T, O_m] with categorical_obs); actions [batch, T, F] or
+        [batch, F]."""
+        agent = copy.copy(self)
+        seq = beliefs_A if beliefs_B is None else beliefs_B
+        if actions is not None:
+            actions = torch.as_tensor(actions, device=self.device)
+            if actions.dim() == 2:
+                actions = actions[:, None, :]
+        T = int(seq[0].shape[1])
+        act1 = actions[:, :max(T - 1, 0)] if actions is not None else None
+        if self.learn_A:
+            lr = lr_pA if not isinstance(lr_pA, torch.Tensor) else lr_pA.expand(self.batch_size)
+            qA, E_qA = learning.update_obs_likelihood_dirichlet(
+                self.pA, self.A, observations, beliefs_A, A_dependencies=self.A_dependencies,
+                categorical_obs=self.categorical_obs, num_obs=self.num_obs, lr=lr)
+            agent.pA, agent.A = qA, E_qA
+        if self.learn_B:
+            # some transition of the batch is valid (pymdp/agent.py:486-499)
+            can_update = (act1 is not None and T > 1 and int(act1.shape[1]) == T - 1
+                          and bool((act1 >= 0).all(-1).any()))
+            if can_update:
+                joints = [[seq[f][:, 1:]] + [seq[d][:, :-1] for d in self.B_dependencies[f]]
+                          for f in range(self.num_factors)]
+                lr = lr_pB if not isinstance(lr_pB, torch.Tensor) else lr_pB.expand(self.batch_size)
+                qB, E_qB = learning.update_state_transition_dirichlet(
+                    self.pB, self.B, joints, act1, num_controls=self.num_controls, lr=lr, mask_invalid=True)
+                agent.pB, agent.B = qB, E_qB
+                if self.use_inductive and self.H is not None:
+                    # inductive matrices from the learned B (pymdp/agent.py:606-615)
+                    I = []
+                    for f in range(self.num_factors):
+                        h = torch.as_tensor(self.H[f], dtype=torch.float32, device=self.device)
+                        I.append(torch.stack([
+                            _generate_I(h if h.dim() == 1 else h[b], E_qB[f][b], float(self.inductive_threshold[b]),
+                                        self.inductive_depth) for b in range(self.batch_size)]))
+                    agent.I = I
+        return agent
 
     def multiaction_probabilities(self, q_pi):
         """pymdp/agent.py:908-937."""

@@ -56,6 +56,21 @@ def plain_fpi_large(seed: int = 0, device="cpu", batch: int = 1024) -> dict:
                 batch=batch)
 
 
+def learning_large(seed: int = 0, device="cpu", batch: int = 1024) -> dict:
+    """Dirichlet learning of A on the large plain-FPI model: 1024 agents,
+    S=[16,16,8,8], 6 modalities x 32 outcomes (2 MiB of counts per modality
+    per agent), pA ~ U[1, 2), one time step of Dirichlet(1) beliefs and
+    uniform observation indices."""
+    g = utils._gen(seed, device)
+    S, O = [16, 16, 8, 8], [32] * 6
+    deps = [[0, 1, 2, 3] for _ in O]
+    pA = [1.0 + torch.rand((batch, o) + tuple(S), generator=g, device=device) for o in O]
+    qs = [q[:, None, :] for q in utils.random_factorized_categorical(S, batch=batch, alpha=1.0, generator=g,
+                                                                      device=device)]
+    obs = [torch.randint(0, o, (batch, 1), generator=g, device=device) for o in O]
+    return dict(name="learning_large", S=S, O=O, pA=pA, qs=qs, obs=obs, A_dependencies=deps, batch=batch)
+
+
 def plain_efe(seed: int = 0, device="cpu", batch: int = 64, S=(32, 16, 8), U=(8, 4, 1), T: int = 3,
               O=(64, 64, 64, 64)) -> dict:
     g = utils._gen(seed, device)
