@@ -64,13 +64,15 @@ def test_update_pA_parity(dev, O, S, deps, T, shared, categorical):
         _check(_np(EA[0][b]), e64, False)
 
 
-@pytest.mark.parametrize("mask", [False, True])
-def test_update_pB_parity(dev, mask):
-    print(f"update_pB mask={mask}", flush=True)
+@pytest.mark.parametrize("mask,S,U,B_deps", [
+    (False, [6, 3, 70], [4, 1, 2], [[0, 1], [1], [2]]),
+    (True, [6, 3, 70], [4, 1, 2], [[0, 1], [1], [2]]),
+    (True, [40, 12, 3], [4, 1, 3], [[0, 1], [1], [2]]),  # full 128-column tiles, 16-byte path
+])
+def test_update_pB_parity(dev, mask, S, U, B_deps):
+    print(f"update_pB mask={mask} S={S}", flush=True)
     rng = np.random.default_rng(9)
     batch, T = 5, 4
-    S, U = [6, 3, 70], [4, 1, 2]
-    B_deps = [[0, 1], [1], [2]]
     qs = [rng.dirichlet(np.ones(s), size=(batch, T)).astype(np.float32) for s in S]
     acts = rng.integers(-1, 4, size=(batch, T - 1, 3)).astype(np.int32)
     acts[:, :, 1] = np.where(acts[:, :, 1] < 0, -1, 0)
