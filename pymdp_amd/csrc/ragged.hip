@@ -155,6 +155,27 @@ __device__ __forceinline__ void rg_stage(long long n, int tid, Ld ld, St st) {
   }
 }
 
+// *dst = *src for a plain record in global memory, copied into LDS with every
+// load issued before the first store.  The kernels read their agent / unit
+// records from LDS: a field read is then an LDS access instead of a global
+// round trip that the compiler repeats after every global store it cannot
+// prove unaliased (and chains when one field indexes another).  The caller
+// places a barrier before the copy is read.
+template <typename T>
+__device__ __forceinline__ void rg_copy_lds(const T* src, T* dst, int tid) {
+  static_assert(sizeof(T) % 4 == 0, "record of whole dwords");
+  constexpr int n = (int)(sizeof(T) / 4), NB = (n + kNT - 1) / kNT;
+  const int* sp = reinterpret_cast<const int*>(src);
+  int* dp = reinterpret_cast<int*>(dst);
+  int v[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q)
+    if (tid + q * kNT < n) v[q] = sp[tid + q * kNT];
+#pragma unroll
+  for (int q = 0; q < NB; ++q)
+    if (tid + q * kNT < n) dp[tid + q * kNT] = v[q];
+}
+
 // ---------------------------------------------------------------------------
 // FPI units
 // ---------------------------------------------------------------------------
@@ -286,6 +307,8 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
   __shared__ double s_ltab[256];
   __shared__ int s_obs[kM];
   __shared__ int s_item, s_broken;
+  __shared__ Agent s_ag;
+  __shared__ Unit s_un;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
   for (int i = tid; i < 256; i += kNT) s_ltab[i] = g_logtab[i];
 
@@ -297,8 +320,12 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
     __syncthreads();
     const int idx = s_item;
     if (idx >= b.n_units) break;
-    const Unit& u = b.units[idx];
-    const Agent& a = b.ag[u.agent];
+    rg_copy_lds(b.units + idx, &s_un, tid);
+    __syncthreads();
+    rg_copy_lds(b.ag + s_un.agent, &s_ag, tid);
+    __syncthreads();
+    const Unit& u = s_un;
+    const Agent& a = s_ag;
     const int totS = a.totS;
     if (b.trace && tid == 0) b.trace[2 * idx] = wall_clock64();
 
@@ -660,6 +687,7 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
   __shared__ double s_h[kE1H];
   __shared__ double s_q[kE1Q];
   __shared__ double s_hap[kNT];
+  __shared__ long long s_cb[16];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
   const Tile tl = b.t1[blockIdx.x];
   const Agent& a = b.ag[tl.agent];
@@ -700,23 +728,31 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
         ++rl;
       }
     }
+    // while the A loads are in flight, wave 0 finds where each column's
+    // history vector lives (one lookup per column, not per element)
+    if (tid < 16) {
+      long long cb = -1;
+      if (tid < nc) {
+        const int n = tl.c0 + tid, f = a.ef[m][2];
+        int t = 0;
+        while (t + 1 < a.T && n >= a.tstart[m][t + 1]) ++t;
+        cb = a.hist_off + a.hist_lvl[t][f] + (long long)(n - a.tstart[m][t]) * a.S[f];
+      }
+      s_cb[tid] = cb;
+    }
 #pragma unroll
     for (int q = 0; q < NB; ++q)
       if (tid + q * kNT < tot) s_a[off[q]] = v[q];
-    const int f = a.ef[m][2];
-    const double* hist = b.hist + a.hist_off;
+  }
+  lds_barrier();
+  {
+    const double* hist = b.hist;
     rg_stage<double, (16 * 64 + kNT - 1) / kNT>(
         16 * d2p, tid,
         [&](long long e) {
           const int c = (int)e / d2p, k = (int)e - c * d2p;
-          double q = 0.0;
-          if (c < nc && k < d2) {
-            const int n = tl.c0 + c;
-            int t = 0;
-            while (t + 1 < a.T && n >= a.tstart[m][t + 1]) ++t;
-            q = hist[a.hist_lvl[t][f] + (long long)(n - a.tstart[m][t]) * a.S[f] + k];
-          }
-          return q;
+          const long long cb = s_cb[c];
+          return cb >= 0 && k < d2 ? hist[cb + k] : 0.0;
         },
         [&](long long e, double q) {
           const int c = (int)e / d2p, k = (int)e - c * d2p;
