@@ -108,6 +108,24 @@ int pymdp_hip_infer_states(const float* const* A_ptrs, const int64_t* A_batch_st
                            const int64_t* A_dep_flat, const int64_t* A_dep_offsets,
                            int32_t num_iter, int32_t method_exact, void* stream);
 
+/*
+ * Variational free energy of the posterior qs [batch, sum S] (e.g. the
+ * output of pymdp_hip_infer_states): the return_info diagnostics of
+ * update_posterior_states for "fpi" / "exact" (pymdp/inference.py:456-483,
+ * pymdp/maths.py:448-762 single-step branch).  Arguments as for
+ * pymdp_hip_infer_states; vfe_out [batch, 4] = (neg_entropy,
+ * prior_cross_entropy, accuracy, vfe), vfe = neg_entropy +
+ * prior_cross_entropy - accuracy (no parameter KL: infer_states passes none).
+ * Limits: F, M <= 16, sum S <= 2048, modality rank <= 8.
+ */
+int pymdp_hip_fpi_vfe(const float* const* A_ptrs, const int64_t* A_batch_stride,
+                      const void* obs, int32_t obs_is_index,
+                      const float* const* prior_ptrs, const int64_t* prior_batch_stride,
+                      const float* qs, float* vfe_out, int64_t batch,
+                      const int64_t* S, int64_t F, const int64_t* O, int64_t M,
+                      const int64_t* obs_offsets, const int64_t* A_dep_flat,
+                      const int64_t* A_dep_offsets, void* stream);
+
 /* Flag bits of the neg-EFE entries; identical to the reference
  * (pymdp/ffi/neg_efe/neg_efe_layout.h:23-42, pymdp/ffi/_efe.py:26-37). */
 #define PYMDP_HIP_FLAG_USE_UTILITY (1 << 0)

@@ -531,3 +531,24 @@ def infer_parameters(pA, A, pB, B, beliefs_A, observations, actions, B_dependenc
                                                          for d in B_dependencies[f]] for f in range(len(pB))]
         pB, B = update_state_transition_dirichlet(pB, B, joints, act1, num_controls, lr_pB, dt=dt)
     return pA, A, pB, B
+
+
+# ---------------------------------------------------------------------------
+# Variational free energy of a one-step posterior (return_info diagnostics)
+# ---------------------------------------------------------------------------
+def stable_cross_entropy(x, y, dt=np.float32):
+    """pymdp/maths.py:47-62: -sum x ln max(y, MINVAL) (0 where x == 0)."""
+    x, y = _a(x, dt), _a(y, dt)
+    return -np.sum(np.where(x == 0, dt(0), x * np.log(np.maximum(y, dt(MINVAL)))))
+
+
+def calc_vfe_one_step(qs, prior, obs, A, A_dependencies, distr_obs=True, dt=np.float32):
+    """pymdp/maths.py:448-762, single-step branch (`qs[f]` of shape [S_f]; the
+    kwargs pymdp/inference.py:314-322 builds for "fpi" / "exact"): returns
+    (neg_entropy, prior_cross_entropy, accuracy, vfe) with
+    vfe = sum_f -H(q_f) + sum_f CE(q_f, prior_f) - sum_m E_q[ll_m]."""
+    neg_entropy = sum(-stable_entropy(q, dt) for q in qs)
+    cross = sum(stable_cross_entropy(q, p, dt) for q, p in zip(qs, prior))
+    lls = compute_log_likelihood_per_modality(obs, A, dt, distr_obs)
+    accuracy = sum(factor_dot(ll, [_a(qs[f], dt) for f in deps]) for ll, deps in zip(lls, A_dependencies))
+    return neg_entropy, cross, accuracy, neg_entropy + cross - accuracy

@@ -47,6 +47,15 @@ SIGNATURES: dict[str, tuple] = {
         _i64p, _i64p,                        # obs_offsets, lp_offsets
         _i64p, _i64p,                        # A_dep_flat, A_dep_offsets
         ctypes.c_int32, ctypes.c_int32, _vp]),
+    "pymdp_hip_fpi_vfe": (ctypes.c_int, [
+        _vp, _i64p,                          # A_ptrs, A_batch_stride
+        _vp, ctypes.c_int32,                 # obs, obs_is_index
+        _vp, _i64p,                          # prior_ptrs, prior_batch_stride
+        _vp, _vp, ctypes.c_int64,            # qs, vfe_out, batch
+        _i64p, ctypes.c_int64,               # S, F
+        _i64p, ctypes.c_int64,               # O, M
+        _i64p, _i64p, _i64p,                 # obs_offsets, A_dep_flat, A_dep_offsets
+        _vp]),
     "pymdp_hip_neg_efe": (ctypes.c_int, [
         _vp, ctypes.c_int64, ctypes.c_int64,     # policy_host, P, T
         _vp,                                     # qs_flat

@@ -8,8 +8,10 @@ without the leading batch axis is shared by every agent (no broadcast copy
 is made: the kernels read it with batch stride 0).
 
 Not on the hot path and therefore not provided: sequence inference methods
-(mmp / vmp / ovf) and the smoothing-based learning they enable, `return_info`
-VFE diagnostics and multi-action flattening (`B_action_dependencies`).
+(mmp / vmp / ovf) and the smoothing-based learning they enable, and
+multi-action flattening (`B_action_dependencies`).  `infer_states(...,
+return_info=True)` returns the VFE diagnostics of the posterior
+(`pymdp_hip_fpi_vfe`).
 `infer_parameters` (Dirichlet learning of A and B, pymdp/agent.py:416-620)
 runs the fused learning kernels of `pymdp_amd.learning`.  Parameter information gain
 (`use_param_info_gain` with `pA` / `pB`) is computed by the neg-EFE kernels.
@@ -26,7 +28,7 @@ import torch
 from . import control, learning
 from .control import Policies, construct_policies
 from .ffi._efe import update_posterior_policies_ffi
-from .ffi._fpi import infer_states_ffi
+from .ffi._fpi import fpi_vfe_ffi, infer_states_ffi
 
 
 def _resolve_a(num_factors, num_modalities, A_dependencies):
@@ -170,9 +172,9 @@ class Agent:
     def infer_states(self, observations, empirical_prior, *, past_actions=None, qs_hist=None, valid_steps=None,
                      mask=None, preprocess_fn=None, return_info=False):
         """pymdp/agent.py:675-823: posterior qs[f] of shape [batch, 1, S_f]
-        (appended to qs_hist when given)."""
-        if return_info:
-            raise NotImplementedError("return_info VFE diagnostics are not on the HIP hot path")
+        (appended to qs_hist when given); with return_info also the VFE
+        diagnostics of the new posterior (pymdp/inference.py:456-483):
+        {"vfe_t", "vfe", "vfe_components"}, each entry [batch]."""
         if not isinstance(empirical_prior, (list, tuple)):
             raise ValueError("`empirical_prior` must be a list/tuple with one entry per hidden-state factor.")
         if len(empirical_prior) != self.num_factors:
@@ -202,11 +204,22 @@ class Agent:
             obs = [o if o.dim() == 2 else o[:, -1] for o in obs]
         qs = infer_states_ffi(A, obs, prior, self.A_dependencies, self.num_iter, distr_obs=distr,
                               method_exact=(self.inference_algo == "exact"), batch=self.batch_size)
+        info = None
+        if return_info:
+            v = fpi_vfe_ffi(A, obs, prior, torch.cat(qs, -1), self.A_dependencies, distr_obs=distr,
+                            batch=self.batch_size)
+            zero = torch.zeros_like(v[:, 3])
+            info = {"vfe_t": v[:, 3], "vfe": v[:, 3], "vfe_components": {
+                "neg_entropy_t": v[:, 0], "prior_cross_entropy_t": v[:, 1], "transition_cross_entropy_t": zero,
+                "accuracy_t": v[:, 2], "state_vfe": v[:, 3], "parameter_kl_A": zero, "parameter_kl_B": zero,
+                "parameter_kl": zero}}
         qs = [q[:, None, :] for q in qs]
         if qs_hist is not None:
             qs = [torch.cat([h, q], 1) for h, q in zip(qs_hist, qs)]
             if self.inference_horizon is not None:
                 qs = [q[:, -self.inference_horizon:] for q in qs]
+        if return_info:
+            return qs, info
         return qs
 
     def update_empirical_prior(self, action, qs):

@@ -126,6 +126,45 @@ def infer_states_ffi(A: Sequence[torch.Tensor], obs: Sequence[torch.Tensor], pri
     prior[f] [batch, S_f] probabilities (any row stride; last axis contiguous)
     Returns qs[f] [batch, S_f] (views into one packed [batch, sum S] buffer).
     """
+    k = _pack_infer_args(A, obs, prior, A_dependencies, num_iter, distr_obs, batch)
+    batch, F, lp_offsets = k["batch"], k["F"], k["lp_offsets"]
+    totS = int(lp_offsets[-1])
+    if out is None:
+        out = torch.empty((batch, totS), dtype=torch.float32, device=prior[0].device)
+    a = k["i64"]
+    rc = _native.lib().pymdp_hip_infer_states(
+        k["Ap"], a["As"].ptr, k["obs_t"].data_ptr(), k["is_index"], k["Pp"], a["Ps"].ptr, out.data_ptr(), batch,
+        a["S"].ptr, F, a["O"].ptr, k["M"], a["oo"].ptr, a["lo"].ptr, a["df"].ptr, a["do"].ptr,
+        int(num_iter), int(bool(method_exact)), _native.stream_handle(prior[0].device))
+    _native.check(rc)
+    return [out[:, int(lp_offsets[f]):int(lp_offsets[f + 1])] for f in range(F)]
+
+
+def fpi_vfe_ffi(A: Sequence[torch.Tensor], obs: Sequence[torch.Tensor], prior: Sequence[torch.Tensor],
+                qs: torch.Tensor, A_dependencies: Sequence[Sequence[int]], *, distr_obs: bool = True,
+                batch: int | None = None) -> torch.Tensor:
+    """Variational free energy of a one-step posterior (`pymdp_hip_fpi_vfe`),
+    the return_info diagnostics of pymdp/inference.py:456-483 for "fpi" /
+    "exact": [batch, 4] = (neg_entropy, prior_cross_entropy, accuracy, vfe).
+    A, obs, prior as for infer_states_ffi; qs the packed [batch, sum S]
+    posterior."""
+    k = _pack_infer_args(A, obs, prior, A_dependencies, 1, distr_obs, batch)
+    batch = k["batch"]
+    totS = int(k["lp_offsets"][-1])
+    qs = qs.to(torch.float32).reshape(batch, totS).contiguous()
+    out = torch.empty((batch, 4), dtype=torch.float32, device=qs.device)
+    a = k["i64"]
+    rc = _native.lib().pymdp_hip_fpi_vfe(
+        k["Ap"], a["As"].ptr, k["obs_t"].data_ptr(), k["is_index"], k["Pp"], a["Ps"].ptr, qs.data_ptr(),
+        out.data_ptr(), batch, a["S"].ptr, k["F"], a["O"].ptr, k["M"], a["oo"].ptr, a["df"].ptr, a["do"].ptr,
+        _native.stream_handle(qs.device))
+    _native.check(rc)
+    return out
+
+
+def _pack_infer_args(A, obs, prior, A_dependencies, num_iter, distr_obs, batch):
+    """Device pointers, strides and int64 attribute arrays of the fused
+    infer_states ABI (shared with the VFE entry)."""
     F = len(prior)
     M = len(A)
     if len(A_dependencies) != M or (not isinstance(obs, torch.Tensor) and len(obs) != M):
@@ -183,20 +222,11 @@ def infer_states_ffi(A: Sequence[torch.Tensor], obs: Sequence[torch.Tensor], pri
     obs_offsets[1:] = np.cumsum(O)
     lp_offsets = np.zeros(F + 1, dtype=np.int64)
     lp_offsets[1:] = np.cumsum(S)
-    totS = int(lp_offsets[-1])
-    if out is None:
-        out = torch.empty((batch, totS), dtype=torch.float32, device=prior[0].device)
     A_dep_flat, A_dep_offsets = flatten_dep_lists(A_dependencies)
-    a = {k: _native.I64(v) for k, v in dict(As=A_stride, Ps=prior_stride, S=S, O=O, oo=obs_offsets,
-                                               lo=lp_offsets, df=A_dep_flat, do=A_dep_offsets).items()}
-    Ap = _native.ptr_array(A_ptrs)
-    Pp = _native.ptr_array(prior_ptrs)
-    rc = _native.lib().pymdp_hip_infer_states(
-        Ap, a["As"].ptr, obs_t.data_ptr(), is_index, Pp, a["Ps"].ptr, out.data_ptr(), batch,
-        a["S"].ptr, F, a["O"].ptr, M, a["oo"].ptr, a["lo"].ptr, a["df"].ptr, a["do"].ptr,
-        int(num_iter), int(bool(method_exact)), _native.stream_handle(prior[0].device))
-    _native.check(rc)
-    return [out[:, int(lp_offsets[f]):int(lp_offsets[f + 1])] for f in range(F)]
+    i64 = {k: _native.I64(v) for k, v in dict(As=A_stride, Ps=prior_stride, S=S, O=O, oo=obs_offsets,
+                                                 lo=lp_offsets, df=A_dep_flat, do=A_dep_offsets).items()}
+    return dict(batch=batch, F=F, M=M, lp_offsets=lp_offsets, obs_t=obs_t, is_index=is_index, i64=i64,
+                Ap=_native.ptr_array(A_ptrs), Pp=_native.ptr_array(prior_ptrs))
 
 
-__all__ = ["can_handle_fpi", "fpi_ffi", "infer_states_ffi"]
+__all__ = ["can_handle_fpi", "fpi_ffi", "infer_states_ffi", "fpi_vfe_ffi"]
