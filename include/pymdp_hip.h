@@ -126,6 +126,29 @@ int pymdp_hip_fpi_vfe(const float* const* A_ptrs, const int64_t* A_batch_stride,
                       const int64_t* obs_offsets, const int64_t* A_dep_flat,
                       const int64_t* A_dep_offsets, void* stream);
 
+/*
+ * Sequence inference over a window of T steps: marginal (method 0) or
+ * variational (method 1) message passing, the sequence branch of
+ * update_posterior_states (pymdp/inference.py:194-271 -> pymdp/algos.py:
+ * 157-500), for factor-local transitions.  A as for pymdp_hip_infer_states;
+ * B_ptrs[f] [batch?, S_f, S_f, U_f]; obs [batch, T, M] int32 indices or
+ * [batch, T, sum O] distributions (obs_offsets); prior_ptrs[f] [batch?, S_f];
+ * actions [batch, T-1, F] int32 or NULL (every U_f must then be 1);
+ * valid_steps [batch] int32 or NULL (no masks: every transition counts, a
+ * negative action selects the uniform transition); q_out [batch, T, sum S].
+ * Limits: T <= 64, F, M <= 8, S_f <= 64, T * sum S <= 1024,
+ * T * sum_m prod S[deps_m] <= 4096, modality rank <= 4.  Errors "seq_ffi: ".
+ */
+int pymdp_hip_sequence_infer(int32_t method, const float* const* A_ptrs, const int64_t* A_batch_stride,
+                             const float* const* B_ptrs, const int64_t* B_batch_stride,
+                             const void* obs, int32_t obs_is_index,
+                             const float* const* prior_ptrs, const int64_t* prior_batch_stride,
+                             const int32_t* actions, const int32_t* valid_steps, float* q_out,
+                             int64_t batch, int64_t T, const int64_t* S, const int64_t* U, int64_t F,
+                             const int64_t* O, int64_t M, const int64_t* obs_offsets,
+                             const int64_t* A_dep_flat, const int64_t* A_dep_offsets,
+                             int32_t num_iter, float tau, void* stream);
+
 /* Flag bits of the neg-EFE entries; identical to the reference
  * (pymdp/ffi/neg_efe/neg_efe_layout.h:23-42, pymdp/ffi/_efe.py:26-37). */
 #define PYMDP_HIP_FLAG_USE_UTILITY (1 << 0)

@@ -56,6 +56,17 @@ SIGNATURES: dict[str, tuple] = {
         _i64p, ctypes.c_int64,               # O, M
         _i64p, _i64p, _i64p,                 # obs_offsets, A_dep_flat, A_dep_offsets
         _vp]),
+    "pymdp_hip_sequence_infer": (ctypes.c_int, [
+        ctypes.c_int32,                      # method
+        _vp, _i64p, _vp, _i64p,              # A ptrs/stride, B ptrs/stride
+        _vp, ctypes.c_int32,                 # obs, obs_is_index
+        _vp, _i64p,                          # prior ptrs/stride
+        _vp, _vp, _vp,                       # actions, valid_steps, q_out
+        ctypes.c_int64, ctypes.c_int64,      # batch, T
+        _i64p, _i64p, ctypes.c_int64,        # S, U, F
+        _i64p, ctypes.c_int64, _i64p,        # O, M, obs_offsets
+        _i64p, _i64p,                        # A_dep_flat, A_dep_offsets
+        ctypes.c_int32, ctypes.c_float, _vp]),
     "pymdp_hip_neg_efe": (ctypes.c_int, [
         _vp, ctypes.c_int64, ctypes.c_int64,     # policy_host, P, T
         _vp,                                     # qs_flat

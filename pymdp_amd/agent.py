@@ -7,9 +7,10 @@ argument meaning and return shapes.  Model tensors are torch tensors on a HIP de
 without the leading batch axis is shared by every agent (no broadcast copy
 is made: the kernels read it with batch stride 0).
 
-Not on the hot path and therefore not provided: sequence inference methods
-(mmp / vmp / ovf) and the smoothing-based learning they enable, and
-multi-action flattening (`B_action_dependencies`).  `infer_states(...,
+Sequence inference ("mmp", "vmp") runs the window kernel
+(`pymdp_hip_sequence_infer`).  Not provided: online filtering / smoothing
+("ovf") and the smoothing-based learning it enables, and multi-action
+flattening (`B_action_dependencies`).  `infer_states(...,
 return_info=True)` returns the VFE diagnostics of the posterior
 (`pymdp_hip_fpi_vfe`).
 `infer_parameters` (Dirichlet learning of A and B, pymdp/agent.py:416-620)
@@ -29,6 +30,7 @@ from . import control, learning
 from .control import Policies, construct_policies
 from .ffi._efe import update_posterior_policies_ffi
 from .ffi._fpi import fpi_vfe_ffi, infer_states_ffi
+from .ffi._seq import sequence_infer_ffi
 
 
 def _resolve_a(num_factors, num_modalities, A_dependencies):
@@ -101,8 +103,8 @@ class Agent:
         self.num_obs = [int(a.shape[-len(self.A_dependencies[m]) - 1]) for m, a in enumerate(A)]
         self.num_iter = num_iter
         self.inference_algo = inference_algo.lower()
-        if self.inference_algo not in ("fpi", "exact"):
-            raise NotImplementedError(f"inference_algo {inference_algo!r} is not on the HIP hot path")
+        if self.inference_algo not in ("fpi", "exact", "mmp", "vmp"):
+            raise NotImplementedError(f"inference_algo {inference_algo!r} is not built (fpi, exact, mmp, vmp)")
         self.inference_horizon = inference_horizon
         self.policy_len = policy_len
         self.action_selection = action_selection
@@ -180,6 +182,8 @@ class Agent:
         if len(empirical_prior) != self.num_factors:
             raise ValueError(f"`empirical_prior` has {len(empirical_prior)} factor(s), expected {self.num_factors}")
         prior = [p if p.dim() == 2 else p.reshape(self.batch_size, -1) for p in empirical_prior]
+        if self.inference_algo in ("mmp", "vmp"):
+            return self._infer_states_sequence(observations, prior, past_actions, valid_steps, return_info)
         fn = preprocess_fn or self.preprocess_fn
         if fn is not None:
             obs, distr = fn(observations), True
@@ -222,8 +226,32 @@ class Agent:
             return qs, info
         return qs
 
+    def _infer_states_sequence(self, observations, prior, past_actions, valid_steps, return_info):
+        """Sequence branch of pymdp/agent.py:675-823: observations[m] is the
+        window [batch, T] (indices) or [batch, T, O_m] (categorical_obs);
+        returns qs[f] [batch, T, S_f]."""
+        if return_info:
+            raise NotImplementedError("return_info for sequence methods (sequence VFE) is not built")
+        if self.categorical_obs:
+            obs, distr = [torch.as_tensor(o, device=self.device).float() for o in observations], True
+        else:
+            obs = [torch.as_tensor(o, device=self.device).reshape(self.batch_size, -1) for o in observations]
+            distr = False
+        if past_actions is not None:
+            past_actions = torch.as_tensor(past_actions, device=self.device)
+        if self.inference_horizon is not None:
+            h = self.inference_horizon
+            obs = [o[:, -h:] for o in obs]
+            if past_actions is not None:
+                past_actions = past_actions[:, :0] if h <= 1 else past_actions[:, -(h - 1):]
+        return sequence_infer_ffi(self.inference_algo, self.A, self.B, obs, prior, self.A_dependencies,
+                                  self.B_dependencies, past_actions=past_actions, valid_steps=valid_steps,
+                                  num_iter=self.num_iter, distr_obs=distr, batch=self.batch_size)
+
     def update_empirical_prior(self, action, qs):
-        """pymdp/agent.py:825-851."""
+        """pymdp/agent.py:825-851 (sequence methods keep D)."""
+        if self.inference_algo in ("mmp", "vmp"):
+            return self.D
         qs_last = [q[:, -1] for q in qs]
         return control.compute_expected_state(qs_last, self.B, torch.as_tensor(action, device=self.device),
                                               self.B_dependencies)
