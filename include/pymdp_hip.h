@@ -390,6 +390,11 @@ int pymdp_hip_ragged_info(void* plan, int64_t* out16);
  * the three tile lists concatenated. */
 int pymdp_hip_ragged_tables(void* plan, int64_t* sizes16, void* agents_out,
                             void* units_out, void* tiles_out, int32_t* iarena_out);
+/* Host copies of the stage-1 tile records (per stage-1 tile: params offset
+ * of its A block, stage-1 output offset, history offset of each of its 16
+ * columns, sizes); count_out / record_bytes receive the count and the record
+ * size; out may be NULL. */
+int pymdp_hip_ragged_e1_records(void* plan, void* out, int64_t* count_out, int64_t* record_bytes);
 /* One step on `stream`: five launches (FPI units, three EFE stages, per-agent
  * assembly), no allocation, no synchronisation.  Steps that overlap on one
  * device race on the plan's scratch. */

@@ -103,10 +103,20 @@ struct Tile {
   int agent, m, t, r0, r1, c0, c1, pad;
 };
 
+// Stage-1 tile record, built on the host: every address the tile needs, so
+// that its A and history loads depend on this one record load only.
+struct E1Rec {
+  long long a_off;   // params offset of the tile's first A element (position-major)
+  long long z_off;   // z1 offset of the tile's first position
+  long long cb[16];  // hist offset of each history column's vector (-1 past nc)
+  int O, d2, d01, nr, nc, c0, pad0, pad1;
+};
+
 struct Bufs {
   const Agent* ag;
   const Unit* units;
   const Tile* t1;
+  const E1Rec* e1;
   const Tile* t2;
   const Tile* t3;
   const int* iarena;
@@ -210,20 +220,23 @@ __device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const doub
   const double* q0 = R >= 2 ? s_qv + a.S_off[a.gfac[g][0]] : nullptr;
   const double* q1 = R == 3 ? s_qv + a.S_off[a.gfac[g][1]] : nullptr;
   const int d1 = R == 3 ? a.gdim[g][1] : 1;
-  // row dot products (R >= 2): 2^lt lanes per row, two accumulators each
+  // row dot products (R >= 2): 2^lt lanes per row, four accumulators each
+  // (eight LDS reads in flight per wait)
   if (R >= 2) {
     const int lt = rows_lg(n), tpr = 1 << lt;
     for (int e = tid; e < nr * tpr; e += kNT) {
       const int rl = e >> lt, sub = e & (tpr - 1);
       const double* x = X + rl * ld;
-      double v0 = 0.0, v1 = 0.0;
+      double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
       int k = sub;
-      for (; k + tpr < n; k += 2 * tpr) {
+      for (; k + 3 * tpr < n; k += 4 * tpr) {
         v0 = fma(x[k], qc[k], v0);
         v1 = fma(x[k + tpr], qc[k + tpr], v1);
+        v2 = fma(x[k + 2 * tpr], qc[k + 2 * tpr], v2);
+        v3 = fma(x[k + 3 * tpr], qc[k + 3 * tpr], v3);
       }
-      if (k < n) v0 = fma(x[k], qc[k], v0);
-      double v = group_sum_low(v0 + v1, lt);
+      for (; k < n; k += tpr) v0 = fma(x[k], qc[k], v0);
+      double v = group_sum_low((v0 + v1) + (v2 + v3), lt);
       if (sub == 0) {
         if (R == 2)
           s_part[a.S_off[a.gfac[g][0]] + r0 + rl] += v;
@@ -237,30 +250,48 @@ __device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const doub
   const int nrg = kNT / n;
   const int col = tid % n, rg = tid / n;
   if (rg < nrg) {
-    double acc0 = 0.0, acc1 = 0.0;
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
     if (R == 1) {
       if (rg == 0) acc0 = X[col];
     } else if (R == 2) {
       int rl = rg;
-      for (; rl + nrg < nr; rl += 2 * nrg) {
+      for (; rl + 3 * nrg < nr; rl += 4 * nrg) {
         acc0 = fma(q0[r0 + rl], X[rl * ld + col], acc0);
         acc1 = fma(q0[r0 + rl + nrg], X[(rl + nrg) * ld + col], acc1);
+        acc2 = fma(q0[r0 + rl + 2 * nrg], X[(rl + 2 * nrg) * ld + col], acc2);
+        acc3 = fma(q0[r0 + rl + 3 * nrg], X[(rl + 3 * nrg) * ld + col], acc3);
       }
-      if (rl < nr) acc0 = fma(q0[r0 + rl], X[rl * ld + col], acc0);
+      for (; rl < nr; rl += nrg) acc0 = fma(q0[r0 + rl], X[rl * ld + col], acc0);
     } else {
+      // two rows per step: (i, j) of rows rl and rl + nrg advance by 2 nrg
       const int di = nrg / d1, dj = nrg - di * d1;
       int i = (r0 + rg) / d1, j = (r0 + rg) - i * d1;
-      for (int rl = rg; rl < nr; rl += nrg) {
+      int i2 = i + di, j2 = j + dj;
+      if (j2 >= d1) {
+        j2 -= d1;
+        ++i2;
+      }
+      const int ei = (2 * nrg) / d1, ej = 2 * nrg - ei * d1;
+      int rl = rg;
+      for (; rl + nrg < nr; rl += 2 * nrg) {
         acc0 = fma(q0[i] * q1[j], X[rl * ld + col], acc0);
-        i += di;
-        j += dj;
+        acc1 = fma(q0[i2] * q1[j2], X[(rl + nrg) * ld + col], acc1);
+        i += ei;
+        j += ej;
         if (j >= d1) {
           j -= d1;
           ++i;
         }
+        i2 += ei;
+        j2 += ej;
+        if (j2 >= d1) {
+          j2 -= d1;
+          ++i2;
+        }
       }
+      if (rl < nr) acc0 = fma(q0[i] * q1[j], X[rl * ld + col], acc0);
     }
-    s_red[tid] = acc0 + acc1;
+    s_red[tid] = (acc0 + acc1) + (acc2 + acc3);
   }
   __syncthreads();
   if (tid < n) {
@@ -687,35 +718,38 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
   __shared__ double s_h[kE1H];
   __shared__ double s_q[kE1Q];
   __shared__ double s_hap[kNT];
-  __shared__ long long s_cb[16];
+  __shared__ E1Rec s_rec;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
-  const Tile tl = b.t1[blockIdx.x];
-  const Agent& a = b.ag[tl.agent];
   long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_units + blockIdx.x) : nullptr;
   if (tr && tid == 0) tr[0] = wall_clock64();
-  const int m = tl.m, O = a.O[m], O1 = O + 1;
-  const int d01 = a.ed[m][0] * a.ed[m][1], d2 = a.ed[m][2];
+  rg_copy_lds(b.e1 + blockIdx.x, &s_rec, tid);
+  lds_barrier();
+  const E1Rec& rc = s_rec;
+  const int O = rc.O, O1 = O + 1, d01 = rc.d01, d2 = rc.d2;
   const int d2p = (d2 + 3) & ~3, lda = d2p + 1;
-  const int nr = tl.r1 - tl.r0, nc = tl.c1 - tl.c0;
-  const long long K = a.K[m];
+  const int nr = rc.nr, nc = rc.nc;
   {
     // A_m is position-major, so the tile's rows (position rl, outcome o) are
-    // one contiguous chunk: element e = (rl, o, k) at Am[e], LDS row o nr + rl
-    const float* Am = b.params + a.A_off[m] + (long long)tl.r0 * O * d2;
+    // one contiguous chunk: element e = (rl, o, k) at Am[e], LDS row o nr + rl.
+    // The history columns are loaded in the same round trip.
+    const float* Am = b.params + rc.a_off;
     constexpr int NB = (kE1A + kNT - 1) / kNT;
+    constexpr int NQ = (16 * 64 + kNT - 1) / kNT;
     const int per = O * d2;  // floats per position
     const int tot = nr * per;
     const int drl = kNT / per, dm = kNT - drl * per;  // per-step advance of (rl, rem)
     const int dq = dm / d2, dk = dm - dq * d2;
     int rl = tid / per, rem = tid - rl * per;
     int o = rem / d2, k = rem - o * d2;
+    // loads are unconditional (past the block's end they repeat its last
+    // element and are not stored), so all of them are issued back to back
     float v[NB];
     int off[NB];
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
       const int e = tid + q * kNT;
       off[q] = (o * nr + rl) * lda + k;
-      if (e < tot) v[q] = __builtin_nontemporal_load(Am + e);
+      v[q] = __builtin_nontemporal_load(Am + min(e, tot - 1));
       k += dk;
       o += dq;
       if (k >= d2) {
@@ -728,36 +762,26 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
         ++rl;
       }
     }
-    // while the A loads are in flight, wave 0 finds where each column's
-    // history vector lives (one lookup per column, not per element)
-    if (tid < 16) {
-      long long cb = -1;
-      if (tid < nc) {
-        const int n = tl.c0 + tid, f = a.ef[m][2];
-        int t = 0;
-        while (t + 1 < a.T && n >= a.tstart[m][t + 1]) ++t;
-        cb = a.hist_off + a.hist_lvl[t][f] + (long long)(n - a.tstart[m][t]) * a.S[f];
-      }
-      s_cb[tid] = cb;
+    double hv[NQ];
+    bool hok[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int e = tid + q * kNT, c = min(e / d2p, 15), kk = e - (e / d2p) * d2p;
+      const long long cb = rc.cb[c];
+      hok[q] = e < 16 * d2p && cb >= 0 && kk < d2;
+      hv[q] = b.hist[hok[q] ? cb + kk : 0];
     }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+      if (!hok[q]) hv[q] = 0.0;
 #pragma unroll
     for (int q = 0; q < NB; ++q)
       if (tid + q * kNT < tot) s_a[off[q]] = v[q];
-  }
-  lds_barrier();
-  {
-    const double* hist = b.hist;
-    rg_stage<double, (16 * 64 + kNT - 1) / kNT>(
-        16 * d2p, tid,
-        [&](long long e) {
-          const int c = (int)e / d2p, k = (int)e - c * d2p;
-          const long long cb = s_cb[c];
-          return cb >= 0 && k < d2 ? hist[cb + k] : 0.0;
-        },
-        [&](long long e, double q) {
-          const int c = (int)e / d2p, k = (int)e - c * d2p;
-          s_q[c * lda + k] = q;
-        });
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int e = tid + q * kNT, c = e / d2p, kk = e - c * d2p;
+      if (e < 16 * d2p) s_q[c * lda + kk] = hv[q];
+    }
   }
   lds_barrier();
   // sum_o A ln A per (ij, k): outcome axis split over thread groups, partial
@@ -799,7 +823,7 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
   const int R = O1 * nr;
   const int nblk = (R + 15) >> 4;
   const int li = lane & 15, lk = lane >> 4;
-  double* Z = b.z1 + a.z1_off[m] + tl.r0;
+  double* Z = b.z1 + rc.z_off;
   const double* qrow = s_q + li * lda + lk;
   for (int blk = wave; blk < nblk; blk += kNW) {
     const int r = (blk << 4) + li;  // this lane's A-operand row
@@ -819,7 +843,7 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
       const int rr = (blk << 4) + 4 * v + lk;
       if (rr < R && li < nc) {
         const int o = rr / nr, rl = rr - o * nr;
-        Z[((long long)(tl.c0 + li) * O1 + o) * d01 + rl] = acc[v];
+        Z[((long long)(rc.c0 + li) * O1 + o) * d01 + rl] = acc[v];
       }
     }
   }
@@ -1011,6 +1035,7 @@ struct Plan {
   std::vector<Agent> agents;
   std::vector<Unit> units;
   std::vector<Tile> t1, t2, t3;
+  std::vector<E1Rec> e1;
   std::vector<int32_t> iarena;
   std::vector<int64_t> layout;  // see pymdp_hip_ragged_layout
   long long hist_total = 0, score_total = 0, z1_total = 0, z2_total = 0, part_total = 0;
@@ -1024,6 +1049,7 @@ struct Plan {
   Agent* d_ag = nullptr;
   Unit* d_units = nullptr;
   Tile *d_t1 = nullptr, *d_t2 = nullptr, *d_t3 = nullptr;
+  E1Rec* d_e1 = nullptr;
   int32_t* d_iarena = nullptr;
   double *d_hist = nullptr, *d_score = nullptr, *d_z1 = nullptr, *d_z2 = nullptr, *d_part = nullptr;
   int* d_team = nullptr;
@@ -1040,7 +1066,7 @@ static void free_plan(Plan* p) {
   int prev = 0;
   (void)hipGetDevice(&prev);
   (void)hipSetDevice(p->device);
-  for (void* x : {(void*)p->d_ag, (void*)p->d_units, (void*)p->d_t1, (void*)p->d_t2, (void*)p->d_t3,
+  for (void* x : {(void*)p->d_ag, (void*)p->d_units, (void*)p->d_t1, (void*)p->d_e1, (void*)p->d_t2, (void*)p->d_t3,
                   (void*)p->d_iarena, (void*)p->d_hist, (void*)p->d_score, (void*)p->d_z1, (void*)p->d_z2,
                   (void*)p->d_part, (void*)p->d_team, (void*)p->d_cnt})
     (void)hipFree(x);
@@ -1428,6 +1454,30 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
   auto by_cost = [](const TileC& x, const TileC& y) { return x.cost > y.cost; };
   for (auto* v : {&t1, &t2, &t3}) std::stable_sort(v->begin(), v->end(), by_cost);
   for (auto& x : t1) pl->t1.push_back(x.t);
+  for (const Tile& tl : pl->t1) {
+    const Agent& a = pl->agents[tl.agent];
+    const int m = tl.m, f = a.ef[m][2];
+    E1Rec r;
+    std::memset(&r, 0, sizeof(r));
+    r.O = a.O[m];
+    r.d2 = a.ed[m][2];
+    r.d01 = a.ed[m][0] * a.ed[m][1];
+    r.nr = tl.r1 - tl.r0;
+    r.nc = tl.c1 - tl.c0;
+    r.c0 = tl.c0;
+    r.a_off = a.A_off[m] + (long long)tl.r0 * a.O[m] * a.ed[m][2];
+    r.z_off = a.z1_off[m] + tl.r0;
+    for (int c = 0; c < 16; ++c) {
+      r.cb[c] = -1;
+      if (c < r.nc) {
+        const int n = tl.c0 + c;
+        int t = 0;
+        while (t + 1 < a.T && n >= a.tstart[m][t + 1]) ++t;
+        r.cb[c] = a.hist_off + a.hist_lvl[t][f] + (long long)(n - a.tstart[m][t]) * a.S[f];
+      }
+    }
+    pl->e1.push_back(r);
+  }
   for (auto& x : t2) pl->t2.push_back(x.t);
   for (auto& x : t3) pl->t3.push_back(x.t);
   if (pl->t1.size() > INT32_MAX || pl->t2.size() > INT32_MAX || pl->t3.size() > INT32_MAX) {
@@ -1489,6 +1539,7 @@ static int upload(Plan* pl) {
   if ((e = alloc((void**)&pl->d_ag, sizeof(Agent) * pl->agents.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_units, sizeof(Unit) * pl->units.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_t1, sizeof(Tile) * pl->t1.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_e1, sizeof(E1Rec) * pl->e1.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_t2, sizeof(Tile) * pl->t2.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_t3, sizeof(Tile) * pl->t3.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_iarena, sizeof(int32_t) * pl->iarena.size())) != hipSuccess)
@@ -1507,6 +1558,7 @@ static int upload(Plan* pl) {
   if ((e = up(pl->d_ag, pl->agents.data(), sizeof(Agent) * pl->agents.size())) != hipSuccess ||
       (e = up(pl->d_units, pl->units.data(), sizeof(Unit) * pl->units.size())) != hipSuccess ||
       (e = up(pl->d_t1, pl->t1.data(), sizeof(Tile) * pl->t1.size())) != hipSuccess ||
+      (e = up(pl->d_e1, pl->e1.data(), sizeof(E1Rec) * pl->e1.size())) != hipSuccess ||
       (e = up(pl->d_t2, pl->t2.data(), sizeof(Tile) * pl->t2.size())) != hipSuccess ||
       (e = up(pl->d_t3, pl->t3.data(), sizeof(Tile) * pl->t3.size())) != hipSuccess ||
       (e = up(pl->d_iarena, pl->iarena.data(), sizeof(int32_t) * pl->iarena.size())) != hipSuccess)
@@ -1556,6 +1608,15 @@ extern "C" int pymdp_hip_ragged_tables(void* plan, int64_t* sizes16, void* agent
   return PYMDP_HIP_OK;
 }
 
+extern "C" int pymdp_hip_ragged_e1_records(void* plan, void* out, int64_t* count_out, int64_t* record_bytes) {
+  Plan* pl = static_cast<Plan*>(plan);
+  if (!pl) return einval("plan is NULL");
+  if (count_out) *count_out = (int64_t)pl->e1.size();
+  if (record_bytes) *record_bytes = (int64_t)sizeof(E1Rec);
+  if (out) std::memcpy(out, pl->e1.data(), sizeof(E1Rec) * pl->e1.size());
+  return PYMDP_HIP_OK;
+}
+
 extern "C" int pymdp_hip_ragged_layout(void* plan, int64_t* out, int64_t n_out) {
   Plan* pl = static_cast<Plan*>(plan);
   if (!pl) return einval("plan is NULL");
@@ -1595,6 +1656,7 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
   b.ag = pl->d_ag;
   b.units = pl->d_units;
   b.t1 = pl->d_t1;
+  b.e1 = pl->d_e1;
   b.t2 = pl->d_t2;
   b.t3 = pl->d_t3;
   b.iarena = pl->d_iarena;

@@ -41,6 +41,8 @@ AGENT_DT = np.dtype([
 
 UNIT_DT = np.dtype([("agent", "i4"), ("slice", "i4"), ("nseg", "i4"), ("pad", "i4"), ("g", "i4", (kM,)),
                     ("r0", "i4", (kM,)), ("r1", "i4", (kM,)), ("lds", "i4", (kM,))])
+E1REC_DT = np.dtype([("a_off", "i8"), ("z_off", "i8"), ("cb", "i8", (16,)), ("O", "i4"), ("d2", "i4"),
+                     ("d01", "i4"), ("nr", "i4"), ("nc", "i4"), ("c0", "i4"), ("pad0", "i4"), ("pad1", "i4")])
 TILE_DT = np.dtype([("agent", "i4"), ("m", "i4"), ("t", "i4"), ("r0", "i4"), ("r1", "i4"), ("c0", "i4"),
                     ("c1", "i4"), ("pad", "i4")])
 
@@ -61,6 +63,18 @@ def tables(plan):
     _native.check(L.pymdp_hip_ragged_tables(plan.handle, None, ag.ctypes.data, units.ctypes.data,
                                             tiles.ctypes.data, iarena.ctypes.data))
     return ag, units, (tiles[:n1], tiles[n1:n1 + n2], tiles[n1 + n2:]), iarena[:int(sizes[7])]
+
+
+def e1_records(plan):
+    """Stage-1 tile records copied from the plan."""
+    L = _native.lib()
+    cnt, size = np.zeros(1, np.int64), np.zeros(1, np.int64)
+    _native.check(L.pymdp_hip_ragged_e1_records(plan.handle, None, cnt.ctypes.data_as(_native._i64p),
+                                                 size.ctypes.data_as(_native._i64p)))
+    assert int(size[0]) == E1REC_DT.itemsize, (int(size[0]), E1REC_DT.itemsize)
+    rec = np.zeros(int(cnt[0]), E1REC_DT)
+    _native.check(L.pymdp_hip_ragged_e1_records(plan.handle, rec.ctypes.data, None, None))
+    return rec
 
 
 def _chk(idx, n, what):
@@ -90,6 +104,8 @@ class Emu:
         self.carry = carry
         self.info = plan.info()
         self.ag, self.units, (self.t1, self.t2, self.t3), self.iarena = tables(plan)
+        self.e1rec = e1_records(plan)
+        assert len(self.e1rec) == len(self.t1)
         self.params = params.astype(np.float32)
         self.obs = obs.astype(np.int32)
         self.hist = np.full(max(1, self.info["hist_doubles"]), np.nan)
@@ -282,7 +298,7 @@ class Emu:
         assert np.isfinite(q).all()
         return q
 
-    def e1(self, tl):
+    def e1(self, tl, rec):
         ai, m, _, r0, r1, c0, c1 = (int(tl[k]) for k in ("agent", "m", "t", "r0", "r1", "c0", "c1"))
         a = self.ag[ai]
         O = int(a["O"][m])
@@ -301,8 +317,13 @@ class Emu:
             _chk(np.array([P * G - 1]), kNT, "s_hap (E1)")
         K = int(a["K"][m])
         assert K == d01 * d2
-        # position-major A_m: (position, o, k) at (position O + o) d2 + k
-        idx = int(a["A_off"][m]) + ((r0 + np.arange(nr))[None, :, None] * O + np.arange(O)[:, None, None]) * d2 + \
+        # the tile's record: sizes and addresses as derived from the agent
+        assert (int(rec["O"]), int(rec["d2"]), int(rec["d01"]), int(rec["nr"]), int(rec["nc"]), int(rec["c0"])) == \
+            (O, d2, d01, nr, nc, c0)
+        assert int(rec["a_off"]) == int(a["A_off"][m]) + r0 * O * d2
+        assert int(rec["z_off"]) == int(a["z1_off"][m]) + r0
+        # position-major A_m: (position, o, k) at (position O + o) d2 + k, from the record's offset
+        idx = int(rec["a_off"]) + ((np.arange(nr))[None, :, None] * O + np.arange(O)[:, None, None]) * d2 + \
             np.arange(d2)[None, None, :]
         At = self.params[_chk(idx, self.params.size, "A (E1)")]  # [O, nr, d2]
         xl = np.where(At == 0, np.float32(0), At * np.log(np.maximum(At, MINVAL)).astype(np.float32))
@@ -317,7 +338,12 @@ class Emu:
                 t += 1
             h = n - int(a["tstart"][m][t])
             assert 0 <= h < a["H"][t][f]
-            cols.append(self._hist_table(a, m, t, 2, h, h + 1)[0])
+            col = self._hist_table(a, m, t, 2, h, h + 1)[0]
+            cb = int(rec["cb"][n - c0])
+            assert cb == int(a["hist_off"]) + int(a["hist_lvl"][t][f]) + h * int(a["S"][f])
+            np.testing.assert_array_equal(self.hist[_chk(cb + np.arange(d2), self.hist.size, "hist (E1)")], col)
+            cols.append(col)
+        assert (rec["cb"][nc:] == -1).all()
         Q = np.stack(cols)  # [nc, d2]
         Y = np.einsum("ork,ck->cor", X, Q)  # [nc, O1, nr]
         zi = int(a["z1_off"][m]) + ((c0 + np.arange(nc))[:, None, None] * O1 + np.arange(O1)[None, :, None]) * d01 + \
@@ -421,8 +447,8 @@ class Emu:
             by_agent.setdefault(int(u["agent"]), []).append(u)
         for ai, us in by_agent.items():
             self.fpi_agent(ai, us)
-        for tl in self.t1:
-            self.e1(tl)
+        for tl, rec in zip(self.t1, self.e1rec):
+            self.e1(tl, rec)
         for tl in self.t2:
             self.e2(tl)
         for tl in self.t3:
@@ -522,6 +548,17 @@ def check_bounds_only(plan):
         assert int(tl["c1"]) <= int(a["tstart"][m][int(a["T"])])
         lda = ((d2 + 3) & ~3) + 1
         assert nc <= 16 and int(a["O"][m]) * nr * lda <= kE1A and nr * lda <= kE1H
+    # stage-1 records: the A block, the history columns and the output block stay inside their buffers
+    rec = e1_records(plan)
+    assert len(rec) == len(t1)
+    if len(rec):
+        O_, d2_, d01_ = rec["O"].astype(np.int64), rec["d2"].astype(np.int64), rec["d01"].astype(np.int64)
+        assert (rec["a_off"] >= 0).all() and (rec["a_off"] + rec["nr"] * O_ * d2_ <= params_n).all()
+        valid = np.arange(16)[None, :] < rec["nc"][:, None]
+        assert ((rec["cb"] >= 0) == valid).all()
+        assert (rec["cb"][valid] + np.repeat(d2_, 16).reshape(-1, 16)[valid] <= info["hist_doubles"]).all()
+        last = rec["z_off"] + ((rec["c0"] + rec["nc"] - 1) * (O_ + 1) + O_) * d01_ + rec["nr"] - 1
+        assert (rec["z_off"] >= 0).all() and (last < info["z1_doubles"]).all()
     for tl in t2:
         a = ag[int(tl["agent"])]
         m, t = int(tl["m"]), int(tl["t"])
