@@ -13,8 +13,10 @@
 //   2. efe_rollout_kernel   predicted marginals once per action history
 //   3. efe_kron_kernel      Kronecker vector of every realised inner tuple
 //   4. efe_gemm1_kernel     W[n] = [A_m ; HAx_m] x_inner (x q_inner[n])
-//   5. efe_score_kernel     qo = W[n] q_outer[h0] for every outer history, fused
-//                           entropy + utility + H_A epilogue -> one score each
+//   5. efe_score_mfma_kernel  qo = W[n] q_outer[h0] for every outer history on
+//                           FP64 MFMA, fused entropy + utility + H_A epilogue ->
+//                           one score each (efe_score_kernel: the FMA form, for
+//                           pA or a W[n] too large for LDS)
 //   6. efe_scatter_kernel   G[p] = sum_{t,m} score[t, m, n(p) H0 + h0(p)]
 //                           (+ inductive), then q_pi = softmax(gamma G + ln E)
 // All per-agent intermediates live in one workspace slab per agent.
@@ -555,6 +557,95 @@ __global__ __launch_bounds__(kNT) void efe_score_kernel(const Desc* __restrict__
   if (has_b) out[hb] = (sig ? -ent_b + hq_b : 0.0) + (util ? ut_b : 0.0) - pg_b;
 }
 
+// The same scores with the outcome GEMM on FP64 MFMA (v_mfma_f64_16x16x4f64):
+// per wave, 16 outer histories at a time; rows of the 16 x 16 tiles are
+// outcomes (the O outcome rows of W[n] and its H_A column), columns are
+// histories, k runs over the first dependency's states.  Lane layout: A
+// operand W[k + l/16][o0 + l%16] (from LDS), B operand q_h[k + l/16] for
+// h = h0 + l%16 (global), accumulator element v of lane l = outcome row
+// 4 v + l/16, history column l%16.  Each lane then adds its outcome logs,
+// utilities and the H_A row; lanes with the same column combine by
+// cross-lane sums.  W must fit in LDS; no pA term.
+typedef double efe_d4 __attribute__((ext_vector_type(4)));
+
+template <int KMAX>  // k steps held per lane: S0 <= 4 KMAX
+__global__ __launch_bounds__(kNT) void efe_score_mfma_kernel(const Desc* __restrict__ dp,
+                                                             const Item2* __restrict__ items, const Args a) {
+  extern __shared__ double wl[];  // W[n] [S0][Ox], then C[t] [O] (launch size: both)
+  __shared__ double s_ltab[256];
+  const Desc& d = *dp;
+  const Item2 it = items[blockIdx.x];
+  const int b = blockIdx.y;
+  const int t = it.t, m = it.m, n = it.n;
+  const int g = d.mod_group[m];
+  const int O = d.O[m];
+  const int O1 = d.Ox[m];
+  const int S0 = d.gS0[g];
+  const int f0 = d.gdeps[g][0];
+  const int H0 = d.g_H0[t][g];
+  const bool sig = (a.flags & kFlagUseStatesInfoGain) != 0;
+  const bool util = (a.flags & kFlagUseUtility) != 0;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
+  double* ws = a.ws + (long long)b * d.ws_stride;
+  const double* Wg = ws + d.w_off[t][m] + (long long)n * S0 * O1;
+  const float* Ct = a.C[m] + (long long)b * a.C_stride[m] + (a.C_time[m] ? (long long)t * O : 0);
+  double* s_c = wl + S0 * O1;
+  for (int i = tid; i < S0 * O1; i += kNT) wl[i] = Wg[i];
+  for (int i = tid; i < 256; i += kNT) s_ltab[i] = g_logtab[i];
+  for (int i = tid; i < O; i += kNT) s_c[i] = util ? (double)Ct[i] : 0.0;
+  __syncthreads();
+  const double* qbase = ws + d.qs_off[t][f0];
+  double* out = ws + d.sc_off[t][m] + (long long)n * H0;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nk = (S0 + 3) >> 2;       // k steps of 4 states
+  const int nrow = O + 1;             // outcome rows + the H_A row
+  const int ntile = (nrow + 15) >> 4;
+  const int hbase = it.hb * kScoreCols;
+  const int hend = min(H0, hbase + kScoreCols);
+  for (int h0 = hbase + wave * 16; h0 < hend; h0 += kNT / 4) {
+    // B fragments of this wave's 16 histories, all k steps
+    const int h = h0 + li;
+    double bq[KMAX];
+#pragma unroll
+    for (int ks = 0; ks < KMAX; ++ks) {
+      const int s = 4 * ks + lk;
+      bq[ks] = (ks < nk && s < S0 && h < H0) ? qbase[(long long)h * S0 + s] : 0.0;
+    }
+    double ent = 0.0, ut = 0.0, hq = 0.0;
+    for (int ot = 0; ot < ntile; ++ot) {
+      const int oa = ot * 16 + li;  // this lane's A-operand row (outcome)
+      efe_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int ks = 0; ks < KMAX; ++ks) {
+        if (ks < nk) {
+          const int s = 4 * ks + lk;
+          const double x = (s < S0 && oa < nrow) ? wl[s * O1 + oa] : 0.0;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, bq[ks], acc, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int o = ot * 16 + 4 * v + lk;
+        const double qo = acc[v];
+        if (o < O) {
+          if (qo != 0.0) ent += qo * log_f64d((double)fmaxf((float)qo, kMinVal), s_ltab);
+          ut = fma(qo, s_c[o], ut);
+        } else if (o == O) {
+          hq += qo;
+        }
+      }
+    }
+    // lanes li, li + 16, li + 32, li + 48 hold the same history's partial sums
+#pragma unroll
+    for (int sh = 16; sh < kWave; sh <<= 1) {
+      ent += __shfl_xor(ent, sh, kWave);
+      ut += __shfl_xor(ut, sh, kWave);
+      hq += __shfl_xor(hq, sh, kWave);
+    }
+    if (lk == 0 && h < H0) out[h] = (sig ? -ent + hq : 0.0) + (util ? ut : 0.0);
+  }
+}
+
 // Inductive value per factor history (pymdp/control.py:1021-1061):
 //   path_f * sum_s (1 - I_f[m_f, s]) ln eps * qs[t][f][h][s]
 // with idx = argmax qs_init_f, m_f = max(argmax_i I_f[i, idx] - 1, 0).
@@ -1041,15 +1132,26 @@ static int run_neg_efe(const CallIn& c) {
                      plan->d_tab, a);
   hipLaunchKernelGGL(efe_gemm1_kernel, dim3(plan->g1_blocks, d.M, B), dim3(256), 0, st, plan->d_desc, a);
   {
-    size_t wmax = 0;
-    for (int m = 0; m < d.M; ++m) wmax = std::max(wmax, (size_t)d.gS0[d.mod_group[m]] * d.Ox[m]);
-    if (wmax * sizeof(double) <= 48 * 1024) {
-      hipLaunchKernelGGL(efe_score_kernel<true>, dim3(plan->n_items2, B), dim3(kNT), wmax * sizeof(double), st,
-                         plan->d_desc, plan->d_items2, a);
-    } else {
-      hipLaunchKernelGGL(efe_score_kernel<false>, dim3(plan->n_items2, B), dim3(kNT), 0, st, plan->d_desc,
-                         plan->d_items2, a);
+    size_t wmax = 0, wcmax = 0;
+    int s0max = 0;
+    for (int m = 0; m < d.M; ++m) {
+      wmax = std::max(wmax, (size_t)d.gS0[d.mod_group[m]] * d.Ox[m]);
+      wcmax = std::max(wcmax, (size_t)d.gS0[d.mod_group[m]] * d.Ox[m] + d.O[m]);
+      s0max = std::max(s0max, d.gS0[d.mod_group[m]]);
     }
+    const bool lwc = wcmax * sizeof(double) <= 48 * 1024;
+    const size_t lbc = wcmax * sizeof(double);
+    const bool lw = wmax * sizeof(double) <= 48 * 1024;
+    const size_t lb = lw ? wmax * sizeof(double) : 0;
+    const dim3 gr(plan->n_items2, B);
+    if (!has_pA && lwc && s0max <= 32)
+      hipLaunchKernelGGL((efe_score_mfma_kernel<8>), gr, dim3(kNT), lbc, st, plan->d_desc, plan->d_items2, a);
+    else if (!has_pA && lwc && s0max <= 64)
+      hipLaunchKernelGGL((efe_score_mfma_kernel<16>), gr, dim3(kNT), lbc, st, plan->d_desc, plan->d_items2, a);
+    else if (lw)
+      hipLaunchKernelGGL(efe_score_kernel<true>, gr, dim3(kNT), lb, st, plan->d_desc, plan->d_items2, a);
+    else
+      hipLaunchKernelGGL(efe_score_kernel<false>, gr, dim3(kNT), 0, st, plan->d_desc, plan->d_items2, a);
   }
   if (ind) hipLaunchKernelGGL(efe_inductive_kernel, dim3(B, d.F), dim3(kNT), 0, st, plan->d_desc, a);
   hipLaunchKernelGGL(efe_scatter_kernel, dim3(B), dim3(1024), 0, st, plan->d_desc, plan->d_tab, a);
