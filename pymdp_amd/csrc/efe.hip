@@ -357,6 +357,8 @@ __global__ __launch_bounds__(kNT) void efe_kron_kernel(const Desc* __restrict__ 
 //    tuples nn of every level at once (one LDS-tiled FP64 GEMM per agent and
 //    modality, A read once per 64-tuple column tile).  Arow(o', s0) = A_m[o', s0, :]
 //    for o' < O and HAx_m[s0, :] for o' == O; rank-1 modalities copy A.
+typedef double g1_d4 __attribute__((ext_vector_type(4)));
+
 __global__ __launch_bounds__(256) void efe_gemm1_kernel(const Desc* __restrict__ dp, const Args a) {
   const Desc& d = *dp;
   const int m = blockIdx.y;
@@ -399,13 +401,18 @@ __global__ __launch_bounds__(256) void efe_gemm1_kernel(const Desc* __restrict__
   }
   __shared__ double As[G1_BK][G1_BM + 1];
   __shared__ double Qs[G1_BK][G1_BN + 1];
-  const int tr = tid >> 4;  // 16 x 16 threads, 4 x 4 outputs each
-  const int tc = tid & 15;
-  double acc[4][4];
+  // FP64 MFMA (v_mfma_f64_16x16x4f64): wave w owns the 32 x 32 quarter
+  // (rows 32 (w / 2), columns 32 (w % 2)) of the 64 x 64 tile as 2 x 2
+  // blocks; lane l feeds A row l % 16 / B column l % 16 at k = l / 16 and
+  // holds element v of a block at row 4 v + l / 16, column l % 16
+  const int lane = tid & (kWave - 1), wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int rw = 32 * (wave >> 1), cw = 32 * (wave & 1);
+  g1_d4 acc[2][2];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    for (int j = 0; j < 2; ++j) acc[i][j] = g1_d4{0.0, 0.0, 0.0, 0.0};
   for (int k0 = 0; k0 < Kr; k0 += G1_BK) {
 #pragma unroll
     for (int l = 0; l < (G1_BM * G1_BK) / 256; ++l) {
@@ -429,30 +436,29 @@ __global__ __launch_bounds__(256) void efe_gemm1_kernel(const Desc* __restrict__
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < G1_BK; ++kk) {
-      double av[4], qv[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) av[i] = As[kk][tr * 4 + i];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) qv[j] = Qs[kk][tc * 4 + j];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fma(av[i], qv[j], acc[i][j]);
+    for (int kk = 0; kk < G1_BK; kk += 4) {
+      const double a0 = As[kk + lk][rw + li], a1 = As[kk + lk][rw + 16 + li];
+      const double b0 = Qs[kk + lk][cw + li], b1 = Qs[kk + lk][cw + 16 + li];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int r = row0 + tr * 4 + i;
-    if (r >= rows) continue;
-    const int o = r / S0, s0 = r - o * S0;
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int nn = col0 + tc * 4 + j;
-      if (nn < ntot) W[((long long)nn * S0 + s0) * Ox + o] = acc[i][j];
+    for (int v = 0; v < 4; ++v) {
+      const int r = row0 + rw + 16 * i + 4 * v + lk;
+      if (r >= rows) continue;
+      const int o = r / S0, s0 = r - o * S0;
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int nn = col0 + cw + 16 * j + li;
+        if (nn < ntot) W[((long long)nn * S0 + s0) * Ox + o] = acc[i][j][v];
+      }
     }
-  }
 }
 
 // 5. Scores: for every outer history h0 of an inner tuple n,
