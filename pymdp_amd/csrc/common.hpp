@@ -149,6 +149,60 @@ __device__ __forceinline__ T group_sum_low(T v, int log2w) {
   return v;
 }
 
+// DPP cross-lane reductions of doubles (VALU moves between lanes instead of
+// ds_bpermute round trips through the LDS unit).  Source lanes are always in
+// the destination's own aligned group, so every lane of each reduced group
+// must be active; the wave-wide forms need the whole wave active.
+//   0xB1: quad_perm [1,0,3,2] (lane ^ 1)     0x4E: quad_perm [2,3,0,1] (lane ^ 2)
+//   0x141: row_half_mirror (lane i <-> 7 - i within 8)
+//   0x140: row_mirror (lane i <-> 15 - i within 16)
+// After the xor-1 and xor-2 steps every aligned group of 4 is uniform, so a
+// half-row mirror pairs each lane with the other group of 4 of its 8, and a
+// row mirror each lane with the other group of 8 of its 16.
+template <int Ctrl>
+__device__ __forceinline__ double dpp_mov_f64(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)bits, Ctrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), Ctrl, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)bits, l);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum of the whole wave in every lane: rows of 16 by DPP, then the four row
+// sums read from lanes 0, 16, 32, 48 (fixed order).
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_mov_f64<0xB1>(v);
+  v += dpp_mov_f64<0x4E>(v);
+  v += dpp_mov_f64<0x141>(v);
+  v += dpp_mov_f64<0x140>(v);
+  return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+__device__ __forceinline__ double wave_max_dpp(double v) {
+  v = fmax(v, dpp_mov_f64<0xB1>(v));
+  v = fmax(v, dpp_mov_f64<0x4E>(v));
+  v = fmax(v, dpp_mov_f64<0x141>(v));
+  v = fmax(v, dpp_mov_f64<0x140>(v));
+  return fmax(fmax(readlane_f64(v, 0), readlane_f64(v, 16)), fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+
+// group_sum_low for doubles: groups of up to 16 lanes by DPP, wider groups
+// finish with shuffles.  Every lane of a group ends with the group's sum.
+__device__ __forceinline__ double group_sum_low_dpp(double v, int log2w) {
+  if (log2w > 0) v += dpp_mov_f64<0xB1>(v);
+  if (log2w > 1) v += dpp_mov_f64<0x4E>(v);
+  if (log2w > 2) v += dpp_mov_f64<0x141>(v);
+  if (log2w > 3) v += dpp_mov_f64<0x140>(v);
+  for (int b = 4; b < log2w; ++b) v += __shfl_xor(v, 1 << b, kWave);
+  return v;
+}
+
 // Sum over lane bits [log2w, 6): lanes that share (lane & (2^log2w - 1)).
 template <typename T>
 __device__ __forceinline__ T group_sum_high(T v, int log2w) {

@@ -193,9 +193,9 @@ __device__ void rg_softmax(const Agent& a, const double* src, double* dst, int l
   for (int f = wave; f < a.F; f += kNW) {
     const int off = a.S_off[f], n = a.S[f];
     const double x = lane < n ? src[off + lane] : -INFINITY;
-    const double mx = wave_max(x);
+    const double mx = wave_max_dpp(x);
     const double e = lane < n ? exp(x - mx) : 0.0;
-    const double s = wave_sum(e);
+    const double s = wave_sum_dpp(e);
     if (lane < n) dst[off + lane] = e / s;
   }
 }
@@ -236,7 +236,7 @@ __device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const doub
         v3 = fma(x[k + 3 * tpr], qc[k + 3 * tpr], v3);
       }
       for (; k < n; k += tpr) v0 = fma(x[k], qc[k], v0);
-      double v = group_sum_low((v0 + v1) + (v2 + v3), lt);
+      double v = group_sum_low_dpp((v0 + v1) + (v2 + v3), lt);
       if (sub == 0) {
         if (R == 2)
           s_part[a.S_off[a.gfac[g][0]] + r0 + rl] += v;
@@ -308,7 +308,7 @@ __device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const doub
       const int ii = e >> lg0, j = e & ((1 << lg0) - 1);
       const int r = (i_lo + ii) * d1 + j;
       double v = (j < d1 && r >= r0 && r < r0 + nr) ? q1[j] * s_y[r - r0] : 0.0;
-      v = group_sum_low(v, lg0);
+      v = group_sum_low_dpp(v, lg0);
       if (j == 0) s_part[a.S_off[a.gfac[g][0]] + i_lo + ii] += v;
     }
     // m_1[j] = sum_i q_0[i] y_ij: one group of 2^lg1 lanes per j, each lane
@@ -321,7 +321,7 @@ __device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const doub
         const int r = (i_lo + ii) * d1 + j;
         if (r >= r0 && r < r0 + nr) v = fma(q0p[i_lo + ii], s_y[r - r0], v);
       }
-      v = group_sum_low(v, lg1);
+      v = group_sum_low_dpp(v, lg1);
       if (sub == 0) s_part[a.S_off[a.gfac[g][1]] + j] += v;
     }
   }
@@ -420,16 +420,21 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
       if (a.team > 1) {
         // exchange partial marginals with the other units of this agent
         // (double-buffered by sweep parity: a unit overwrites its slot of
-        // buffer it & 1 only after every unit has read it at sweep it - 1)
+        // buffer it & 1 only after every unit has read it at sweep it - 1).
+        // The partials move as agent-scope relaxed atomic stores and loads
+        // (device-coherence scope bits on each access), so no agent-scope
+        // fence is needed: such a fence writes back (release) or invalidates
+        // (acquire) the whole L2 of the XCD on every sweep.  Wave 0 stores
+        // this unit's partials, waits until every store is acknowledged
+        // (s_waitcnt vmcnt(0)), then adds its arrival; lane 0 spins on the
+        // count; the barrier below orders every thread's partial loads after
+        // the spin has seen all arrivals.
         double* part = b.part + a.part_off + (long long)(it & 1) * a.team * totS;
-        // wave 0 publishes this unit's partials: its stores, an agent-scope
-        // release fence, the arrival count; lane 0 spins with relaxed loads
-        // and the wave acquires once; the barrier below orders every other
-        // thread's loads after that acquire
         if (wave == 0) {
-          for (int i = lane; i < totS; i += kWave) part[u.slice * totS + i] = s_part[i];
+          for (int i = lane; i < totS; i += kWave)
+            __hip_atomic_store(part + u.slice * totS + i, s_part[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           if (!s_broken) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             if (lane == 0) {
               int* cnt = b.team_cnt + u.agent;
               __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -444,7 +449,6 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
                 }
               }
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
           }
         }
         __syncthreads();
@@ -454,7 +458,10 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
           for (int s0 = 0; s0 < a.team; s0 += 16) {
             double v[16];
 #pragma unroll
-            for (int q = 0; q < 16; ++q) v[q] = s0 + q < a.team ? pp[(long long)(s0 + q) * totS] : 0.0;
+            for (int q = 0; q < 16; ++q)
+              v[q] = s0 + q < a.team ? __hip_atomic_load(pp + (long long)(s0 + q) * totS, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)
+                                     : 0.0;
 #pragma unroll
             for (int q = 0; q < 16; ++q)
               if (s0 + q < a.team) s += v[q];
