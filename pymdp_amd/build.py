@@ -35,6 +35,10 @@ HIPCC_FLAGS = [
     # Packed FP32 instructions are disabled for all device code.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
+    # MFMA accumulators in ordinary VGPRs: with the default AGPR form the
+    # compiler copies a loop-carried accumulator AGPR <-> VGPR around every
+    # MFMA of a k loop (sixteen moves and a wait per step).
+    "-mllvm", "-amdgpu-mfma-vgpr-form=true",
     "-I", str(REPO_DIR / "include"),
     "-I", str(CSRC),
 ]

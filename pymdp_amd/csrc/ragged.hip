@@ -711,6 +711,11 @@ constexpr int kE1A = 4352;    // floats of the A block
 constexpr int kE1H = 512;     // doubles of the sum_o A ln A rows
 constexpr int kE1Q = 16 * 65;  // doubles of the history block
 
+// floor(n / d) for 0 <= n, 1 <= d, n + d <= 2^22, from inv_d = 1.0f / d: the
+// quotient of n + 1/2 lies at least 1/(2d) from an integer, and the two float
+// roundings move it by at most (n/d + 1) 2^-23 < 1/(2d).
+__device__ __forceinline__ int rg_quot(int n, float inv_d) { return (int)(((float)n + 0.5f) * inv_d); }
+
 __host__ __device__ __forceinline__ int rg_e1_rows(int O, int d2, int d01) {
   // (i, j) positions of one tile: the A block and the sum_o A ln A rows fit
   const int lda = ((d2 + 3) & ~3) + 1;
@@ -735,6 +740,7 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
   const int O = rc.O, O1 = O + 1, d01 = rc.d01, d2 = rc.d2;
   const int d2p = (d2 + 3) & ~3, lda = d2p + 1;
   const int nr = rc.nr, nc = rc.nc;
+  const float inv_d2 = 1.0f / (float)d2, inv_d2p = 1.0f / (float)d2p;
   {
     // A_m is position-major, so the tile's rows (position rl, outcome o) are
     // one contiguous chunk: element e = (rl, o, k) at Am[e], LDS row o nr + rl.
@@ -747,7 +753,7 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
     const int drl = kNT / per, dm = kNT - drl * per;  // per-step advance of (rl, rem)
     const int dq = dm / d2, dk = dm - dq * d2;
     int rl = tid / per, rem = tid - rl * per;
-    int o = rem / d2, k = rem - o * d2;
+    int o = rg_quot(rem, inv_d2), k = rem - o * d2;
     // loads are unconditional (past the block's end they repeat its last
     // element and are not stored), so all of them are issued back to back
     float v[NB];
@@ -773,7 +779,7 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
     bool hok[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const int e = tid + q * kNT, c = min(e / d2p, 15), kk = e - (e / d2p) * d2p;
+      const int e = tid + q * kNT, cq = rg_quot(e, inv_d2p), c = min(cq, 15), kk = e - cq * d2p;
       const long long cb = rc.cb[c];
       hok[q] = e < 16 * d2p && cb >= 0 && kk < d2;
       hv[q] = b.hist[hok[q] ? cb + kk : 0];
@@ -786,7 +792,7 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
       if (tid + q * kNT < tot) s_a[off[q]] = v[q];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const int e = tid + q * kNT, c = e / d2p, kk = e - c * d2p;
+      const int e = tid + q * kNT, c = rg_quot(e, inv_d2p), kk = e - c * d2p;
       if (e < 16 * d2p) s_q[c * lda + kk] = hv[q];
     }
   }
@@ -797,9 +803,10 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
     const int P = nr * d2;
     const int G = std::max(1, std::min(O, kNT / P));
     const int och = (O + G - 1) / G;
+    const float inv_p = 1.0f / (float)P;
     for (int e = tid; e < P * G; e += kNT) {
-      const int g = e / P, pp = e - g * P;
-      const int rl = pp / d2, k = pp - rl * d2;
+      const int g = rg_quot(e, inv_p), pp = e - g * P;
+      const int rl = rg_quot(pp, inv_d2), k = pp - rl * d2;
       const float* col = s_a + rl * lda + k;
       const int o1 = std::min(O, (g + 1) * och);
       double s0 = 0.0, s1 = 0.0;
@@ -820,37 +827,56 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
       for (int pp = tid; pp < P; pp += kNT) {
         double sacc = 0.0;
         for (int g = 0; g < G; ++g) sacc += s_hap[g * P + pp];
-        const int rl = pp / d2, k = pp - rl * d2;
+        const int rl = rg_quot(pp, inv_d2), k = pp - rl * d2;
         s_h[rl * lda + k] = sacc;
       }
     }
   }
   lds_barrier();
-  // MFMA over 16-row blocks of the O1 * nr rows
-  const int R = O1 * nr;
+  // MFMA over 16-row blocks of the O1 * nr rows, two blocks per wave at a
+  // time (independent accumulators hide the MFMA latency).  The k loop has
+  // no branches: each lane reads its row of both operand blocks at clamped
+  // addresses and selects; columns k >= d2 of X are zeroed (the LDS padding
+  // is never written) as are those of Q.
+  const int R = O1 * nr, RA = O * nr;
   const int nblk = (R + 15) >> 4;
   const int li = lane & 15, lk = lane >> 4;
-  double* Z = b.z1 + rc.z_off;
+  const float inv_nr = 1.0f / (float)nr;
+  // this lane's output column (history c0 + li) base in Z1
+  double* zc = b.z1 + rc.z_off + (long long)(rc.c0 + li) * O1 * d01;
   const double* qrow = s_q + li * lda + lk;
-  for (int blk = wave; blk < nblk; blk += kNW) {
-    const int r = (blk << 4) + li;  // this lane's A-operand row
-    const bool rv = r < R;
-    const bool isha = r >= O * nr;
-    const float* xa = s_a + (rv && !isha ? r : 0) * lda + lk;
-    const double* xh = s_h + (rv && isha ? r - O * nr : 0) * lda + lk;
-    rg_d4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int blk = wave; blk < nblk; blk += 2 * kNW) {
+    const int blk1 = blk + kNW;  // may be past the last block: computed, not stored
+    const int r0 = (blk << 4) + li, r1 = (blk1 << 4) + li;
+    const float* xa0 = s_a + (r0 < RA ? r0 : 0) * lda + lk;
+    const double* xh0 = s_h + (r0 >= RA && r0 < R ? r0 - RA : 0) * lda + lk;
+    const float* xa1 = s_a + (r1 < RA ? r1 : 0) * lda + lk;
+    const double* xh1 = s_h + (r1 >= RA && r1 < R ? r1 - RA : 0) * lda + lk;
+    const bool h0 = r0 >= RA, h1 = r1 >= RA;
+    rg_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
     for (int k0 = 0; k0 < d2p; k0 += 4) {
-      const bool kv = rv && (k0 + lk) < d2;
-      double x = 0.0;
-      if (kv) x = isha ? xh[k0] : (double)xa[k0];
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, qrow[k0], acc, 0, 0, 0);
+      const bool kv = (k0 + lk) < d2;
+      const double a0 = (double)xa0[k0], g0 = xh0[k0];
+      const double a1 = (double)xa1[k0], g1 = xh1[k0];
+      const double q = qrow[k0];
+      const double x0 = kv ? (h0 ? g0 : a0) : 0.0;
+      const double x1 = kv ? (h1 ? g1 : a1) : 0.0;
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, q, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, q, acc1, 0, 0, 0);
     }
+    if (li < nc) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int rr = (blk << 4) + 4 * v + lk;
-      if (rr < R && li < nc) {
-        const int o = rr / nr, rl = rr - o * nr;
-        Z[((long long)(rc.c0 + li) * O1 + o) * d01 + rl] = acc[v];
+      for (int v = 0; v < 4; ++v) {
+        const int rr0 = (blk << 4) + 4 * v + lk;
+        if (rr0 < R) {
+          const int o = rg_quot(rr0, inv_nr);
+          zc[o * d01 + (rr0 - o * nr)] = acc0[v];
+        }
+        const int rr1 = (blk1 << 4) + 4 * v + lk;
+        if (rr1 < R) {
+          const int o = rg_quot(rr1, inv_nr);
+          zc[o * d01 + (rr1 - o * nr)] = acc1[v];
+        }
       }
     }
   }
