@@ -189,10 +189,22 @@ __device__ __forceinline__ void rg_copy_lds(const T* src, T* dst, int tid) {
 // ---------------------------------------------------------------------------
 // FPI units
 // ---------------------------------------------------------------------------
-__device__ void rg_softmax(const Agent& a, const double* src, double* dst, int lane, int wave) {
+// dst_f = softmax(x_f) per factor, one wave per factor.  x = src, or with
+// part != nullptr x = src + part (the log prior plus this sweep's partial
+// marginals), part then being cleared for the next sweep: that folds the
+// log q update of a single-unit agent into its softmax (no separate pass and
+// barrier).
+__device__ void rg_softmax(const Agent& a, const double* src, double* part, double* dst, int lane, int wave) {
   for (int f = wave; f < a.F; f += kNW) {
     const int off = a.S_off[f], n = a.S[f];
-    const double x = lane < n ? src[off + lane] : -INFINITY;
+    double x = -INFINITY;
+    if (lane < n) {
+      x = src[off + lane];
+      if (part) {
+        x += part[off + lane];
+        part[off + lane] = 0.0;
+      }
+    }
     const double mx = wave_max_dpp(x);
     const double e = lane < n ? exp(x - mx) : 0.0;
     const double s = wave_sum_dpp(e);
@@ -412,8 +424,14 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
     }
     __syncthreads();
 
+    // a single unit's log q of sweep it > 0 is s_lp + s_part, formed inside
+    // the softmax; a team's is summed over its units into s_logq
+    const bool single = a.team == 1;
     for (int it = 0; it < a.iters; ++it) {
-      rg_softmax(a, s_logq, s_qv, lane, wave);
+      if (single && it > 0)
+        rg_softmax(a, s_lp, s_part, s_qv, lane, wave);
+      else
+        rg_softmax(a, s_logq, nullptr, s_qv, lane, wave);
       __syncthreads();
       for (int sg = 0; sg < u.nseg; ++sg)
         rg_fpi_segment(a, u.g[sg], u.r0[sg], u.r1[sg] - u.r0[sg], s_x + u.lds[sg], s_y, s_red, s_qv, s_part, tid);
@@ -469,15 +487,13 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
           s_logq[i] = s_lp[i] + s;
           s_part[i] = 0.0;
         }
-      } else {
-        for (int i = tid; i < totS; i += kNT) {
-          s_logq[i] = s_lp[i] + s_part[i];
-          s_part[i] = 0.0;
-        }
+        __syncthreads();
       }
-      __syncthreads();
     }
-    rg_softmax(a, s_logq, s_qv, lane, wave);
+    if (single && a.iters > 0)
+      rg_softmax(a, s_lp, s_part, s_qv, lane, wave);
+    else
+      rg_softmax(a, s_logq, nullptr, s_qv, lane, wave);
     __syncthreads();
 
     if (u.slice == 0) {
