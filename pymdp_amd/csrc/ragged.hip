@@ -723,8 +723,8 @@ __device__ __forceinline__ void rg_load_hist(const Agent& a, const double* hist,
 // X[l % 16][k + l / 16] and Q[col l % 16][k + l / 16]; accumulator element v
 // of lane l is row 4 v + l / 16, column l % 16).
 typedef double rg_d4 __attribute__((ext_vector_type(4)));
-constexpr int kE1A = 4352;    // floats of the A block
-constexpr int kE1H = 512;     // doubles of the sum_o A ln A rows
+constexpr int kE1A = 8704;    // floats of the A block
+constexpr int kE1H = 1024;    // doubles of the sum_o A ln A rows
 constexpr int kE1Q = 16 * 65;  // doubles of the history block
 
 // Workgroups are dispatched to the XCDs round-robin (workgroup w on XCD

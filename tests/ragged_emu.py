@@ -19,7 +19,7 @@ kF, kM, kR, kT, kDim, kNT = 8, 8, 3, 4, 64, 256
 kCap, kRowsCap, kTeamMax = 6144, 1024, 128
 kTile, kTile2, kCols = 4608, 2304, 32
 kQTab = kCols * 65
-kE1A, kE1H, kE1Q = 4352, 512, 16 * 65
+kE1A, kE1H, kE1Q = 8704, 1024, 16 * 65
 
 AGENT_DT = np.dtype([
     ("F", "i4"), ("M", "i4"), ("T", "i4"), ("P", "i4"), ("G", "i4"), ("iters", "i4"), ("totS", "i4"),
