@@ -235,7 +235,11 @@ __device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const doub
   // row dot products (R >= 2): 2^lt lanes per row, four accumulators each
   // (eight LDS reads in flight per wait)
   if (R >= 2) {
-    const int lt = rows_lg(n), tpr = 1 << lt;
+    // lanes per row: about 8 columns each, fewer when that would take more
+    // than one round of the workgroup's threads
+    int lt = rows_lg(n);
+    while (lt > 0 && (nr << lt) > kNT) --lt;
+    const int tpr = 1 << lt;
     for (int e = tid; e < nr * tpr; e += kNT) {
       const int rl = e >> lt, sub = e & (tpr - 1);
       const double* x = X + rl * ld;
@@ -312,29 +316,41 @@ __device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const doub
     s_part[a.S_off[fl] + tid] += sacc;
   }
   if (R == 3) {
-    // m_0[i] = sum_j q_1[j] y_ij: one group of 2^lg0 >= d1 lanes per i
+    // m_0[i] = sum_j q_1[j] y_ij (thread per i) and m_1[j] = sum_i q_0[i] y_ij
+    // (thread per j) in one pass: serial FMA chains over LDS, two / four
+    // partial sums each, no cross-lane reductions.  Rows r = i d1 + j outside
+    // [r0, r0 + nr) belong to other segments and count 0.
     const int i_lo = r0 / d1, i_hi = (r0 + nr - 1) / d1, ni = i_hi - i_lo + 1;
-    const int lg0 = ceil_log2(d1);
-    const double* q0p = q0;
-    for (int e = tid; e < (ni << lg0); e += kNT) {
-      const int ii = e >> lg0, j = e & ((1 << lg0) - 1);
-      const int r = (i_lo + ii) * d1 + j;
-      double v = (j < d1 && r >= r0 && r < r0 + nr) ? q1[j] * s_y[r - r0] : 0.0;
-      v = group_sum_low_dpp(v, lg0);
-      if (j == 0) s_part[a.S_off[a.gfac[g][0]] + i_lo + ii] += v;
-    }
-    // m_1[j] = sum_i q_0[i] y_ij: one group of 2^lg1 lanes per j, each lane
-    // over every 2^lg1-th i
-    const int lg1 = std::min(6, ceil_log2(ni));
-    for (int e = tid; e < (d1 << lg1); e += kNT) {
-      const int j = e >> lg1, sub = e & ((1 << lg1) - 1);
-      double v = 0.0;
-      for (int ii = sub; ii < ni; ii += 1 << lg1) {
-        const int r = (i_lo + ii) * d1 + j;
-        if (r >= r0 && r < r0 + nr) v = fma(q0p[i_lo + ii], s_y[r - r0], v);
+    for (int e = tid; e < ni + d1; e += kNT) {
+      if (e < ni) {
+        const int i = i_lo + e, rb = i * d1 - r0;  // y index of (i, j) is rb + j
+        const int j0 = std::max(0, -rb), j1 = std::min(d1, nr - rb);
+        double v0 = 0.0, v1 = 0.0;
+        int j = j0;
+        for (; j + 1 < j1; j += 2) {
+          v0 = fma(q1[j], s_y[rb + j], v0);
+          v1 = fma(q1[j + 1], s_y[rb + j + 1], v1);
+        }
+        if (j < j1) v0 = fma(q1[j], s_y[rb + j], v0);
+        s_part[a.S_off[a.gfac[g][0]] + i] += v0 + v1;
+      } else {
+        const int j = e - ni;
+        auto term = [&](int ii, double acc) {
+          const int r = (i_lo + ii) * d1 + j - r0;
+          const bool ok = r >= 0 && r < nr;
+          return fma(ok ? q0[i_lo + ii] : 0.0, s_y[ok ? r : 0], acc);
+        };
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+        int ii = 0;
+        for (; ii + 3 < ni; ii += 4) {
+          v0 = term(ii, v0);
+          v1 = term(ii + 1, v1);
+          v2 = term(ii + 2, v2);
+          v3 = term(ii + 3, v3);
+        }
+        for (; ii < ni; ++ii) v0 = term(ii, v0);
+        s_part[a.S_off[a.gfac[g][1]] + j] += (v0 + v1) + (v2 + v3);
       }
-      v = group_sum_low_dpp(v, lg1);
-      if (sub == 0) s_part[a.S_off[a.gfac[g][1]] + j] += v;
     }
   }
   __syncthreads();

@@ -154,8 +154,16 @@ class Emu:
                 i = r // d1
                 j = r - i * d1
                 i_lo, i_hi = r0 // d1, (r0 + nr - 1) // d1
-                # lane groups of the kernel's segmented reductions fit a wave
-                assert _ceil_log2(d1) <= 6 and min(6, _ceil_log2(i_hi - i_lo + 1)) <= 6
+                ni = i_hi - i_lo + 1
+                # the kernel's thread-per-i / thread-per-j pass: y indices
+                # rb + j over j in [max(0, -rb), min(d1, nr - rb)) and the
+                # clamped m_1 rows stay inside s_y; q_0 indices stay in range
+                for ii in range(ni):
+                    rb = (i_lo + ii) * d1 - r0
+                    j0, j1 = max(0, -rb), min(d1, nr - rb)
+                    if j1 > j0:
+                        _chk(np.array([rb + j0, rb + j1 - 1]), nr, "m0 y")
+                _chk(np.array([i_lo, i_hi]), int(a["gdim"][g][0]), "m1 q0")
                 f0, f1 = int(a["gfac"][g][0]), int(a["gfac"][g][1])
                 np.add.at(part, S_off[f1] + j, q0[i] * y)
                 np.add.at(part, S_off[f0] + i, q1[j] * y)
