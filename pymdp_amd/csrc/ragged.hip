@@ -61,7 +61,7 @@ constexpr int kCap = 6144;        // log-likelihood doubles one FPI unit keeps i
 constexpr int kRowsCap = 1024;    // rows of one FPI segment
 constexpr int kTeamMax = 128;     // FPI units of one agent
 constexpr int kTile = 4608;       // doubles of a stage-3 operand tile in LDS
-constexpr int kTile2 = 2304;      // doubles of a stage-2 operand tile in LDS
+constexpr int kTile2 = 4608;      // doubles of a stage-2 operand tile in LDS
 constexpr int kCols = 32;         // histories per stage-2 / stage-3 tile
 constexpr int kQTab = kCols * 65;  // doubles of a stage-2 / stage-3 history table in LDS
 constexpr int kE3Combos = 128;    // history combinations scored per E3 tile
@@ -685,17 +685,17 @@ __device__ __forceinline__ void rg_gemm_lds(const double* X, int rows, const dou
 // with row stride ld: every thread issues NB loads before storing, and the
 // (row, column) of its elements advance by the thread count with one carry
 // compare each (no per-element division).
-template <typename T, int NB, typename Src>
+template <typename T, int NB, typename Src, int NT = kNT>
 __device__ __forceinline__ void rg_stage_rows(const Src* src, int rows, int w, int ld, T* dst, int tid) {
   const long long n = (long long)rows * w;
-  const int dr = kNT / w, dc = kNT - dr * w;
+  const int dr = NT / w, dc = NT - dr * w;
   int r = tid / w, c = tid - r * w;
-  for (long long e0 = tid; e0 < n; e0 += (long long)NB * kNT) {
+  for (long long e0 = tid; e0 < n; e0 += (long long)NB * NT) {
     T v[NB];
     int off[NB];
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
-      const long long e = e0 + (long long)q * kNT;
+      const long long e = e0 + (long long)q * NT;
       off[q] = r * ld + c;
       if (e < n) v[q] = (T)src[e];
       c += dc;
@@ -707,7 +707,7 @@ __device__ __forceinline__ void rg_stage_rows(const Src* src, int rows, int w, i
     }
 #pragma unroll
     for (int q = 0; q < NB; ++q)
-      if (e0 + (long long)q * kNT < n) dst[off[q]] = v[q];
+      if (e0 + (long long)q * NT < n) dst[off[q]] = v[q];
   }
 }
 
@@ -718,15 +718,16 @@ __device__ __forceinline__ int rg_hcount(const Agent& a, int m, int t, int p) {
 
 // History vectors [c0, c1) of padded dim p of modality m at level t into
 // Q [c][k] (row stride d | 1); padding dims have one history, q = 1.
+template <int NT = kNT>
 __device__ __forceinline__ void rg_load_hist(const Agent& a, const double* hist, int m, int t, int p, int c0,
                                              int c1, double* Q, int tid) {
   const int d = a.ed[m][p], ld = d | 1, f = a.ef[m][p];
   if (f < 0) {
-    for (int e = tid; e < c1 - c0; e += kNT) Q[e * ld] = 1.0;
+    for (int e = tid; e < c1 - c0; e += NT) Q[e * ld] = 1.0;
     return;
   }
   const double* src = hist + a.hist_lvl[t][f] + (long long)c0 * a.S[f];
-  rg_stage_rows<double, (kCols * 64 + kNT - 1) / kNT>(src, c1 - c0, d, ld, Q, tid);
+  rg_stage_rows<double, (kCols * 64 + NT - 1) / NT, double, NT>(src, c1 - c0, d, ld, Q, tid);
 }
 
 // Stage 1: for positions ij in [r0, r1) (ij = i * d1 + j) and concatenated
@@ -964,7 +965,10 @@ __global__ __launch_bounds__(kNT) void rg_e2_kernel(Bufs b) {
 // Stage 3 at level t: pairs (h2, h1) in [r0, r1) (pair = h2 * H1 + h1),
 // first-dim histories h0 in [c0, c1): qo[o] = Z2[pair, o, :] . q0[h0], then
 // score = sum_{o<O} (qo C[o] - qo ln qo) + qo[O]  (one wave per combination).
-__global__ __launch_bounds__(kNT) void rg_e3_kernel(Bufs b) {
+// 512 threads: twice the waves of the other stages per tile at the same LDS
+// (the tile is LDS-bound at 3 per CU; measured 0.44 -> 0.37 ms).
+constexpr int kNT3 = 512, kNW3 = kNT3 / kWave;
+__global__ __launch_bounds__(kNT3) void rg_e3_kernel(Bufs b) {
   __shared__ double s_t[kTile];
   __shared__ double s_q[kQTab];
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
@@ -979,14 +983,14 @@ __global__ __launch_bounds__(kNT) void rg_e3_kernel(Bufs b) {
   // a rank-1 modality's stage-1 output already has the stage-2 layout
   const double* zb = a.gR[a.mod_group[m]] == 1 ? b.z1 : b.z2;
   const double* src = zb + a.z2_off[m][t] + (long long)tl.r0 * O1 * d0;
-  rg_stage_rows<double, (kTile + kNT - 1) / kNT>(src, np * O1, d0, ld0, s_t, tid);
-  rg_load_hist(a, b.hist + a.hist_off, m, t, 0, tl.c0, tl.c1, s_q, tid);
+  rg_stage_rows<double, (kTile + kNT3 - 1) / kNT3, double, kNT3>(src, np * O1, d0, ld0, s_t, tid);
+  rg_load_hist<kNT3>(a, b.hist + a.hist_off, m, t, 0, tl.c0, tl.c1, s_q, tid);
   const float* Cm = b.params + a.C_off[m] + (a.C_time[m] ? (long long)t * O : 0);
   const float c_lo = lane < O ? Cm[lane] : 0.0f;
   const float c_hi = lane + kWave < O ? Cm[lane + kWave] : 0.0f;
   __syncthreads();
   double* sc = b.score + a.score_off + a.score_base[t][m];
-  for (int cc = wave; cc < np * nh; cc += kNW) {
+  for (int cc = wave; cc < np * nh; cc += kNW3) {
     const int pl = cc / nh, hl = cc - pl * nh;
     const double* q0 = s_q + hl * ld0;
     double v = 0.0;
@@ -1767,7 +1771,7 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
     hipLaunchKernelGGL(rg_fpi_kernel<kF * kDim>, dim3(grid1), dim3(kNT), 0, s, b);
   if (b.n_t1) hipLaunchKernelGGL(rg_e1_kernel, dim3(b.n_t1), dim3(kNT), 0, s, b);
   if (b.n_t2) hipLaunchKernelGGL(rg_e2_kernel, dim3(b.n_t2), dim3(kNT), 0, s, b);
-  if (b.n_t3) hipLaunchKernelGGL(rg_e3_kernel, dim3(b.n_t3), dim3(kNT), 0, s, b);
+  if (b.n_t3) hipLaunchKernelGGL(rg_e3_kernel, dim3(b.n_t3), dim3(kNT3), 0, s, b);
   hipLaunchKernelGGL(rg_assemble_kernel, dim3((unsigned)pl->n_agents), dim3(kNT), 0, s, b);
   PMH_HIP_CHECK(hipGetLastError(), "ragged: ");
   return 0;

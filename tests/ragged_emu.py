@@ -17,7 +17,7 @@ from pymdp_amd import _native
 MINVAL = np.float32(2.220446049250313e-16)
 kF, kM, kR, kT, kDim, kNT = 8, 8, 3, 4, 64, 256
 kCap, kRowsCap, kTeamMax = 6144, 1024, 128
-kTile, kTile2, kCols = 4608, 2304, 32
+kTile, kTile2, kCols = 4608, 4608, 32
 kQTab = kCols * 65
 kE1A, kE1H, kE1Q = 8704, 1024, 16 * 65
 
