@@ -262,8 +262,11 @@ __device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const doub
     }
   }
   // weighted column sums: thread (col, rg) over rows rg, rg + nrg, ...;
-  // the row's (i, j) advances incrementally (no division per row)
-  const int nrg = kNT / n;
+  // the row's (i, j) advances incrementally (no division per row).  The nrg
+  // row groups are summed serially below, so nrg is capped near sqrt(nr):
+  // rows per thread + partials per column stays small (a narrow last axis
+  // would otherwise give up to 128 groups of one row each).
+  const int nrg = std::min(kNT / n, 1 + (int)sqrtf((float)nr));
   const int col = tid % n, rg = tid / n;
   if (rg < nrg) {
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;

@@ -139,7 +139,7 @@ class Emu:
             q1 = qv[S_off[a["gfac"][g][1]]:]
             i = r // d1
             w = q0[_chk(i, int(a["gdim"][g][0]), "w q0")] * q1[_chk(r - i * d1, d1, "w q1")]
-        nrg = kNT // n
+        nrg = min(kNT // n, 1 + int(np.sqrt(np.float32(nr))))
         assert nrg * n <= kNT
         part[S_off[fl]:S_off[fl] + n] += w @ rows
         if R >= 2:
