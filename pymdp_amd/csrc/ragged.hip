@@ -1034,21 +1034,42 @@ __global__ __launch_bounds__(kNT) void rg_assemble_kernel(Bufs b) {
   const double* sc = b.score + a.score_off;
   double* logit = b.score + a.logit_off;
   double lmax = -INFINITY;
-  for (int p = tid; p < P; p += kNT) {
-    double G = 0.0;
+  // four policies per thread at a time, so that their history-index and
+  // score gathers for one (t, m) are in flight together
+  constexpr int kPB = 4;
+  for (int p0 = tid; p0 < P; p0 += kPB * kNT) {
+    double G[kPB];
+#pragma unroll
+    for (int q = 0; q < kPB; ++q) G[q] = 0.0;
     for (int t = 0; t < T; ++t)
       for (int m = 0; m < a.M; ++m) {
         const int g = a.mod_group[m];
-        long long ix = a.score_base[t][m];
-        for (int j = 0; j < a.gR[g]; ++j)
-          ix += (long long)p2h[((long long)p * T + t) * F + a.gfac[g][j]] * a.score_stride[t][m][j];
-        G += sc[ix];
+        long long ix[kPB];
+#pragma unroll
+        for (int q = 0; q < kPB; ++q) ix[q] = a.score_base[t][m];
+        for (int j = 0; j < a.gR[g]; ++j) {
+          const long long st = a.score_stride[t][m][j];
+          const int fj = a.gfac[g][j];
+#pragma unroll
+          for (int q = 0; q < kPB; ++q) {
+            const int p = min(p0 + q * kNT, P - 1);
+            ix[q] += (long long)p2h[((long long)p * T + t) * F + fj] * st;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < kPB; ++q) G[q] += sc[ix[q]];
       }
-    b.out[a.G_off + p] = (float)G;
-    double l = (double)a.gamma * G;
-    if (a.has_E) l += log_stable_f64(b.params[a.E_off + p]);
-    logit[p] = l;
-    lmax = fmax(lmax, l);
+#pragma unroll
+    for (int q = 0; q < kPB; ++q) {
+      const int p = p0 + q * kNT;
+      if (p < P) {
+        b.out[a.G_off + p] = (float)G[q];
+        double l = (double)a.gamma * G[q];
+        if (a.has_E) l += log_stable_f64(b.params[a.E_off + p]);
+        logit[p] = l;
+        lmax = fmax(lmax, l);
+      }
+    }
   }
   lmax = wave_max(lmax);
   if (lane == 0) s_red[wave] = lmax;
@@ -1056,8 +1077,13 @@ __global__ __launch_bounds__(kNT) void rg_assemble_kernel(Bufs b) {
   double mx = s_red[0];
   for (int w = 1; w < kNW; ++w) mx = fmax(mx, s_red[w]);
   __syncthreads();
+  // exp(l - max) replaces the logit (one exp per policy)
   double se = 0.0;
-  for (int p = tid; p < P; p += kNT) se += exp(logit[p] - mx);
+  for (int p = tid; p < P; p += kNT) {
+    const double e = exp(logit[p] - mx);
+    logit[p] = e;
+    se += e;
+  }
   se = wave_sum(se);
   if (lane == 0) s_red[wave] = se;
   __syncthreads();
@@ -1067,7 +1093,7 @@ __global__ __launch_bounds__(kNT) void rg_assemble_kernel(Bufs b) {
   float best = -1.0f;
   int bi = 0x7fffffff;
   for (int p = tid; p < P; p += kNT) {
-    const float q = (float)(exp(logit[p] - mx) / tot);
+    const float q = (float)(logit[p] / tot);
     b.out[a.qpi_off + p] = q;
     if (q > best) {
       best = q;
