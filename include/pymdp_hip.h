@@ -345,7 +345,9 @@ int pymdp_hip_update_pB(const float* pB, int64_t pB_batch_stride, float* qB_out,
  * agents: S, U [F each], O [M each], A_dep_rank [M each] with A_dep_flat,
  * B_dep_rank [F each] with B_dep_flat, C_time [M each] (1 = C_m is [T, O_m]),
  * E_present [1 each] (0 = uniform E), gamma [1 each], policies [P*T*F each,
- * int32].  Limits: F, M <= 8, T <= 4, S, O <= 64, dependency ranks <= 3.
+ * int32].  Limits: F, M <= 8, T <= 4, S, O <= 64, dependency ranks <= 3,
+ * num_iter <= 1022 (the sweep number tags the partial marginals exchanged
+ * between the workgroups of a large agent).
  * The plan is built on the host; pymdp_hip_ragged_plan_upload allocates and
  * fills its device tables on the current device (the only allocation).
  */
@@ -405,8 +407,10 @@ int pymdp_hip_ragged_step(void* plan, const float* params, const int32_t* obs,
  * captured. */
 int pymdp_hip_ragged_autotune(void* plan, const float* params, const int32_t* obs,
                               float* out, int32_t* actions, void* stream, float* best_ms);
-/* *flag_out = 1 if a team barrier of an earlier step gave up waiting (the
- * posteriors of that agent are NaN), else 0.  Synchronises the device. */
+/* *flag_out = 1 if a team exchange (the per-sweep wait of a large agent's
+ * workgroups for each other's partial marginals) of an earlier step gave up
+ * waiting (the posteriors of that agent are NaN), else 0.  Synchronises the
+ * device. */
 int pymdp_hip_ragged_status(void* plan, int32_t* flag_out);
 /* Prior of later steps' FPI: carry = 0 reads D from the params arena (the
  * default), carry = 1 reads the predicted prior the previous step wrote at

@@ -66,6 +66,14 @@ constexpr int kCols = 32;         // histories per stage-2 / stage-3 tile
 constexpr int kQTab = kCols * 65;  // doubles of a stage-2 / stage-3 history table in LDS
 constexpr int kE3Combos = 128;    // history combinations scored per E3 tile
 constexpr long long kSpinMax = 1LL << 22;
+// Team exchange tags: the low kTagBits mantissa bits of every published
+// partial carry the sweep's sequence number (step * iters + sweep), cleared
+// again on read (a relative change below 2^-42); a fresh partial buffer holds
+// kTagFresh, which no sweep of the first step carries (iters <= kTagMaxIters).
+constexpr int kTagBits = 10;
+constexpr unsigned long long kTagMask = (1ull << kTagBits) - 1;
+constexpr unsigned long long kTagFresh = kTagMask;
+constexpr int kTagMaxIters = (int)kTagMask - 1;
 
 struct Agent {
   int F, M, T, P, G, iters, totS, has_E, team;
@@ -129,8 +137,7 @@ struct Bufs {
   double* z1;
   double* z2;
   double* part;
-  int* team_cnt;
-  int* cnt;          // [0] FPI queue, [3] team-barrier timeout flag
+  int* cnt;          // [0] FPI queue, [1] step sequence number, [3] team-exchange timeout flag
   long long* trace;  // optional: [unit or tile][start, end] wall clock ticks
   int n_agents, n_units, n_t1, n_t2, n_t3;
   int carry;  // 1: the FPI prior is the previous step's predicted prior (out + pred_off), 0: D
@@ -368,11 +375,12 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
   __shared__ double s_red[kNT];
   __shared__ double s_ltab[256];
   __shared__ int s_obs[kM];
-  __shared__ int s_item, s_broken;
+  __shared__ int s_item, s_broken, s_seq;
   __shared__ Agent s_ag;
   __shared__ Unit s_un;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
   for (int i = tid; i < 256; i += kNT) s_ltab[i] = g_logtab[i];
+  if (tid == 0) s_seq = b.cnt[1];  // this step's sequence number (team-exchange tags)
 
   for (;;) {
     if (tid == 0) {
@@ -457,51 +465,57 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
       if (a.team > 1) {
         // exchange partial marginals with the other units of this agent
         // (double-buffered by sweep parity: a unit overwrites its slot of
-        // buffer it & 1 only after every unit has read it at sweep it - 1).
-        // The partials move as agent-scope relaxed atomic stores and loads
-        // (device-coherence scope bits on each access), so no agent-scope
-        // fence is needed: such a fence writes back (release) or invalidates
-        // (acquire) the whole L2 of the XCD on every sweep.  Wave 0 stores
-        // this unit's partials, waits until every store is acknowledged
-        // (s_waitcnt vmcnt(0)), then adds its arrival; lane 0 spins on the
-        // count; the barrier below orders every thread's partial loads after
-        // the spin has seen all arrivals.
-        double* part = b.part + a.part_off + (long long)(it & 1) * a.team * totS;
-        if (wave == 0) {
-          for (int i = lane; i < totS; i += kWave)
-            __hip_atomic_store(part + u.slice * totS + i, s_part[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (!s_broken) {
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            if (lane == 0) {
-              int* cnt = b.team_cnt + u.agent;
-              __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              const int target = a.team * (it + 1);
-              long long spins = 0;
-              while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins > kSpinMax) {
-                  s_broken = 1;
-                  atomicOr(&b.cnt[3], 1);
-                  break;
-                }
-              }
-            }
+        // buffer it & 1 only after it has read every unit's sweep it - 1
+        // partials, which each unit publishes only after reading sweep it - 2
+        // from buffer it & 1).  Each partial is published as one agent-scope
+        // relaxed atomic 64-bit store (device-coherence scope bits, no
+        // L2-flushing fence) with the sweep's sequence number in its low
+        // mantissa bits; readers poll the partials themselves with
+        // agent-scope relaxed atomic loads until every tag matches, so a
+        // sweep costs one round trip instead of a store acknowledgement, an
+        // arrival count and a load.  A slot holds the sweep's tag or that of
+        // two sweeps earlier (or the fresh-buffer tag), never a match by
+        // accident; the poll is bounded (on timeout the agent's qs are NaN
+        // and the plan's status flag is set).
+        const unsigned long long tag =
+            ((unsigned long long)(unsigned)s_seq * (unsigned)a.iters + (unsigned)it) & kTagMask;
+        unsigned long long* part = reinterpret_cast<unsigned long long*>(b.part + a.part_off) +
+                                   (long long)(it & 1) * a.team * totS;
+        if (wave == 0)
+          for (int i = lane; i < totS; i += kWave) {
+            const unsigned long long bits = ((unsigned long long)__double_as_longlong(s_part[i]) & ~kTagMask) | tag;
+            __hip_atomic_store(part + u.slice * totS + i, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           }
-        }
-        __syncthreads();
         for (int i = tid; i < totS; i += kNT) {
-          const double* pp = part + i;
+          const unsigned long long* pp = part + i;
           double s = 0.0;
           for (int s0 = 0; s0 < a.team; s0 += 16) {
-            double v[16];
+            const int nq = min(16, a.team - s0);
+            unsigned long long v[16];
 #pragma unroll
             for (int q = 0; q < 16; ++q)
-              v[q] = s0 + q < a.team ? __hip_atomic_load(pp + (long long)(s0 + q) * totS, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT)
-                                     : 0.0;
+              v[q] = q < nq ? __hip_atomic_load(pp + (long long)(s0 + q) * totS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                            : tag;
+            long long spins = 0;
+            for (;;) {
+              bool ready = true;
+#pragma unroll
+              for (int q = 0; q < 16; ++q) ready = ready && (v[q] & kTagMask) == tag;
+              if (ready || s_broken) break;  // after a timeout no unit of the step waits again
+              if (++spins > kSpinMax) {
+                s_broken = 1;
+                atomicOr(&b.cnt[3], 1);
+                break;
+              }
+              __builtin_amdgcn_s_sleep(2);
+#pragma unroll
+              for (int q = 0; q < 16; ++q)
+                if ((v[q] & kTagMask) != tag)
+                  v[q] = __hip_atomic_load(pp + (long long)(s0 + q) * totS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
 #pragma unroll
             for (int q = 0; q < 16; ++q)
-              if (s0 + q < a.team) s += v[q];
+              if (q < nq) s += __longlong_as_double((long long)(v[q] & ~kTagMask));
           }
           s_logq[i] = s_lp[i] + s;
           s_part[i] = 0.0;
@@ -1124,8 +1138,6 @@ __global__ __launch_bounds__(kNT) void rg_assemble_kernel(Bufs b) {
     const int* pol0 = b.iarena + a.pol0_off;
     for (int f = 0; f < F; ++f) b.act[a.act_off + f] = pol0[(long long)bx * F + f];
     s_best = bx;
-    // this agent's team barrier starts from zero at the next step
-    b.team_cnt[blockIdx.x] = 0;
   }
   __syncthreads();
   // empirical prior for the next step (pymdp/agent.py:825-851): B_f[:, deps,
@@ -1138,8 +1150,12 @@ __global__ __launch_bounds__(kNT) void rg_assemble_kernel(Bufs b) {
       for (int i = tid; i < a.S[f]; i += kNT) b.out[a.pred_off + a.S_off[f] + i] = (float)h[i];
     }
   }
-  // the FPI queue starts from zero at the next step
-  if (blockIdx.x == 0 && tid == 0) b.cnt[0] = 0;
+  // the FPI queue starts from zero at the next step, whose team-exchange
+  // tags continue the sweep sequence
+  if (blockIdx.x == 0 && tid == 0) {
+    b.cnt[0] = 0;
+    b.cnt[1] = (int)((unsigned)b.cnt[1] + 1u);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1168,7 +1184,6 @@ struct Plan {
   E1Rec* d_e1 = nullptr;
   int32_t* d_iarena = nullptr;
   double *d_hist = nullptr, *d_score = nullptr, *d_z1 = nullptr, *d_z2 = nullptr, *d_part = nullptr;
-  int* d_team = nullptr;
   int* d_cnt = nullptr;
 };
 
@@ -1184,7 +1199,7 @@ static void free_plan(Plan* p) {
   (void)hipSetDevice(p->device);
   for (void* x : {(void*)p->d_ag, (void*)p->d_units, (void*)p->d_t1, (void*)p->d_e1, (void*)p->d_t2, (void*)p->d_t3,
                   (void*)p->d_iarena, (void*)p->d_hist, (void*)p->d_score, (void*)p->d_z1, (void*)p->d_z2,
-                  (void*)p->d_part, (void*)p->d_team, (void*)p->d_cnt})
+                  (void*)p->d_part, (void*)p->d_cnt})
     (void)hipFree(x);
   (void)hipSetDevice(prev);
   delete p;
@@ -1219,6 +1234,7 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
   *plan_out = nullptr;
   if (n_agents <= 0) return einval("n_agents must be positive");
   if (num_iter < 0) return einval("num_iter must be >= 0");
+  if (num_iter > kTagMaxIters) return einval("num_iter must be <= 1022");
   Plan* pl = new Plan();
   pl->n_agents = n_agents;
   long long pA = 0, pO = 0, pS = 0;  // running offsets: A_dep_flat, modality, factor
@@ -1666,7 +1682,6 @@ static int upload(Plan* pl) {
   if ((e = alloc((void**)&pl->d_z1, sizeof(double) * pl->z1_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_z2, sizeof(double) * pl->z2_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_part, sizeof(double) * pl->part_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
-  if ((e = alloc((void**)&pl->d_team, sizeof(int) * pl->n_agents)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_cnt, sizeof(int) * 4)) != hipSuccess) return fail_dev(e, "hipMalloc");
   auto up = [&](void* d, const void* h, size_t bytes) {
     return bytes ? hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) : hipSuccess;
@@ -1679,7 +1694,9 @@ static int upload(Plan* pl) {
       (e = up(pl->d_t3, pl->t3.data(), sizeof(Tile) * pl->t3.size())) != hipSuccess ||
       (e = up(pl->d_iarena, pl->iarena.data(), sizeof(int32_t) * pl->iarena.size())) != hipSuccess)
     return fail_dev(e, "hipMemcpy");
-  if ((e = hipMemset(pl->d_team, 0, sizeof(int) * pl->n_agents)) != hipSuccess) return fail_dev(e, "hipMemset");
+  // every partial of a fresh plan carries the fresh-buffer tag (all ones)
+  if ((e = hipMemset(pl->d_part, 0xFF, sizeof(double) * std::max<long long>(1, pl->part_total))) != hipSuccess)
+    return fail_dev(e, "hipMemset");
   if ((e = hipMemset(pl->d_cnt, 0, sizeof(int) * 4)) != hipSuccess) return fail_dev(e, "hipMemset");
   if ((e = hipDeviceSynchronize()) != hipSuccess) return fail_dev(e, "hipDeviceSynchronize");
   return PYMDP_HIP_OK;
@@ -1785,7 +1802,6 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
   b.z1 = pl->d_z1;
   b.z2 = pl->d_z2;
   b.part = pl->d_part;
-  b.team_cnt = pl->d_team;
   b.cnt = pl->d_cnt;
   b.n_agents = (int)pl->n_agents;
   b.n_units = (int)pl->units.size();

@@ -233,7 +233,7 @@ class RaggedBatch:
         return float(ms.value)
 
     def status(self) -> int:
-        """1 if an FPI team barrier of an earlier step timed out (that
+        """1 if an FPI team exchange of an earlier step timed out (that
         agent's posterior is NaN); synchronises the device."""
         flag = ctypes.c_int32(0)
         with torch.cuda.device(self.device):
