@@ -130,6 +130,14 @@ def test_invalid_agents_rejected(over, msg):
         ragged.RaggedPlan([_one_agent(**over)], 16)
 
 
+def test_sweep_count_limit():
+    """The sweep number tags the partial marginals a team exchanges (10
+    mantissa bits, one value kept for a fresh buffer): num_iter <= 1022."""
+    ragged.RaggedPlan([_one_agent()], 1022)
+    with pytest.raises(ValueError, match="num_iter must be <= 1022"):
+        ragged.RaggedPlan([_one_agent()], 1023)
+
+
 def test_plan_layout_is_disjoint():
     w = workloads.ragged(seed=2, n_agents=50, tensors=False)
     plan = ragged.RaggedPlan(w["agents"], 16)
