@@ -216,16 +216,24 @@ __global__ __launch_bounds__(kNT) void efe_wb_kernel(const Desc* __restrict__ dp
 }
 
 // 2. Rollout: qs[t][f][h] = B_f[:, parents..., u_h] . (x parent marginals)
+// One workgroup per agent.  Per (level, factor), B_f is staged in LDS as
+// [u][s][k] rows of stride Kb | 1 (in global memory its k axis is strided by
+// U, and lanes of consecutive s would all hit one bank), and a one-dependency
+// factor's level t-1 parent vectors are staged beside it, when they fit;
+// otherwise the dot products read global memory.  The k order of every dot
+// product is the same either way.
+constexpr int kRollB = 10240;  // floats of one factor's staged B
+constexpr int kRollP = 2048;  // doubles of staged parent vectors
 __global__ __launch_bounds__(kNT) void efe_rollout_kernel(const Desc* __restrict__ dp, const int* __restrict__ tab,
                                                           const Args a) {
+  __shared__ float s_b[kRollB];
+  __shared__ double s_p[kRollP];
   const Desc& d = *dp;
   const int b = blockIdx.x;
   double* ws = a.ws + (long long)b * d.ws_stride;
   // level -1 (qs_init) into the workspace in FP64
   const float* qin = a.qs + (long long)b * a.qs_stride;
   for (int i = threadIdx.x; i < a.qs_off[d.F]; i += kNT) ws[d.qs0_off + i] = (double)qin[i];
-  __threadfence_block();
-  __syncthreads();
   const double* qs0 = ws + d.qs0_off;
   for (int t = 0; t < d.T; ++t) {
     for (int f = 0; f < d.F; ++f) {
@@ -234,21 +242,42 @@ __global__ __launch_bounds__(kNT) void efe_rollout_kernel(const Desc* __restrict
       const int U = d.U[f];
       const int rank = d.Brank[f];
       const int Kb = d.Kb[f];
+      const int ldb = Kb | 1;
       const float* Bf = a.B[f] + (long long)b * a.B_stride[f];
       const int* par = tab + d.f_par_off[t][f];
       const int* act = tab + d.f_act_off[t][f];
       double* out = ws + d.qs_off[t][f];
       const long long total = (long long)n * S;
+      const int pf0 = d.Bdeps[f][0], Sp = d.S[pf0];
+      const double* qsrc = (t == 0) ? qs0 + a.qs_off[pf0] : ws + d.qs_off[t - 1][pf0];  // rank-1 parents
+      const long long np = (t == 0) ? 1 : d.fn[t - 1][pf0];
+      const bool stB = (long long)U * S * ldb <= kRollB;
+      const bool stP = rank == 1 && np * Sp <= kRollP;
+      // earlier levels' workspace writes and earlier factors' LDS reads are
+      // complete past this barrier
+      __syncthreads();
+      if (stB) {
+        const int KU = Kb * U;
+        for (int e = threadIdx.x; e < S * KU; e += kNT) {
+          const int s = e / KU, r = e - s * KU, k = r / U, u = r - k * U;
+          s_b[(u * S + s) * ldb + k] = Bf[e];
+        }
+      }
+      if (stP)
+        for (int e = threadIdx.x; e < np * Sp; e += kNT) s_p[e] = qsrc[e];
+      __syncthreads();
       for (long long i = threadIdx.x; i < total; i += kNT) {
         const int h = (int)(i / S);
         const int s = (int)(i - (long long)h * S);
         const int u = act[h];
-        const float* row = Bf + (long long)s * Kb * U + u;
+        // B_f[s, k.., u]: s_b row (u S + s) or global Bf + s Kb U + u with k strided by U
+        const float* row = stB ? s_b + (u * S + s) * ldb : Bf + (long long)s * Kb * U + u;
+        const int ks = stB ? 1 : U;
         double acc = 0.0;
         if (rank == 1) {
-          const int pf = d.Bdeps[f][0];
-          const double* q = (t == 0) ? qs0 + a.qs_off[pf] : ws + d.qs_off[t - 1][pf] + (long long)par[h] * d.S[pf];
-          for (int k = 0; k < Kb; ++k) acc = fma((double)row[(long long)k * U], q[k], acc);
+          const long long po = (t == 0) ? 0 : (long long)par[h] * Sp;
+          const double* q = stP ? s_p + po : qsrc + po;
+          for (int k = 0; k < Kb; ++k) acc = fma((double)row[k * ks], q[k], acc);
         } else {
           const double* qp[8];
           for (int r = 0; r < rank; ++r) {
@@ -264,15 +293,14 @@ __global__ __launch_bounds__(kNT) void efe_rollout_kernel(const Desc* __restrict
               rem /= sz;
               w *= qp[r][c];
             }
-            acc = fma((double)row[(long long)k * U], w, acc);
+            acc = fma((double)row[k * ks], w, acc);
           }
         }
         out[(long long)h * S + s] = acc;
       }
     }
-    __threadfence_block();
-    __syncthreads();
   }
+  __syncthreads();
 }
 
 // 2b. pB term of every history (pymdp/control.py:554-602):
