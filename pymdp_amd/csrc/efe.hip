@@ -740,28 +740,56 @@ __global__ __launch_bounds__(1024) void efe_scatter_kernel(const Desc* __restric
   const double lnE_uniform = log_stable_f64(1.0f / (float)P);
   __shared__ double red[16];
   double mx = -INFINITY;
-  for (int p = threadIdx.x; p < P; p += blockDim.x) {
-    double sum = 0.0;
+  // four policies per thread at a time, so that their index and score
+  // gathers for one (t, m) are in flight together (same sums per policy)
+  constexpr int kPB = 4;
+  const int nt = blockDim.x;
+  for (int p0 = threadIdx.x; p0 < P; p0 += kPB * nt) {
+    int pc[kPB];
+    double sum[kPB];
+#pragma unroll
+    for (int q = 0; q < kPB; ++q) {
+      pc[q] = min(p0 + q * nt, P - 1);
+      sum[q] = 0.0;
+    }
     for (int t = 0; t < d.T; ++t) {
-      double st = 0.0;
+      double st[kPB];
+#pragma unroll
+      for (int q = 0; q < kPB; ++q) st[q] = 0.0;
       for (int m = 0; m < d.M; ++m) {
-        const int g = d.mod_group[m];
-        st += ws[d.sc_off[t][m] + tab[d.g_p2s_off[t][g] + p]];
+        const int* ix = tab + d.g_p2s_off[t][d.mod_group[m]];
+        const double* sc = ws + d.sc_off[t][m];
+        int j[kPB];
+#pragma unroll
+        for (int q = 0; q < kPB; ++q) j[q] = ix[pc[q]];
+#pragma unroll
+        for (int q = 0; q < kPB; ++q) st[q] += sc[j[q]];
       }
       if (ind) {
-        for (int f = 0; f < d.F; ++f) st += ws[d.fsc_off[t][f] + tab[d.f_p2h_off[t][f] + p]];
+        for (int f = 0; f < d.F; ++f)
+#pragma unroll
+          for (int q = 0; q < kPB; ++q) st[q] += ws[d.fsc_off[t][f] + tab[d.f_p2h_off[t][f] + pc[q]]];
       }
       if (d.has_pB) {
         for (int f = 0; f < d.F; ++f)
-          if (a.pB[f]) st -= ws[d.pbs_off[t][f] + tab[d.f_p2h_off[t][f] + p]];
+          if (a.pB[f])
+#pragma unroll
+            for (int q = 0; q < kPB; ++q) st[q] -= ws[d.pbs_off[t][f] + tab[d.f_p2h_off[t][f] + pc[q]]];
       }
-      sum += st;
+#pragma unroll
+      for (int q = 0; q < kPB; ++q) sum[q] += st[q];
     }
-    G[p] = (float)sum;
-    const double lnE = a.E ? log_stable_f64(a.E[(long long)b * a.E_stride + p]) : lnE_uniform;
-    const double logit = gamma * sum + lnE;
-    Gd[p] = logit;
-    mx = fmax(mx, logit);
+#pragma unroll
+    for (int q = 0; q < kPB; ++q) {
+      const int p = p0 + q * nt;
+      if (p < P) {
+        G[p] = (float)sum[q];
+        const double lnE = a.E ? log_stable_f64(a.E[(long long)b * a.E_stride + p]) : lnE_uniform;
+        const double logit = gamma * sum[q] + lnE;
+        Gd[p] = logit;
+        mx = fmax(mx, logit);
+      }
+    }
   }
   if (a.q_pi == nullptr) return;
   mx = wave_max(mx);
