@@ -148,8 +148,17 @@ __global__ __launch_bounds__(kNT) void efe_ha_kernel(const Desc* __restrict__ dp
   const long long k = (long long)blockIdx.x * kNT + threadIdx.x;
   if (k >= K) return;
   const float* Am = a.A[m] + (long long)b * a.A_stride[m];
+  // eight outcome rows' loads in flight before their logs (same sum order)
   double acc = 0.0;
-  for (int o = 0; o < O; ++o) acc += xlogx_f64(Am[(long long)o * K + k]);
+  int o = 0;
+  for (; o + 8 <= O; o += 8) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = Am[(long long)(o + j) * K + k];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc += xlogx_f64(v[j]);
+  }
+  for (; o < O; ++o) acc += xlogx_f64(Am[(long long)o * K + k]);
   a.ws[(long long)b * d.ws_stride + d.ha_off[m] + k] = acc;
 }
 
