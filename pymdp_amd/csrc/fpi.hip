@@ -101,14 +101,14 @@ __device__ void softmax_factors(const FpiDesc& d, const double* src, double* dst
     const int n = d.S[f];
     double m = -INFINITY;
     for (int i = lane; i < n; i += kWave) m = fmax(m, src[off + i]);
-    m = wave_max(m);
+    m = wave_max_dpp(m);
     double s = 0.0;
     for (int i = lane; i < n; i += kWave) {
       const double e = exp(src[off + i] - m);
       dst[off + i] = e;
       s += e;
     }
-    s = wave_sum(s);
+    s = wave_sum_dpp(s);
     for (int i = lane; i < n; i += kWave) dst[off + i] = dst[off + i] / s;
   }
 }
