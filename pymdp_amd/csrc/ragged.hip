@@ -632,70 +632,39 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
 }
 
 // ---------------------------------------------------------------------------
-// EFE stages: batched small GEMMs Y[row, col] = X[row, :] . Q[col, :] with
-// X (rows x nk) and Q (cols x nk) staged in LDS (row stride nk | 1); each
-// thread computes a 4 x 4 block of (row, col) pairs strided by the block
-// counts, so the lanes of a wave read consecutive X rows (no bank
-// conflicts) and mostly the same Q rows (broadcast).
+// EFE stages
 // ---------------------------------------------------------------------------
-template <int RB, int CB, typename Store>
-__device__ __forceinline__ void rg_gemm_blk(const double* X, int rows, const double* Q, int cols, int nk, int tid,
-                                            Store store) {
-  const int ld = nk | 1;
-  const int nrb = (rows + RB - 1) / RB, ncb = (cols + CB - 1) / CB;
-  for (int e = tid; e < nrb * ncb; e += kNT) {
-    const int rb = e % nrb, cb = e / nrb;
-    double acc[RB][CB];
-#pragma unroll
-    for (int u = 0; u < RB; ++u)
-#pragma unroll
-      for (int v = 0; v < CB; ++v) acc[u][v] = 0.0;
-    int ro[RB], co[CB];
-#pragma unroll
-    for (int u = 0; u < RB; ++u) {
-      const int r = rb + u * nrb;
-      ro[u] = (r < rows ? r : 0) * ld;
+// Y[row, col] = X[row, :] . Q[col, :] on FP64 MFMA (v_mfma_f64_16x16x4f64):
+// X (rows x nk) and Q (cols x nk) in LDS with row stride nk | 1; each wave
+// takes 16 x 16 blocks of Y.  Lane l feeds X row l % 16 and Q row l % 16 at
+// k = k0 + l / 16 (k >= nk and rows / columns past the block read index 0
+// and feed zeros) and holds element j of the block at row 4 j + l / 16,
+// column l % 16.
+typedef double rg_d4 __attribute__((ext_vector_type(4)));
+template <int NW, typename Store>
+__device__ __forceinline__ void rg_gemm_mfma(const double* X, int rows, const double* Q, int cols, int nk, int lane,
+                                             int wave, Store store) {
+  const int ld = nk | 1, nkp = (nk + 3) & ~3;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nrb = (rows + 15) >> 4, ncb = (cols + 15) >> 4;
+  for (int blk = wave; blk < nrb * ncb; blk += NW) {
+    const int rb = blk % nrb, cb = blk / nrb;
+    const int r = rb * 16 + li, c = cb * 16 + li;
+    const double* xr = X + (r < rows ? r : 0) * ld;
+    const double* qr = Q + (c < cols ? c : 0) * ld;
+    rg_d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < nkp; k0 += 4) {
+      const int k = k0 + lk;
+      const bool kv = k < nk;
+      const double x = xr[kv ? k : 0], q = qr[kv ? k : 0];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(kv ? x : 0.0, kv ? q : 0.0, acc, 0, 0, 0);
     }
 #pragma unroll
-    for (int v = 0; v < CB; ++v) {
-      const int c = cb + v * ncb;
-      co[v] = (c < cols ? c : 0) * ld;
-    }
-    for (int k = 0; k < nk; ++k) {
-      double x[RB], q[CB];
-#pragma unroll
-      for (int u = 0; u < RB; ++u) x[u] = X[ro[u] + k];
-#pragma unroll
-      for (int v = 0; v < CB; ++v) q[v] = Q[co[v] + k];
-#pragma unroll
-      for (int u = 0; u < RB; ++u)
-#pragma unroll
-        for (int v = 0; v < CB; ++v) acc[u][v] = fma(x[u], q[v], acc[u][v]);
-    }
-#pragma unroll
-    for (int u = 0; u < RB; ++u) {
-      const int r = rb + u * nrb;
-      if (r >= rows) continue;
-#pragma unroll
-      for (int v = 0; v < CB; ++v) {
-        const int c = cb + v * ncb;
-        if (c < cols) store(r, c, acc[u][v]);
-      }
+    for (int j = 0; j < 4; ++j) {
+      const int rr = rb * 16 + 4 * j + lk;
+      if (rr < rows && c < cols) store(rr, c, acc[j]);
     }
   }
-}
-
-// Sixteen (row, column) pairs per thread; the block is tall when the
-// column count is small so that more threads have work.
-template <typename Store>
-__device__ __forceinline__ void rg_gemm_lds(const double* X, int rows, const double* Q, int cols, int nk, int tid,
-                                            Store store) {
-  if (cols >= 4)
-    rg_gemm_blk<4, 4>(X, rows, Q, cols, nk, tid, store);
-  else if (cols >= 2)
-    rg_gemm_blk<8, 2>(X, rows, Q, cols, nk, tid, store);
-  else
-    rg_gemm_blk<16, 1>(X, rows, Q, cols, nk, tid, store);
 }
 
 // Copies a row-major block [rows x w] (contiguous in global memory) into LDS
@@ -756,7 +725,6 @@ __device__ __forceinline__ void rg_load_hist(const Agent& a, const double* hist,
 // history columns with FP64 MFMA (v_mfma_f64_16x16x4f64: lane l holds
 // X[l % 16][k + l / 16] and Q[col l % 16][k + l / 16]; accumulator element v
 // of lane l is row 4 v + l / 16, column l % 16).
-typedef double rg_d4 __attribute__((ext_vector_type(4)));
 constexpr int kE1A = 8704;    // floats of the A block
 constexpr int kE1H = 1024;    // doubles of the sum_o A ln A rows
 constexpr int kE1Q = 16 * 65;  // doubles of the history block
@@ -951,9 +919,12 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
   if (tr && tid == 0) tr[1] = wall_clock64();
 }
 
-// Stage 2 at level t: rows R = (h2, o, i) in [r0, r1) of Z1 (length d1),
+// Stage 2 at level t (FP64 MFMA): rows R = (h2, o, i) in [r0, r1) of Z1 (length d1),
 // columns = middle-dim histories [c0, c1).  Z2[((h2 * H1 + h1) * (O+1) + o) * d0 + i].
-__global__ __launch_bounds__(kNT) void rg_e2_kernel(Bufs b) {
+// 512 threads: the tile is LDS-bound at 3 per CU, so more waves per tile hide
+// more of its load latency.
+constexpr int kNT2 = 512;
+__global__ __launch_bounds__(kNT2) void rg_e2_kernel(Bufs b) {
   __shared__ double s_t[kTile2];
   __shared__ double s_q[kQTab];
   const int tid = threadIdx.x;
@@ -966,12 +937,12 @@ __global__ __launch_bounds__(kNT) void rg_e2_kernel(Bufs b) {
   const int nr = tl.r1 - tl.r0, nc = tl.c1 - tl.c0;
   const int H1 = rg_hcount(a, m, t, 1);
   const double* src = b.z1 + a.z1_off[m] + ((long long)a.tstart[m][t] * O1 * d0 + tl.r0) * d1;
-  rg_stage_rows<double, (kTile2 + kNT - 1) / kNT>(src, nr, d1, ld1, s_t, tid);
-  rg_load_hist(a, b.hist + a.hist_off, m, t, 1, tl.c0, tl.c1, s_q, tid);
+  rg_stage_rows<double, (kTile2 + kNT2 - 1) / kNT2, double, kNT2>(src, nr, d1, ld1, s_t, tid);
+  rg_load_hist<kNT2>(a, b.hist + a.hist_off, m, t, 1, tl.c0, tl.c1, s_q, tid);
   __syncthreads();
   double* Z = b.z2 + a.z2_off[m][t];
   const int r0 = tl.r0, c0 = tl.c0, od = O1 * d0;
-  rg_gemm_lds(s_t, nr, s_q, nc, d1, tid, [&](int r, int c, double v) {
+  rg_gemm_mfma<kNT2 / kWave>(s_t, nr, s_q, nc, d1, tid & (kWave - 1), tid >> 6, [&](int r, int c, double v) {
     const int R = r0 + r;
     const int h2 = R / od, oi = R - h2 * od;
     Z[((long long)h2 * H1 + c0 + c) * od + oi] = v;
@@ -1815,7 +1786,7 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
   else
     hipLaunchKernelGGL(rg_fpi_kernel<kF * kDim>, dim3(grid1), dim3(kNT), 0, s, b);
   if (b.n_t1) hipLaunchKernelGGL(rg_e1_kernel, dim3(b.n_t1), dim3(kNT), 0, s, b);
-  if (b.n_t2) hipLaunchKernelGGL(rg_e2_kernel, dim3(b.n_t2), dim3(kNT), 0, s, b);
+  if (b.n_t2) hipLaunchKernelGGL(rg_e2_kernel, dim3(b.n_t2), dim3(kNT2), 0, s, b);
   if (b.n_t3) hipLaunchKernelGGL(rg_e3_kernel, dim3(b.n_t3), dim3(kNT3), 0, s, b);
   hipLaunchKernelGGL(rg_assemble_kernel, dim3((unsigned)pl->n_agents), dim3(kNT), 0, s, b);
   PMH_HIP_CHECK(hipGetLastError(), "ragged: ");
