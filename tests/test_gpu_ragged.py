@@ -109,6 +109,27 @@ def test_ragged_team_agents_parity(dev):
         _check_agent(rb, i, use_dedup=True)
 
 
+def test_ragged_team_exchange_across_tag_wrap(dev):
+    """The team exchange tags each partial with the sweep sequence number
+    (step * num_iter + sweep) in 10 mantissa bits, which wrap every 64 steps
+    at 16 sweeps: 80 graph replays of a team workload stay bitwise identical
+    to the first step, with no exchange timeout."""
+    w = workloads.ragged(seed=4, device=dev, n_agents=8, state_range=(20, 40), obs_range=(2, 6), max_policies=64,
+                         max_A_bytes=1 << 22, max_modalities=3)
+    print("ragged team tags: 8 agents, 80 steps", flush=True)
+    rb = ragged.RaggedBatch(w["agents"], num_iter=16, device=dev)
+    assert rb.plan.info()["team_max"] > 2
+    rb.step()
+    torch.cuda.synchronize()
+    out1, act1 = rb.out.clone(), rb.act.clone()
+    g = rb.capture()
+    for _ in range(80):
+        g.replay()
+    torch.cuda.synchronize()
+    assert rb.status() == 0
+    assert torch.equal(rb.out, out1) and torch.equal(rb.act, act1)
+
+
 def test_ragged_full_batch(dev):
     w = workloads.ragged(seed=0, device=dev, n_agents=4096)
     print("ragged full: 4096 agents", flush=True)
