@@ -911,10 +911,13 @@ static int launch_fpi(const FpiDesc& d0, FpiPtrs p, int /*unused*/, int64_t batc
     if (fast_shape(&df, &n0, &rows0)) {
       const std::vector<FastChoice> cands = fast_candidates(df, n0, rows0);
       if (!cands.empty()) {
-        // default: fewest threads with <= 32 FP64 row values per lane
+        // default: fewest threads with <= 64 FP64 row values per lane (the
+        // candidates are ordered by thread count; on 1024 agents of 16384
+        // states the 256-thread, 64-value variant took 0.35 ms against
+        // 0.54 ms for 512 threads with 32 values each)
         FastChoice fc = cands.back();
         for (const FastChoice& c : cands)
-          if (c.RPL * c.N0 <= 32) {
+          if (c.RPL * c.N0 <= 64) {
             fc = c;
             break;
           }
