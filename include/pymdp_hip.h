@@ -5,8 +5,13 @@
  * Every entry point takes plain device pointers, int64 attribute arrays that
  * live in HOST memory, and a hipStream_t (passed as void*; NULL = the null
  * stream).  All work is enqueued on that stream; no entry point synchronises
- * the stream or allocates on the warm path (device scratch is grown once per
- * shape and reused).
+ * the stream or allocates on the warm path.  Device scratch belongs to the
+ * library, one grow-only buffer per (device, stream): calls on different
+ * streams may overlap; a buffer that grows is retired, not freed, so a HIP
+ * graph captured earlier stays valid until pymdp_hip_release_scratch().
+ * Under stream capture nothing is allocated: make one call of a shape (and,
+ * for the neg-EFE entries, of a policy set) on the capturing stream outside
+ * capture first.
  *
  * Return value: 0 on success, PYMDP_HIP_EINVAL for an invalid argument
  * (the reference's kInvalidArgument), PYMDP_HIP_EDEVICE for a HIP runtime
@@ -42,6 +47,17 @@ const char* pymdp_hip_last_error(void);
 
 /* Library version string and gfx target it was built for. */
 const char* pymdp_hip_version(void);
+
+/* Frees the device scratch of the calling thread's current device and every
+ * retired scratch buffer and plan table; synchronises the device.  Graphs
+ * captured before the call must not be replayed after it.  No counterpart in
+ * the reference (its XLA handlers free per-executable state with the
+ * executable, pymdp/ffi/neg_efe/neg_efe.h:30-39). */
+int pymdp_hip_release_scratch(void);
+
+/* Bytes of per-stream scratch held (all devices); *retired_count (if not
+ * NULL) receives the number of retired allocations awaiting release. */
+int64_t pymdp_hip_scratch_bytes(int64_t* retired_count);
 
 /* Name of the FPI kernel instance (template arguments = innermost width,
  * rows per lane, workgroup size) used by the most recent FPI call on the
@@ -155,7 +171,17 @@ int pymdp_hip_sequence_infer(int32_t method, const float* const* A_ptrs, const i
 #define PYMDP_HIP_FLAG_USE_STATES_INFO_GAIN (1 << 1)
 #define PYMDP_HIP_FLAG_USE_INDUCTIVE (1 << 2)
 #define PYMDP_HIP_FLAG_USE_PARAM_INFO_GAIN (1 << 3) /* needs pA and / or pB */
-#define PYMDP_HIP_FLAG_MODEL_PARAMS_STATIC (1 << 4) /* accepted, no effect: A-derived tables are rebuilt every call */
+/* Caller promises A, pA and pB are not modified in place between calls: the
+ * tables derived from them (sum_o A ln A, the Dirichlet novelty weights) are
+ * kept per (device, stream) and reused while the device pointers, strides,
+ * batch, policy set and term flags match, as the reference's warm test
+ * (pymdp/ffi/neg_efe/neg_efe_cuda_entry.cc:40-100).  A buffer freed and
+ * reallocated at the same address is indistinguishable: call
+ * pymdp_hip_param_cache_clear() when parameters change. */
+#define PYMDP_HIP_FLAG_MODEL_PARAMS_STATIC (1 << 4)
+
+/* Drops the cached parameter tables of the current device. */
+int pymdp_hip_param_cache_clear(void);
 
 /*
  * Negative expected free energy of every policy, optionally with the policy
@@ -165,7 +191,7 @@ int pymdp_hip_sequence_infer(int32_t method, const float* const* A_ptrs, const i
  *
  *   policy_host [P, T, F] int32 HOST array (policies are static agent
  *                      state; the action-history plan built from it is
- *                      cached by full content hash)
+ *                      cached and matched on the full content)
  *   qs_flat [batch, sum S]            current beliefs qs_init
  *   A_ptrs[M] / A_batch_stride[M]     A[m] [batch?, O_m, S_deps...]
  *   B_ptrs[F] / B_batch_stride[F]     B[f] [batch?, S_f, S_Bdeps..., U_f]

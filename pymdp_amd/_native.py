@@ -29,6 +29,9 @@ SIGNATURES: dict[str, tuple] = {
     "pymdp_hip_last_error": (ctypes.c_char_p, []),
     "pymdp_hip_version": (ctypes.c_char_p, []),
     "pymdp_hip_last_launch": (ctypes.c_char_p, []),
+    "pymdp_hip_release_scratch": (ctypes.c_int, []),
+    "pymdp_hip_scratch_bytes": (ctypes.c_int64, [_i64p]),
+    "pymdp_hip_param_cache_clear": (ctypes.c_int, []),
     "pymdp_hip_fpi": (ctypes.c_int, [
         _vp, _vp, _vp,                       # ll, lp, q
         ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,

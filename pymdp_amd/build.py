@@ -25,7 +25,8 @@ LIB_DIR = PKG_DIR / "_lib"
 LIB_NAME = "libpymdp_hip.so"
 OFFLOAD_ARCH = os.environ.get("PYMDP_HIP_ARCH", "gfx950")
 
-SOURCES = ["fpi.hip", "efe.hip", "ragged.hip", "adjoint.hip", "learning.hip", "vfe.hip", "seq.hip", "history.cpp"]
+SOURCES = ["fpi.hip", "efe.hip", "ragged.hip", "adjoint.hip", "learning.hip", "vfe.hip", "seq.hip", "scratch.hip",
+           "history.cpp"]
 
 HIPCC_FLAGS = [
     "-O3",

@@ -29,6 +29,7 @@
 
 #include "../../include/pymdp_hip.h"
 #include "common.hpp"
+#include "scratch.hpp"
 
 namespace pmh {
 namespace adj {
@@ -606,31 +607,6 @@ static int einval(const char* pfx, const std::string& m) {
   return PYMDP_HIP_EINVAL;
 }
 
-struct Ws {
-  int device = -1;
-  double* ptr = nullptr;
-  size_t bytes = 0;
-};
-static Ws g_ws[64];
-
-static int workspace(size_t bytes, double** out, const char* pfx) {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return einval(pfx, "no HIP device");
-  Ws& w = g_ws[dev];
-  if (w.bytes < bytes) {
-    if (w.ptr) (void)hipFree(w.ptr);
-    w.ptr = nullptr;
-    w.bytes = 0;
-    if (hipMalloc(&w.ptr, bytes) != hipSuccess) {
-      set_error(std::string(pfx) + "hipMalloc of the adjoint workspace failed");
-      return PYMDP_HIP_EDEVICE;
-    }
-    w.bytes = bytes;
-  }
-  *out = w.ptr;
-  return 0;
-}
-
 static int fill_common(Desc* d, const int64_t* S, int64_t F, const int64_t* O, int64_t M, const int64_t* Adf,
                        const int64_t* Ado, const char* pfx) {
   std::memset(d, 0, sizeof(Desc));
@@ -731,7 +707,9 @@ extern "C" int pymdp_hip_fpi_adjoint(const float* const* A_ptrs, const int64_t* 
   if (batch == 0) return 0;
   d.ws_stride = (long long)(d.iters + 1) * d.totS;
   double* ws = nullptr;
-  if ((rc = workspace(sizeof(double) * (size_t)d.ws_stride * (size_t)batch, &ws, pfx))) return rc;
+  if ((rc = scratch_get(kPoolAdjoint, static_cast<hipStream_t>(stream), sizeof(double) * (size_t)d.ws_stride * (size_t)batch,
+                        reinterpret_cast<void**>(&ws), pfx)))
+    return rc;
   p.ws = ws;
   if (ensure_log_table()) {
     set_error("fpi_ffi: cannot initialise the log table");

@@ -21,6 +21,7 @@
 //                           (+ inductive), then q_pi = softmax(gamma G + ln E)
 // All per-agent intermediates live in one workspace slab per agent.
 #include "common.hpp"
+#include "scratch.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -54,6 +55,7 @@ constexpr int kFlagUseUtility = 1 << 0;
 constexpr int kFlagUseStatesInfoGain = 1 << 1;
 constexpr int kFlagUseInductive = 1 << 2;
 constexpr int kFlagUseParamInfoGain = 1 << 3;
+constexpr int kFlagModelParamsStatic = 1 << 4;
 
 // Plan descriptor, resident in device memory (one per policy-set key).
 struct Desc {
@@ -88,6 +90,7 @@ struct Desc {
   long long qs0_off;  // qs_init in FP64
   long long g_off;    // FP64 logits / exp of the policy posterior [P]
   long long ws_stride;
+  long long param_end;  // doubles of the A / pA / pB-derived tables at the start of a slab
 };
 
 // Work items of the two GEMM stages (device arrays, per plan).
@@ -126,6 +129,11 @@ struct Args {
   float* G;     // [batch, P]
   float* q_pi;  // [batch, P] or null
   double* ws;   // FP64 workspace [batch, ws_stride]
+  // A / pA / pB-derived tables (H_A, W_A, W_B at ha_off / wa_off / wb_off):
+  // the workspace itself (pc = ws, pc_stride = ws_stride), or a cached
+  // buffer [batch, pc_stride] kept across calls with static parameters
+  double* pc;
+  long long pc_stride;
   int qs_off[kMaxF + 1];
   int flags;
 };
@@ -159,7 +167,7 @@ __global__ __launch_bounds__(kNT) void efe_ha_kernel(const Desc* __restrict__ dp
     for (int j = 0; j < 8; ++j) acc += xlogx_f64(v[j]);
   }
   for (; o < O; ++o) acc += xlogx_f64(Am[(long long)o * K + k]);
-  a.ws[(long long)b * d.ws_stride + d.ha_off[m] + k] = acc;
+  a.pc[(long long)b * a.pc_stride + d.ha_off[m] + k] = acc;
 }
 
 // digamma of a positive double: upward recurrence psi(x) = psi(x + 1) - 1/x
@@ -202,7 +210,7 @@ __global__ __launch_bounds__(kNT) void efe_wa_kernel(const Desc* __restrict__ dp
   double s = 0.0;
   for (int o = 0; o < O; ++o) s += (double)fmaxf(pa[(long long)o * K + k], kMinVal);
   s = fmax(s, (double)kMinVal);
-  double* w = a.ws + (long long)b * d.ws_stride + d.wa_off[m];
+  double* w = a.pc + (long long)b * a.pc_stride + d.wa_off[m];
   for (int o = 0; o < O; ++o) w[(long long)o * K + k] = wnorm_f64(pa[(long long)o * K + k], s);
 }
 
@@ -220,7 +228,7 @@ __global__ __launch_bounds__(kNT) void efe_wb_kernel(const Desc* __restrict__ dp
   double s = 0.0;
   for (int i = 0; i < S; ++i) s += (double)fmaxf(pb[(long long)i * KU + c], kMinVal);
   s = fmax(s, (double)kMinVal);
-  double* w = a.ws + (long long)b * d.ws_stride + d.wb_off[f];
+  double* w = a.pc + (long long)b * a.pc_stride + d.wb_off[f];
   for (int i = 0; i < S; ++i) w[(long long)i * KU + c] = wnorm_f64(pb[(long long)i * KU + c], s);
 }
 
@@ -323,7 +331,7 @@ __global__ __launch_bounds__(kNT) void efe_pb_kernel(const Desc* __restrict__ dp
   double* ws = a.ws + (long long)b * d.ws_stride;
   const double* qs0 = ws + d.qs0_off;
   const int S = d.S[f], U = d.U[f], rank = d.Brank[f], Kb = d.Kb[f];
-  const double* W = ws + d.wb_off[f];
+  const double* W = a.pc + (long long)b * a.pc_stride + d.wb_off[f];
   for (int t = 0; t < d.T; ++t) {
     const int n = d.fn[t][f];
     const int* par = tab + d.f_par_off[t][f];
@@ -408,7 +416,7 @@ __global__ __launch_bounds__(256) void efe_gemm1_kernel(const Desc* __restrict__
   const bool sig = (a.flags & kFlagUseStatesInfoGain) != 0;
   const int Ox = d.Ox[m];
   const int rows = Ox * S0;  // rows j = o' * S0 + s0: A, sum_o A ln A, W_A
-  const double* wa = d.has_pA && a.pA[m] ? a.ws + (long long)b * d.ws_stride + d.wa_off[m] : nullptr;
+  const double* wa = d.has_pA && a.pA[m] ? a.pc + (long long)b * a.pc_stride + d.wa_off[m] : nullptr;
   const int ntot = d.g_ntot[g];
   const int tiles_m = (rows + G1_BM - 1) / G1_BM;
   const int tiles_n = (ntot + G1_BN - 1) / G1_BN;
@@ -417,7 +425,7 @@ __global__ __launch_bounds__(256) void efe_gemm1_kernel(const Desc* __restrict__
   const int col0 = ((int)blockIdx.x / tiles_m) * G1_BN;
   double* ws = a.ws + (long long)b * d.ws_stride;
   const float* Am = a.A[m] + (long long)b * a.A_stride[m];
-  const double* ha = ws + d.ha_off[m];
+  const double* ha = a.pc + (long long)b * a.pc_stride + d.ha_off[m];
   const double* Q = ws + d.qin_off[0][g];  // all levels, [ntot, Kr]
   double* W = ws + d.w_off[0][m];           // all levels, [ntot, S0, Ox]
   const int tid = threadIdx.x;
@@ -827,8 +835,10 @@ __global__ __launch_bounds__(1024) void efe_scatter_kernel(const Desc* __restric
 // Host: plan cache (device copy of the history plan, per device and key)
 // ---------------------------------------------------------------------------
 struct DevPlan {
-  uint64_t key = 0;
+  std::vector<int64_t> meta;    // P, T, F, M, S, U, O, dependency spans, pA / pB presence
+  std::vector<int32_t> policy;  // the policy matrix [P, T, F] the plan was built from
   int device = -1;
+  bool captured = false;  // used under stream capture: device tables are retired, not freed
   EfeHostPlan host;
   Desc desc{};
   Desc* d_desc = nullptr;
@@ -839,22 +849,25 @@ struct DevPlan {
   int kron_blocks = 1;
   int g1_blocks = 1;
   ~DevPlan() {
-    if (d_desc) (void)hipFree(d_desc);
-    if (d_tab) (void)hipFree(d_tab);
-    if (d_items1) (void)hipFree(d_items1);
-    if (d_items2) (void)hipFree(d_items2);
+    // hipFree synchronises the device, so kernels launched with this plan
+    // have finished; a graph captured with it may be replayed later, so then
+    // the tables outlive the plan until pymdp_hip_release_scratch
+    void* ptrs[4] = {d_desc, d_tab, d_items1, d_items2};
+    for (void* q : ptrs) {
+      if (!q) continue;
+      if (captured)
+        scratch_retire(q);
+      else
+        (void)hipFree(q);
+    }
   }
 };
 
+// Plans are shared: a call holds its plan for its whole duration, so a
+// concurrent call that evicts it from the cache cannot free it under it.
 std::mutex g_mu;
-std::list<std::unique_ptr<DevPlan>> g_plans;  // most recent first, small LRU
-constexpr size_t kMaxPlans = 8;
-
-struct Ws {
-  double* p = nullptr;
-  size_t bytes = 0;
-};
-std::map<int, Ws> g_ws;
+std::list<std::shared_ptr<DevPlan>> g_plans;  // most recent first, small LRU
+constexpr size_t kMaxPlans = 16;
 
 static int efe_err(const std::string& msg) {
   set_error("efe_ffi: " + msg);
@@ -870,9 +883,8 @@ static int efe_err(const std::string& msg) {
     }                                                                                 \
   } while (0)
 
-static int make_dev_plan(EfeHostPlan&& hp, int device, bool has_pA, bool has_pB, uint64_t key, DevPlan** out) {
-  auto dp = std::make_unique<DevPlan>();
-  dp->key = key;
+static int make_dev_plan(EfeHostPlan&& hp, int device, bool has_pA, bool has_pB, std::shared_ptr<DevPlan>* out) {
+  auto dp = std::make_shared<DevPlan>();
   dp->device = device;
   dp->host = std::move(hp);
   EfeHostPlan& h = dp->host;
@@ -931,6 +943,7 @@ static int make_dev_plan(EfeHostPlan&& hp, int device, bool has_pA, bool has_pB,
   }
   if (has_pB)
     for (int f = 0; f < d.F; ++f) d.wb_off[f] = walloc((long long)d.S[f] * d.Kb[f] * d.U[f]);
+  d.param_end = ws;  // the parameter tables come first: [0, param_end) of a slab
   {
     long long totS = 0;
     for (int f = 0; f < d.F; ++f) totS += d.S[f];
@@ -1013,63 +1026,54 @@ static int make_dev_plan(EfeHostPlan&& hp, int device, bool has_pA, bool has_pB,
   EFE_HIP(hipMemcpy(dp->d_tab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
   if (!it1.empty()) EFE_HIP(hipMemcpy(dp->d_items1, it1.data(), sizeof(Item1) * it1.size(), hipMemcpyHostToDevice));
   if (!it2.empty()) EFE_HIP(hipMemcpy(dp->d_items2, it2.data(), sizeof(Item2) * it2.size(), hipMemcpyHostToDevice));
-  *out = dp.release();
+  *out = std::move(dp);
   return 0;
 }
 
-// Finds or builds the device plan for this policy set (content-keyed).
+// Finds or builds the device plan for this policy set.  The cache compares
+// the full content (shapes, dependency spans and every policy entry), so a
+// plan is never reused for a different policy set.
 static int get_plan(const int32_t* policy, int64_t P, int64_t T, int64_t F, const int64_t* S, const int64_t* U,
                     const int64_t* O, int64_t M, const int64_t* Adf, const int64_t* Ado, const int64_t* Bdf,
-                    const int64_t* Bdo, bool has_pA, bool has_pB, DevPlan** out) {
+                    const int64_t* Bdo, bool has_pA, bool has_pB, hipStream_t stream, std::shared_ptr<DevPlan>* out) {
   int device = 0;
   EFE_HIP(hipGetDevice(&device));
-  // content key first (cheap relative to the tree build)
-  uint64_t h = fnv1a64(policy, sizeof(int32_t) * (size_t)(P * T * F));
-  h = fnv1a64(S, sizeof(int64_t) * F, h);
-  h = fnv1a64(U, sizeof(int64_t) * F, h);
-  h = fnv1a64(O, sizeof(int64_t) * M, h);
-  h = fnv1a64(Ado, sizeof(int64_t) * (M + 1), h);
-  h = fnv1a64(Adf, sizeof(int64_t) * Ado[M], h);
-  h = fnv1a64(Bdo, sizeof(int64_t) * (F + 1), h);
-  h = fnv1a64(Bdf, sizeof(int64_t) * Bdo[F], h);
-  const int64_t dims[4] = {P, T, F, M};
-  const uint64_t hkey = fnv1a64(dims, sizeof(dims), h);  // the history plan's own key
-  const int64_t pflags = (has_pA ? 1 : 0) | (has_pB ? 2 : 0);
-  const uint64_t key = fnv1a64(&pflags, sizeof(pflags), hkey);
-  std::lock_guard<std::mutex> lk(g_mu);
-  for (auto it = g_plans.begin(); it != g_plans.end(); ++it) {
-    if ((*it)->key == key && (*it)->device == device) {
-      g_plans.splice(g_plans.begin(), g_plans, it);
-      *out = g_plans.front().get();
-      return 0;
+  std::vector<int64_t> meta = {P, T, F, M, (has_pA ? 1 : 0) | (has_pB ? 2 : 0)};
+  meta.insert(meta.end(), S, S + F);
+  meta.insert(meta.end(), U, U + F);
+  meta.insert(meta.end(), O, O + M);
+  meta.insert(meta.end(), Ado, Ado + M + 1);
+  meta.insert(meta.end(), Adf, Adf + Ado[M]);
+  meta.insert(meta.end(), Bdo, Bdo + F + 1);
+  meta.insert(meta.end(), Bdf, Bdf + Bdo[F]);
+  const size_t npol = (size_t)(P * T * F);
+  const bool capturing = stream_capturing(stream);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto it = g_plans.begin(); it != g_plans.end(); ++it) {
+      const DevPlan& q = **it;
+      if (q.device == device && q.meta == meta && std::memcmp(q.policy.data(), policy, sizeof(int32_t) * npol) == 0) {
+        g_plans.splice(g_plans.begin(), g_plans, it);
+        if (capturing) g_plans.front()->captured = true;
+        *out = g_plans.front();
+        return 0;
+      }
     }
   }
+  if (capturing)
+    return efe_err("the first call with a policy set must happen outside stream capture (it builds the plan)");
   EfeHostPlan hp;
   std::string err = build_efe_host_plan(policy, P, T, F, S, U, O, M, Adf, Ado, Bdf, Bdo, &hp);
   if (!err.empty()) return efe_err(err);
-  if (hp.key != hkey) return efe_err("internal: plan key mismatch");
-  DevPlan* dp = nullptr;
-  int rc = make_dev_plan(std::move(hp), device, has_pA, has_pB, key, &dp);
+  std::shared_ptr<DevPlan> dp;
+  int rc = make_dev_plan(std::move(hp), device, has_pA, has_pB, &dp);
   if (rc) return rc;
-  g_plans.emplace_front(dp);
-  while (g_plans.size() > kMaxPlans) g_plans.pop_back();
-  *out = dp;
-  return 0;
-}
-
-static int get_ws(size_t bytes, double** out) {
-  int device = 0;
-  EFE_HIP(hipGetDevice(&device));
+  dp->meta = std::move(meta);
+  dp->policy.assign(policy, policy + npol);
   std::lock_guard<std::mutex> lk(g_mu);
-  Ws& w = g_ws[device];
-  if (w.bytes < bytes) {
-    if (w.p) EFE_HIP(hipFree(w.p));
-    w.p = nullptr;
-    w.bytes = 0;
-    EFE_HIP(hipMalloc(&w.p, bytes));
-    w.bytes = bytes;
-  }
-  *out = w.p;
+  g_plans.push_front(dp);
+  while (g_plans.size() > kMaxPlans) g_plans.pop_back();
+  *out = std::move(dp);
   return 0;
 }
 
@@ -1111,6 +1115,80 @@ struct CallIn {
   hipStream_t stream;
 };
 
+// Tables derived from A / pA / pB (H_A, W_A, W_B) kept across calls when the
+// caller sets PYMDP_HIP_FLAG_MODEL_PARAMS_STATIC, i.e. promises the parameter
+// buffers are not modified in place between calls.  The warm test is the
+// reference's: the same device pointers (and strides, batch, plan and term
+// flags) as the call that filled the cache
+// (pymdp/ffi/neg_efe/neg_efe_cuda_entry.cc:40-100).  An entry belongs to one
+// stream, so its fill is ordered before every use.
+struct ParamCache {
+  std::shared_ptr<DevPlan> plan;
+  int device = -1;
+  uintptr_t stream = 0;
+  std::vector<int64_t> key;  // batch, term flags, A / pA / pB pointers and strides
+  double* buf = nullptr;
+  long long stride = 0;
+  bool captured = false;
+  ~ParamCache() {
+    if (!buf) return;
+    if (captured)
+      scratch_retire(buf);
+    else
+      (void)hipFree(buf);
+  }
+};
+std::list<std::shared_ptr<ParamCache>> g_pcache;  // most recent first
+constexpr size_t kMaxParamCaches = 4;
+
+// Finds the cache entry of this call: *fill = false on a hit (tables valid);
+// on a miss a new entry is allocated and *fill = true.  Under stream capture a
+// miss allocates nothing: *out stays empty and the tables go to the
+// workspace.
+static int get_param_cache(const std::shared_ptr<DevPlan>& plan, const CallIn& c, bool has_pA, bool has_pB,
+                           std::shared_ptr<ParamCache>* out, bool* fill) {
+  int device = 0;
+  EFE_HIP(hipGetDevice(&device));
+  const Desc& d = plan->desc;
+  std::vector<int64_t> key = {c.batch, c.flags & (kFlagUseStatesInfoGain | kFlagUseParamInfoGain)};
+  for (int m = 0; m < d.M; ++m) {
+    key.push_back((int64_t)(uintptr_t)c.A[m]);
+    key.push_back(c.A_stride[m]);
+    key.push_back(has_pA ? (int64_t)(uintptr_t)c.pA[m] : 0);
+    key.push_back(has_pA ? c.pA_stride[m] : 0);
+  }
+  for (int f = 0; f < d.F; ++f) {
+    key.push_back(has_pB ? (int64_t)(uintptr_t)c.pB[f] : 0);
+    key.push_back(has_pB ? c.pB_stride[f] : 0);
+  }
+  const uintptr_t sid = reinterpret_cast<uintptr_t>(c.stream);
+  const bool capturing = stream_capturing(c.stream);
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto it = g_pcache.begin(); it != g_pcache.end(); ++it) {
+    ParamCache& e = **it;
+    if (e.plan == plan && e.device == device && e.stream == sid && e.key == key) {
+      g_pcache.splice(g_pcache.begin(), g_pcache, it);
+      if (capturing) e.captured = true;
+      *out = g_pcache.front();
+      *fill = false;
+      return 0;
+    }
+  }
+  *fill = true;
+  if (capturing) return 0;
+  auto e = std::make_shared<ParamCache>();
+  e->plan = plan;
+  e->device = device;
+  e->stream = sid;
+  e->key = std::move(key);
+  e->stride = (d.param_end + 15) / 16 * 16;
+  EFE_HIP(hipMalloc(&e->buf, sizeof(double) * (size_t)e->stride * (size_t)std::max<int64_t>(1, c.batch)));
+  g_pcache.push_front(e);
+  while (g_pcache.size() > kMaxParamCaches) g_pcache.pop_back();
+  *out = std::move(e);
+  return 0;
+}
+
 static int run_neg_efe(const CallIn& c) {
   if (c.batch < 0) return efe_err("batch must be non-negative");
   if (c.F <= 0 || c.M <= 0) return efe_err("neg-EFE FFI requires at least one factor and one modality");
@@ -1122,8 +1200,9 @@ static int run_neg_efe(const CallIn& c) {
   const bool has_pA = pig && c.pA != nullptr;
   const bool has_pB = pig && c.pB != nullptr;
   if (pig && !has_pA && !has_pB) return efe_err("use_param_info_gain=True requires at least one of pA or pB");
-  DevPlan* plan = nullptr;
-  int rc = get_plan(c.policy, c.P, c.T, c.F, c.S, c.U, c.O, c.M, c.Adf, c.Ado, c.Bdf, c.Bdo, has_pA, has_pB, &plan);
+  std::shared_ptr<DevPlan> plan;
+  int rc = get_plan(c.policy, c.P, c.T, c.F, c.S, c.U, c.O, c.M, c.Adf, c.Ado, c.Bdf, c.Bdo, has_pA, has_pB, c.stream,
+                    &plan);
   if (rc) return rc;
   if (c.batch == 0) return 0;
   if (ensure_log_table()) return efe_err("cannot initialise the log table");
@@ -1177,12 +1256,25 @@ static int run_neg_efe(const CallIn& c) {
   a.q_pi = c.q_pi;
   a.flags = c.flags;
   double* ws = nullptr;
-  rc = get_ws(sizeof(double) * (size_t)d.ws_stride * (size_t)c.batch, &ws);
+  rc = scratch_get(kPoolEfe, c.stream, sizeof(double) * (size_t)d.ws_stride * (size_t)c.batch,
+                   reinterpret_cast<void**>(&ws), "efe_ffi: ");
   if (rc) return rc;
   a.ws = ws;
+  a.pc = ws;
+  a.pc_stride = d.ws_stride;
   const unsigned B = (unsigned)c.batch;
   hipStream_t st = c.stream;
-  if (c.flags & kFlagUseStatesInfoGain) {
+  bool tables = true;  // the A / pA / pB-derived tables must be formed by this call
+  std::shared_ptr<ParamCache> pcache;
+  if ((c.flags & kFlagModelParamsStatic) && d.param_end > 0) {
+    rc = get_param_cache(plan, c, has_pA, has_pB, &pcache, &tables);
+    if (rc) return rc;
+    if (pcache) {
+      a.pc = pcache->buf;
+      a.pc_stride = pcache->stride;
+    }
+  }
+  if (tables && (c.flags & kFlagUseStatesInfoGain)) {
     long long maxK = 1;
     for (int g = 0; g < d.G; ++g) maxK = std::max(maxK, (long long)d.gS0[g] * d.gKrest[g]);
     hipLaunchKernelGGL(efe_ha_kernel, dim3((unsigned)((maxK + kNT - 1) / kNT), d.M, B), dim3(kNT), 0, st,
@@ -1191,10 +1283,10 @@ static int run_neg_efe(const CallIn& c) {
   long long maxK = 1, maxKU = 1;
   for (int g = 0; g < d.G; ++g) maxK = std::max(maxK, (long long)d.gS0[g] * d.gKrest[g]);
   for (int f = 0; f < d.F; ++f) maxKU = std::max(maxKU, (long long)d.Kb[f] * d.U[f]);
-  if (has_pA)
+  if (tables && has_pA)
     hipLaunchKernelGGL(efe_wa_kernel, dim3((unsigned)((maxK + kNT - 1) / kNT), d.M, B), dim3(kNT), 0, st,
                        plan->d_desc, a);
-  if (has_pB)
+  if (tables && has_pB)
     hipLaunchKernelGGL(efe_wb_kernel, dim3((unsigned)((maxKU + kNT - 1) / kNT), d.F, B), dim3(kNT), 0, st,
                        plan->d_desc, a);
   hipLaunchKernelGGL(efe_rollout_kernel, dim3(B), dim3(kNT), 0, st, plan->d_desc, plan->d_tab, a);
@@ -1234,6 +1326,17 @@ static int run_neg_efe(const CallIn& c) {
 }  // namespace pmh
 
 using namespace pmh;
+
+// Drops every cached parameter table of the current device (the next call
+// with PYMDP_HIP_FLAG_MODEL_PARAMS_STATIC forms them again).
+extern "C" int pymdp_hip_param_cache_clear(void) {
+  set_error("");
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) return efe::efe_err("no HIP device");
+  std::lock_guard<std::mutex> lk(efe::g_mu);
+  efe::g_pcache.remove_if([&](const std::shared_ptr<efe::ParamCache>& e) { return e->device == device; });
+  return PYMDP_HIP_OK;
+}
 
 extern "C" int pymdp_hip_neg_efe(const int32_t* policy_host, int64_t P, int64_t T, const float* qs_flat,
                                  const float* const* A_ptrs, const int64_t* A_batch_stride,

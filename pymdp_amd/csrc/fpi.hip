@@ -28,6 +28,7 @@
 //    the build reads one row of A per modality (O_m times less HBM traffic),
 //    and the skipped terms are exact zeros (0 * finite log).
 #include "common.hpp"
+#include "scratch.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -448,41 +449,47 @@ __global__ __launch_bounds__(NT) void fpi_fast_kernel(const FpiDesc d, const Fpi
     for (int s = 0; s < N0; ++s) X0[i][s] = 0.0;
   if (d.mode && d.obs_index) {
     // One-hot observations: LL[s] = sum_m ln max(A_m[o_m, s], MINVAL)
-    //                              = ln prod_m max(A_m[o_m, s], MINVAL),
-    // the product in FP64 (>= MINVAL^M, far inside the FP64 range), so each
-    // state costs one FP64 log instead of one per modality.
+    //                              = sum_c ln prod_{m in chunk c} max(A_m[o_m, s], MINVAL)
+    // over chunks of kProdChunk modalities.  A chunk's FP64 product is at
+    // least MINVAL^16 = 2^-832, a normal double (MINVAL^20 = 2^-1040 would
+    // already be subnormal, where log_f64d is not valid), so each state costs
+    // one FP64 log per 16 modalities instead of one per modality.
+    constexpr int kProdChunk = 16;
     const int* oi = static_cast<const int*>(p.obs) + (long long)b * d.M;
 #pragma unroll
     for (int i = 0; i < RPL; ++i) {
       const int r = tid + NT * i;
-      double pr[N0];
+      for (int m0 = 0; m0 < d.M; m0 += kProdChunk) {
+        double pr[N0];
 #pragma unroll
-      for (int s = 0; s < N0; ++s) pr[s] = 1.0;
-      for (int m = 0; m < d.M; ++m) {
-        const int o = oi[m];
-        if (o < 0 || o >= d.O[m]) continue;  // all-zero one_hot row contributes 0
-        const float* row = p.A[m] + (long long)b * d.A_stride[m] + (long long)o * d.K[m] + (long long)r * n;
-        float t[N0];
-        if (N0 % 4 == 0 && n == N0) {
+        for (int s = 0; s < N0; ++s) pr[s] = 1.0;
+        const int m1 = min(d.M, m0 + kProdChunk);
+        for (int m = m0; m < m1; ++m) {
+          const int o = oi[m];
+          if (o < 0 || o >= d.O[m]) continue;  // all-zero one_hot row contributes 0
+          const float* row = p.A[m] + (long long)b * d.A_stride[m] + (long long)o * d.K[m] + (long long)r * n;
+          float t[N0];
+          if (N0 % 4 == 0 && n == N0) {
 #pragma unroll
-          for (int s4 = 0; s4 < N0; s4 += 4) {
-            typedef float v4f __attribute__((ext_vector_type(4)));
-            v4f v = {1.f, 1.f, 1.f, 1.f};
-            if (r < rows0) v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(row + s4));
-            t[s4] = v.x;
-            t[s4 + 1] = v.y;
-            t[s4 + 2] = v.z;
-            t[s4 + 3] = v.w;
+            for (int s4 = 0; s4 < N0; s4 += 4) {
+              typedef float v4f __attribute__((ext_vector_type(4)));
+              v4f v = {1.f, 1.f, 1.f, 1.f};
+              if (r < rows0) v = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(row + s4));
+              t[s4] = v.x;
+              t[s4 + 1] = v.y;
+              t[s4 + 2] = v.z;
+              t[s4 + 3] = v.w;
+            }
+          } else {
+#pragma unroll
+            for (int s = 0; s < N0; ++s) t[s] = (r < rows0 && s < n) ? __builtin_nontemporal_load(row + s) : 1.0f;
           }
-        } else {
 #pragma unroll
-          for (int s = 0; s < N0; ++s) t[s] = (r < rows0 && s < n) ? __builtin_nontemporal_load(row + s) : 1.0f;
+          for (int s = 0; s < N0; ++s) pr[s] *= (double)fmaxf(t[s], kMinVal);
         }
 #pragma unroll
-        for (int s = 0; s < N0; ++s) pr[s] *= (double)fmaxf(t[s], kMinVal);
+        for (int s = 0; s < N0; ++s) X0[i][s] += (r < rows0 && s < n) ? log_f64d(pr[s], ltab) : 0.0;
       }
-#pragma unroll
-      for (int s = 0; s < N0; ++s) X0[i][s] = (r < rows0 && s < n) ? log_f64d(pr[s], ltab) : 0.0;
     }
   } else {
     for (int m = 0; m < d.M; ++m) {
@@ -613,13 +620,6 @@ namespace {
 
 thread_local std::string g_err;
 
-struct Workspace {
-  double* ptr = nullptr;
-  size_t bytes = 0;
-};
-std::mutex g_ws_mu;
-std::map<int, Workspace> g_ws;  // per device, grow-only
-
 int einval(const char* prefix, const std::string& msg) {
   set_error(std::string(prefix) + msg);
   return PYMDP_HIP_EINVAL;
@@ -629,22 +629,6 @@ int einval(const char* prefix, const std::string& msg) {
 
 void set_error(const std::string& msg) { g_err = msg; }
 const char* get_error() { return g_err.c_str(); }
-
-static int get_workspace(size_t bytes, double** out) {
-  int dev = 0;
-  PMH_HIP_CHECK(hipGetDevice(&dev), "fpi_ffi: ");
-  std::lock_guard<std::mutex> lk(g_ws_mu);
-  Workspace& w = g_ws[dev];
-  if (w.bytes < bytes) {
-    if (w.ptr) PMH_HIP_CHECK(hipFree(w.ptr), "fpi_ffi: ");
-    w.ptr = nullptr;
-    w.bytes = 0;
-    PMH_HIP_CHECK(hipMalloc(&w.ptr, bytes), "fpi_ffi: ");
-    w.bytes = bytes;
-  }
-  *out = w.ptr;
-  return 0;
-}
 
 // Validate the reference's FPI attributes (pymdp/ffi/fpi/fpi_entry.h:24-82)
 // and build the launch descriptor.  ll_sizes[m] is the per-agent element count
@@ -979,7 +963,8 @@ static int launch_fpi(const FpiDesc& d0, FpiPtrs p, int /*unused*/, int64_t batc
     hipLaunchKernelGGL((fpi_kernel<32, 512>), grid, dim3(512), lds, stream, d, p);
   } else {
     double* ws = nullptr;
-    int rc = get_workspace(sizeof(double) * (size_t)d.ws_stride * (size_t)batch, &ws);
+    int rc = scratch_get(kPoolFpi, stream, sizeof(double) * (size_t)d.ws_stride * (size_t)batch,
+                         reinterpret_cast<void**>(&ws), "fpi_ffi: ");
     if (rc) return rc;
     p.ws = ws;
     hipLaunchKernelGGL((fpi_kernel<0, 512>), grid, dim3(512), lds, stream, d, p);

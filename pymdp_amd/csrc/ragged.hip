@@ -501,9 +501,11 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
               bool ready = true;
 #pragma unroll
               for (int q = 0; q < 16; ++q) ready = ready && (v[q] & kTagMask) == tag;
-              if (ready || s_broken) break;  // after a timeout no unit of the step waits again
+              // after a timeout no thread of the unit waits again: the flag is
+              // read with an LDS atomic so the poll cannot be hoisted
+              if (ready || __hip_atomic_load(&s_broken, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
               if (++spins > kSpinMax) {
-                s_broken = 1;
+                __hip_atomic_store(&s_broken, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 atomicOr(&b.cnt[3], 1);
                 break;
               }

@@ -81,6 +81,26 @@ def _flags(use_utility, use_states_info_gain, use_param_info_gain, use_inductive
     return flags
 
 
+# Identity of the parameter tensors whose derived tables the library holds
+# under FLAG_MODEL_PARAMS_STATIC.  The library's warm test is device-pointer
+# identity; torch may hand a freed tensor's address to a new tensor, so the
+# flag is passed only after checking that the same live tensor objects, with
+# unchanged in-place version counters, fill the pointers; otherwise the cache
+# is dropped first.
+_static_owner: list = []
+
+
+def _static_params_guard(tensors) -> None:
+    import weakref
+    global _static_owner
+    cur = [(t.data_ptr(), t._version) for t in tensors]
+    same = len(_static_owner) == len(tensors) and all(
+        ref() is t and key == k for (ref, key), t, k in zip(_static_owner, tensors, cur))
+    if not same:
+        _native.check(_native.lib().pymdp_hip_param_cache_clear())
+        _static_owner = [(weakref.ref(t), k) for t, k in zip(tensors, cur)]
+
+
 def update_posterior_policies_ffi(policy_matrix, qs_init: Sequence[torch.Tensor], A: Sequence[torch.Tensor],
                                   B: Sequence[torch.Tensor], C: Sequence[torch.Tensor], E: torch.Tensor | None,
                                   A_dependencies, B_dependencies, *, gamma=16.0, I=None,
@@ -116,14 +136,19 @@ def update_posterior_policies_ffi(policy_matrix, qs_init: Sequence[torch.Tensor]
         raise ValueError(f"efe_ffi: policy horizon {T} above the supported {MAX_HORIZON}")
     batch = int(qs_init[0].shape[0])
     dev = qs_init[0].device
+    for f, q in enumerate(qs_init):
+        if not q.is_cuda or q.device != dev:
+            raise ValueError(f"qs_init[{f}] must be a HIP tensor on {dev}")
+        if q.dim() != 2 or int(q.shape[0]) != batch:
+            raise ValueError(f"qs_init[{f}] has shape {tuple(q.shape)}; expected ({batch}, S_{f})")
     S = np.array([int(q.shape[-1]) for q in qs_init], dtype=np.int64)
     O = np.array([int(a.shape[-len(d) - 1]) for a, d in zip(A, A_dependencies)], dtype=np.int64)
     U = np.array([int(b.shape[-1]) for b in B], dtype=np.int64)
     qs_flat = torch.cat([q.to(torch.float32).reshape(batch, -1) for q in qs_init], -1).contiguous()
 
     def _ptr_stride(t: torch.Tensor, core_ndim: int, name: str):
-        if t.dtype != torch.float32 or not t.is_cuda:
-            raise ValueError(f"{name} must be a float32 HIP tensor")
+        if t.dtype != torch.float32 or not t.is_cuda or t.device != dev:
+            raise ValueError(f"{name} must be a float32 HIP tensor on {dev}")
         if not t.is_contiguous():
             raise ValueError(f"{name} must be contiguous")
         if t.dim() == core_ndim:
@@ -146,14 +171,17 @@ def update_posterior_policies_ffi(policy_matrix, qs_init: Sequence[torch.Tensor]
         if len(pA) != M:
             raise ValueError("pA must have one entry per observation modality")
         pA = [x.to(torch.float32).contiguous() for x in pA]
-        for m, (x, a) in enumerate(zip(pA, A)):
-            if tuple(x.shape[-a.dim():]) != tuple(a.shape[-a.dim():]) and tuple(x.shape) != tuple(a.shape[1:]):
+        for m, (x, a, d) in enumerate(zip(pA, A, A_dependencies)):
+            if tuple(x.shape[-len(d) - 1:]) != tuple(a.shape[-len(d) - 1:]):
                 raise ValueError(f"pA[{m}] must have the shape of A[{m}]")
         pA_p, pA_s = zip(*[_ptr_stride(x, len(d) + 1, f"pA[{m}]") for m, (x, d) in enumerate(zip(pA, A_dependencies))])
     if use_param_info_gain and pB is not None:
         if len(pB) != F:
             raise ValueError("pB must have one entry per hidden-state factor")
         pB = [x.to(torch.float32).contiguous() for x in pB]
+        for f, (x, b, d) in enumerate(zip(pB, B, B_dependencies)):
+            if tuple(x.shape[-len(d) - 2:]) != tuple(b.shape[-len(d) - 2:]):
+                raise ValueError(f"pB[{f}] must have the shape of B[{f}]")
         pB_p, pB_s = zip(*[_ptr_stride(x, len(d) + 2, f"pB[{f}]") for f, (x, d) in enumerate(zip(pB, B_dependencies))])
     C_p, C_s, C_t = [], [], []
     for m, c in enumerate(C):
@@ -168,14 +196,19 @@ def update_posterior_policies_ffi(policy_matrix, qs_init: Sequence[torch.Tensor]
         C_s.append(s)
         C_t.append(1 if timed else 0)
     if E is not None:
-        if E.dtype != torch.float32 or E.shape[-1] != P:
-            raise ValueError("E must be float32 with a trailing policy axis of size P")
+        # [P] or [1, P]: one E for every agent (stride 0); [batch, P]: per agent
+        if E.dtype != torch.float32 or not E.is_cuda or E.device != dev:
+            raise ValueError(f"E must be a float32 HIP tensor on {dev}")
+        if E.dim() not in (1, 2) or int(E.shape[-1]) != P or (E.dim() == 2 and int(E.shape[0]) not in (1, batch)):
+            raise ValueError(f"E has shape {tuple(E.shape)}; expected ({P},), (1, {P}) or ({batch}, {P})")
         E = E.contiguous()
-        E_ptr, E_stride = E.data_ptr(), (P if E.dim() == 2 else 0)
+        E_ptr, E_stride = E.data_ptr(), (P if E.dim() == 2 and int(E.shape[0]) == batch and batch > 1 else 0)
     else:
         E_ptr, E_stride = None, 0
     if isinstance(gamma, torch.Tensor):
         g_t = gamma.to(device=dev, dtype=torch.float32).reshape(-1)
+        if g_t.numel() not in (1, batch):
+            raise ValueError(f"gamma must be a scalar or have one entry per agent ({batch}); got {g_t.numel()}")
         if g_t.numel() != batch:
             g_t = g_t.expand(batch)
         g_t = g_t.contiguous()
@@ -213,6 +246,8 @@ def update_posterior_policies_ffi(policy_matrix, qs_init: Sequence[torch.Tensor]
                                                Adf=Adf, Ado=Ado, Bdf=Bdf, Bdo=Bdo, pAs=pA_s or [0],
                                                pBs=pB_s or [0]).items()}
     flags = _flags(use_utility, use_states_info_gain, use_param_info_gain, use_inductive, ffi_cache_params_static)
+    if ffi_cache_params_static:
+        _static_params_guard(list(A) + list(pA or []) + list(pB or []))
     pA_arr = _native.ptr_array(pA_p) if pA_p is not None else None
     pB_arr = _native.ptr_array(pB_p) if pB_p is not None else None
     rc = _native.lib().pymdp_hip_neg_efe(

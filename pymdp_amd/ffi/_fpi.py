@@ -100,6 +100,10 @@ def fpi_ffi(log_likelihoods: Sequence[torch.Tensor], log_prior: Sequence[torch.T
     lp_offsets = np.zeros(F + 1, dtype=np.int64)
     lp_offsets[1:] = np.cumsum(S)
     dev = log_prior[0].device
+    for name, ts in (("log_likelihoods", log_likelihoods), ("log_prior", log_prior)):
+        for i, t in enumerate(ts):
+            if not t.is_cuda or t.device != dev:
+                raise ValueError(f"fpi_ffi: {name}[{i}] must be a HIP tensor on {dev}")
     ll_flat = torch.cat([ll.to(torch.float32).reshape(batch, -1) for ll in log_likelihoods], -1).contiguous()
     lp_flat = torch.cat([lp.to(torch.float32).reshape(batch, -1) for lp in log_prior], -1).contiguous()
     q_flat = torch.empty_like(lp_flat)
@@ -151,6 +155,8 @@ def fpi_vfe_ffi(A: Sequence[torch.Tensor], obs: Sequence[torch.Tensor], prior: S
     k = _pack_infer_args(A, obs, prior, A_dependencies, 1, distr_obs, batch)
     batch = k["batch"]
     totS = int(k["lp_offsets"][-1])
+    if not qs.is_cuda or qs.device != prior[0].device or qs.numel() != batch * totS:
+        raise ValueError(f"qs must be a HIP tensor on {prior[0].device} holding [{batch}, {totS}] values")
     qs = qs.to(torch.float32).reshape(batch, totS).contiguous()
     out = torch.empty((batch, 4), dtype=torch.float32, device=qs.device)
     a = k["i64"]
@@ -198,16 +204,34 @@ def _pack_infer_args(A, obs, prior, A_dependencies, num_iter, distr_obs, batch):
             raise ValueError(f"A[{m}] must be contiguous")
         A_ptrs.append(a.data_ptr())
         A_stride.append(0 if shared else int(O[m]) * K)
+    dev = prior[0].device
     prior_ptrs, prior_stride = [], []
     for f, p in enumerate(prior):
-        if p.dtype != torch.float32 or not p.is_cuda:
-            raise ValueError(f"prior[{f}] must be a float32 HIP tensor")
+        if p.dtype != torch.float32 or not p.is_cuda or p.device != dev:
+            raise ValueError(f"prior[{f}] must be a float32 HIP tensor on {dev}")
         if p.stride(-1) != 1:
             raise ValueError(f"prior[{f}] last axis must be contiguous")
+        if p.dim() not in (1, 2) or (p.dim() == 2 and int(p.shape[0]) not in (1, batch)):
+            raise ValueError(f"prior[{f}] has shape {tuple(p.shape)}; expected (S_{f},), (1, S_{f}) "
+                             f"or ({batch}, S_{f})")
         prior_ptrs.append(p.data_ptr())
-        prior_stride.append(int(p.stride(0)) if p.dim() == 2 else 0)
+        # [S_f] and [1, S_f] are one prior for every agent
+        prior_stride.append(int(p.stride(0)) if p.dim() == 2 and int(p.shape[0]) == batch and batch > 1 else 0)
+    for m, a in enumerate(A):
+        if a.device != dev:
+            raise ValueError(f"A[{m}] must be on {dev}, the device of the prior")
+    obs_list = [obs] if isinstance(obs, torch.Tensor) else list(obs)
+    for m, o in enumerate(obs_list):
+        if not isinstance(o, torch.Tensor) or not o.is_cuda or o.device != dev:
+            raise ValueError(f"obs[{m}] must be a HIP tensor on {dev}")
     if distr_obs:
         obs_t = torch.cat([o.to(torch.float32).reshape(batch, -1) for o in obs], -1).contiguous()
+        if int(obs_t.shape[1]) != int(O.sum()):
+            raise ValueError(f"distributional observations have {int(obs_t.shape[1])} entries per agent; "
+                             f"expected sum(O) = {int(O.sum())}")
+        for m, o in enumerate(obs):
+            if int(o.shape[-1]) != int(O[m]):
+                raise ValueError(f"obs[{m}] has {int(o.shape[-1])} outcomes; A[{m}] has {int(O[m])}")
         is_index = 0
     elif isinstance(obs, torch.Tensor):
         # pre-packed int32 [batch, M] index observations (no packing kernels)

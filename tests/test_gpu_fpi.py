@@ -78,6 +78,24 @@ def test_plain_fpi_large_parity(dev):
         assert torch.isfinite(q).all()
         assert torch.allclose(q.sum(-1), torch.ones_like(q[:, 0]), atol=1e-5)
     _check(qs, w["A"], w["obs"], w["D"], w["A_dependencies"], [0, 1, 511, 1023], w["num_iter"])
+    # the FP32 spread of the reference path on agents 0, 512 and 1023: a
+    # second FP32 restatement with another summation order
+    from tests.fp32_alt import run_factorized_fpi_alt32
+    for b in (0, 512, 1023):
+        A_b = [_np(a[b]) for a in w["A"]]
+        ob = [np.eye(a.shape[0], dtype=np.float32)[int(o[b])] for a, o in zip(A_b, w["obs"])]
+        pb = [_np(d[b]) for d in w["D"]]
+        deps = w["A_dependencies"]
+        o32 = R.run_factorized_fpi(A_b, ob, pb, deps, num_iter=16, dt=np.float32)
+        o64 = R.run_factorized_fpi(A_b, ob, pb, deps, num_iter=16, dt=np.float64)
+        alt = run_factorized_fpi_alt32(A_b, ob, pb, deps, 16)
+        d = lambda x, y: max(float(np.abs(np.asarray(u, np.float64) - np.asarray(v, np.float64)).max())
+                             for u, v in zip(x, y))
+        gq = [_np(q[b]) for q in qs]
+        print(f"agent {b}: |gpu-fp64|={d(gq, o64):.2e} |gpu-fp32|={d(gq, o32):.2e} "
+              f"|gpu-fp32alt|={d(gq, alt):.2e} |fp32-fp32alt|={d(o32, alt):.2e} |fp32-fp64|={d(o32, o64):.2e}",
+              flush=True)
+        assert d(gq, o64) <= TOL
     # determinism: a second launch gives bitwise the same result
     qs2 = infer_states_ffi(w["A"], w["obs"], w["D"], w["A_dependencies"], w["num_iter"], distr_obs=False)
     for a, b in zip(qs, qs2):

@@ -172,3 +172,32 @@ def test_bruteforce_catches_transposed_dependency():
     # ... but reading it with the original deps must not
     with pytest.raises(Exception):
         R.run_factorized_fpi(A_t, obs, prior, [[0, 1]], num_iter=4, dt=np.float64)
+
+
+def test_fp32_summation_order_sensitivity_on_large_fpi():
+    """On the 1024-agent large-FPI model (S=[16,16,8,8], 6 x 32 outcomes,
+    Dirichlet(0.5) A) two FP32 restatements of the reference path that differ
+    only in summation order disagree by more than 1e-5 on some agents, by as
+    much as each differs from FP64: there the FP32 result is not defined to
+    1e-5, and parity is judged against FP64 (<= 1e-5) plus the FP32 spread.
+    Agents 11 and 12 of a CPU-generated 16-agent batch (seed 0) oscillate."""
+    from pymdp_amd import workloads
+    from tests.fp32_alt import run_factorized_fpi_alt32
+    w = workloads.plain_fpi_large(seed=0, device="cpu", batch=16)
+    A, D, obs, deps = w["A"], w["D"], w["obs"], w["A_dependencies"]
+    worst_alt = worst_64 = 0.0
+    for b in (11, 12):
+        Ab = [a[b].numpy() for a in A]
+        ob = [np.eye(a.shape[0], dtype=np.float32)[int(o[b])] for a, o in zip(Ab, obs)]
+        pb = [d[b].numpy() for d in D]
+        q32 = R.run_factorized_fpi(Ab, ob, pb, deps, 16, np.float32)
+        q64 = R.run_factorized_fpi(Ab, ob, pb, deps, 16, np.float64)
+        qa = run_factorized_fpi_alt32(Ab, ob, pb, deps, 16)
+        ealt = max(float(np.abs(x.astype(np.float64) - y.astype(np.float64)).max()) for x, y in zip(q32, qa))
+        e64 = max(float(np.abs(x.astype(np.float64) - y).max()) for x, y in zip(q32, q64))
+        worst_alt, worst_64 = max(worst_alt, ealt), max(worst_64, e64)
+        # the alternative order is itself a faithful restatement: close to FP64
+        assert max(float(np.abs(x.astype(np.float64) - y).max()) for x, y in zip(qa, q64)) < 1e-4
+    print(f"|fp32 - fp32_alt| = {worst_alt:.2e}, |fp32 - fp64| = {worst_64:.2e}")
+    assert worst_alt > 1e-5
+    assert worst_alt > 0.2 * worst_64
