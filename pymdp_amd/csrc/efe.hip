@@ -865,8 +865,11 @@ struct DevPlan {
 
 // Plans are shared: a call holds its plan for its whole duration, so a
 // concurrent call that evicts it from the cache cannot free it under it.
-std::mutex g_mu;
-std::list<std::shared_ptr<DevPlan>> g_plans;  // most recent first, small LRU
+// The registries are never destroyed: at process exit their destructors
+// would call into the HIP runtime and into another translation unit's
+// statics (the retired-scratch list) in an unspecified order.
+std::mutex& g_mu = *new std::mutex;
+std::list<std::shared_ptr<DevPlan>>& g_plans = *new std::list<std::shared_ptr<DevPlan>>;  // most recent first
 constexpr size_t kMaxPlans = 16;
 
 static int efe_err(const std::string& msg) {
@@ -1138,7 +1141,7 @@ struct ParamCache {
       (void)hipFree(buf);
   }
 };
-std::list<std::shared_ptr<ParamCache>> g_pcache;  // most recent first
+std::list<std::shared_ptr<ParamCache>>& g_pcache = *new std::list<std::shared_ptr<ParamCache>>;  // most recent first
 constexpr size_t kMaxParamCaches = 4;
 
 // Finds the cache entry of this call: *fill = false on a hit (tables valid);

@@ -13,8 +13,11 @@
 namespace pmh {
 namespace {
 
-std::mutex g_mu;
-ScratchBook g_book;
+// Never destroyed (like the plan registries that retire tables into it), so
+// no destructor runs at process exit, when the order of static destruction
+// across translation units is unspecified.
+std::mutex& g_mu = *new std::mutex;
+ScratchBook& g_book = *new ScratchBook;
 
 }  // namespace
 

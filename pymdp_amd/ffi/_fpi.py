@@ -176,7 +176,11 @@ def _pack_infer_args(A, obs, prior, A_dependencies, num_iter, distr_obs, batch):
     if len(A_dependencies) != M or (not isinstance(obs, torch.Tensor) and len(obs) != M):
         raise ValueError("A, obs and A_dependencies must have one entry per modality")
     if batch is None:
-        batch = int(prior[0].shape[0]) if prior[0].dim() == 2 else 1
+        # the largest leading axis of the batched operands; [S_f] / [1, S_f]
+        # priors and shared A broadcast over it
+        lead = [int(p.shape[0]) for p in prior if p.dim() == 2]
+        lead += [int(a.shape[0]) for a, d in zip(A, A_dependencies) if a.dim() == len(d) + 2]
+        batch = max(lead) if lead else 1
     S = np.array([int(p.shape[-1]) for p in prior], dtype=np.int64)
     O = np.array([int(a.shape[0] if a.dim() == len(d) + 1 else a.shape[1])
                   for a, d in zip(A, A_dependencies)], dtype=np.int64)
