@@ -125,6 +125,27 @@ int pymdp_hip_infer_states(const float* const* A_ptrs, const int64_t* A_batch_st
                            int32_t num_iter, int32_t method_exact, void* stream);
 
 /*
+ * pymdp_hip_infer_states with no device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as pymdp_hip_infer_states.  The
+ * per-agent workspace of the streaming FPI variant (dependency tuples of
+ * more than about 16K joint states) lives in a fixed 64 MiB device array
+ * that each GPU holds until the process ends; a call needing more is
+ * rejected with PYMDP_HIP_EINVAL.  Every call on a GPU shares that array:
+ * calls that overlap on one device race.
+ */
+int pymdp_hip_infer_states_noalloc(const float* const* A_ptrs, const int64_t* A_batch_stride,
+                                   const void* obs, int32_t obs_is_index,
+                                   const float* const* prior_ptrs, const int64_t* prior_batch_stride,
+                                   float* q_out, int64_t batch,
+                                   const int64_t* S, int64_t F,
+                                   const int64_t* O, int64_t M,
+                                   const int64_t* obs_offsets, const int64_t* lp_offsets,
+                                   const int64_t* A_dep_flat, const int64_t* A_dep_offsets,
+                                   int32_t num_iter, int32_t method_exact, void* stream);
+
+/*
  * Variational free energy of the posterior qs [batch, sum S] (e.g. the
  * output of pymdp_hip_infer_states): the return_info diagnostics of
  * update_posterior_states for "fpi" / "exact" (pymdp/inference.py:456-483,

@@ -50,6 +50,10 @@ SIGNATURES: dict[str, tuple] = {
         _i64p, _i64p,                        # obs_offsets, lp_offsets
         _i64p, _i64p,                        # A_dep_flat, A_dep_offsets
         ctypes.c_int32, ctypes.c_int32, _vp]),
+    "pymdp_hip_infer_states_noalloc": (ctypes.c_int, [
+        _vp, _i64p, _vp, ctypes.c_int32, _vp, _i64p, _vp, ctypes.c_int64,
+        _i64p, ctypes.c_int64, _i64p, ctypes.c_int64, _i64p, _i64p, _i64p, _i64p,
+        ctypes.c_int32, ctypes.c_int32, _vp]),
     "pymdp_hip_fpi_vfe": (ctypes.c_int, [
         _vp, _i64p,                          # A_ptrs, A_batch_stride
         _vp, ctypes.c_int32,                 # obs, obs_is_index

@@ -880,7 +880,11 @@ static int autotune_fast(const FpiDesc& d, const FpiPtrs& p, const std::vector<F
   return 0;
 }
 
-static int launch_fpi(const FpiDesc& d0, FpiPtrs p, int /*unused*/, int64_t batch, hipStream_t stream) {
+// fixed_ws / fixed_cap: a caller-provided workspace (the allocation-free
+// entry point's fixed scratch) instead of the library's stream scratch;
+// shapes that need more than fixed_cap bytes are rejected.
+static int launch_fpi(const FpiDesc& d0, FpiPtrs p, int /*unused*/, int64_t batch, hipStream_t stream,
+                      double* fixed_ws = nullptr, size_t fixed_cap = 0) {
   const char* pfx = "fpi_ffi: ";
   FpiDesc d = d0;
   if (batch == 0) return 0;
@@ -915,7 +919,7 @@ static int launch_fpi(const FpiDesc& d0, FpiPtrs p, int /*unused*/, int64_t batc
             cached = true;
           }
         }
-        if (!cached && autotune_enabled() && cands.size() > 1) {
+        if (!cached && !fixed_ws && autotune_enabled() && cands.size() > 1) {
           hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
           if (hipStreamIsCapturing(stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
             int rc = autotune_fast(df, p, cands, batch, stream, &fc);
@@ -963,9 +967,16 @@ static int launch_fpi(const FpiDesc& d0, FpiPtrs p, int /*unused*/, int64_t batc
     hipLaunchKernelGGL((fpi_kernel<32, 512>), grid, dim3(512), lds, stream, d, p);
   } else {
     double* ws = nullptr;
-    int rc = scratch_get(kPoolFpi, stream, sizeof(double) * (size_t)d.ws_stride * (size_t)batch,
-                         reinterpret_cast<void**>(&ws), "fpi_ffi: ");
-    if (rc) return rc;
+    const size_t need = sizeof(double) * (size_t)d.ws_stride * (size_t)batch;
+    if (fixed_ws) {
+      if (need > fixed_cap)
+        return einval(pfx, "this call needs " + std::to_string(need) + " bytes of scratch; the allocation-free entry "
+                           "point holds at most " + std::to_string(fixed_cap) + " (use fewer agents per call)");
+      ws = fixed_ws;
+    } else {
+      int rc = scratch_get(kPoolFpi, stream, need, reinterpret_cast<void**>(&ws), "fpi_ffi: ");
+      if (rc) return rc;
+    }
     p.ws = ws;
     hipLaunchKernelGGL((fpi_kernel<0, 512>), grid, dim3(512), lds, stream, d, p);
   }
@@ -1023,14 +1034,53 @@ extern "C" int pymdp_hip_fpi(const float* ll_flat, const float* lp_flat, float* 
   return launch_fpi(d, p, jslots, batch, (hipStream_t)stream);
 }
 
-extern "C" int pymdp_hip_infer_states(const float* const* A_ptrs, const int64_t* A_batch_stride,
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant of pymdp_hip_infer_states.  The streaming FPI
+// variant keeps each agent's FP64 log-likelihood and stage buffers in device
+// scratch; here that scratch is this fixed array (one copy per device, held
+// until the process ends) instead of library-allocated memory.  Its device
+// address is looked up once per device with hipGetSymbolAddress: in host
+// code the array's name is not a device address.  The kernels write every
+// workspace element before reading it, so nothing is cleared between calls.
+// Calls that overlap on one device share the array and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+}  // namespace
+
+static int noalloc_scratch(double** out) {
+  const char* pfx = "fpi_ffi: ";
+  int dev = 0;
+  PMH_HIP_CHECK(hipGetDevice(&dev), pfx);
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return einval(pfx, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_error(std::string(pfx) + "hipGetSymbolAddress(scratch): " + hipGetErrorString(e));
+      return PYMDP_HIP_EDEVICE;
+    }
+    g_noalloc_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_noalloc_ptr[dev];
+  return 0;
+}
+
+static int infer_states_impl(const float* const* A_ptrs, const int64_t* A_batch_stride,
                                       const void* obs, int32_t obs_is_index,
                                       const float* const* prior_ptrs, const int64_t* prior_batch_stride,
                                       float* q_out, int64_t batch,
                                       const int64_t* S, int64_t F, const int64_t* O, int64_t M,
                                       const int64_t* obs_offsets, const int64_t* lp_offsets,
                                       const int64_t* A_dep_flat, const int64_t* A_dep_offsets, int32_t num_iter,
-                                      int32_t method_exact, void* stream) {
+                                      int32_t method_exact, void* stream, double* fixed_ws, size_t fixed_cap) {
   const char* pfx = "fpi_ffi: ";
   pmh::set_error("");
   if (batch < 0) return einval(pfx, "batch must be non-negative");
@@ -1064,5 +1114,31 @@ extern "C" int pymdp_hip_infer_states(const float* const* A_ptrs, const int64_t*
     p.prior[f] = prior_ptrs[f];
     d.prior_stride[f] = prior_batch_stride[f];
   }
-  return launch_fpi(d, p, jslots, batch, (hipStream_t)stream);
+  return launch_fpi(d, p, jslots, batch, (hipStream_t)stream, fixed_ws, fixed_cap);
+}
+
+extern "C" int pymdp_hip_infer_states(const float* const* A_ptrs, const int64_t* A_batch_stride,
+                                      const void* obs, int32_t obs_is_index,
+                                      const float* const* prior_ptrs, const int64_t* prior_batch_stride,
+                                      float* q_out, int64_t batch,
+                                      const int64_t* S, int64_t F, const int64_t* O, int64_t M,
+                                      const int64_t* obs_offsets, const int64_t* lp_offsets,
+                                      const int64_t* A_dep_flat, const int64_t* A_dep_offsets, int32_t num_iter,
+                                      int32_t method_exact, void* stream) {
+  return infer_states_impl(A_ptrs, A_batch_stride, obs, obs_is_index, prior_ptrs, prior_batch_stride, q_out, batch, S, F, O, M, obs_offsets, lp_offsets, A_dep_flat, A_dep_offsets, num_iter, method_exact, stream, nullptr, 0);
+}
+
+extern "C" int pymdp_hip_infer_states_noalloc(const float* const* A_ptrs, const int64_t* A_batch_stride,
+                                      const void* obs, int32_t obs_is_index,
+                                      const float* const* prior_ptrs, const int64_t* prior_batch_stride,
+                                      float* q_out, int64_t batch,
+                                      const int64_t* S, int64_t F, const int64_t* O, int64_t M,
+                                      const int64_t* obs_offsets, const int64_t* lp_offsets,
+                                      const int64_t* A_dep_flat, const int64_t* A_dep_offsets, int32_t num_iter,
+                                      int32_t method_exact, void* stream) {
+  pmh::set_error("");
+  double* ws = nullptr;
+  const int rc = noalloc_scratch(&ws);
+  if (rc) return rc;
+  return infer_states_impl(A_ptrs, A_batch_stride, obs, obs_is_index, prior_ptrs, prior_batch_stride, q_out, batch, S, F, O, M, obs_offsets, lp_offsets, A_dep_flat, A_dep_offsets, num_iter, method_exact, stream, ws, kNoAllocScratchBytes);
 }
