@@ -270,6 +270,8 @@ def calc_inductive_value_t(qs, qs_next, I, epsilon=1e-3, dt=np.float32):
 
 def generate_I_matrix(H, B, threshold, depth):
     """pymdp/control.py:965-1019 (factor-local B)."""
+    if depth < 1:
+        raise ValueError("`depth` must be >= 1")
     out = []
     for f in range(len(H)):
         reach = (np.asarray(B[f]) > threshold).astype(np.float32).sum(axis=-1)

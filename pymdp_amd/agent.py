@@ -152,13 +152,20 @@ class Agent:
                                                    device=dev).expand(bs).contiguous()
         self.inductive_epsilon = torch.as_tensor(inductive_epsilon, dtype=torch.float32,
                                                  device=dev).expand(bs).contiguous()
+        if use_inductive and self.inductive_depth < 1:
+            raise ValueError("`inductive_depth` must be >= 1 when `use_inductive` is True")
         if use_inductive and H is not None:
+            # per agent, as the reference's vmap of generate_I_matrix over
+            # (H, B, inductive_threshold) (pymdp/agent.py:366-376)
             I = []
             for f in range(self.num_factors):
                 h_f = torch.as_tensor(H[f], dtype=torch.float32, device=dev)
-                b_f = B[f] if B[f].dim() == 3 else B[f][0]
-                I.append(_generate_I(h_f if h_f.dim() == 1 else h_f[0], b_f, float(inductive_threshold),
-                                     inductive_depth))
+                b_f = B[f].to(dev)
+                I.append(torch.stack([
+                    _generate_I(h_f if h_f.dim() == 1 else h_f[b],
+                                b_f if b_f.dim() == len(self.B_dependencies[f]) + 2 else b_f[b],
+                                float(self.inductive_threshold[b]), self.inductive_depth)
+                    for b in range(bs)]))
         elif use_inductive and I is not None:
             I = [_dev(i) for i in I]
         else:
@@ -335,7 +342,11 @@ class Agent:
 
 
 def _generate_I(H_f: torch.Tensor, B_f: torch.Tensor, threshold: float, depth: int) -> torch.Tensor:
-    """pymdp/control.py:965-1019 for one factor (factor-local B)."""
+    """pymdp/control.py:965-1019 for one factor (factor-local B): row i marks
+    the states that can reach the goal set H_f in i backward steps through
+    transitions of probability above `threshold` under some action."""
+    if depth < 1:
+        raise ValueError("`depth` must be >= 1")
     reach = ((B_f > threshold).float().sum(-1) > 0).float()
     pred = reach.T
     rows = [H_f]
