@@ -10,11 +10,18 @@ import numpy as np
 from oracle import numpy_ref as R
 
 
-def run_factorized_fpi_alt32(A, obs, prior, deps, num_iter):
+def run_factorized_fpi_alt32(A, obs, prior, deps, num_iter, ll_rounding="float32"):
+    """ll_rounding = "float32": ln A in FP32 (NumPy's float32 log), as the
+    oracle's FP32 path; "rounded64": ln A computed in FP64 and rounded once to
+    FP32 (the correctly rounded FP32 log), a perturbation of at most one ulp
+    of each log-likelihood term."""
     dt = np.float32
     if len(prior) == 1:
         return R.run_factorized_fpi(A, obs, prior, deps, num_iter, dt)
-    lls = [R.compute_log_likelihood_single_modality(o, a, dt) for o, a in zip(obs, A)]
+    if ll_rounding == "rounded64":
+        lls = [R.compute_log_likelihood_single_modality(o, a, np.float64).astype(dt) for o, a in zip(obs, A)]
+    else:
+        lls = [R.compute_log_likelihood_single_modality(o, a, dt) for o, a in zip(obs, A)]
     lp = [R.log_stable(p, dt) for p in prior]
     log_q = [np.zeros_like(x) for x in lp]
     for _ in range(num_iter):

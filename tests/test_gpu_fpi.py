@@ -89,12 +89,13 @@ def test_plain_fpi_large_parity(dev):
         o32 = R.run_factorized_fpi(A_b, ob, pb, deps, num_iter=16, dt=np.float32)
         o64 = R.run_factorized_fpi(A_b, ob, pb, deps, num_iter=16, dt=np.float64)
         alt = run_factorized_fpi_alt32(A_b, ob, pb, deps, 16)
+        rnd = run_factorized_fpi_alt32(A_b, ob, pb, deps, 16, ll_rounding="rounded64")
         d = lambda x, y: max(float(np.abs(np.asarray(u, np.float64) - np.asarray(v, np.float64)).max())
                              for u, v in zip(x, y))
         gq = [_np(q[b]) for q in qs]
         print(f"agent {b}: |gpu-fp64|={d(gq, o64):.2e} |gpu-fp32|={d(gq, o32):.2e} "
-              f"|gpu-fp32alt|={d(gq, alt):.2e} |fp32-fp32alt|={d(o32, alt):.2e} |fp32-fp64|={d(o32, o64):.2e}",
-              flush=True)
+              f"|gpu-fp32alt|={d(gq, alt):.2e} |fp32-fp32alt|={d(o32, alt):.2e} |fp32-fp64|={d(o32, o64):.2e} "
+              f"|fp32-fp32(ln A rounded from fp64)|={d(o32, rnd):.2e}", flush=True)
         assert d(gq, o64) <= TOL
     # determinism: a second launch gives bitwise the same result
     qs2 = infer_states_ffi(w["A"], w["obs"], w["D"], w["A_dependencies"], w["num_iter"], distr_obs=False)
