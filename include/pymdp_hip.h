@@ -428,10 +428,10 @@ int pymdp_hip_ragged_plan_upload(void* plan);
  * With out == NULL returns the number of values. */
 int pymdp_hip_ragged_layout(void* plan, int64_t* out, int64_t n_out);
 /* out16 = (agents, FPI units, stage-1 tiles, stage-2 tiles, stage-3 tiles,
- *          FPI grid, history doubles, score doubles, stage-1 doubles,
- *          stage-2 doubles, team-exchange doubles, largest FPI team,
+ *          FPI grid, history doubles, score doubles, stage-1 floats,
+ *          stage-2 floats, team-exchange doubles, largest FPI team,
  *          EFE multiply-adds per step, FPI multiply-adds per step, FPI grid
- *          capacity (workgroups resident at once), 0). */
+ *          capacity (workgroups resident at once), sum_o A ln A doubles). */
 int pymdp_hip_ragged_info(void* plan, int64_t* out16);
 /* Host copies of the plan's device tables (for host-side verification):
  * sizes16 = (sizeof agent record, sizeof unit, units, sizeof tile, stage-1
@@ -440,11 +440,20 @@ int pymdp_hip_ragged_info(void* plan, int64_t* out16);
 int pymdp_hip_ragged_tables(void* plan, int64_t* sizes16, void* agents_out,
                             void* units_out, void* tiles_out, int32_t* iarena_out);
 /* Host copies of the stage-1 tile records (per stage-1 tile: params offset
- * of its A block, stage-1 output offset, history offset of each of its 16
- * columns, sizes); count_out / record_bytes receive the count and the record
- * size; out may be NULL. */
+ * of its A block, offset of its rows of the sum_o A ln A table, stage-1
+ * output offset, history offset of each of its 16 columns, sizes); count_out
+ * / record_bytes receive the count and the record size; out may be NULL. */
 int pymdp_hip_ragged_e1_records(void* plan, void* out, int64_t* count_out, int64_t* record_bytes);
-/* One step on `stream`: five launches (FPI units, three EFE stages, per-agent
+/* Static parameters (no counterpart in the reference's ragged-free API; the
+ * counterpart of PYMDP_HIP_FLAG_MODEL_PARAMS_STATIC): with is_static = 1 the
+ * caller promises the A blocks of the params arena are not modified in place
+ * between steps, so the sum_o A ln A table is formed by the first step with a
+ * given arena pointer and reused; with 0 (the default) every step forms it.
+ * Either call invalidates the table.  Graphs captured while the table was
+ * valid do not form it. */
+int pymdp_hip_ragged_set_params_static(void* plan, int32_t is_static);
+/* One step on `stream`: five or six launches (the sum_o A ln A table unless
+ * it is valid under static parameters, FPI units, three EFE stages, per-agent
  * assembly), no allocation, no synchronisation.  Steps that overlap on one
  * device race on the plan's scratch. */
 int pymdp_hip_ragged_step(void* plan, const float* params, const int32_t* obs,

@@ -94,6 +94,7 @@ struct Agent {
   long long obs_off, qs_off, G_off, qpi_off, act_off;
   long long hist_off, roll_off, p2h_off, pol0_off, score_off, logit_off, part_off;
   long long z1_off[kM], z2_off[kM][kT];
+  long long ha_off[kM];  // sum_o A ln A table of modality m ([pos][d2] doubles)
   long long pred_off;  // predicted prior B . q for the chosen action, [totS] in `out`
 };
 
@@ -112,12 +113,21 @@ struct Tile {
 };
 
 // Stage-1 tile record, built on the host: every address the tile needs, so
-// that its A and history loads depend on this one record load only.
+// that its A, sum_o A ln A and history loads depend on this one record load
+// only.
 struct E1Rec {
   long long a_off;   // params offset of the tile's first A element (position-major)
+  long long h_off;   // ha offset of the tile's first position's sum_o A ln A row
   long long z_off;   // z1 offset of the tile's first position
   long long cb[16];  // hist offset of each history column's vector (-1 past nc)
   int O, d2, d01, nr, nc, c0, pad0, pad1;
+};
+
+// One block of the sum_o A ln A table: elements [e0, e0 + kNT) of modality
+// m's [pos][k] table (n elements).
+struct HaTask {
+  long long a_off, ha_off;
+  int O, d2, n, e0;
 };
 
 struct Bufs {
@@ -125,6 +135,7 @@ struct Bufs {
   const Unit* units;
   const Tile* t1;
   const E1Rec* e1;
+  const HaTask* hat;
   const Tile* t2;
   const Tile* t3;
   const int* iarena;
@@ -134,8 +145,9 @@ struct Bufs {
   int* act;
   double* hist;
   double* score;
-  double* z1;
-  double* z2;
+  double* ha;  // sum_o A ln A rows of every modality, [pos][d2]
+  float* z1;
+  float* z2;
   double* part;
   int* cnt;          // [0] FPI queue, [1] step sequence number, [3] team-exchange timeout flag
   long long* trace;  // optional: [unit or tile][start, end] wall clock ticks
@@ -676,16 +688,20 @@ __device__ __forceinline__ void rg_gemm_mfma(const double* X, int rows, const do
 template <typename T, int NB, typename Src, int NT = kNT>
 __device__ __forceinline__ void rg_stage_rows(const Src* src, int rows, int w, int ld, T* dst, int tid) {
   const long long n = (long long)rows * w;
+  if (n <= 0) return;
   const int dr = NT / w, dc = NT - dr * w;
   int r = tid / w, c = tid - r * w;
   for (long long e0 = tid; e0 < n; e0 += (long long)NB * NT) {
-    T v[NB];
+    // loads unconditional at clamped addresses, converted when stored: a load
+    // (or its conversion) under a per-lane condition would be waited for at
+    // the branch's join, one round trip per element
+    Src v[NB];
     int off[NB];
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
       const long long e = e0 + (long long)q * NT;
       off[q] = r * ld + c;
-      if (e < n) v[q] = (T)src[e];
+      v[q] = src[e < n ? e : n - 1];
       c += dc;
       r += dr;
       if (c >= w) {
@@ -695,7 +711,7 @@ __device__ __forceinline__ void rg_stage_rows(const Src* src, int rows, int w, i
     }
 #pragma unroll
     for (int q = 0; q < NB; ++q)
-      if (e0 + (long long)q * NT < n) dst[off[q]] = v[q];
+      if (e0 + (long long)q * NT < n) dst[off[q]] = (T)v[q];
   }
 }
 
@@ -718,11 +734,14 @@ __device__ __forceinline__ void rg_load_hist(const Agent& a, const double* hist,
   rg_stage_rows<double, (kCols * 64 + NT - 1) / NT, double, NT>(src, c1 - c0, d, ld, Q, tid);
 }
 
+
 // Stage 1: for positions ij in [r0, r1) (ij = i * d1 + j) and concatenated
 // last-dim histories n in [c0, c1) (at most 16):
 //   Z1[((n * (O+1) + o) * d0 + i) * d1 + j] = sum_k A[o, ij, k] q_n[k]   (o < O)
 //   Z1[((n * (O+1) + O) * d0 + i) * d1 + j] = sum_k (sum_o A ln A)[ij, k] q_n[k]
-// The tile's A block is staged in LDS as floats, sum_o A ln A as doubles;
+// (Z1 in FP32).  The tile's A block is staged in LDS as floats, its rows of
+// the sum_o A ln A table (rg_ha_kernel) as doubles, all with coalesced loads
+// issued together;
 // each wave multiplies 16-row blocks of [A ; sum_o A ln A] by the 16
 // history columns with FP64 MFMA (v_mfma_f64_16x16x4f64: lane l holds
 // X[l % 16][k + l / 16] and Q[col l % 16][k + l / 16]; accumulator element v
@@ -766,7 +785,6 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
   __shared__ float s_a[kE1A];
   __shared__ double s_h[kE1H];
   __shared__ double s_q[kE1Q];
-  __shared__ double s_hap[kNT];
   __shared__ E1Rec s_rec;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
   const int tile = rg_xcd_tile(blockIdx.x, gridDim.x);
@@ -813,6 +831,13 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
         ++rl;
       }
     }
+    // the tile's sum_o A ln A rows: [nr][d2] contiguous in the table
+    constexpr int NH = (kE1H + kNT - 1) / kNT;
+    const double* Hr = b.ha + rc.h_off;
+    const int toth = nr * d2;
+    double hh[NH];
+#pragma unroll
+    for (int q = 0; q < NH; ++q) hh[q] = Hr[min(tid + q * kNT, toth - 1)];
     double hv[NQ];
     bool hok[NQ];
 #pragma unroll
@@ -833,41 +858,10 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
       const int e = tid + q * kNT, c = rg_quot(e, inv_d2p), kk = e - c * d2p;
       if (e < 16 * d2p) s_q[c * lda + kk] = hv[q];
     }
-  }
-  lds_barrier();
-  // sum_o A ln A per (ij, k): outcome axis split over thread groups, partial
-  // sums combined in a fixed order
-  {
-    const int P = nr * d2;
-    const int G = std::max(1, std::min(O, kNT / P));
-    const int och = (O + G - 1) / G;
-    const float inv_p = 1.0f / (float)P;
-    for (int e = tid; e < P * G; e += kNT) {
-      const int g = rg_quot(e, inv_p), pp = e - g * P;
-      const int rl = rg_quot(pp, inv_d2), k = pp - rl * d2;
-      const float* col = s_a + rl * lda + k;
-      const int o1 = std::min(O, (g + 1) * och);
-      double s0 = 0.0, s1 = 0.0;
-      int o = g * och;
-      for (; o + 2 <= o1; o += 2) {
-        const float v0 = col[(o * nr) * lda], v1 = col[((o + 1) * nr) * lda];
-        s0 += (double)xlogx_f(v0);
-        s1 += (double)xlogx_f(v1);
-      }
-      if (o < o1) s0 += (double)xlogx_f(col[(o * nr) * lda]);
-      if (G == 1)
-        s_h[rl * lda + k] = s0 + s1;
-      else
-        s_hap[e] = s0 + s1;
-    }
-    if (G > 1) {
-      lds_barrier();
-      for (int pp = tid; pp < P; pp += kNT) {
-        double sacc = 0.0;
-        for (int g = 0; g < G; ++g) sacc += s_hap[g * P + pp];
-        const int rl = rg_quot(pp, inv_d2), k = pp - rl * d2;
-        s_h[rl * lda + k] = sacc;
-      }
+#pragma unroll
+    for (int q = 0; q < NH; ++q) {
+      const int e = tid + q * kNT, rl = rg_quot(e, inv_d2), k = e - rl * d2;
+      if (e < toth) s_h[rl * lda + k] = hh[q];
     }
   }
   lds_barrier();
@@ -881,7 +875,7 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
   const int li = lane & 15, lk = lane >> 4;
   const float inv_nr = 1.0f / (float)nr;
   // this lane's output column (history c0 + li) base in Z1
-  double* zc = b.z1 + rc.z_off + (long long)(rc.c0 + li) * O1 * d01;
+  float* zc = b.z1 + rc.z_off + (long long)(rc.c0 + li) * O1 * d01;
   const double* qrow = s_q + li * lda + lk;
   for (int blk = wave; blk < nblk; blk += 2 * kNW) {
     const int blk1 = blk + kNW;  // may be past the last block: computed, not stored
@@ -908,17 +902,43 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
         const int rr0 = (blk << 4) + 4 * v + lk;
         if (rr0 < R) {
           const int o = rg_quot(rr0, inv_nr);
-          zc[o * d01 + (rr0 - o * nr)] = acc0[v];
+          zc[o * d01 + (rr0 - o * nr)] = (float)acc0[v];
         }
         const int rr1 = (blk1 << 4) + 4 * v + lk;
         if (rr1 < R) {
           const int o = rg_quot(rr1, inv_nr);
-          zc[o * d01 + (rr1 - o * nr)] = acc1[v];
+          zc[o * d01 + (rr1 - o * nr)] = (float)acc1[v];
         }
       }
     }
   }
   if (tr && tid == 0) tr[1] = wall_clock64();
+}
+
+// sum_o A ln A of every (position, k) of a modality, summed in FP64 (the
+// ambiguity row of the stage-1 product): one thread per element, the
+// outcome loop with eight loads in flight.
+__global__ __launch_bounds__(kNT) void rg_ha_kernel(Bufs b) {
+  const HaTask tk = b.hat[blockIdx.x];
+  const int e = tk.e0 + (int)threadIdx.x;
+  if (e >= tk.n) return;
+  const int pos = e / tk.d2, k = e - pos * tk.d2;
+  const long long st = tk.d2;
+  const float* col = b.params + tk.a_off + (long long)pos * tk.O * st + k;
+  double s0 = 0.0, s1 = 0.0;
+  int o = 0;
+  for (; o + 8 <= tk.O; o += 8) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = col[(o + j) * st];
+#pragma unroll
+    for (int j = 0; j < 8; j += 2) {
+      s0 += (double)xlogx_f(v[j]);
+      s1 += (double)xlogx_f(v[j + 1]);
+    }
+  }
+  for (; o < tk.O; ++o) s0 += (double)xlogx_f(col[o * st]);
+  b.ha[tk.ha_off + e] = s0 + s1;
 }
 
 // Stage 2 at level t (FP64 MFMA): rows R = (h2, o, i) in [r0, r1) of Z1 (length d1),
@@ -938,16 +958,16 @@ __global__ __launch_bounds__(kNT2) void rg_e2_kernel(Bufs b) {
   const int d0 = a.ed[m][0], d1 = a.ed[m][1], ld1 = d1 | 1;
   const int nr = tl.r1 - tl.r0, nc = tl.c1 - tl.c0;
   const int H1 = rg_hcount(a, m, t, 1);
-  const double* src = b.z1 + a.z1_off[m] + ((long long)a.tstart[m][t] * O1 * d0 + tl.r0) * d1;
-  rg_stage_rows<double, (kTile2 + kNT2 - 1) / kNT2, double, kNT2>(src, nr, d1, ld1, s_t, tid);
+  const float* src = b.z1 + a.z1_off[m] + ((long long)a.tstart[m][t] * O1 * d0 + tl.r0) * d1;
+  rg_stage_rows<double, (kTile2 + kNT2 - 1) / kNT2, float, kNT2>(src, nr, d1, ld1, s_t, tid);
   rg_load_hist<kNT2>(a, b.hist + a.hist_off, m, t, 1, tl.c0, tl.c1, s_q, tid);
   __syncthreads();
-  double* Z = b.z2 + a.z2_off[m][t];
+  float* Z = b.z2 + a.z2_off[m][t];
   const int r0 = tl.r0, c0 = tl.c0, od = O1 * d0;
   rg_gemm_mfma<kNT2 / kWave>(s_t, nr, s_q, nc, d1, tid & (kWave - 1), tid >> 6, [&](int r, int c, double v) {
     const int R = r0 + r;
     const int h2 = R / od, oi = R - h2 * od;
-    Z[((long long)h2 * H1 + c0 + c) * od + oi] = v;
+    Z[((long long)h2 * H1 + c0 + c) * od + oi] = (float)v;
   });
   if (tr && tid == 0) tr[1] = wall_clock64();
 }
@@ -971,9 +991,9 @@ __global__ __launch_bounds__(kNT3) void rg_e3_kernel(Bufs b) {
   const int np = tl.r1 - tl.r0, nh = tl.c1 - tl.c0;
   const int H0 = rg_hcount(a, m, t, 0);
   // a rank-1 modality's stage-1 output already has the stage-2 layout
-  const double* zb = a.gR[a.mod_group[m]] == 1 ? b.z1 : b.z2;
-  const double* src = zb + a.z2_off[m][t] + (long long)tl.r0 * O1 * d0;
-  rg_stage_rows<double, (kTile + kNT3 - 1) / kNT3, double, kNT3>(src, np * O1, d0, ld0, s_t, tid);
+  const float* zb = a.gR[a.mod_group[m]] == 1 ? b.z1 : b.z2;
+  const float* src = zb + a.z2_off[m][t] + (long long)tl.r0 * O1 * d0;
+  rg_stage_rows<double, (kTile + kNT3 - 1) / kNT3, float, kNT3>(src, np * O1, d0, ld0, s_t, tid);
   rg_load_hist<kNT3>(a, b.hist + a.hist_off, m, t, 0, tl.c0, tl.c1, s_q, tid);
   const float* Cm = b.params + a.C_off[m] + (a.C_time[m] ? (long long)t * O : 0);
   const float c_lo = lane < O ? Cm[lane] : 0.0f;
@@ -1141,6 +1161,10 @@ struct Plan {
   std::vector<Unit> units;
   std::vector<Tile> t1, t2, t3;
   std::vector<E1Rec> e1;
+  std::vector<HaTask> hat;
+  long long ha_total = 0;
+  bool params_static = false;     // see pymdp_hip_ragged_set_params_static
+  const float* ha_params = nullptr;  // params arena the sum_o A ln A table was formed from
   std::vector<int32_t> iarena;
   std::vector<int64_t> layout;  // see pymdp_hip_ragged_layout
   long long hist_total = 0, score_total = 0, z1_total = 0, z2_total = 0, part_total = 0;
@@ -1155,8 +1179,11 @@ struct Plan {
   Unit* d_units = nullptr;
   Tile *d_t1 = nullptr, *d_t2 = nullptr, *d_t3 = nullptr;
   E1Rec* d_e1 = nullptr;
+  HaTask* d_hat = nullptr;
+  double* d_ha = nullptr;
   int32_t* d_iarena = nullptr;
-  double *d_hist = nullptr, *d_score = nullptr, *d_z1 = nullptr, *d_z2 = nullptr, *d_part = nullptr;
+  double *d_hist = nullptr, *d_score = nullptr, *d_part = nullptr;
+  float *d_z1 = nullptr, *d_z2 = nullptr;
   int* d_cnt = nullptr;
 };
 
@@ -1170,7 +1197,7 @@ static void free_plan(Plan* p) {
   int prev = 0;
   (void)hipGetDevice(&prev);
   (void)hipSetDevice(p->device);
-  for (void* x : {(void*)p->d_ag, (void*)p->d_units, (void*)p->d_t1, (void*)p->d_e1, (void*)p->d_t2, (void*)p->d_t3,
+  for (void* x : {(void*)p->d_ag, (void*)p->d_units, (void*)p->d_t1, (void*)p->d_e1, (void*)p->d_hat, (void*)p->d_ha, (void*)p->d_t2, (void*)p->d_t3,
                   (void*)p->d_iarena, (void*)p->d_hist, (void*)p->d_score, (void*)p->d_z1, (void*)p->d_z2,
                   (void*)p->d_part, (void*)p->d_cnt})
     (void)hipFree(x);
@@ -1501,11 +1528,16 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
       z1 += (long long)N1 * O1 * d0 * d1;
       // E1 tiles: blocks of (i, j) positions whose A rows and sum_o A ln A
       // rows fit the tile's LDS x blocks of at most 16 concatenated history
-      // columns
+      // columns; the modality's sum_o A ln A table ([pos][d2], rg_ha_kernel)
       {
         const int d01 = d0 * d1;
         const int nr = rg_e1_rows(a.O[m], d2, d01);
         if (a.O[m] * (((d2 + 3) & ~3) + 1) > kE1A) return fail("EFE tile does not fit");
+        const long long nel = (long long)d01 * d2;
+        a.ha_off[m] = pl->ha_total;
+        for (long long e0 = 0; e0 < nel; e0 += kNT)
+          pl->hat.push_back(HaTask{a.A_off[m], pl->ha_total, a.O[m], d2, (int)nel, (int)e0});
+        pl->ha_total += nel;
         for (int r0 = 0; r0 < d01; r0 += nr)
           for (int c0 = 0; c0 < N1; c0 += 16) {
             const int r1 = std::min(d01, r0 + nr), c1 = std::min(N1, c0 + 16);
@@ -1571,6 +1603,7 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
     r.nc = tl.c1 - tl.c0;
     r.c0 = tl.c0;
     r.a_off = a.A_off[m] + (long long)tl.r0 * a.O[m] * a.ed[m][2];
+    r.h_off = a.ha_off[m] + (long long)tl.r0 * a.ed[m][2];
     r.z_off = a.z1_off[m] + tl.r0;
     for (int c = 0; c < 16; ++c) {
       r.cb[c] = -1;
@@ -1645,6 +1678,8 @@ static int upload(Plan* pl) {
   if ((e = alloc((void**)&pl->d_units, sizeof(Unit) * pl->units.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_t1, sizeof(Tile) * pl->t1.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_e1, sizeof(E1Rec) * pl->e1.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_hat, sizeof(HaTask) * pl->hat.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_ha, sizeof(double) * pl->ha_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_t2, sizeof(Tile) * pl->t2.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_t3, sizeof(Tile) * pl->t3.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_iarena, sizeof(int32_t) * pl->iarena.size())) != hipSuccess)
@@ -1652,8 +1687,8 @@ static int upload(Plan* pl) {
   if ((e = alloc((void**)&pl->d_hist, sizeof(double) * pl->hist_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_score, sizeof(double) * pl->score_total)) != hipSuccess)
     return fail_dev(e, "hipMalloc");
-  if ((e = alloc((void**)&pl->d_z1, sizeof(double) * pl->z1_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
-  if ((e = alloc((void**)&pl->d_z2, sizeof(double) * pl->z2_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_z1, sizeof(float) * pl->z1_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_z2, sizeof(float) * pl->z2_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_part, sizeof(double) * pl->part_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_cnt, sizeof(int) * 4)) != hipSuccess) return fail_dev(e, "hipMalloc");
   auto up = [&](void* d, const void* h, size_t bytes) {
@@ -1663,6 +1698,7 @@ static int upload(Plan* pl) {
       (e = up(pl->d_units, pl->units.data(), sizeof(Unit) * pl->units.size())) != hipSuccess ||
       (e = up(pl->d_t1, pl->t1.data(), sizeof(Tile) * pl->t1.size())) != hipSuccess ||
       (e = up(pl->d_e1, pl->e1.data(), sizeof(E1Rec) * pl->e1.size())) != hipSuccess ||
+      (e = up(pl->d_hat, pl->hat.data(), sizeof(HaTask) * pl->hat.size())) != hipSuccess ||
       (e = up(pl->d_t2, pl->t2.data(), sizeof(Tile) * pl->t2.size())) != hipSuccess ||
       (e = up(pl->d_t3, pl->t3.data(), sizeof(Tile) * pl->t3.size())) != hipSuccess ||
       (e = up(pl->d_iarena, pl->iarena.data(), sizeof(int32_t) * pl->iarena.size())) != hipSuccess)
@@ -1753,6 +1789,7 @@ extern "C" int pymdp_hip_ragged_info(void* plan, int64_t* out16) {
   out16[12] = (int64_t)pl->efe_macs;
   out16[13] = (int64_t)pl->fpi_macs;
   out16[14] = pl->grid_cap;
+  out16[15] = pl->ha_total;
   return PYMDP_HIP_OK;
 }
 
@@ -1763,6 +1800,8 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
   b.units = pl->d_units;
   b.t1 = pl->d_t1;
   b.e1 = pl->d_e1;
+  b.hat = pl->d_hat;
+  b.ha = pl->d_ha;
   b.t2 = pl->d_t2;
   b.t3 = pl->d_t3;
   b.iarena = pl->d_iarena;
@@ -1783,6 +1822,12 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
   b.n_t3 = (int)pl->t3.size();
   b.trace = pl->trace;
   b.carry = pl->carry;
+  // the sum_o A ln A table: every step, or with static parameters only when
+  // the params arena changed
+  if (!pl->hat.empty() && (!pl->params_static || pl->ha_params != params)) {
+    hipLaunchKernelGGL(rg_ha_kernel, dim3((unsigned)pl->hat.size()), dim3(kNT), 0, s, b);
+    pl->ha_params = pl->params_static ? params : nullptr;
+  }
   if (pl->totS_max <= 256)
     hipLaunchKernelGGL(rg_fpi_kernel<256>, dim3(grid1), dim3(kNT), 0, s, b);
   else
@@ -1869,6 +1914,15 @@ extern "C" int pymdp_hip_ragged_status(void* plan, int32_t* flag_out) {
   PMH_HIP_CHECK(hipDeviceSynchronize(), "ragged: ");
   PMH_HIP_CHECK(hipMemcpy(h, pl->d_cnt, sizeof(h), hipMemcpyDeviceToHost), "ragged: ");
   *flag_out = h[3];
+  return PYMDP_HIP_OK;
+}
+
+extern "C" int pymdp_hip_ragged_set_params_static(void* plan, int32_t is_static) {
+  Plan* pl = static_cast<Plan*>(plan);
+  if (!pl) return einval("plan is NULL");
+  if (is_static != 0 && is_static != 1) return einval("params_static must be 0 or 1");
+  pl->params_static = is_static != 0;
+  pl->ha_params = nullptr;  // formed again by the next step
   return PYMDP_HIP_OK;
 }
 

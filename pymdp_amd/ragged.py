@@ -102,7 +102,7 @@ class RaggedPlan:
         out = np.zeros(16, np.int64)
         _native.check(_native.lib().pymdp_hip_ragged_info(self.handle, out.ctypes.data_as(_native._i64p)))
         keys = ["agents", "units", "tiles1", "tiles2", "tiles3", "grid_fpi", "hist_doubles", "score_doubles",
-                "z1_doubles", "z2_doubles", "part_doubles", "team_max", "efe_macs", "fpi_macs", "grid_cap"]
+                "z1_doubles", "z2_doubles", "part_doubles", "team_max", "efe_macs", "fpi_macs", "grid_cap", "ha_doubles"]
         return dict(zip(keys, (int(x) for x in out)))
 
     def upload(self):
@@ -188,6 +188,7 @@ class RaggedBatch:
         self.act = torch.zeros(max(1, int(self.plan.sizes[3])), dtype=torch.int32, device=self.device)
         self.graph = None
         self.carry = False
+        self.params_static = False
         # gather / scatter indices D -> predicted-prior slots (reset_prior)
         d_idx, p_idx = [], []
         fi = 0
@@ -213,6 +214,14 @@ class RaggedBatch:
         carry-mode step starts from D as the reference's first rollout step."""
         with torch.cuda.device(self.device):
             self.out[self._p_idx] = self.params[self._d_idx]
+
+    def set_params_static(self, static: bool):
+        """static=True: the params arena is not modified in place between
+        steps, so the sum_o A ln A table of the stage-1 product is formed once
+        (by the next step) and reused; False (the default): every step forms
+        it.  Call again after changing parameters in place."""
+        _native.check(_native.lib().pymdp_hip_ragged_set_params_static(self.plan.handle, 1 if static else 0))
+        self.params_static = bool(static)
 
     def set_obs(self, obs: Sequence[Sequence[int]]):
         self.obs.copy_(pack_obs(self.plan, obs, self.device))

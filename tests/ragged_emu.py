@@ -36,12 +36,13 @@ AGENT_DT = np.dtype([
     ("obs_off", "i8"), ("qs_off", "i8"), ("G_off", "i8"), ("qpi_off", "i8"), ("act_off", "i8"),
     ("hist_off", "i8"), ("roll_off", "i8"), ("p2h_off", "i8"), ("pol0_off", "i8"), ("score_off", "i8"),
     ("logit_off", "i8"), ("part_off", "i8"), ("z1_off", "i8", (kM,)), ("z2_off", "i8", (kM, kT)),
+    ("ha_off", "i8", (kM,)),
     ("pred_off", "i8"),
 ], align=True)
 
 UNIT_DT = np.dtype([("agent", "i4"), ("slice", "i4"), ("nseg", "i4"), ("pad", "i4"), ("g", "i4", (kM,)),
                     ("r0", "i4", (kM,)), ("r1", "i4", (kM,)), ("lds", "i4", (kM,))])
-E1REC_DT = np.dtype([("a_off", "i8"), ("z_off", "i8"), ("cb", "i8", (16,)), ("O", "i4"), ("d2", "i4"),
+E1REC_DT = np.dtype([("a_off", "i8"), ("h_off", "i8"), ("z_off", "i8"), ("cb", "i8", (16,)), ("O", "i4"), ("d2", "i4"),
                      ("d01", "i4"), ("nr", "i4"), ("nc", "i4"), ("c0", "i4"), ("pad0", "i4"), ("pad1", "i4")])
 TILE_DT = np.dtype([("agent", "i4"), ("m", "i4"), ("t", "i4"), ("r0", "i4"), ("r1", "i4"), ("c0", "i4"),
                     ("c1", "i4"), ("pad", "i4")])
@@ -110,8 +111,9 @@ class Emu:
         self.obs = obs.astype(np.int32)
         self.hist = np.full(max(1, self.info["hist_doubles"]), np.nan)
         self.score = np.full(max(1, self.info["score_doubles"]), np.nan)
-        self.z1 = np.full(max(1, self.info["z1_doubles"]), np.nan)
-        self.z2 = np.full(max(1, self.info["z2_doubles"]), np.nan)
+        # stage-1 / stage-2 outputs are FP32
+        self.z1 = np.full(max(1, self.info["z1_doubles"]), np.nan, np.float32)
+        self.z2 = np.full(max(1, self.info["z2_doubles"]), np.nan, np.float32)
         self.out = np.full(max(1, int(plan.sizes[2])), np.nan, np.float32) if out is None else out.copy()
         self.act = np.full(max(1, int(plan.sizes[3])), -1, np.int32)
         check_units(self.ag, self.units, self.info)
@@ -319,10 +321,7 @@ class Emu:
         _chk(np.array([(O * nr - 1) * lda + d2 - 1]), kE1A, "s_a (E1)")
         _chk(np.array([(nr - 1) * lda + d2 - 1]), kE1H, "s_h (E1)")
         _chk(np.array([15 * lda + ((d2 + 3) & ~3) - 1]), kE1Q, "s_q (E1)")
-        P = nr * d2
-        G = max(1, min(O, kNT // P))
-        if G > 1:
-            _chk(np.array([P * G - 1]), kNT, "s_hap (E1)")
+        _chk(np.array([nr * d2 - 1]), kE1H, "sum A ln A block (E1)")
         K = int(a["K"][m])
         assert K == d01 * d2
         # the tile's record: sizes and addresses as derived from the agent
@@ -330,6 +329,8 @@ class Emu:
             (O, d2, d01, nr, nc, c0)
         assert int(rec["a_off"]) == int(a["A_off"][m]) + r0 * O * d2
         assert int(rec["z_off"]) == int(a["z1_off"][m]) + r0
+        assert int(rec["h_off"]) == int(a["ha_off"][m]) + r0 * d2
+        _chk(int(rec["h_off"]) + np.arange(nr * d2), self.info["ha_doubles"], "sum A ln A table (E1)")
         # position-major A_m: (position, o, k) at (position O + o) d2 + k, from the record's offset
         idx = int(rec["a_off"]) + ((np.arange(nr))[None, :, None] * O + np.arange(O)[:, None, None]) * d2 + \
             np.arange(d2)[None, None, :]
@@ -567,6 +568,7 @@ def check_bounds_only(plan):
         assert (rec["cb"][valid] + np.repeat(d2_, 16).reshape(-1, 16)[valid] <= info["hist_doubles"]).all()
         last = rec["z_off"] + ((rec["c0"] + rec["nc"] - 1) * (O_ + 1) + O_) * d01_ + rec["nr"] - 1
         assert (rec["z_off"] >= 0).all() and (last < info["z1_doubles"]).all()
+        assert (rec["h_off"] >= 0).all() and (rec["h_off"] + rec["nr"] * d2_ <= info["ha_doubles"]).all()
     for tl in t2:
         a = ag[int(tl["agent"])]
         m, t = int(tl["m"]), int(tl["t"])
