@@ -209,18 +209,18 @@ __global__ __launch_bounds__(NT) void fpi_kernel(const FpiDesc d, const FpiPtrs 
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
               const int j = j0 + i;
-              t[i] = 1.0f;
-              if (j >= jb && j < je) {
-                const int r = (((j - jb) * NW + wave) << rpw_lg) + lr;
-                if (r < rows && col < n) t[i] = __builtin_nontemporal_load(row + (long long)r * n + col);
-              }
+              // every load of the chunk unconditional at a clamped row and
+              // column (slots outside [jb, je) included), masked at use
+              const int r = (((j - jb) * NW + wave) << rpw_lg) + lr;
+              t[i] = __builtin_nontemporal_load(row + (long long)min(max(r, 0), rows - 1) * n + min(col, n - 1));
             }
 #pragma unroll
             for (int i = 0; i < CH; ++i) {
               const int j = j0 + i;
               if (j >= jb && j < je) {
                 const int r = (((j - jb) * NW + wave) << rpw_lg) + lr;
-                if (r < rows && col < n) X0[j] = fma((double)w, log_stable_f64(t[i]), X0[j]);
+                const double v = fma((double)w, log_stable_f64(t[i]), X0[j]);
+                X0[j] = (r < rows && col < n) ? v : X0[j];
               }
             }
           }
@@ -229,10 +229,9 @@ __global__ __launch_bounds__(NT) void fpi_kernel(const FpiDesc d, const FpiPtrs 
         const float* llm = p.ll + (long long)b * d.ll_stride + d.ll_off[m];
 #pragma unroll
         for (int j = 0; j < JMAX; ++j) {
-          if (j >= jb && j < je) {
-            const int r = (((j - jb) * NW + wave) << rpw_lg) + lr;
-            if (r < rows && col < n) X0[j] += (double)llm[(long long)r * n + col];
-          }
+          const int r = (((j - jb) * NW + wave) << rpw_lg) + lr;
+          const float v = llm[(long long)min(max(r, 0), rows - 1) * n + min(col, n - 1)];
+          X0[j] += (j >= jb && j < je && r < rows && col < n) ? (double)v : 0.0;
         }
       }
     }

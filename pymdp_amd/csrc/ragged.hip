@@ -430,27 +430,39 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
       const int n = a.gdim[g][R - 1], ld = n | 1;
       const int r0 = u.r0[sg], nr = u.r1[sg] - r0;
       double* X = s_x + u.lds[sg];
-      // four elements per thread in flight: every load of a batch is issued
-      // before any product is used
-      for (int e0 = tid; e0 < nr * n; e0 += 4 * kNT) {
-        double prod[4] = {1.0, 1.0, 1.0, 1.0};
-        for (int m = 0; m < a.M; ++m) {
-          if (a.mod_group[m] != g) continue;
-          const int o = s_obs[m];
-          if (o < 0 || o >= a.O[m]) continue;
-          // A_m is position-major: row (position r, outcome o) at (r O + o) n
-          const float* Ar = b.params + a.A_off[m] + ((long long)r0 * a.O[m] + o) * n;
-          const long long rs = (long long)a.O[m] * n;
-          float v[4];
+      // four elements per thread and every modality of the group in flight:
+      // all loads of a batch are issued (unconditionally, at clamped
+      // addresses; the modality guards are uniform) before any product is
+      // formed
+      const int ne = nr * n;
+      for (int e0 = tid; e0 < ne; e0 += 4 * kNT) {
+        int rl[4], kk[4];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int e = e0 + q * kNT;
-            const int rl = e / n;
-            v[q] = e < nr * n ? Ar[rl * rs + (e - rl * n)] : 1.0f;
-          }
-#pragma unroll
-          for (int q = 0; q < 4; ++q) prod[q] *= (double)fmaxf(v[q], kMinVal);
+        for (int q = 0; q < 4; ++q) {
+          const int e = min(e0 + q * kNT, ne - 1);
+          rl[q] = e / n;
+          kk[q] = e - rl[q] * n;
         }
+        float v[kM][4];
+#pragma unroll
+        for (int m = 0; m < kM; ++m) {
+          const int o = m < a.M ? s_obs[m] : -1;
+          if (m < a.M && a.mod_group[m] == g && o >= 0 && o < a.O[m]) {
+            // A_m is position-major: row (position r, outcome o) at (r O + o) n
+            const float* Ar = b.params + a.A_off[m] + ((long long)r0 * a.O[m] + o) * n;
+            const long long rs = (long long)a.O[m] * n;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[m][q] = Ar[rl[q] * rs + kk[q]];
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[m][q] = 1.0f;
+          }
+        }
+        double prod[4] = {1.0, 1.0, 1.0, 1.0};
+#pragma unroll
+        for (int m = 0; m < kM; ++m)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) prod[q] *= (double)fmaxf(v[m][q], kMinVal);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int e = e0 + q * kNT;
