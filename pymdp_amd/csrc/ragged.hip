@@ -793,7 +793,12 @@ __host__ __device__ __forceinline__ int rg_e1_rows(int O, int d2, int d01) {
   return nr < d01 ? nr : d01;
 }
 
-__global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
+// NT threads per tile: 512 (eight waves) puts twice the loads of a tile in
+// flight and, at three tiles per CU (the LDS bound), six waves per SIMD.
+constexpr int kNT1 = 512;
+template <int NT>
+__global__ __launch_bounds__(NT) void rg_e1_kernel(Bufs b) {
+  constexpr int NW = NT / kWave;
   __shared__ float s_a[kE1A];
   __shared__ double s_h[kE1H];
   __shared__ double s_q[kE1Q];
@@ -814,11 +819,11 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
     // one contiguous chunk: element e = (rl, o, k) at Am[e], LDS row o nr + rl.
     // The history columns are loaded in the same round trip.
     const float* Am = b.params + rc.a_off;
-    constexpr int NB = (kE1A + kNT - 1) / kNT;
-    constexpr int NQ = (16 * 64 + kNT - 1) / kNT;
+    constexpr int NB = (kE1A + NT - 1) / NT;
+    constexpr int NQ = (16 * 64 + NT - 1) / NT;
     const int per = O * d2;  // floats per position
     const int tot = nr * per;
-    const int drl = kNT / per, dm = kNT - drl * per;  // per-step advance of (rl, rem)
+    const int drl = NT / per, dm = NT - drl * per;  // per-step advance of (rl, rem)
     const int dq = dm / d2, dk = dm - dq * d2;
     int rl = tid / per, rem = tid - rl * per;
     int o = rg_quot(rem, inv_d2), k = rem - o * d2;
@@ -828,7 +833,7 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
     int off[NB];
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
-      const int e = tid + q * kNT;
+      const int e = tid + q * NT;
       off[q] = (o * nr + rl) * lda + k;
       v[q] = __builtin_nontemporal_load(Am + min(e, tot - 1));
       k += dk;
@@ -844,17 +849,17 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
       }
     }
     // the tile's sum_o A ln A rows: [nr][d2] contiguous in the table
-    constexpr int NH = (kE1H + kNT - 1) / kNT;
+    constexpr int NH = (kE1H + NT - 1) / NT;
     const double* Hr = b.ha + rc.h_off;
     const int toth = nr * d2;
     double hh[NH];
 #pragma unroll
-    for (int q = 0; q < NH; ++q) hh[q] = Hr[min(tid + q * kNT, toth - 1)];
+    for (int q = 0; q < NH; ++q) hh[q] = Hr[min(tid + q * NT, toth - 1)];
     double hv[NQ];
     bool hok[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const int e = tid + q * kNT, cq = rg_quot(e, inv_d2p), c = min(cq, 15), kk = e - cq * d2p;
+      const int e = tid + q * NT, cq = rg_quot(e, inv_d2p), c = min(cq, 15), kk = e - cq * d2p;
       const long long cb = rc.cb[c];
       hok[q] = e < 16 * d2p && cb >= 0 && kk < d2;
       hv[q] = b.hist[hok[q] ? cb + kk : 0];
@@ -864,15 +869,15 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
       if (!hok[q]) hv[q] = 0.0;
 #pragma unroll
     for (int q = 0; q < NB; ++q)
-      if (tid + q * kNT < tot) s_a[off[q]] = v[q];
+      if (tid + q * NT < tot) s_a[off[q]] = v[q];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const int e = tid + q * kNT, c = rg_quot(e, inv_d2p), kk = e - c * d2p;
+      const int e = tid + q * NT, c = rg_quot(e, inv_d2p), kk = e - c * d2p;
       if (e < 16 * d2p) s_q[c * lda + kk] = hv[q];
     }
 #pragma unroll
     for (int q = 0; q < NH; ++q) {
-      const int e = tid + q * kNT, rl = rg_quot(e, inv_d2), k = e - rl * d2;
+      const int e = tid + q * NT, rl = rg_quot(e, inv_d2), k = e - rl * d2;
       if (e < toth) s_h[rl * lda + k] = hh[q];
     }
   }
@@ -889,8 +894,8 @@ __global__ __launch_bounds__(kNT) void rg_e1_kernel(Bufs b) {
   // this lane's output column (history c0 + li) base in Z1
   float* zc = b.z1 + rc.z_off + (long long)(rc.c0 + li) * O1 * d01;
   const double* qrow = s_q + li * lda + lk;
-  for (int blk = wave; blk < nblk; blk += 2 * kNW) {
-    const int blk1 = blk + kNW;  // may be past the last block: computed, not stored
+  for (int blk = wave; blk < nblk; blk += 2 * NW) {
+    const int blk1 = blk + NW;  // may be past the last block: computed, not stored
     const int r0 = (blk << 4) + li, r1 = (blk1 << 4) + li;
     const float* xa0 = s_a + (r0 < RA ? r0 : 0) * lda + lk;
     const double* xh0 = s_h + (r0 >= RA && r0 < R ? r0 - RA : 0) * lda + lk;
@@ -1844,7 +1849,7 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
     hipLaunchKernelGGL(rg_fpi_kernel<256>, dim3(grid1), dim3(kNT), 0, s, b);
   else
     hipLaunchKernelGGL(rg_fpi_kernel<kF * kDim>, dim3(grid1), dim3(kNT), 0, s, b);
-  if (b.n_t1) hipLaunchKernelGGL(rg_e1_kernel, dim3(b.n_t1), dim3(kNT), 0, s, b);
+  if (b.n_t1) hipLaunchKernelGGL(rg_e1_kernel<kNT1>, dim3(b.n_t1), dim3(kNT1), 0, s, b);
   if (b.n_t2) hipLaunchKernelGGL(rg_e2_kernel, dim3(b.n_t2), dim3(kNT2), 0, s, b);
   if (b.n_t3) hipLaunchKernelGGL(rg_e3_kernel, dim3(b.n_t3), dim3(kNT3), 0, s, b);
   hipLaunchKernelGGL(rg_assemble_kernel, dim3((unsigned)pl->n_agents), dim3(kNT), 0, s, b);
