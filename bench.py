@@ -59,6 +59,43 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
+CPU_RUNS, CPU_WARM = 20, 3
+
+
+def _cpu_threads():
+    return len(os.sched_getaffinity(0))
+
+
+_CPU_FLAGS = None
+
+
+def _cpu_native():
+    """Builds the oracle's C/OpenMP restatement with -march=native for this
+    host (once per process) and returns the flags."""
+    global _CPU_FLAGS
+    if _CPU_FLAGS is None:
+        from oracle import cpu_ref
+        _CPU_FLAGS = cpu_ref.use_native()
+    return _CPU_FLAGS
+
+
+def _log(msg):
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _median_runs(fn, runs=CPU_RUNS, warm=CPU_WARM):
+    """Median wall time (s) of `runs` calls of fn after `warm` untimed ones."""
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    ts.sort()
+    n = len(ts)
+    return ts[n // 2] if n % 2 else 0.5 * (ts[n // 2 - 1] + ts[n // 2])
 
 
 def _args():
@@ -67,7 +104,8 @@ def _args():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="headline",
-                    choices=["headline", "plain_fpi_large", "plain_efe", "ragged", "grad", "learning"])
+                    choices=["headline", "plain_fpi_small", "plain_fpi_large", "plain_efe", "ragged", "grad",
+                             "learning"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--calibrate-copy", action="store_true")
@@ -160,6 +198,71 @@ def calibrate_copy():
                       "ms": dt * 1e3, "GBps": 8 * n / dt / 1e9}))
 
 
+def bench_plain_fpi_small(a, torch):
+    """64 agents, S=[4,3], O=[5,6], 16 sweeps (the reference's CPU-runnable
+    case): one step = one fused infer_states launch for all agents."""
+    import numpy as np
+
+    from pymdp_amd import workloads
+    from pymdp_amd.ffi import infer_states_ffi
+
+    dev = torch.device("cuda:0")
+    w = workloads.plain_fpi_small(seed=0, device=dev)
+    B, S, O, deps = w["batch"], w["S"], w["O"], w["A_dependencies"]
+    obs = torch.stack([o.to(torch.int32) for o in w["obs"]], -1).contiguous()
+    out = torch.empty((B, sum(S)), device=dev)
+
+    def step():
+        return infer_states_ffi(w["A"], obs, w["D"], deps, w["num_iter"], distr_obs=False, out=out)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(a.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t0) * 1e3 / a.steps
+    ms_events = e0.elapsed_time(e1) / a.steps
+    from pymdp_amd import _native
+    kernel = _native.lib().pymdp_hip_last_launch().decode()
+    K = int(np.prod(S))
+    alg_bytes = B * (len(O) * K * 4 + len(O) * 4 + 2 * sum(S) * 4)
+    achieved = alg_bytes / (ms_events * 1e-3) / 1e9
+    from oracle import numpy_ref as R
+    q = step()
+    torch.cuda.synchronize()
+    e64 = 0.0
+    for b in range(B):
+        A_b = [x[b].cpu().numpy() for x in w["A"]]
+        ob = [np.eye(o, dtype=np.float32)[int(x[b])] for o, x in zip(O, w["obs"])]
+        r64 = R.run_factorized_fpi(A_b, ob, [d[b].cpu().numpy() for d in w["D"]], deps, w["num_iter"], np.float64)
+        e64 = max(e64, max(float(np.abs(q[f][b].cpu().numpy() - r64[f]).max()) for f in range(len(S))))
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline_fpi(w, B)
+    return {
+        "metric": "agent-steps/s (plain FPI, 64 agents, S=[4,3], O=[5,6], 16 sweeps)",
+        "value": B / (ms_step * 1e-3), "unit": "agent-steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 data, f64 sweep accumulation",
+        "data": "synthetic (seeded Dirichlet(1) A, uniform D, uniform observation indices; generated on device)",
+        "config": {"workload": "plain_fpi_small", "agents": B, "num_states": S, "num_obs": O,
+                   "num_iter": w["num_iter"], "obs": "index (one-hot)"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kernel, "kernel_ms": ms_events,
+                     "alg_bytes_per_launch": alg_bytes,
+                     "note": "one launch of 64 one-wave agents: the launch latency, not HBM, sets the time"},
+        "parity": {"agents_checked": B, "max_abs_err_vs_fp64": e64},
+        "cpu_baseline": cpu,
+    }
+
+
 def bench_plain_fpi_large(a, torch):
     import numpy as np
 
@@ -217,7 +320,7 @@ def bench_plain_fpi_large(a, torch):
 
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline_fpi(w, a.cpu_seconds)
+        cpu = cpu_baseline_fpi(w)
     res = {
         "metric": "agent-steps/s (plain FPI, 1024 agents, S=[16,16,8,8], 6x32 outcomes, 16 sweeps)",
         "value": B / (ms_step * 1e-3),
@@ -324,28 +427,23 @@ def bench_learning(a, torch):
     }
 
 
-def cpu_baseline_fpi(w, seconds):
-    """The oracle's C/OpenMP restatement of the reference CPU FFI FPI, on a
-    bounded sample (the first 64 agents, repeated for ~`seconds`)."""
+def cpu_baseline_fpi(w, n=256):
+    """The oracle's C/OpenMP restatement of the reference CPU FFI FPI
+    (-march=native, OpenMP over agents on every core of this host) on the
+    first n agents: median of 20 runs after 3 warm-ups."""
     import numpy as np
 
     from oracle import cpu_ref
-    n = 64
+    flags = _cpu_native()
+    n = min(n, int(w["batch"]))
     A = [x[:n].cpu().numpy() for x in w["A"]]
     D = [d[:n].cpu().numpy() for d in w["D"]]
     obs = np.stack([o[:n].cpu().numpy() for o in w["obs"]], 1).astype(np.int32)
-    threads = len(os.sched_getaffinity(0))
-    cpu_ref.infer_states(A, obs, D, w["A_dependencies"], w["num_iter"], threads)  # warm-up
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        cpu_ref.infer_states(A, obs, D, w["A_dependencies"], w["num_iter"], threads)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    return {"value": reps * n / el, "unit": "agent-steps/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} agents of the workload, {reps} repetitions, {el:.1f} s, "
-                      f"oracle/cpu_baseline.c (OpenMP over agents, -O3 -march=x86-64-v3)"}
+    threads = _cpu_threads()
+    t = _median_runs(lambda: cpu_ref.infer_states(A, obs, D, w["A_dependencies"], w["num_iter"], threads))
+    return {"value": n / t, "unit": "agent-steps/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} agents of the workload per run, median of {CPU_RUNS} runs after {CPU_WARM} "
+                      f"warm-ups ({t * 1e3:.2f} ms per run), oracle/cpu_baseline.c (OpenMP over agents, {flags})"}
 
 
 def _efe_macs(pm, S, O, A_dependencies, B_dependencies, batch):
@@ -415,22 +513,7 @@ def bench_plain_efe(a, torch):
         eq = max(eq, float(np.abs(q_pi[b].cpu().numpy() - p64).max()))
     cpu = None
     if not a.no_cpu_baseline:
-        n, t0 = 0, time.perf_counter()
-        A_h = [x[:2].cpu().numpy() for x in w["A"]]
-        B_h = [x[:2].cpu().numpy() for x in w["B"]]
-        C_h = [x[:2].cpu().numpy() for x in w["C"]]
-        qs_h = [x[:2].cpu().numpy() for x in w["qs"]]
-        t0 = time.perf_counter()
-        while time.perf_counter() - t0 < min(a.cpu_seconds, 10.0) or n == 0:
-            b = n % 2
-            G32 = DR.neg_efe_dedup(pm, [q[b] for q in qs_h], [x[b] for x in A_h], [x[b] for x in B_h],
-                                   [x[b] for x in C_h], w["A_dependencies"], w["B_dependencies"], dt=np.float32)
-            DR.q_pi_from_G(G32, w["gamma"], dt=np.float32)
-            n += 1
-        el = time.perf_counter() - t0
-        cpu = {"value": n / el, "unit": "agent-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{n} agent steps over the first 2 agents, {el:.1f} s, NumPy FP32 restatement of the "
-                         f"history-dedup neg-EFE + q_pi (oracle), single thread"}
+        cpu = cpu_baseline_efe(w, pm)
     return {
         "metric": "agent-steps/s (plain EFE, 64 agents, 32768 policies)",
         "value": B / (ms_step * 1e-3), "unit": "agent-steps/s", "n_gpus": 1, "steps": a.steps,
@@ -451,6 +534,26 @@ def bench_plain_efe(a, torch):
     }
 
 
+def cpu_baseline_efe(w, pm):
+    """The oracle's C/OpenMP restatement of the reference CPU factor-history
+    neg-EFE + q_pi + action (-march=native), one agent per core: the first
+    (cores) agents per run, median of 20 runs after 3 warm-ups."""
+    from oracle import cpu_ref
+    flags = _cpu_native()
+    threads = _cpu_threads()
+    n = min(threads, int(w["batch"]))
+    ags = [dict(S=w["S"], U=w["U"], O=w["O"], A=[x[b].cpu().numpy() for x in w["A"]],
+                B=[x[b].cpu().numpy() for x in w["B"]], C=[x[b].cpu().numpy() for x in w["C"]],
+                qs=[x[b].cpu().numpy() for x in w["qs"]], policies=pm, A_dependencies=w["A_dependencies"],
+                B_dependencies=w["B_dependencies"], gamma=w["gamma"]) for b in range(n)]
+    host = cpu_ref.Agents(ags, 16, do_fpi=False)
+    t = _median_runs(lambda: host.run(threads))
+    return {"value": n / t, "unit": "agent-steps/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} agents per run (one per core), median of {CPU_RUNS} runs after {CPU_WARM} "
+                      f"warm-ups ({t:.3f} s per run), oracle/cpu_baseline.c factor-history neg-EFE + q_pi + "
+                      f"argmax (shared inner-factor contraction at every level, OpenMP over agents, {flags})"}
+
+
 def _plain_agent_step(ag, tensors, torch):
     """One agent through the plain per-agent entry points (infer_states_ffi,
     update_posterior_policies_ffi, argmax action) at batch size 1."""
@@ -460,7 +563,8 @@ def _plain_agent_step(ag, tensors, torch):
     qs = infer_states_ffi(A, obs, D, ag["A_dependencies"], 16, distr_obs=False)
     q_pi, G = update_posterior_policies_ffi(ag["policies"], qs, A, B, C, None, ag["A_dependencies"],
                                             ag["B_dependencies"], gamma=ag["gamma"])
-    return pol0[torch.argmax(q_pi[0])]
+    # index_select keeps the choice on the device (no host sync; capturable)
+    return torch.index_select(pol0, 0, torch.argmax(q_pi[0]).view(1))[0]
 
 
 def bench_ragged(a, torch):
@@ -532,6 +636,31 @@ def bench_ragged(a, torch):
     plain_s = time.perf_counter() - t0
     plain_rate = len(sample) / plain_s
     same = all(torch.equal(pa.to(torch.int32), rb.agent_outputs(i)[3]) for pa, i in zip(plain_act, sample))
+    _log("ragged: plain per-agent calls captured per agent")
+    # the same per-agent calls timed as GPU time: each agent's call sequence
+    # captured once in a graph (after one eager call of that shape on the
+    # capture stream), all graphs replayed back to back between two events
+    cap = torch.cuda.Stream()
+    graphs = []
+    with torch.cuda.stream(cap):
+        for i, tt in zip(sample, tens):
+            _plain_agent_step(ags[i], tt, torch)
+            gi = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gi, stream=cap):
+                _plain_agent_step(ags[i], tt, torch)
+            graphs.append(gi)
+    torch.cuda.synchronize()
+    for gi in graphs:
+        gi.replay()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for gi in graphs:
+        gi.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    plain_gpu_ms = e0.elapsed_time(e1)
+    plain_gpu_rate = len(sample) / (plain_gpu_ms * 1e-3)
+    del graphs
 
     # parity of a few agents against the oracle (FPI restatement + history-dedup EFE, FP64)
     from oracle import dedup_ref as DR
@@ -552,7 +681,8 @@ def bench_ragged(a, torch):
 
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline_ragged(ags, a.cpu_seconds)
+        _log("ragged: CPU baseline")
+        cpu = cpu_baseline_ragged(ags, rb)
     info = rb.plan.info()
     return {
         "metric": "agent-steps/s (batched ragged FPI+EFE step, 4096 agents, HIP graph)",
@@ -578,9 +708,14 @@ def bench_ragged(a, torch):
                      "kernel_ms": ms_events, "alg_bytes_per_launch": alg},
         "plain_per_agent": {"value": plain_rate, "unit": "agent-steps/s",
                             "sample": f"every 16th agent ({len(sample)}), batch-1 infer_states_ffi + "
-                                      f"update_posterior_policies_ffi + argmax, {plain_s:.2f} s",
-                            "same_actions_as_batched": bool(same)},
+                                      f"update_posterior_policies_ffi + argmax, {plain_s:.2f} s wall",
+                            "same_actions_as_batched": bool(same),
+                            "gpu_time": {"value": plain_gpu_rate, "unit": "agent-steps/s",
+                                         "ms_total": plain_gpu_ms,
+                                         "how": "each agent's calls captured in one graph, the graphs replayed "
+                                                "back to back between two HIP events (no host work)"}},
         "speedup_vs_plain": (n / (ms_step * 1e-3)) / plain_rate,
+        "speedup_vs_plain_gpu_time": (n / (ms_events * 1e-3)) / plain_gpu_rate,
         "params_changing_each_step": {"ms_per_step": ms_dynamic, "value": n / (ms_dynamic * 1e-3),
                                       "note": "sum_o A ln A formed inside every step (rg_ha_kernel)"},
         "parity": {"agents_checked": 4, "max_abs_err_vs_fp64": e64, "team_barrier_timeouts": rb.status()},
@@ -589,7 +724,7 @@ def bench_ragged(a, torch):
 
 
 def bench_grad(a, torch):
-    from pymdp_amd import autograd_baseline as AB
+    from oracle import autograd_ref as AB
     from pymdp_amd import differentiable as DF
     from pymdp_amd import workloads
 
@@ -625,10 +760,27 @@ def bench_grad(a, torch):
         return (time.perf_counter() - t0) * 1e3 / k
 
     ms_hip = timed(hip, a.steps)
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(a.steps):
+        hip()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms_events = e0.elapsed_time(e1) / a.steps
     ms_ref = timed(ref, max(2, a.steps // 5))
     num = max(float((x.grad - y.grad).abs().max()) for gx, gy in zip(ph, pr) for x, y in zip(gx, gy))
     den = max(float(y.grad.abs().max()) for gy in pr for y in gy)
     B = w["batch"]
+    # algorithmic bytes of one gradient: every logit read once and its
+    # gradient written once, the observation and action trajectories read
+    n_logits = sum(int(x.numel()) for k in keys for x in w[k])
+    alg_bytes = 2 * 4 * n_logits + 4 * int(w["obs"].numel()) + 8 * int(w["actions"].numel())
+    achieved = alg_bytes / (ms_events * 1e-3) / 1e9
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline_grad(w, keys, AB, torch)
     return {
         "metric": "agent-trajectory gradients/s (512 agents x 32 steps, dL/d logits of A, B, C, D)",
         "value": B / (ms_hip * 1e-3), "unit": "agent-gradients/s", "n_gpus": 1, "steps": a.steps,
@@ -643,36 +795,73 @@ def bench_grad(a, torch):
                               "what": "torch autograd through the pure-torch FP32 forward on the same GPU"},
         "speedup_vs_autograd": ms_ref / ms_hip,
         "parity": {"max_abs_grad_diff_vs_autograd": num, "max_abs_grad": den},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "trajectory loss forward + HIP adjoints (32 time steps, all kernels of one gradient)",
+                     "kernel_ms": ms_events, "alg_bytes_per_launch": alg_bytes,
+                     "note": "32 dependent time steps of 512 small agents: latency-bound; the bytes are every "
+                             "logit read once and its gradient written once"},
+        "cpu_baseline": cpu,
     }
 
 
-def cpu_baseline_ragged(ags, seconds):
-    """The oracle on the host (one core): FPI restatement + the reference CPU
-    FFI's history-dedup neg-EFE, agents in order until ~`seconds` elapse."""
+def cpu_baseline_grad(w, keys, AB, torch, n=32):
+    """torch autograd through the pure-torch FP32 forward on the host cores
+    (torch.set_num_threads = every core of this host), on the first n
+    agents: median of 20 runs after 3 warm-ups."""
+    threads = _cpu_threads()
+    old = torch.get_num_threads()
+    torch.set_num_threads(threads)
+    try:
+        params = [[x[:n].detach().cpu().clone().requires_grad_() for x in w[k]] for k in keys]
+        obs = w["obs"][:, :n].cpu()
+        actions = w["actions"][:, :n].cpu()
+
+        def run():
+            for grp in params:
+                for x in grp:
+                    x.grad = None
+            loss = AB.trajectory_loss(*params, obs, actions, w["policies"], w["A_dependencies"],
+                                      num_iter=w["num_iter"], gamma=w["gamma"])
+            loss.backward()
+
+        t = _median_runs(run)
+    finally:
+        torch.set_num_threads(old)
+    return {"value": n / t, "unit": "agent-gradients/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} agents per run, median of {CPU_RUNS} runs after {CPU_WARM} warm-ups "
+                      f"({t:.3f} s per run), torch autograd through the FP32 forward of oracle/autograd_ref.py "
+                      f"on the host (torch.set_num_threads({threads}))"}
+
+
+def cpu_baseline_ragged(ags, rb, n=1024):
+    """The oracle's C/OpenMP restatement of the reference CPU path per agent
+    (FPI, factor-history neg-EFE, q_pi, argmax action; -march=native,
+    OpenMP over agents on every core of this host) on the first n agents:
+    median of 20 runs after 3 warm-ups.  Its actions are compared with the
+    batched GPU step's on the first 256 of them."""
     import numpy as np
 
-    from oracle import dedup_ref as DR
-    from oracle import numpy_ref as R
+    from oracle import cpu_ref
+    flags = _cpu_native()
+    threads = _cpu_threads()
     host = []
-    for ag in ags[:2048]:
-        host.append(([x.cpu().numpy() for x in ag["A"]], [x.cpu().numpy() for x in ag["B"]],
-                     [x.cpu().numpy() for x in ag["C"]], [x.cpu().numpy() for x in ag["D"]]))
-    done, t0 = 0, time.perf_counter()
-    while done < len(host) and time.perf_counter() - t0 < seconds:
-        ag = ags[done]
-        A, B, C, D = host[done]
-        ob = [np.eye(o, dtype=np.float32)[x] for o, x in zip(ag["O"], ag["obs"])]
-        qs = R.run_factorized_fpi(A, ob, D, ag["A_dependencies"], 16, np.float32)
-        G = DR.neg_efe_dedup(ag["policies"], qs, A, B, C, ag["A_dependencies"], ag["B_dependencies"],
-                             dt=np.float32)
-        q = DR.q_pi_from_G(G, ag["gamma"], dt=np.float32)
-        R.sample_policy(ag["policies"], q)
-        done += 1
-    el = time.perf_counter() - t0
-    return {"value": done / el, "unit": "agent-steps/s", "cores": 1, "kind": "port",
-            "sample": f"first {done} agents of the workload, {el:.1f} s, NumPy FP32 restatement (FPI + "
-                      f"history-dedup neg-EFE + q_pi + action), single thread"}
+    for ag in ags[:n]:
+        d = dict(ag)
+        for k in "ABCD":
+            d[k] = [x.cpu().numpy() for x in ag[k]]
+        host.append(d)
+    H = cpu_ref.Agents(host, 16, do_fpi=True)
+    t = _median_runs(lambda: H.run(threads))
+    same = True
+    for i, o in enumerate(H.out[:256]):
+        act = np.asarray(ags[i]["policies"])[int(o["best"][0]), 0, :]
+        same &= bool(np.array_equal(act, rb.agent_outputs(i)[3].cpu().numpy()))
+    return {"value": n / t, "unit": "agent-steps/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} agents per run, median of {CPU_RUNS} runs after {CPU_WARM} warm-ups "
+                      f"({t * 1e3:.1f} ms per run), oracle/cpu_baseline.c FPI + factor-history neg-EFE + q_pi + "
+                      f"argmax (OpenMP over agents, {flags})",
+            "same_actions_as_gpu_first_256": same}
 
 
 HEADLINE_METRIC = "agent-steps/s & % HBM peak: batched vs plain FPI+EFE at <=1e-5 err; grads/s"
@@ -680,25 +869,35 @@ HEADLINE_METRIC = "agent-steps/s & % HBM peak: batched vs plain FPI+EFE at <=1e-
 
 def bench_headline(a, torch):
     import copy
+    _log("ragged step")
     res = bench_ragged(a, torch)
     res["metric"] = HEADLINE_METRIC
     keep = ("value", "unit", "ms_per_step", "steps", "config", "roofline", "parity", "cpu_baseline",
             "autograd_baseline", "speedup_vs_autograd", "grads_per_s")
     b = copy.copy(a)
+    b.steps, b.warmup = 50, 5
+    _log("plain_fpi_small")
+    small = bench_plain_fpi_small(b, torch)
+    res["plain_fpi_small"] = {k: small[k] for k in keep if k in small}
+    b = copy.copy(a)
     b.steps, b.warmup = 20, 3
+    _log("plain_fpi_large")
     fpi = bench_plain_fpi_large(b, torch)
     res["plain_fpi_large"] = {k: fpi[k] for k in keep if k in fpi}
     b = copy.copy(a)
     b.steps, b.warmup = 5, 1
+    _log("plain_efe")
     efe = bench_plain_efe(b, torch)
     res["plain_efe"] = {k: efe[k] for k in keep if k in efe}
     b = copy.copy(a)
     b.steps, b.warmup = 5, 1
+    _log("grad")
     g = bench_grad(b, torch)
     res["grad"] = {k: g[k] for k in keep if k in g}
     res["grads_per_s"] = g["grads_per_s"]
     b = copy.copy(a)
     b.steps, b.warmup = 10, 2
+    _log("learning")
     lrn = bench_learning(b, torch)
     res["learning"] = {k: lrn[k] for k in keep if k in lrn}
     return res
@@ -721,6 +920,8 @@ def main():
         return
     if a.workload == "headline":
         res = bench_headline(a, torch)
+    elif a.workload == "plain_fpi_small":
+        res = bench_plain_fpi_small(a, torch)
     elif a.workload == "plain_fpi_large":
         res = bench_plain_fpi_large(a, torch)
     elif a.workload == "ragged":

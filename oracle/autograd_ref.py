@@ -1,8 +1,9 @@
 """Pure-torch forward of the trajectory loss of pymdp_amd.differentiable,
 differentiated by torch autograd: the comparison baseline for the HIP
 adjoints (the counterpart of the reference's with_jax_grad backward, which
-differentiates the JAX reference path, pymdp/ffi/_core.py:282-310).  No
-product entry point calls it.
+differentiates the JAX reference path, pymdp/ffi/_core.py:282-310), timed
+on the GPU and on the host cores.  Test and benchmark infrastructure only:
+no product entry point calls it.
 
 Same arithmetic as the reference JAX path: FPI of pymdp/algos.py:88-155
 (Jacobi sweeps from log_q = 0), neg-EFE of pymdp/control.py:604-690 at

@@ -9,7 +9,7 @@ import pytest
 import torch
 
 from oracle import grad_ref as GR
-from pymdp_amd import autograd_baseline as AB
+from oracle import autograd_ref as AB
 from pymdp_amd import differentiable as DF
 from pymdp_amd import workloads
 

@@ -1,12 +1,12 @@
 """CPU pins of the gradient oracle and the autograd reference forward:
-the pure-torch forward of pymdp_amd.autograd_baseline equals the FP64
+the pure-torch forward of oracle/autograd_ref.py equals the FP64
 NumPy restatement in oracle/grad_ref.py, and its autograd gradient equals
 FP64 central differences of that restatement (h = 1e-6)."""
 import numpy as np
 
 
 from oracle import grad_ref as GR
-from pymdp_amd import autograd_baseline as AB
+from oracle import autograd_ref as AB
 from pymdp_amd import workloads
 
 
