@@ -743,14 +743,17 @@ static int assign_slots(FpiDesc* d, int NW) {
 
 // Fast-path eligibility and launch configurations.  Candidates are
 // (N0 = padded innermost width, RPL = rows per lane, NT = workgroup size) with
-// RPL * N0 <= 64 FP64 values per lane; the default picks the fewest threads
-// that keep RPL * N0 <= 32, the autotuner times every candidate once per
-// shape (see autotune_fast).
+// RPL * N0 <= 64 FP64 values per lane and NT in {64, 128, 256, 512}; the
+// default rule is in launch_fpi, the autotuner times every candidate once
+// per shape (see autotune_fast).
 struct FastChoice {
   int N0, RPL, NT;
 };
 
 #define PMH_FAST_INSTANCES(X)                                                            \
+  X(4, 1, 64) X(4, 2, 64) X(4, 4, 64) X(4, 1, 128) X(4, 2, 128) X(4, 4, 128)                 \
+  X(8, 1, 64) X(8, 2, 64) X(8, 4, 64) X(8, 1, 128) X(8, 2, 128) X(8, 4, 128)                 \
+  X(16, 1, 64) X(16, 2, 64) X(16, 1, 128) X(16, 2, 128)                                      \
   X(4, 1, 256) X(4, 2, 256) X(4, 4, 256) X(4, 8, 256) X(4, 16, 256) X(4, 4, 512) X(4, 8, 512) \
   X(8, 1, 256) X(8, 2, 256) X(8, 4, 256) X(8, 8, 256) X(8, 2, 512) X(8, 4, 512)              \
   X(16, 1, 256) X(16, 2, 256) X(16, 4, 256) X(16, 1, 512) X(16, 2, 512)
@@ -795,7 +798,7 @@ static size_t fast_lds(const FpiDesc& d, int n0, int nt) {
 
 static std::vector<FastChoice> fast_candidates(const FpiDesc& d, int n0, long long rows0) {
   std::vector<FastChoice> c;
-  for (int nt : {256, 512}) {
+  for (int nt : {64, 128, 256, 512}) {
     int rpl = 1;
     while ((long long)rpl * nt < rows0) rpl *= 2;
     if (rpl * n0 > 64 || !has_fast_instance(n0, rpl, nt)) continue;
@@ -898,13 +901,16 @@ static int launch_fpi(const FpiDesc& d0, FpiPtrs p, int /*unused*/, int64_t batc
     if (fast_shape(&df, &n0, &rows0)) {
       const std::vector<FastChoice> cands = fast_candidates(df, n0, rows0);
       if (!cands.empty()) {
-        // default: fewest threads with <= 64 FP64 row values per lane (the
-        // candidates are ordered by thread count; on 1024 agents of 16384
-        // states the 256-thread, 64-value variant took 0.35 ms against
-        // 0.54 ms for 512 threads with 32 values each)
+        // default: the fewest of 256 / 512 threads with <= 64 FP64 row
+        // values per lane (on 1024 agents of 16384 states the 256-thread,
+        // 64-value variant took 0.35 ms against 0.54 ms for 512 threads with
+        // 32 values each; on 64 agents of 12 states one or two waves per
+        // agent were no faster than four, the sweeps' barrier-separated
+        // phases set the time); the 64 / 128-thread variants are autotuner
+        // candidates
         FastChoice fc = cands.back();
         for (const FastChoice& c : cands)
-          if (c.RPL * c.N0 <= 64) {
+          if (c.NT >= 256 && c.RPL * c.N0 <= 64) {
             fc = c;
             break;
           }
