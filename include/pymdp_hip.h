@@ -455,7 +455,8 @@ int pymdp_hip_ragged_set_params_static(void* plan, int32_t is_static);
 /* One step on `stream`: five or six launches (the sum_o A ln A table unless
  * it is valid under static parameters, FPI units, three EFE stages, per-agent
  * assembly), no allocation, no synchronisation.  Steps that overlap on one
- * device race on the plan's scratch. */
+ * device race on the plan's scratch.  `params` must be 16-byte aligned
+ * (EINVAL otherwise; device allocations are). */
 int pymdp_hip_ragged_step(void* plan, const float* params, const int32_t* obs,
                           float* out, int32_t* actions, void* stream);
 /* Times the persistent FPI grid (1, 2, ... workgroups per CU up to the

@@ -693,37 +693,54 @@ __device__ __forceinline__ void rg_gemm_mfma(const double* X, int rows, const do
   }
 }
 
+// floor(n / d) for 0 <= n, 1 <= d, n + d <= 2^22, from inv_d = 1.0f / d: the
+// quotient of n + 1/2 lies at least 1/(2d) from an integer, and the two float
+// roundings move it by at most (n/d + 1) 2^-23 < 1/(2d).
+__device__ __forceinline__ int rg_quot(int n, float inv_d) { return (int)(((float)n + 0.5f) * inv_d); }
+
 // Copies a row-major block [rows x w] (contiguous in global memory) into LDS
-// with row stride ld: every thread issues NB loads before storing, and the
-// (row, column) of its elements advance by the thread count with one carry
-// compare each (no per-element division).
-template <typename T, int NB, typename Src, int NT = kNT>
-__device__ __forceinline__ void rg_stage_rows(const Src* src, int rows, int w, int ld, T* dst, int tid) {
-  const long long n = (long long)rows * w;
+// with row stride ld, converted to T, with 16-byte vector loads: every
+// thread issues NV loads before storing any element.  The block is read as
+// vectors from the 16-byte aligned address at or below its first element,
+// so up to 16 bytes before and after the block are read (the buffers staged
+// this way are 16-byte aligned allocations with 16 bytes of tail padding)
+// and only the block's elements are stored.  n = rows * w <= 2^22 - w.
+template <typename S>
+struct rg_vec16;
+template <>
+struct rg_vec16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct rg_vec16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <typename T, int NV, typename Src, int NT = kNT>
+__device__ __forceinline__ void rg_stage_rows_v(const Src* src, int rows, int w, int ld, T* dst, int tid) {
+  constexpr int E = 16 / (int)sizeof(Src);
+  typedef typename rg_vec16<Src>::type V;
+  const int n = rows * w;
   if (n <= 0) return;
-  const int dr = NT / w, dc = NT - dr * w;
-  int r = tid / w, c = tid - r * w;
-  for (long long e0 = tid; e0 < n; e0 += (long long)NB * NT) {
-    // loads unconditional at clamped addresses, converted when stored: a load
-    // (or its conversion) under a per-lane condition would be waited for at
-    // the branch's join, one round trip per element
-    Src v[NB];
-    int off[NB];
+  const int shift = (int)((reinterpret_cast<uintptr_t>(src) / sizeof(Src)) & (E - 1));
+  const V* sv = reinterpret_cast<const V*>(src - shift);
+  const int nv = (shift + n + E - 1) / E;
+  const float inv_w = 1.0f / (float)w;
+  for (int j0 = tid; j0 < nv; j0 += NV * NT) {
+    V v[NV];
 #pragma unroll
-    for (int q = 0; q < NB; ++q) {
-      const long long e = e0 + (long long)q * NT;
-      off[q] = r * ld + c;
-      v[q] = src[e < n ? e : n - 1];
-      c += dc;
-      r += dr;
-      if (c >= w) {
-        c -= w;
-        ++r;
+    for (int q = 0; q < NV; ++q) v[q] = sv[min(j0 + q * NT, nv - 1)];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const int j = j0 + q * NT;
+#pragma unroll
+      for (int c = 0; c < E; ++c) {
+        const int e = j * E + c - shift;
+        if (j < nv && e >= 0 && e < n) {
+          const int r = rg_quot(e, inv_w);
+          dst[r * ld + (e - r * w)] = (T)v[q][c];
+        }
       }
     }
-#pragma unroll
-    for (int q = 0; q < NB; ++q)
-      if (e0 + (long long)q * NT < n) dst[off[q]] = (T)v[q];
   }
 }
 
@@ -743,7 +760,7 @@ __device__ __forceinline__ void rg_load_hist(const Agent& a, const double* hist,
     return;
   }
   const double* src = hist + a.hist_lvl[t][f] + (long long)c0 * a.S[f];
-  rg_stage_rows<double, (kCols * 64 + NT - 1) / NT, double, NT>(src, c1 - c0, d, ld, Q, tid);
+  rg_stage_rows_v<double, (kCols * 64 / 2 + 1 + NT - 1) / NT, double, NT>(src, c1 - c0, d, ld, Q, tid);
 }
 
 
@@ -779,10 +796,6 @@ __device__ __forceinline__ int rg_xcd_tile(int w, int n) {
   return ((i / kXcdChunk) * kXcds + x) * kXcdChunk + (i % kXcdChunk);
 }
 
-// floor(n / d) for 0 <= n, 1 <= d, n + d <= 2^22, from inv_d = 1.0f / d: the
-// quotient of n + 1/2 lies at least 1/(2d) from an integer, and the two float
-// roundings move it by at most (n/d + 1) 2^-23 < 1/(2d).
-__device__ __forceinline__ int rg_quot(int n, float inv_d) { return (int)(((float)n + 0.5f) * inv_d); }
 
 __host__ __device__ __forceinline__ int rg_e1_rows(int O, int d2, int d01) {
   // (i, j) positions of one tile: the A block and the sum_o A ln A rows fit
@@ -793,100 +806,105 @@ __host__ __device__ __forceinline__ int rg_e1_rows(int O, int d2, int d01) {
   return nr < d01 ? nr : d01;
 }
 
-// NT threads per tile: 512 (eight waves) puts twice the loads of a tile in
-// flight and, at three tiles per CU (the LDS bound), six waves per SIMD.
-constexpr int kNT1 = 512;
-template <int NT>
-__global__ __launch_bounds__(NT) void rg_e1_kernel(Bufs b) {
-  constexpr int NW = NT / kWave;
-  __shared__ float s_a[kE1A];
-  __shared__ double s_h[kE1H];
-  __shared__ double s_q[kE1Q];
-  __shared__ E1Rec s_rec;
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
-  const int tile = rg_xcd_tile(blockIdx.x, gridDim.x);
-  long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_units + tile) : nullptr;
-  if (tr && tid == 0) tr[0] = wall_clock64();
-  rg_copy_lds(b.e1 + tile, &s_rec, tid);
-  lds_barrier();
-  const E1Rec& rc = s_rec;
+// Stage-1 tile staging: the tile's A block (floats), its rows of the
+// sum_o A ln A table and its history columns (doubles) into LDS, by NP
+// threads (ptid in [0, NP)).  A_m is position-major, so the tile's rows
+// (position rl, outcome o) are one contiguous chunk: element e = (rl, o, k)
+// at Am[e], LDS row o nr + rl.  All loads are unconditional (past the
+// block's end they repeat its last element and are not stored) and issued
+// back to back before any LDS store.
+template <int NP>
+__device__ __forceinline__ void rg_e1_stage(const Bufs& b, const E1Rec& rc, float* s_a, double* s_h, double* s_q,
+                                            int ptid) {
+  const int O = rc.O, d2 = rc.d2, nr = rc.nr;
+  const int d2p = (d2 + 3) & ~3, lda = d2p + 1;
+  const float inv_d2 = 1.0f / (float)d2, inv_d2p = 1.0f / (float)d2p;
+  // the A block as 16-byte vectors from the aligned address at or below its
+  // first element (the arena is 16-byte aligned, agent blocks start at
+  // multiples of four floats and an A block is followed by at least four
+  // floats of the same agent, so every vector stays in the arena); vector j
+  // holds elements 4 j - shift .. 4 j - shift + 3 of the block
+  constexpr int NV = (kE1A / 4 + 1 + NP - 1) / NP;
+  constexpr int NQ = (16 * 64 + NP - 1) / NP;
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const int per = O * d2;  // floats per position
+  const int tot = nr * per;
+  const int shift = (int)(rc.a_off & 3);
+  const f4* Av = reinterpret_cast<const f4*>(b.params + (rc.a_off - shift));
+  const int nv = (shift + tot + 3) >> 2;
+  const float inv_per = 1.0f / (float)per;
+  f4 v[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) v[q] = __builtin_nontemporal_load(Av + min(ptid + q * NP, nv - 1));
+  // the tile's sum_o A ln A rows: [nr][d2] contiguous in the table
+  constexpr int NH = (kE1H + NP - 1) / NP;
+  const double* Hr = b.ha + rc.h_off;
+  const int toth = nr * d2;
+  double hh[NH];
+#pragma unroll
+  for (int q = 0; q < NH; ++q) hh[q] = Hr[min(ptid + q * NP, toth - 1)];
+  double hv[NQ];
+  bool hok[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int e = ptid + q * NP, cq = rg_quot(e, inv_d2p), c = min(cq, 15), kk = e - cq * d2p;
+    const long long cb = rc.cb[c];
+    hok[q] = e < 16 * d2p && cb >= 0 && kk < d2;
+    hv[q] = b.hist[hok[q] ? cb + kk : 0];
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q)
+    if (!hok[q]) hv[q] = 0.0;
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const int j = ptid + q * NP;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int e = 4 * j + c - shift;
+      if (j < nv && e >= 0 && e < tot) {
+        const int rl = rg_quot(e, inv_per), rem = e - rl * per;
+        const int o = rg_quot(rem, inv_d2), k = rem - o * d2;
+        s_a[(o * nr + rl) * lda + k] = v[q][c];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int e = ptid + q * NP, c = rg_quot(e, inv_d2p), kk = e - c * d2p;
+    if (e < 16 * d2p) s_q[c * lda + kk] = hv[q];
+  }
+#pragma unroll
+  for (int q = 0; q < NH; ++q) {
+    const int e = ptid + q * NP, rl2 = rg_quot(e, inv_d2), k2 = e - rl2 * d2;
+    if (e < toth) s_h[rl2 * lda + k2] = hh[q];
+  }
+  // zero the padding columns [d2, d2p) of every A and sum_o A ln A row, so
+  // the product's k loop reads them unconditionally (the history block's
+  // padding columns are zero too)
+  if (d2p > d2) {
+    const int npad = d2p - d2;
+    for (int e = ptid; e < (O + 1) * nr * npad; e += NP) {
+      const int r = e / npad, k2 = d2 + (e - r * npad);
+      if (r < O * nr)
+        s_a[r * lda + k2] = 0.0f;
+      else
+        s_h[(r - O * nr) * lda + k2] = 0.0;
+    }
+  }
+}
+
+// Stage-1 tile product from LDS by NW waves (wave index cw in [0, NW)):
+// MFMA over 16-row blocks of the O1 * nr rows, two blocks per wave at a
+// time (independent accumulators hide the MFMA latency).  The k loop has no
+// branches and no per-lane condition: each lane reads its row of both
+// operand blocks at clamped rows and selects between them; the padding
+// columns [d2, d2p) of X and Q are zero in LDS.
+template <int NW>
+__device__ __forceinline__ void rg_e1_mfma(const Bufs& b, const E1Rec& rc, const float* s_a, const double* s_h,
+                                           const double* s_q, int lane, int cw) {
   const int O = rc.O, O1 = O + 1, d01 = rc.d01, d2 = rc.d2;
   const int d2p = (d2 + 3) & ~3, lda = d2p + 1;
   const int nr = rc.nr, nc = rc.nc;
-  const float inv_d2 = 1.0f / (float)d2, inv_d2p = 1.0f / (float)d2p;
-  {
-    // A_m is position-major, so the tile's rows (position rl, outcome o) are
-    // one contiguous chunk: element e = (rl, o, k) at Am[e], LDS row o nr + rl.
-    // The history columns are loaded in the same round trip.
-    const float* Am = b.params + rc.a_off;
-    constexpr int NB = (kE1A + NT - 1) / NT;
-    constexpr int NQ = (16 * 64 + NT - 1) / NT;
-    const int per = O * d2;  // floats per position
-    const int tot = nr * per;
-    const int drl = NT / per, dm = NT - drl * per;  // per-step advance of (rl, rem)
-    const int dq = dm / d2, dk = dm - dq * d2;
-    int rl = tid / per, rem = tid - rl * per;
-    int o = rg_quot(rem, inv_d2), k = rem - o * d2;
-    // loads are unconditional (past the block's end they repeat its last
-    // element and are not stored), so all of them are issued back to back
-    float v[NB];
-    int off[NB];
-#pragma unroll
-    for (int q = 0; q < NB; ++q) {
-      const int e = tid + q * NT;
-      off[q] = (o * nr + rl) * lda + k;
-      v[q] = __builtin_nontemporal_load(Am + min(e, tot - 1));
-      k += dk;
-      o += dq;
-      if (k >= d2) {
-        k -= d2;
-        ++o;
-      }
-      rl += drl;
-      if (o >= O) {
-        o -= O;
-        ++rl;
-      }
-    }
-    // the tile's sum_o A ln A rows: [nr][d2] contiguous in the table
-    constexpr int NH = (kE1H + NT - 1) / NT;
-    const double* Hr = b.ha + rc.h_off;
-    const int toth = nr * d2;
-    double hh[NH];
-#pragma unroll
-    for (int q = 0; q < NH; ++q) hh[q] = Hr[min(tid + q * NT, toth - 1)];
-    double hv[NQ];
-    bool hok[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int e = tid + q * NT, cq = rg_quot(e, inv_d2p), c = min(cq, 15), kk = e - cq * d2p;
-      const long long cb = rc.cb[c];
-      hok[q] = e < 16 * d2p && cb >= 0 && kk < d2;
-      hv[q] = b.hist[hok[q] ? cb + kk : 0];
-    }
-#pragma unroll
-    for (int q = 0; q < NQ; ++q)
-      if (!hok[q]) hv[q] = 0.0;
-#pragma unroll
-    for (int q = 0; q < NB; ++q)
-      if (tid + q * NT < tot) s_a[off[q]] = v[q];
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-      const int e = tid + q * NT, c = rg_quot(e, inv_d2p), kk = e - c * d2p;
-      if (e < 16 * d2p) s_q[c * lda + kk] = hv[q];
-    }
-#pragma unroll
-    for (int q = 0; q < NH; ++q) {
-      const int e = tid + q * NT, rl = rg_quot(e, inv_d2), k = e - rl * d2;
-      if (e < toth) s_h[rl * lda + k] = hh[q];
-    }
-  }
-  lds_barrier();
-  // MFMA over 16-row blocks of the O1 * nr rows, two blocks per wave at a
-  // time (independent accumulators hide the MFMA latency).  The k loop has
-  // no branches: each lane reads its row of both operand blocks at clamped
-  // addresses and selects; columns k >= d2 of X are zeroed (the LDS padding
-  // is never written) as are those of Q.
   const int R = O1 * nr, RA = O * nr;
   const int nblk = (R + 15) >> 4;
   const int li = lane & 15, lk = lane >> 4;
@@ -894,7 +912,7 @@ __global__ __launch_bounds__(NT) void rg_e1_kernel(Bufs b) {
   // this lane's output column (history c0 + li) base in Z1
   float* zc = b.z1 + rc.z_off + (long long)(rc.c0 + li) * O1 * d01;
   const double* qrow = s_q + li * lda + lk;
-  for (int blk = wave; blk < nblk; blk += 2 * NW) {
+  for (int blk = cw; blk < nblk; blk += 2 * NW) {
     const int blk1 = blk + NW;  // may be past the last block: computed, not stored
     const int r0 = (blk << 4) + li, r1 = (blk1 << 4) + li;
     const float* xa0 = s_a + (r0 < RA ? r0 : 0) * lda + lk;
@@ -903,15 +921,23 @@ __global__ __launch_bounds__(NT) void rg_e1_kernel(Bufs b) {
     const double* xh1 = s_h + (r1 >= RA && r1 < R ? r1 - RA : 0) * lda + lk;
     const bool h0 = r0 >= RA, h1 = r1 >= RA;
     rg_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < d2p; k0 += 4) {
-      const bool kv = (k0 + lk) < d2;
+    // two k steps per iteration: their LDS reads are in flight together
+    int k0 = 0;
+    for (; k0 + 4 < d2p; k0 += 8) {
+      const double a0 = (double)xa0[k0], g0 = xh0[k0], a0b = (double)xa0[k0 + 4], g0b = xh0[k0 + 4];
+      const double a1 = (double)xa1[k0], g1 = xh1[k0], a1b = (double)xa1[k0 + 4], g1b = xh1[k0 + 4];
+      const double q = qrow[k0], qb = qrow[k0 + 4];
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(h0 ? g0 : a0, q, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(h1 ? g1 : a1, q, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(h0 ? g0b : a0b, qb, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(h1 ? g1b : a1b, qb, acc1, 0, 0, 0);
+    }
+    if (k0 < d2p) {
       const double a0 = (double)xa0[k0], g0 = xh0[k0];
       const double a1 = (double)xa1[k0], g1 = xh1[k0];
       const double q = qrow[k0];
-      const double x0 = kv ? (h0 ? g0 : a0) : 0.0;
-      const double x1 = kv ? (h1 ? g1 : a1) : 0.0;
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, q, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, q, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(h0 ? g0 : a0, q, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(h1 ? g1 : a1, q, acc1, 0, 0, 0);
     }
     if (li < nc) {
 #pragma unroll
@@ -929,6 +955,28 @@ __global__ __launch_bounds__(NT) void rg_e1_kernel(Bufs b) {
       }
     }
   }
+}
+
+// One tile per workgroup of NT threads: every thread stages, then every
+// wave multiplies.  512 threads (eight waves) put twice the loads of a tile
+// in flight and, at three tiles per CU (the LDS bound), six waves per SIMD.
+constexpr int kNT1 = 512;
+template <int NT>
+__global__ __launch_bounds__(NT) void rg_e1_kernel(Bufs b) {
+  constexpr int NW = NT / kWave;
+  __shared__ float s_a[kE1A];
+  __shared__ double s_h[kE1H];
+  __shared__ double s_q[kE1Q];
+  __shared__ E1Rec s_rec;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
+  const int tile = rg_xcd_tile(blockIdx.x, gridDim.x);
+  long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_units + tile) : nullptr;
+  if (tr && tid == 0) tr[0] = wall_clock64();
+  rg_copy_lds(b.e1 + tile, &s_rec, tid);
+  lds_barrier();
+  rg_e1_stage<NT>(b, s_rec, s_a, s_h, s_q, tid);
+  lds_barrier();
+  rg_e1_mfma<NW>(b, s_rec, s_a, s_h, s_q, lane, wave);
   if (tr && tid == 0) tr[1] = wall_clock64();
 }
 
@@ -976,7 +1024,7 @@ __global__ __launch_bounds__(kNT2) void rg_e2_kernel(Bufs b) {
   const int nr = tl.r1 - tl.r0, nc = tl.c1 - tl.c0;
   const int H1 = rg_hcount(a, m, t, 1);
   const float* src = b.z1 + a.z1_off[m] + ((long long)a.tstart[m][t] * O1 * d0 + tl.r0) * d1;
-  rg_stage_rows<double, (kTile2 + kNT2 - 1) / kNT2, float, kNT2>(src, nr, d1, ld1, s_t, tid);
+  rg_stage_rows_v<double, (kTile2 / 4 + 1 + kNT2 - 1) / kNT2, float, kNT2>(src, nr, d1, ld1, s_t, tid);
   rg_load_hist<kNT2>(a, b.hist + a.hist_off, m, t, 1, tl.c0, tl.c1, s_q, tid);
   __syncthreads();
   float* Z = b.z2 + a.z2_off[m][t];
@@ -1010,7 +1058,7 @@ __global__ __launch_bounds__(kNT3) void rg_e3_kernel(Bufs b) {
   // a rank-1 modality's stage-1 output already has the stage-2 layout
   const float* zb = a.gR[a.mod_group[m]] == 1 ? b.z1 : b.z2;
   const float* src = zb + a.z2_off[m][t] + (long long)tl.r0 * O1 * d0;
-  rg_stage_rows<double, (kTile + kNT3 - 1) / kNT3, float, kNT3>(src, np * O1, d0, ld0, s_t, tid);
+  rg_stage_rows_v<double, (kTile / 4 + 1 + kNT3 - 1) / kNT3, float, kNT3>(src, np * O1, d0, ld0, s_t, tid);
   rg_load_hist<kNT3>(a, b.hist + a.hist_off, m, t, 0, tl.c0, tl.c1, s_q, tid);
   const float* Cm = b.params + a.C_off[m] + (a.C_time[m] ? (long long)t * O : 0);
   const float c_lo = lane < O ? Cm[lane] : 0.0f;
@@ -1701,11 +1749,13 @@ static int upload(Plan* pl) {
   if ((e = alloc((void**)&pl->d_t3, sizeof(Tile) * pl->t3.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_iarena, sizeof(int32_t) * pl->iarena.size())) != hipSuccess)
     return fail_dev(e, "hipMalloc");
-  if ((e = alloc((void**)&pl->d_hist, sizeof(double) * pl->hist_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  // 16 bytes of tail padding: stages read these buffers as 16-byte vectors
+  if ((e = alloc((void**)&pl->d_hist, sizeof(double) * pl->hist_total + 16)) != hipSuccess)
+    return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_score, sizeof(double) * pl->score_total)) != hipSuccess)
     return fail_dev(e, "hipMalloc");
-  if ((e = alloc((void**)&pl->d_z1, sizeof(float) * pl->z1_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
-  if ((e = alloc((void**)&pl->d_z2, sizeof(float) * pl->z2_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_z1, sizeof(float) * pl->z1_total + 16)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_z2, sizeof(float) * pl->z2_total + 16)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_part, sizeof(double) * pl->part_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_cnt, sizeof(int) * 4)) != hipSuccess) return fail_dev(e, "hipMalloc");
   auto up = [&](void* d, const void* h, size_t bytes) {
@@ -1863,6 +1913,7 @@ extern "C" int pymdp_hip_ragged_step(void* plan, const float* params, const int3
   if (!pl) return einval("plan is NULL");
   if (!pl->d_ag) return einval("plan is not uploaded");
   if (!params || !obs || !out || !act) return einval("arena pointer is NULL");
+  if (reinterpret_cast<uintptr_t>(params) % 16 != 0) return einval("the params arena must be 16-byte aligned");
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev != pl->device) return einval("plan was created on another device");
   if (ensure_log_table()) {
@@ -1881,6 +1932,8 @@ extern "C" int pymdp_hip_ragged_autotune(void* plan, const float* params, const 
   Plan* pl = static_cast<Plan*>(plan);
   if (!pl) return einval("plan is NULL");
   if (!pl->d_ag) return einval("plan is not uploaded");
+  if (!params || !obs || !out || !act) return einval("arena pointer is NULL");
+  if (reinterpret_cast<uintptr_t>(params) % 16 != 0) return einval("the params arena must be 16-byte aligned");
   hipStream_t s = static_cast<hipStream_t>(stream);
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
