@@ -701,6 +701,7 @@ def bench_ragged(a, torch):
                    "num_iter": w["num_iter"], "gamma": ags[0]["gamma"], "fpi_units": info["units"],
                    "fpi_team_max": info["team_max"], "fpi_grid": info["grid_fpi"],
                    "efe_tiles": [info["tiles1"], info["tiles2"], info["tiles3"]],
+                   "tile_sizes": rb.plan.tile_config(),
                    "efe_macs_per_step": info["efe_macs"], "fpi_macs_per_step": info["fpi_macs"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("ragged step"),

@@ -459,11 +459,25 @@ int pymdp_hip_ragged_set_params_static(void* plan, int32_t is_static);
  * (EINVAL otherwise; device allocations are). */
 int pymdp_hip_ragged_step(void* plan, const float* params, const int32_t* obs,
                           float* out, int32_t* actions, void* stream);
-/* Times the persistent FPI grid (1, 2, ... workgroups per CU up to the
- * occupancy limit) and keeps the fastest; not while `stream` is being
- * captured. */
+/* The plan's autotuner (no counterpart in the reference): times the
+ * persistent FPI grid (1, 2, ... workgroups per CU up to the occupancy
+ * limit), then the EFE tile sizes one stage at a time (stage-1 A floats per
+ * tile 4352 / 6528 / 8704, stage-2 and stage-3 doubles per tile 2304 / 4608,
+ * stage-3 combinations 64 / 128, histories per stage-2 / 3 tile 16 / 32),
+ * each over whole steps, and keeps the fastest; *best_ms = that step time.
+ * Not while `stream` is being captured; graphs captured before the call
+ * must be captured again (the tile tables are rewritten). */
 int pymdp_hip_ragged_autotune(void* plan, const float* params, const int32_t* obs,
                               float* out, int32_t* actions, void* stream, float* best_ms);
+/* out8 = (stage-1 A floats per tile, stage-2 doubles per tile, stage-3
+ *         doubles per tile, stage-3 combinations per tile, histories per
+ *         stage-2 / 3 tile, 1 if autotuned, FPI grid, 0). */
+int pymdp_hip_ragged_tile_config(void* plan, int32_t* out8);
+/* Sets the EFE tile sizes (cfg5 = the first five values of
+ * pymdp_hip_ragged_tile_config, each within the autotuner's range; EINVAL
+ * otherwise) and rewrites the tile tables.  Results do not depend on the
+ * tiling.  Not while a graph that replays the plan may run. */
+int pymdp_hip_ragged_set_tiles(void* plan, const int32_t* cfg5);
 /* *flag_out = 1 if a team exchange (the per-sweep wait of a large agent's
  * workgroups for each other's partial marginals) of an earlier step gave up
  * waiting (the posteriors of that agent are NaN), else 0.  Synchronises the

@@ -797,10 +797,11 @@ __device__ __forceinline__ int rg_xcd_tile(int w, int n) {
 }
 
 
-__host__ __device__ __forceinline__ int rg_e1_rows(int O, int d2, int d01) {
-  // (i, j) positions of one tile: the A block and the sum_o A ln A rows fit
+__host__ __device__ __forceinline__ int rg_e1_rows(int O, int d2, int d01, int abudget) {
+  // (i, j) positions of one tile: the A block (at most abudget <= kE1A LDS
+  // floats) and the sum_o A ln A rows fit
   const int lda = ((d2 + 3) & ~3) + 1;
-  int nr = kE1A / (O * lda);
+  int nr = abudget / (O * lda);
   nr = nr < kE1H / lda ? nr : kE1H / lda;
   if (nr < 1) nr = 1;
   return nr < d01 ? nr : d01;
@@ -1219,8 +1220,26 @@ __global__ __launch_bounds__(kNT) void rg_assemble_kernel(Bufs b) {
 // ---------------------------------------------------------------------------
 // Host plan
 // ---------------------------------------------------------------------------
+// EFE tile sizes (see build_tiles); the defaults are the LDS capacities
+struct TileCfg {
+  int e1_a = kE1A;            // A floats per stage-1 tile
+  int e2_t = kTile2;          // Z1 doubles per stage-2 tile
+  int e3_t = kTile;           // Z2 doubles per stage-3 tile
+  int e3_combos = kE3Combos;  // history combinations scored per stage-3 tile
+  int cols = kCols;           // histories per stage-2 / stage-3 tile
+};
+
+// autotuner candidates (each at most its LDS capacity)
+constexpr int kTuneE1A[] = {4352, 6528, kE1A};
+constexpr int kTuneE2T[] = {2304, kTile2};
+constexpr int kTuneE3T[] = {2304, kTile};
+constexpr int kTuneCombos[] = {64, kE3Combos};
+constexpr int kTuneCols[] = {16, kCols};
+
 struct Plan {
   int device = 0;
+  TileCfg tiles;                        // the tiling in use
+  size_t cap_t1 = 0, cap_t2 = 0, cap_t3 = 0;  // device tile-array capacities
   int64_t n_agents = 0, sumF = 0, sumM = 0;
   std::vector<Agent> agents;
   std::vector<Unit> units;
@@ -1288,6 +1307,98 @@ static double efe_cost(const std::vector<int>& dims, const std::vector<int>& Hs,
 using namespace pmh;
 using namespace pmh::rg;
 
+// EFE tiling from the agents' layouts.  Stage 1: blocks of (i, j) positions
+// whose A rows (at most e1_a LDS floats) and sum_o A ln A rows fit the
+// tile's LDS x blocks of at most 16 concatenated last-dim histories; stage
+// 2: blocks of Z1 rows (at most e2_t LDS doubles) x at most `cols` middle-dim
+// histories of one level; stage 3: blocks of (h2, h1) pairs (at most e3_t
+// LDS doubles and e3_combos scored combinations) x at most `cols` first-dim
+// histories.  Every tile list is ordered by decreasing cost.
+static void build_tiles(Plan* pl, const TileCfg& cfg) {
+  struct TileC {
+    Tile t;
+    double cost;
+  };
+  std::vector<TileC> t1, t2, t3;
+  pl->efe_macs = 0.0;
+  for (int ai = 0; ai < (int)pl->agents.size(); ++ai) {
+    const Agent& a = pl->agents[ai];
+    for (int m = 0; m < a.M; ++m) {
+      const int R = a.gR[a.mod_group[m]];
+      const int O1 = a.O[m] + 1;
+      const int d0 = a.ed[m][0], d1 = a.ed[m][1], d2 = a.ed[m][2];
+      auto Hc = [&](int t, int p) { return a.ef[m][p] < 0 ? 1 : a.H[t][a.ef[m][p]]; };
+      const int N1 = a.tstart[m][a.T];
+      const int d01 = d0 * d1;
+      const int nr1 = rg_e1_rows(a.O[m], d2, d01, cfg.e1_a);
+      for (int r0 = 0; r0 < d01; r0 += nr1)
+        for (int c0 = 0; c0 < N1; c0 += 16) {
+          const int r1 = std::min(d01, r0 + nr1), c1 = std::min(N1, c0 + 16);
+          t1.push_back(TileC{Tile{ai, m, 0, r0, r1, c0, c1, 0}, (double)(r1 - r0) * O1 * d2 * 16});
+        }
+      pl->efe_macs += (double)d01 * O1 * d2 * N1;
+      for (int t = 0; t < a.T; ++t) {
+        const int H0 = Hc(t, 0), H1 = Hc(t, 1), H2 = Hc(t, 2);
+        if (R != 1) {
+          const int rows = H2 * O1 * d0, ld1 = d1 | 1;
+          const int nr = std::max(1, std::min(rows, cfg.e2_t / ld1));
+          for (int r0 = 0; r0 < rows; r0 += nr)
+            for (int c0 = 0; c0 < H1; c0 += cfg.cols) {
+              const int r1 = std::min(rows, r0 + nr), c1 = std::min(H1, c0 + cfg.cols);
+              t2.push_back(TileC{Tile{ai, m, t, r0, r1, c0, c1, 0}, (double)(r1 - r0) * d1 * (c1 - c0)});
+            }
+          pl->efe_macs += (double)rows * d1 * H1;
+        }
+        // E3 tiles: (h2, h1) pairs x first-dim histories
+        const int pairs = H2 * H1, ld0 = d0 | 1;
+        const int hb = std::min(H0, cfg.cols);
+        const int np = std::max(1, std::min({pairs, cfg.e3_t / (O1 * ld0), std::max(1, cfg.e3_combos / hb)}));
+        for (int r0 = 0; r0 < pairs; r0 += np)
+          for (int c0 = 0; c0 < H0; c0 += cfg.cols) {
+            const int r1 = std::min(pairs, r0 + np), c1 = std::min(H0, c0 + cfg.cols);
+            t3.push_back(TileC{Tile{ai, m, t, r0, r1, c0, c1, 0}, (double)(r1 - r0) * (c1 - c0) * O1 * d0});
+          }
+        pl->efe_macs += (double)pairs * H0 * O1 * d0;
+      }
+    }
+  }
+  auto by_cost = [](const TileC& x, const TileC& y) { return x.cost > y.cost; };
+  for (auto* v : {&t1, &t2, &t3}) std::stable_sort(v->begin(), v->end(), by_cost);
+  pl->t1.clear();
+  pl->t2.clear();
+  pl->t3.clear();
+  pl->e1.clear();
+  for (auto& x : t1) pl->t1.push_back(x.t);
+  for (const Tile& tl : pl->t1) {
+    const Agent& a = pl->agents[tl.agent];
+    const int m = tl.m, f = a.ef[m][2];
+    E1Rec r;
+    std::memset(&r, 0, sizeof(r));
+    r.O = a.O[m];
+    r.d2 = a.ed[m][2];
+    r.d01 = a.ed[m][0] * a.ed[m][1];
+    r.nr = tl.r1 - tl.r0;
+    r.nc = tl.c1 - tl.c0;
+    r.c0 = tl.c0;
+    r.a_off = a.A_off[m] + (long long)tl.r0 * a.O[m] * a.ed[m][2];
+    r.h_off = a.ha_off[m] + (long long)tl.r0 * a.ed[m][2];
+    r.z_off = a.z1_off[m] + tl.r0;
+    for (int c = 0; c < 16; ++c) {
+      r.cb[c] = -1;
+      if (c < r.nc) {
+        const int n = tl.c0 + c;
+        int t = 0;
+        while (t + 1 < a.T && n >= a.tstart[m][t + 1]) ++t;
+        r.cb[c] = a.hist_off + a.hist_lvl[t][f] + (long long)(n - a.tstart[m][t]) * a.S[f];
+      }
+    }
+    pl->e1.push_back(r);
+  }
+  for (auto& x : t2) pl->t2.push_back(x.t);
+  for (auto& x : t3) pl->t3.push_back(x.t);
+  pl->tiles = cfg;
+}
+
 extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr, const int64_t* S_flat,
                                              const int64_t* U_flat, const int64_t* O_flat,
                                              const int64_t* A_dep_rank, const int64_t* A_dep_flat,
@@ -1311,11 +1422,6 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
     double cost;
   };
   std::vector<UnitC> teams, singles;
-  struct TileC {
-    Tile t;
-    double cost;
-  };
-  std::vector<TileC> t1, t2, t3;
   for (int64_t ai = 0; ai < n_agents; ++ai) {
     const int64_t F = hdr[4 * ai], M = hdr[4 * ai + 1], T = hdr[4 * ai + 2], P = hdr[4 * ai + 3];
     auto fail = [&](const std::string& m) {
@@ -1596,19 +1702,12 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
       // columns; the modality's sum_o A ln A table ([pos][d2], rg_ha_kernel)
       {
         const int d01 = d0 * d1;
-        const int nr = rg_e1_rows(a.O[m], d2, d01);
         if (a.O[m] * (((d2 + 3) & ~3) + 1) > kE1A) return fail("EFE tile does not fit");
         const long long nel = (long long)d01 * d2;
         a.ha_off[m] = pl->ha_total;
         for (long long e0 = 0; e0 < nel; e0 += kNT)
           pl->hat.push_back(HaTask{a.A_off[m], pl->ha_total, a.O[m], d2, (int)nel, (int)e0});
         pl->ha_total += nel;
-        for (int r0 = 0; r0 < d01; r0 += nr)
-          for (int c0 = 0; c0 < N1; c0 += 16) {
-            const int r1 = std::min(d01, r0 + nr), c1 = std::min(N1, c0 + 16);
-            t1.push_back(TileC{Tile{(int)ai, m, 0, r0, r1, c0, c1, 0}, (double)(r1 - r0) * O1 * d2 * 16});
-          }
-        pl->efe_macs += (double)d01 * O1 * d2 * N1;
       }
       for (int t = 0; t < T; ++t) {
         const int H0 = Hc(t, 0), H1 = Hc(t, 1), H2 = Hc(t, 2);
@@ -1618,26 +1717,8 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
         } else {
           a.z2_off[m][t] = z2;
           z2 += (long long)H2 * H1 * O1 * d0;
-          const int rows = H2 * O1 * d0, ld1 = d1 | 1;
-          const int nr = std::max(1, std::min(rows, kTile2 / ld1));
-          for (int r0 = 0; r0 < rows; r0 += nr)
-            for (int c0 = 0; c0 < H1; c0 += kCols) {
-              const int r1 = std::min(rows, r0 + nr), c1 = std::min(H1, c0 + kCols);
-              t2.push_back(TileC{Tile{(int)ai, m, t, r0, r1, c0, c1, 0}, (double)(r1 - r0) * d1 * (c1 - c0)});
-            }
-          pl->efe_macs += (double)rows * d1 * H1;
         }
-        // E3 tiles: (h2, h1) pairs x first-dim histories
-        const int pairs = H2 * H1, ld0 = d0 | 1;
-        if (O1 * ld0 > kTile) return fail("EFE tile does not fit");
-        const int hb = std::min(H0, kCols);
-        const int np = std::max(1, std::min({pairs, kTile / (O1 * ld0), std::max(1, kE3Combos / hb)}));
-        for (int r0 = 0; r0 < pairs; r0 += np)
-          for (int c0 = 0; c0 < H0; c0 += kCols) {
-            const int r1 = std::min(pairs, r0 + np), c1 = std::min(H0, c0 + kCols);
-            t3.push_back(TileC{Tile{(int)ai, m, t, r0, r1, c0, c1, 0}, (double)(r1 - r0) * (c1 - c0) * O1 * d0});
-          }
-        pl->efe_macs += (double)pairs * H0 * O1 * d0;
+        if (O1 * ((d0 | 1)) > kTile) return fail("EFE tile does not fit");
       }
     }
     pl->totS_max = std::max(pl->totS_max, a.totS);
@@ -1653,36 +1734,7 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
     for (auto& u : v.units) pl->units.push_back(u);
   for (auto& v : singles)
     for (auto& u : v.units) pl->units.push_back(u);
-  auto by_cost = [](const TileC& x, const TileC& y) { return x.cost > y.cost; };
-  for (auto* v : {&t1, &t2, &t3}) std::stable_sort(v->begin(), v->end(), by_cost);
-  for (auto& x : t1) pl->t1.push_back(x.t);
-  for (const Tile& tl : pl->t1) {
-    const Agent& a = pl->agents[tl.agent];
-    const int m = tl.m, f = a.ef[m][2];
-    E1Rec r;
-    std::memset(&r, 0, sizeof(r));
-    r.O = a.O[m];
-    r.d2 = a.ed[m][2];
-    r.d01 = a.ed[m][0] * a.ed[m][1];
-    r.nr = tl.r1 - tl.r0;
-    r.nc = tl.c1 - tl.c0;
-    r.c0 = tl.c0;
-    r.a_off = a.A_off[m] + (long long)tl.r0 * a.O[m] * a.ed[m][2];
-    r.h_off = a.ha_off[m] + (long long)tl.r0 * a.ed[m][2];
-    r.z_off = a.z1_off[m] + tl.r0;
-    for (int c = 0; c < 16; ++c) {
-      r.cb[c] = -1;
-      if (c < r.nc) {
-        const int n = tl.c0 + c;
-        int t = 0;
-        while (t + 1 < a.T && n >= a.tstart[m][t + 1]) ++t;
-        r.cb[c] = a.hist_off + a.hist_lvl[t][f] + (long long)(n - a.tstart[m][t]) * a.S[f];
-      }
-    }
-    pl->e1.push_back(r);
-  }
-  for (auto& x : t2) pl->t2.push_back(x.t);
-  for (auto& x : t3) pl->t3.push_back(x.t);
+  build_tiles(pl, TileCfg{});
   if (pl->t1.size() > INT32_MAX || pl->t2.size() > INT32_MAX || pl->t3.size() > INT32_MAX) {
     delete pl;
     return einval("too many EFE tiles");
@@ -1713,6 +1765,20 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
   return PYMDP_HIP_OK;
 }
 
+// Copies the host tile lists (within the device capacities) to the device.
+static hipError_t upload_tiles(Plan* pl) {
+  if (pl->t1.size() > pl->cap_t1 || pl->t2.size() > pl->cap_t2 || pl->t3.size() > pl->cap_t3)
+    return hipErrorInvalidValue;
+  auto up = [&](void* d, const void* h, size_t bytes) {
+    return bytes ? hipMemcpy(d, h, bytes, hipMemcpyHostToDevice) : hipSuccess;
+  };
+  hipError_t e;
+  if ((e = up(pl->d_t1, pl->t1.data(), sizeof(Tile) * pl->t1.size())) != hipSuccess) return e;
+  if ((e = up(pl->d_e1, pl->e1.data(), sizeof(E1Rec) * pl->e1.size())) != hipSuccess) return e;
+  if ((e = up(pl->d_t2, pl->t2.data(), sizeof(Tile) * pl->t2.size())) != hipSuccess) return e;
+  return up(pl->d_t3, pl->t3.data(), sizeof(Tile) * pl->t3.size());
+}
+
 static int upload(Plan* pl) {
   if (pl->d_ag) return 0;
   auto fail_dev = [&](hipError_t e, const char* what) {
@@ -1738,15 +1804,28 @@ static int upload(Plan* pl) {
   }
   pl->grid1 = std::min<long long>(pl->grid_cap, (long long)pl->units.size());
   pl->grid1 = std::max(pl->grid1, std::min(pl->grid_cap, pl->team_max + 1));
+  // tile arrays sized for the smallest candidate tiles (the most tiles), so
+  // the autotuner can re-tile without reallocating
+  {
+    const TileCfg keep = pl->tiles;
+    build_tiles(pl, TileCfg{kTuneE1A[0], kTuneE2T[0], kTuneE3T[0], kTuneCombos[0], kTuneCols[0]});
+    pl->cap_t1 = pl->t1.size();
+    pl->cap_t2 = pl->t2.size();
+    pl->cap_t3 = pl->t3.size();
+    build_tiles(pl, keep);
+    pl->cap_t1 = std::max(pl->cap_t1, pl->t1.size());
+    pl->cap_t2 = std::max(pl->cap_t2, pl->t2.size());
+    pl->cap_t3 = std::max(pl->cap_t3, pl->t3.size());
+  }
   auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 8)); };
   if ((e = alloc((void**)&pl->d_ag, sizeof(Agent) * pl->agents.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_units, sizeof(Unit) * pl->units.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
-  if ((e = alloc((void**)&pl->d_t1, sizeof(Tile) * pl->t1.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
-  if ((e = alloc((void**)&pl->d_e1, sizeof(E1Rec) * pl->e1.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_t1, sizeof(Tile) * pl->cap_t1)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_e1, sizeof(E1Rec) * pl->cap_t1)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_hat, sizeof(HaTask) * pl->hat.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_ha, sizeof(double) * pl->ha_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
-  if ((e = alloc((void**)&pl->d_t2, sizeof(Tile) * pl->t2.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
-  if ((e = alloc((void**)&pl->d_t3, sizeof(Tile) * pl->t3.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_t2, sizeof(Tile) * pl->cap_t2)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_t3, sizeof(Tile) * pl->cap_t3)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_iarena, sizeof(int32_t) * pl->iarena.size())) != hipSuccess)
     return fail_dev(e, "hipMalloc");
   // 16 bytes of tail padding: stages read these buffers as 16-byte vectors
@@ -1763,11 +1842,8 @@ static int upload(Plan* pl) {
   };
   if ((e = up(pl->d_ag, pl->agents.data(), sizeof(Agent) * pl->agents.size())) != hipSuccess ||
       (e = up(pl->d_units, pl->units.data(), sizeof(Unit) * pl->units.size())) != hipSuccess ||
-      (e = up(pl->d_t1, pl->t1.data(), sizeof(Tile) * pl->t1.size())) != hipSuccess ||
-      (e = up(pl->d_e1, pl->e1.data(), sizeof(E1Rec) * pl->e1.size())) != hipSuccess ||
+      (e = upload_tiles(pl)) != hipSuccess ||
       (e = up(pl->d_hat, pl->hat.data(), sizeof(HaTask) * pl->hat.size())) != hipSuccess ||
-      (e = up(pl->d_t2, pl->t2.data(), sizeof(Tile) * pl->t2.size())) != hipSuccess ||
-      (e = up(pl->d_t3, pl->t3.data(), sizeof(Tile) * pl->t3.size())) != hipSuccess ||
       (e = up(pl->d_iarena, pl->iarena.data(), sizeof(int32_t) * pl->iarena.size())) != hipSuccess)
     return fail_dev(e, "hipMemcpy");
   // every partial of a fresh plan carries the fresh-buffer tag (all ones)
@@ -1944,33 +2020,96 @@ extern "C" int pymdp_hip_ragged_autotune(void* plan, const float* params, const 
   }
   int cus = 0;
   PMH_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device), "ragged: ");
-  hipEvent_t e0, e1;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   PMH_HIP_CHECK(hipEventCreate(&e0), "ragged: ");
-  PMH_HIP_CHECK(hipEventCreate(&e1), "ragged: ");
-  float best = 1e30f;
-  int b1 = pl->grid1;
-  const int lo = std::min(pl->grid_cap, pl->team_max + 1);
-  for (int w = 1; w * cus <= pl->grid_cap + cus - 1; ++w) {
-    const int g1 = std::max(lo, std::min(pl->grid_cap, w * cus));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    set_error("ragged: hipEventCreate failed");
+    return PYMDP_HIP_EDEVICE;
+  }
+  constexpr int kReps = 4;
+  // mean step time over kReps steps after one warm-up step
+  auto time_steps = [&](int g1, float* ms) -> int {
     int rc = ragged_launch(pl, params, obs, out, act, s, g1);
     if (rc) return rc;
-    PMH_HIP_CHECK(hipEventRecord(e0, s), "ragged: ");
-    for (int r = 0; r < 3; ++r)
+    if (hipEventRecord(e0, s) != hipSuccess) return PYMDP_HIP_EDEVICE;
+    for (int r = 0; r < kReps; ++r)
       if ((rc = ragged_launch(pl, params, obs, out, act, s, g1))) return rc;
-    PMH_HIP_CHECK(hipEventRecord(e1, s), "ragged: ");
-    PMH_HIP_CHECK(hipEventSynchronize(e1), "ragged: ");
-    float ms = 0.f;
-    PMH_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1), "ragged: ");
-    if (ms < best) {
-      best = ms;
-      b1 = g1;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(ms, e0, e1) != hipSuccess) {
+      set_error("ragged: autotune timing failed");
+      return PYMDP_HIP_EDEVICE;
     }
-  }
+    *ms /= kReps;
+    return 0;
+  };
+  auto run = [&]() -> int {
+    // 1. persistent FPI grid: 1, 2, ... workgroups per CU
+    float best = 1e30f, ms = 0.f;
+    int b1 = pl->grid1, rc = 0;
+    const int lo = std::min(pl->grid_cap, pl->team_max + 1);
+    for (int w = 1; w * cus <= pl->grid_cap + cus - 1; ++w) {
+      const int g1 = std::max(lo, std::min(pl->grid_cap, w * cus));
+      if ((rc = time_steps(g1, &ms))) return rc;
+      if (ms < best) {
+        best = ms;
+        b1 = g1;
+      }
+    }
+    pl->grid1 = b1;
+    // 2. EFE tile sizes, one stage at a time from the current tiling
+    TileCfg bc = pl->tiles;
+    if ((rc = time_steps(pl->grid1, &best))) return rc;
+    auto trial = [&](TileCfg c) -> int {
+      build_tiles(pl, c);
+      if (upload_tiles(pl) != hipSuccess) {
+        set_error("ragged: tile upload failed");
+        return PYMDP_HIP_EDEVICE;
+      }
+      float t = 0.f;
+      const int r = time_steps(pl->grid1, &t);
+      if (r) return r;
+      if (t < best) {
+        best = t;
+        bc = c;
+      }
+      return 0;
+    };
+    for (int x : kTuneE1A) {
+      TileCfg c = bc;
+      c.e1_a = x;
+      if (x != bc.e1_a && (rc = trial(c))) return rc;
+    }
+    for (int x : kTuneE2T) {
+      TileCfg c = bc;
+      c.e2_t = x;
+      if (x != bc.e2_t && (rc = trial(c))) return rc;
+    }
+    for (int x : kTuneE3T)
+      for (int y : kTuneCombos) {
+        TileCfg c = bc;
+        c.e3_t = x;
+        c.e3_combos = y;
+        if ((x != bc.e3_t || y != bc.e3_combos) && (rc = trial(c))) return rc;
+      }
+    for (int x : kTuneCols) {
+      TileCfg c = bc;
+      c.cols = x;
+      if (x != bc.cols && (rc = trial(c))) return rc;
+    }
+    build_tiles(pl, bc);
+    if (upload_tiles(pl) != hipSuccess) {
+      set_error("ragged: tile upload failed");
+      return PYMDP_HIP_EDEVICE;
+    }
+    if (best_ms_out) *best_ms_out = best;
+    return 0;
+  };
+  const int rc = run();
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  pl->grid1 = b1;
+  if (rc) return rc;
   pl->tuned = true;
-  if (best_ms_out) *best_ms_out = best / 3.0f;
   return PYMDP_HIP_OK;
 }
 
@@ -1984,6 +2123,37 @@ extern "C" int pymdp_hip_ragged_status(void* plan, int32_t* flag_out) {
   PMH_HIP_CHECK(hipDeviceSynchronize(), "ragged: ");
   PMH_HIP_CHECK(hipMemcpy(h, pl->d_cnt, sizeof(h), hipMemcpyDeviceToHost), "ragged: ");
   *flag_out = h[3];
+  return PYMDP_HIP_OK;
+}
+
+extern "C" int pymdp_hip_ragged_tile_config(void* plan, int32_t* out8) {
+  Plan* pl = static_cast<Plan*>(plan);
+  if (!pl || !out8) return einval("plan or out8 is NULL");
+  const TileCfg& c = pl->tiles;
+  const int32_t v[8] = {c.e1_a, c.e2_t, c.e3_t, c.e3_combos, c.cols, pl->tuned ? 1 : 0, pl->grid1, 0};
+  std::memcpy(out8, v, sizeof(v));
+  return PYMDP_HIP_OK;
+}
+
+extern "C" int pymdp_hip_ragged_set_tiles(void* plan, const int32_t* cfg5) {
+  Plan* pl = static_cast<Plan*>(plan);
+  if (!pl || !cfg5) return einval("plan or cfg5 is NULL");
+  TileCfg c;
+  c.e1_a = cfg5[0];
+  c.e2_t = cfg5[1];
+  c.e3_t = cfg5[2];
+  c.e3_combos = cfg5[3];
+  c.cols = cfg5[4];
+  if (c.e1_a < kTuneE1A[0] || c.e1_a > kE1A || c.e2_t < kTuneE2T[0] || c.e2_t > kTile2 || c.e3_t < kTuneE3T[0] ||
+      c.e3_t > kTile || c.e3_combos < kTuneCombos[0] || c.e3_combos > kE3Combos || c.cols < kTuneCols[0] ||
+      c.cols > kCols)
+    return einval("tile sizes out of range (stage 1 A floats 4352..8704, stage 2 / 3 doubles 2304..4608, "
+                  "combinations 64..128, columns 16..32)");
+  build_tiles(pl, c);
+  if (pl->d_ag && upload_tiles(pl) != hipSuccess) {
+    set_error("ragged: tile upload failed");
+    return PYMDP_HIP_EDEVICE;
+  }
   return PYMDP_HIP_OK;
 }
 

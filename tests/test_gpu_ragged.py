@@ -167,6 +167,9 @@ def test_ragged_graph_replay_and_autotune(dev):
     torch.cuda.synchronize()
     ref_out, ref_act = rb.out.clone(), rb.act.clone()
     rb.autotune()
+    cfg = rb.plan.tile_config()
+    print("autotuned tiles:", cfg, flush=True)
+    assert cfg["autotuned"] == 1
     rb.step()
     torch.cuda.synchronize()
     assert torch.equal(ref_out, rb.out) and torch.equal(ref_act, rb.act)
@@ -236,3 +239,25 @@ def test_ragged_rollout_carries_prior(dev):
         g.replay()
         torch.cuda.synchronize()
         assert torch.equal(eager[t][0], rb.out) and torch.equal(eager[t][1], rb.act), t
+
+
+def test_ragged_tilings_bitwise_equal(dev):
+    """Every EFE tiling in the autotuner's range gives bitwise the same
+    step (a tile only partitions rows and history columns; each sum keeps its
+    order)."""
+    w = workloads.ragged(seed=4, device=dev, n_agents=256)
+    print("ragged tilings: 256 agents", flush=True)
+    rb = ragged.RaggedBatch(w["agents"], num_iter=16, device=dev)
+    rb.step()
+    torch.cuda.synchronize()
+    ref_out, ref_act = rb.out.clone(), rb.act.clone()
+    for cfg in [(4352, 2304, 2304, 64, 16), (6528, 4608, 2304, 128, 32), (8704, 2304, 4608, 64, 32),
+                (4352, 4608, 4608, 128, 16)]:
+        rb.plan.set_tiles(*cfg)
+        rb.out.zero_()
+        rb.act.zero_()
+        rb.step()
+        torch.cuda.synchronize()
+        assert torch.equal(ref_out, rb.out) and torch.equal(ref_act, rb.act), cfg
+    with pytest.raises(ValueError, match="tile sizes out of range"):
+        rb.plan.set_tiles(9000, 4608, 4608, 128, 32)

@@ -300,3 +300,28 @@ def test_stage_cursors_match_division(d2, nr):
         for q, (r, c) in enumerate(_rowcol(tid, d2, 18)):
             e = tid + q * 256
             assert (r, c) == (e // d2, e % d2)
+
+
+@pytest.mark.parametrize("cfg", [(4352, 2304, 2304, 64, 16), (6528, 4608, 4608, 128, 32)])
+def test_tilings_bounds_and_emulation(cfg):
+    """Every tiling of the autotuner's range keeps every access of the full
+    4096-agent plan in bounds, and the emulated step is bitwise the same as
+    with the default tiles."""
+    w = workloads.ragged(seed=0, n_agents=4096, tensors=False)
+    plan = ragged.RaggedPlan(w["agents"], 16)
+    n0 = plan.info()["tiles1"]
+    plan.set_tiles(*cfg)
+    assert plan.tile_config()["e1_a_floats"] == cfg[0]
+    E.check_bounds_only(plan)
+    if cfg[0] < 8704:
+        assert plan.info()["tiles1"] > n0
+    wk = dict(seed=1, n_agents=24, state_range=(2, 9), obs_range=(2, 9), max_policies=256, max_A_bytes=1 << 16)
+    ags = workloads.ragged(**wk)["agents"]
+    ref = _emulate(ags)[1]
+    plan2 = ragged.RaggedPlan(ags, 16)
+    plan2.set_tiles(*cfg)
+    params = ragged.pack_params(plan2, ags, "cpu").numpy()
+    obs = ragged.pack_obs(plan2, [a["obs"] for a in ags], "cpu").numpy()
+    em = E.Emu(plan2, params, obs).run()
+    np.testing.assert_array_equal(em.out, ref.out)
+    np.testing.assert_array_equal(em.act, ref.act)
