@@ -609,21 +609,22 @@ __global__ __launch_bounds__(kNT) void efe_score_kernel(const Desc* __restrict__
 }
 
 // The same scores with the outcome GEMM on FP64 MFMA (v_mfma_f64_16x16x4f64):
-// per wave, 16 outer histories at a time; rows of the 16 x 16 tiles are
-// outcomes (the O outcome rows of W[n] and its H_A column), columns are
-// histories, k runs over the first dependency's states.  Lane layout: A
-// operand W[k + l/16][o0 + l%16] (from LDS), B operand q_h[k + l/16] for
-// h = h0 + l%16 (global), accumulator element v of lane l = outcome row
-// 4 v + l/16, history column l%16.  Each lane then adds its outcome logs,
-// utilities and the H_A row; lanes with the same column combine by
-// cross-lane sums.  W must fit in LDS; no pA term.
+// per wave, 16 outer histories at a time; rows of the 16 x 16 tiles are the
+// O outcome rows of W[n], columns are histories, k runs over the first
+// dependency's states.  Lane layout: A operand W[k + l/16][o0 + l%16] (from
+// LDS), B operand q_h[k + l/16] for h = h0 + l%16 (global), accumulator
+// element v of lane l = outcome row 4 v + l/16, history column l%16.  The
+// H_A column of W is a separate dot product with the same B fragments
+// (lane-local FMAs), not a 17th MFMA row block.  Each lane then adds its
+// outcome entropies (qo ln qo with ln_fast: exact exponent, hardware FP32
+// log of the mantissa, summed in FP64) and utilities; lanes with the same
+// column combine by cross-lane sums.  W must fit in LDS; no pA term.
 typedef double efe_d4 __attribute__((ext_vector_type(4)));
 
 template <int KMAX>  // k steps held per lane: S0 <= 4 KMAX
 __global__ __launch_bounds__(kNT) void efe_score_mfma_kernel(const Desc* __restrict__ dp,
                                                              const Item2* __restrict__ items, const Args a) {
   extern __shared__ double wl[];  // W[n] [S0][Ox], then C[t] [O] (launch size: both)
-  __shared__ double s_ltab[256];
   const Desc& d = *dp;
   const Item2 it = items[blockIdx.x];
   const int b = blockIdx.y;
@@ -642,15 +643,13 @@ __global__ __launch_bounds__(kNT) void efe_score_mfma_kernel(const Desc* __restr
   const float* Ct = a.C[m] + (long long)b * a.C_stride[m] + (a.C_time[m] ? (long long)t * O : 0);
   double* s_c = wl + S0 * O1;
   for (int i = tid; i < S0 * O1; i += kNT) wl[i] = Wg[i];
-  for (int i = tid; i < 256; i += kNT) s_ltab[i] = g_logtab[i];
   for (int i = tid; i < O; i += kNT) s_c[i] = util ? (double)Ct[i] : 0.0;
   __syncthreads();
   const double* qbase = ws + d.qs_off[t][f0];
   double* out = ws + d.sc_off[t][m] + (long long)n * H0;
   const int li = lane & 15, lk = lane >> 4;
-  const int nk = (S0 + 3) >> 2;       // k steps of 4 states
-  const int nrow = O + 1;             // outcome rows + the H_A row
-  const int ntile = (nrow + 15) >> 4;
+  const int nk = (S0 + 3) >> 2;  // k steps of 4 states
+  const int ntile = (O + 15) >> 4;
   const int hbase = it.hb * kScoreCols;
   const int hend = min(H0, hbase + kScoreCols);
   for (int h0 = hbase + wave * 16; h0 < hend; h0 += kNT / 4) {
@@ -663,6 +662,12 @@ __global__ __launch_bounds__(kNT) void efe_score_mfma_kernel(const Desc* __restr
       bq[ks] = (ks < nk && s < S0 && h < H0) ? qbase[(long long)h * S0 + s] : 0.0;
     }
     double ent = 0.0, ut = 0.0, hq = 0.0;
+    // H_A . q_h over this lane's k values (summed across lk below)
+#pragma unroll
+    for (int ks = 0; ks < KMAX; ++ks) {
+      const int s = 4 * ks + lk;
+      if (ks < nk && s < S0) hq = fma(wl[s * O1 + O], bq[ks], hq);
+    }
     for (int ot = 0; ot < ntile; ++ot) {
       const int oa = ot * 16 + li;  // this lane's A-operand row (outcome)
       efe_d4 acc = {0.0, 0.0, 0.0, 0.0};
@@ -670,7 +675,7 @@ __global__ __launch_bounds__(kNT) void efe_score_mfma_kernel(const Desc* __restr
       for (int ks = 0; ks < KMAX; ++ks) {
         if (ks < nk) {
           const int s = 4 * ks + lk;
-          const double x = (s < S0 && oa < nrow) ? wl[s * O1 + oa] : 0.0;
+          const double x = (s < S0 && oa < O) ? wl[s * O1 + oa] : 0.0;
           acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, bq[ks], acc, 0, 0, 0);
         }
       }
@@ -679,10 +684,8 @@ __global__ __launch_bounds__(kNT) void efe_score_mfma_kernel(const Desc* __restr
         const int o = ot * 16 + 4 * v + lk;
         const double qo = acc[v];
         if (o < O) {
-          if (qo != 0.0) ent += qo * log_f64d((double)fmaxf((float)qo, kMinVal), s_ltab);
+          if (qo != 0.0) ent += qo * ln_fast(fmax(qo, (double)kMinVal));
           ut = fma(qo, s_c[o], ut);
-        } else if (o == O) {
-          hq += qo;
         }
       }
     }

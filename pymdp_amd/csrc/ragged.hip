@@ -155,14 +155,13 @@ struct Bufs {
   int carry;  // 1: the FPI prior is the previous step's predicted prior (out + pred_off), 0: D
 };
 
-// ln of a float clipped at MINVAL (always a normal float) from the hardware
-// log2 (v_log_f32, about 1 ulp) times ln 2: a few instructions instead of the
-// library logf; the outcome entropies and sum_o A ln A it feeds are summed
-// in FP64.
-__device__ __forceinline__ float ln_clip_f(float x) {
-  return __builtin_amdgcn_logf(fmaxf(x, kMinVal)) * 0.693147180559945309f;
+// x ln x of a float with the reference's clip (0 for x == 0) in FP64, the
+// log from ln_fast (exact exponent, hardware log2 of the mantissa: an
+// absolute error below ~1.5e-7 at any magnitude); the outcome entropies and
+// the sum_o A ln A it feeds are summed in FP64.
+__device__ __forceinline__ double xlogx_fast(float x) {
+  return x == 0.0f ? 0.0 : (double)x * ln_fast((double)fmaxf(x, kMinVal));
 }
-__device__ __forceinline__ float xlogx_f(float x) { return x == 0.0f ? 0.0f : x * ln_clip_f(x); }
 
 // dst(e) = src(e) for e in [0, n): every thread issues NB global loads
 // before it stores any of them (one memory round trip per NB elements per
@@ -999,11 +998,11 @@ __global__ __launch_bounds__(kNT) void rg_ha_kernel(Bufs b) {
     for (int j = 0; j < 8; ++j) v[j] = col[(o + j) * st];
 #pragma unroll
     for (int j = 0; j < 8; j += 2) {
-      s0 += (double)xlogx_f(v[j]);
-      s1 += (double)xlogx_f(v[j + 1]);
+      s0 += xlogx_fast(v[j]);
+      s1 += xlogx_fast(v[j + 1]);
     }
   }
-  for (; o < tk.O; ++o) s0 += (double)xlogx_f(col[o * st]);
+  for (; o < tk.O; ++o) s0 += xlogx_fast(col[o * st]);
   b.ha[tk.ha_off + e] = s0 + s1;
 }
 
@@ -1078,9 +1077,8 @@ __global__ __launch_bounds__(kNT3) void rg_e3_kernel(Bufs b) {
       double qo = 0.0;
       for (int i = 0; i < d0; ++i) qo = fma(z[i], q0[i], qo);
       if (o < O) {
-        const float qf = (float)qo;
         double w = qo * (double)(half ? c_hi : c_lo);
-        if (qf != 0.0f) w -= qo * (double)ln_clip_f(qf);
+        if (qo != 0.0) w -= qo * ln_fast(fmax(qo, (double)kMinVal));
         v += w;
       } else {
         v += qo;
