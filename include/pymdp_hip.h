@@ -414,8 +414,9 @@ int pymdp_hip_ragged_plan_upload(void* plan);
  *                            [leading storage axes][O_m][last storage axis]
  *   A_perm [sum M * 3]       storage axis j = position A_perm[j] of the
  *                            modality's given dependency list (-1 = unused)
- *   B_off [sum F]            B_f stored [U_f][S_f][S_deps...] (control axis
- *                            first; given as [S_f][S_deps...][U_f])
+ *   B_off [sum F]            B_f stored [U_f][S_deps...][S_f] (control axis
+ *                            first, next-state axis last; given as
+ *                            [S_f][S_deps...][U_f])
  *   C_off [sum M]            C_m [O_m] or [T, O_m]
  *   D_off [sum F]            prior D_f [S_f]
  *   E_off [n]                E [P] (or -1)
@@ -444,6 +445,14 @@ int pymdp_hip_ragged_tables(void* plan, int64_t* sizes16, void* agents_out,
  * output offset, history offset of each of its 16 columns, sizes); count_out
  * / record_bytes receive the count and the record size; out may be NULL. */
 int pymdp_hip_ragged_e1_records(void* plan, void* out, int64_t* count_out, int64_t* record_bytes);
+/* Host copies of the rollout task records (per task: params offset of B_f,
+ * history offsets of its outputs and of each dependency's parent level,
+ * int-arena offset of its history records, sizes and level); level_start
+ * (kT + 1 = 5 values, may be NULL) receives the first task of each level and
+ * the total; count_out / record_bytes receive the count and the record size;
+ * out may be NULL. */
+int pymdp_hip_ragged_roll_tasks(void* plan, void* out, int64_t* count_out, int64_t* record_bytes,
+                                int64_t* level_start);
 /* Static parameters (no counterpart in the reference's ragged-free API; the
  * counterpart of PYMDP_HIP_FLAG_MODEL_PARAMS_STATIC): with is_static = 1 the
  * caller promises the A blocks of the params arena are not modified in place

@@ -19,7 +19,7 @@ Layout: one float32 parameter arena (every agent's A, B, C, D, E at offsets
 the plan chooses; A_m is stored with its dependency axes in the plan's
 contraction order and position-major: [positions of the leading axes][O_m]
 [last axis], so that every outcome of a block of positions is contiguous;
-B_f with its control axis first), one int32 observation arena, one float32 output arena
+B_f as [U_f][dependency states][S_f]), one int32 observation arena, one float32 output arena
 (qs, G, q_pi, predicted prior per agent) and one int32 action arena.  A step allocates nothing
 and does not synchronise, so it can be captured in a HIP graph
 (RaggedBatch.capture).
@@ -155,7 +155,8 @@ def pack_params(plan: RaggedPlan, agents: Sequence[dict], device) -> torch.Tenso
             mi += 1
         for f in range(len(ag["S"])):
             Bf = ag["B"][f]
-            B = Bf.permute(Bf.dim() - 1, *range(Bf.dim() - 1)).contiguous().reshape(-1)  # u-major
+            # [U_f][dependency states][S_f]: the rollout's lanes run over S_f
+            B = Bf.permute(Bf.dim() - 1, *range(1, Bf.dim() - 1), 0).contiguous().reshape(-1)
             o = int(lay["B_off"][fi])
             arena[o:o + B.numel()] = B.to(device)
             D = ag["D"][f].reshape(-1)

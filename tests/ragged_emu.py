@@ -28,13 +28,13 @@ AGENT_DT = np.dtype([
     ("gR", "i4", (kM,)), ("gfac", "i4", (kM, kR)), ("gdim", "i4", (kM, kR)),
     ("BR", "i4", (kF,)), ("Bdep", "i4", (kF, 3)), ("BK", "i4", (kF,)), ("U", "i4", (kF,)),
     ("C_time", "i4", (kM,)),
-    ("H", "i4", (kT, kF)), ("hist_lvl", "i4", (kT, kF)), ("roll_lvl", "i4", (kT, kF)),
+    ("H", "i4", (kT, kF)), ("hist_lvl", "i4", (kT, kF)),
     ("score_base", "i4", (kT, kM)), ("score_stride", "i4", (kT, kM, kR)),
     ("ed", "i4", (kM, 3)), ("ef", "i4", (kM, 3)), ("tstart", "i4", (kM, kT + 1)),
     ("K", "i8", (kM,)), ("A_off", "i8", (kM,)), ("B_off", "i8", (kF,)), ("C_off", "i8", (kM,)),
     ("D_off", "i8", (kF,)), ("E_off", "i8"),
     ("obs_off", "i8"), ("qs_off", "i8"), ("G_off", "i8"), ("qpi_off", "i8"), ("act_off", "i8"),
-    ("hist_off", "i8"), ("roll_off", "i8"), ("p2h_off", "i8"), ("pol0_off", "i8"), ("score_off", "i8"),
+    ("hist_off", "i8"), ("p2h_off", "i8"), ("pol0_off", "i8"), ("score_off", "i8"),
     ("logit_off", "i8"), ("part_off", "i8"), ("z1_off", "i8", (kM,)), ("z2_off", "i8", (kM, kT)),
     ("ha_off", "i8", (kM,)),
     ("pred_off", "i8"),
@@ -44,6 +44,8 @@ UNIT_DT = np.dtype([("agent", "i4"), ("slice", "i4"), ("nseg", "i4"), ("pad", "i
                     ("r0", "i4", (kM,)), ("r1", "i4", (kM,)), ("lds", "i4", (kM,))])
 E1REC_DT = np.dtype([("a_off", "i8"), ("h_off", "i8"), ("z_off", "i8"), ("cb", "i8", (16,)), ("O", "i4"), ("d2", "i4"),
                      ("d01", "i4"), ("nr", "i4"), ("nc", "i4"), ("c0", "i4"), ("pad0", "i4"), ("pad1", "i4")])
+ROLL_DT = np.dtype([("b_off", "i8"), ("out", "i8"), ("par", "i8", (3,)), ("ent", "i8"), ("Sf", "i4"), ("KB", "i4"),
+                    ("rb", "i4"), ("nh", "i4"), ("t", "i4"), ("dd", "i4", (3,))], align=True)
 TILE_DT = np.dtype([("agent", "i4"), ("m", "i4"), ("t", "i4"), ("r0", "i4"), ("r1", "i4"), ("c0", "i4"),
                     ("c1", "i4"), ("pad", "i4")])
 
@@ -78,6 +80,20 @@ def e1_records(plan):
     return rec
 
 
+def roll_tasks(plan):
+    """(rollout task records, first task of each level) copied from the plan."""
+    L = _native.lib()
+    cnt, size = np.zeros(1, np.int64), np.zeros(1, np.int64)
+    start = np.zeros(kT + 1, np.int64)
+    _native.check(L.pymdp_hip_ragged_roll_tasks(plan.handle, None, cnt.ctypes.data_as(_native._i64p),
+                                                size.ctypes.data_as(_native._i64p),
+                                                start.ctypes.data_as(_native._i64p)))
+    assert int(size[0]) == ROLL_DT.itemsize, (int(size[0]), ROLL_DT.itemsize)
+    rec = np.zeros(int(cnt[0]), ROLL_DT)
+    _native.check(L.pymdp_hip_ragged_roll_tasks(plan.handle, rec.ctypes.data, None, None, None))
+    return rec, start
+
+
 def _chk(idx, n, what):
     idx = np.asarray(idx)
     if idx.size:
@@ -106,6 +122,7 @@ class Emu:
         self.info = plan.info()
         self.ag, self.units, (self.t1, self.t2, self.t3), self.iarena = tables(plan)
         self.e1rec = e1_records(plan)
+        self.rolls, self.roll_start = roll_tasks(plan)
         assert len(self.e1rec) == len(self.t1)
         self.params = params.astype(np.float32)
         self.obs = obs.astype(np.int32)
@@ -253,41 +270,29 @@ class Emu:
         self.out[_chk(int(a["qs_off"]) + np.arange(totS), self.out.size, "qs out")] = qv.astype(np.float32)
         h0 = int(a["hist_off"])
         self.hist[_chk(h0 + np.arange(totS), self.hist.size, "hist post")] = qv
-        for t in range(int(a["T"])):
-            for f in range(F):
-                Sf, rb, KB, Uf = int(S[f]), int(a["BR"][f]), int(a["BK"][f]), int(a["U"][f])
-                nh = int(a["H"][t][f])
-                # LDS staging of the rollout: one-dependency parents at the end
-                # of s_x, whole action blocks (floats) before them
-                nv = 0
-                if rb == 1 and t > 0:
-                    d = int(a["Bdep"][f][0])
-                    if int(a["H"][t - 1][d]) * int(S[d]) <= kCap // 2:
-                        nv = int(a["H"][t - 1][d]) * int(S[d])
-                        _chk(kCap - nv + np.arange(nv), kCap, "s_par")
-                blkl = Sf * (KB | 1)
-                per = max(0, min(Uf, (2 * (kCap - nv)) // max(blkl, 1)))
-                if per:
-                    assert per * blkl <= 2 * (kCap - nv), "action blocks overlap the parents"
-                Bf = P[_chk(int(a["B_off"][f]) + np.arange(Sf * KB * Uf), P.size, "B")].reshape(Uf, Sf, KB)
-                for h in range(nh):
-                    rec = self.iarena[_chk(int(a["roll_off"]) + int(a["roll_lvl"][t][f]) + h * (1 + rb)
-                                           + np.arange(1 + rb), self.iarena.size, "roll rec")]
-                    u = int(rec[0])
-                    assert 0 <= u < Uf
-                    w = np.ones(1)
-                    for j in range(rb):
-                        d = int(a["Bdep"][f][j])
-                        if t == 0:
-                            base = int(S_off[d])
-                        else:
-                            assert 0 <= rec[1 + j] < a["H"][t - 1][d]
-                            base = int(a["hist_lvl"][t - 1][d]) + int(rec[1 + j]) * int(S[d])
-                        qp = self.hist[_chk(h0 + base + np.arange(S[d]), self.hist.size, "hist parent")]
-                        w = np.kron(w, qp)
-                    assert w.size == KB
-                    dst = h0 + int(a["hist_lvl"][t][f]) + h * Sf + np.arange(Sf)
-                    self.hist[_chk(dst, self.hist.size, "hist out")] = Bf[u].astype(np.float64) @ w
+
+    # ---------------------------------------------------------------- rollout
+    def roll(self, rt):
+        """rg_roll_kernel for one task: output o -> state s of entry o / S_f,
+        entries (h, u, parents) of the level's action-sorted history list;
+        B_f stored [U_f][K_B][S_f]."""
+        Sf, KB, rb, nh, t = (int(rt[k]) for k in ("Sf", "KB", "rb", "nh", "t"))
+        ne = 2 + rb
+        assert 1 <= nh and nh * Sf <= kNT and 1 <= rb <= 3 and Sf <= kDim
+        P = self.params
+        ent = self.iarena[_chk(int(rt["ent"]) + np.arange(nh * ne), self.iarena.size, "roll entries")].reshape(nh, ne)
+        for i in range(nh):
+            h, u = int(ent[i][0]), int(ent[i][1])
+            assert u >= 0
+            w = np.ones(1)
+            for j in range(rb):
+                d = int(rt["dd"][j])
+                base = int(rt["par"][j]) + (0 if t == 0 else int(ent[i][2 + j]) * d)
+                w = np.kron(w, self.hist[_chk(base + np.arange(d), self.hist.size, "hist parent")])
+            assert w.size == KB and np.isfinite(w).all(), "rollout read of an unset parent vector"
+            addr = int(rt["b_off"]) + (u * KB + np.arange(KB))[:, None] * Sf + np.arange(Sf)[None, :]
+            Bu = P[_chk(addr, P.size, "B")].astype(np.float64)  # [K_B, S_f]
+            self.hist[_chk(int(rt["out"]) + h * Sf + np.arange(Sf), self.hist.size, "hist out")] = w @ Bu
 
     # ---------------------------------------------------------------- EFE stages
     def _hcount(self, a, m, t, p):
@@ -456,6 +461,11 @@ class Emu:
             by_agent.setdefault(int(u["agent"]), []).append(u)
         for ai, us in by_agent.items():
             self.fpi_agent(ai, us)
+        check_rolls(self.ag, self.rolls, self.roll_start, self.iarena)
+        for t in range(kT):
+            for rt in self.rolls[int(self.roll_start[t]):int(self.roll_start[t + 1])]:
+                assert int(rt["t"]) == t
+                self.roll(rt)
         for tl, rec in zip(self.t1, self.e1rec):
             self.e1(tl, rec)
         for tl in self.t2:
@@ -465,6 +475,34 @@ class Emu:
         for ai in range(self.plan.n):
             self.assemble(ai)
         return self
+
+
+def check_rolls(ag, rolls, start, iarena):
+    """Rollout tasks: grouped by level, and every history vector of every
+    (agent, level, factor) written by exactly one task entry, with that
+    factor's B block and sizes."""
+    assert list(start) == sorted(start) and int(start[-1]) == len(rolls)
+    for t in range(kT):
+        assert (rolls["t"][int(start[t]):int(start[t + 1])] == t).all()
+    want = {}
+    for a in ag:
+        h0 = int(a["hist_off"])
+        for t in range(int(a["T"])):
+            for f in range(int(a["F"])):
+                Sf = int(a["S"][f])
+                for h in range(int(a["H"][t][f])):
+                    want[h0 + int(a["hist_lvl"][t][f]) + h * Sf] = (int(a["B_off"][f]), Sf, int(a["BK"][f]),
+                                                                     int(a["BR"][f]))
+    got = {}
+    for rt in rolls:
+        ne = 2 + int(rt["rb"])
+        b_base = int(rt["b_off"])
+        for i in range(int(rt["nh"])):
+            h = int(iarena[int(rt["ent"]) + i * ne])
+            o = int(rt["out"]) + h * int(rt["Sf"])
+            assert o not in got, "history vector written twice"
+            got[o] = (b_base, int(rt["Sf"]), int(rt["KB"]), int(rt["rb"]))
+    assert got == want
 
 
 def check_units(ag, units, info):
@@ -547,6 +585,21 @@ def check_bounds_only(plan):
                     assert int(a["z2_off"][m][t]) + H[2] * H[1] * O1 * d0 <= info["z2_doubles"]
                 blk = int(a["score_base"][t][m]) + H[0] * H[1] * H[2]
                 assert int(a["score_off"]) + blk <= info["score_doubles"]
+    rolls, start = roll_tasks(plan)
+    check_rolls(ag, rolls, start, iarena)
+    for rt in rolls:
+        Sf, KB, rb, nh = (int(rt[k]) for k in ("Sf", "KB", "rb", "nh"))
+        ne = 2 + rb
+        assert 1 <= nh and 1 <= rb <= 3
+        assert int(rt["ent"]) + nh * ne <= iarena.size
+        ent = iarena[int(rt["ent"]):int(rt["ent"]) + nh * ne].reshape(nh, ne)
+        assert ent[:, 1].min() >= 0
+        assert nh * Sf <= kNT
+        assert int(rt["b_off"]) + (int(ent[:, 1].max()) + 1) * KB * Sf <= params_n
+        for j in range(rb):
+            hi = 0 if int(rt["t"]) == 0 else int(ent[:, 2 + j].max())
+            assert int(rt["par"][j]) + (hi + 1) * int(rt["dd"][j]) <= info["hist_doubles"]
+        assert int(rt["out"]) + (int(ent[:, 0].max()) + 1) * Sf <= info["hist_doubles"]
     for tl in t1:
         a = ag[int(tl["agent"])]
         m = int(tl["m"])
