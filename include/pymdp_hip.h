@@ -432,7 +432,7 @@ int pymdp_hip_ragged_layout(void* plan, int64_t* out, int64_t n_out);
  *          FPI grid, history doubles, score doubles, stage-1 floats,
  *          stage-2 floats, team-exchange doubles, largest FPI team,
  *          EFE multiply-adds per step, FPI multiply-adds per step, FPI grid
- *          capacity (workgroups resident at once), sum_o A ln A doubles). */
+ *          capacity (workgroups resident at once), sum_o A ln A floats). */
 int pymdp_hip_ragged_info(void* plan, int64_t* out16);
 /* Host copies of the plan's device tables (for host-side verification):
  * sizes16 = (sizeof agent record, sizeof unit, units, sizeof tile, stage-1
@@ -440,9 +440,10 @@ int pymdp_hip_ragged_info(void* plan, int64_t* out16);
  * the three tile lists concatenated. */
 int pymdp_hip_ragged_tables(void* plan, int64_t* sizes16, void* agents_out,
                             void* units_out, void* tiles_out, int32_t* iarena_out);
-/* Host copies of the stage-1 tile records (per stage-1 tile: params offset
- * of its A block, offset of its rows of the sum_o A ln A table, stage-1
- * output offset, history offset of each of its 16 columns, sizes); count_out
+/* Host copies of the stage-1 work-item records (per item, a range of rows of
+ * one modality: params offset of its A, offset of its sum_o A ln A table,
+ * stage-1 output offset, history offset and first column of each level,
+ * sizes and the row range); count_out
  * / record_bytes receive the count and the record size; out may be NULL. */
 int pymdp_hip_ragged_e1_records(void* plan, void* out, int64_t* count_out, int64_t* record_bytes);
 /* Host copies of the rollout task records (per task: params offset of B_f,
@@ -470,15 +471,15 @@ int pymdp_hip_ragged_step(void* plan, const float* params, const int32_t* obs,
                           float* out, int32_t* actions, void* stream);
 /* The plan's autotuner (no counterpart in the reference): times the
  * persistent FPI grid (1, 2, ... workgroups per CU up to the occupancy
- * limit), then the EFE tile sizes one stage at a time (stage-1 A floats per
- * tile 4352 / 6528 / 8704, stage-2 and stage-3 doubles per tile 2304 / 4608,
+ * limit), then the EFE tile sizes one stage at a time (stage-1 rows per work
+ * item 128 / 256, stage-2 and stage-3 doubles per tile 2304 / 4608,
  * stage-3 combinations 64 / 128, histories per stage-2 / 3 tile 16 / 32),
  * each over whole steps, and keeps the fastest; *best_ms = that step time.
  * Not while `stream` is being captured; graphs captured before the call
  * must be captured again (the tile tables are rewritten). */
 int pymdp_hip_ragged_autotune(void* plan, const float* params, const int32_t* obs,
                               float* out, int32_t* actions, void* stream, float* best_ms);
-/* out8 = (stage-1 A floats per tile, stage-2 doubles per tile, stage-3
+/* out8 = (stage-1 rows per work item, stage-2 doubles per tile, stage-3
  *         doubles per tile, stage-3 combinations per tile, histories per
  *         stage-2 / 3 tile, 1 if autotuned, FPI grid, 0). */
 int pymdp_hip_ragged_tile_config(void* plan, int32_t* out8);

@@ -112,19 +112,22 @@ struct Tile {
   int agent, m, t, r0, r1, c0, c1, pad;
 };
 
-// Stage-1 tile record, built on the host: every address the tile needs, so
-// that its A, sum_o A ln A and history loads depend on this one record load
-// only.
+// Stage-1 work item: rows [e0, e1) of one modality's row space (see
+// rg_e1_kernel); every address the item needs comes from this record
+// (history column n of level t, ts[t] <= n < ts[t + 1], is the vector at
+// hb[t] + (n - ts[t]) * d2 of the history buffer).
 struct E1Rec {
-  long long a_off;   // params offset of the tile's first A element (position-major)
-  long long h_off;   // ha offset of the tile's first position's sum_o A ln A row
-  long long z_off;   // z1 offset of the tile's first position
-  long long cb[16];  // hist offset of each history column's vector (-1 past nc)
-  int O, d2, d01, nr, nc, c0, pad0, pad1;
+  long long a_off;   // params offset of A_m
+  long long h_off;   // ha offset of the modality's sum_o A ln A table ([pos][d2])
+  long long z_off;   // z1 offset of the modality
+  long long hb[kT];  // hist offset of level t's first last-axis history vector
+  int ts[kT + 1];    // first concatenated column of each level (ts[T..kT] = N1)
+  int O, d2, d01, e0, e1, N1, pad;
 };
+static_assert(sizeof(E1Rec) == 104, "stage-1 record layout");
 
-// One block of the sum_o A ln A table: elements [e0, e0 + kNT) of modality
-// m's [pos][k] table (n elements).
+// One block of the sum_o A ln A table (FP32): elements [e0, e0 + kNT) of
+// modality m's [pos][k] table (n elements).
 struct HaTask {
   long long a_off, ha_off;
   int O, d2, n, e0;
@@ -160,7 +163,7 @@ struct Bufs {
   int* act;
   double* hist;
   double* score;
-  double* ha;  // sum_o A ln A rows of every modality, [pos][d2]
+  float* ha;  // sum_o A ln A rows of every modality, [pos][d2] (FP32)
   float* z1;
   float* z2;
   double* part;
@@ -184,19 +187,19 @@ __device__ __forceinline__ double xlogx_fast(float x) {
 // round trip that the compiler repeats after every global store it cannot
 // prove unaliased (and chains when one field indexes another).  The caller
 // places a barrier before the copy is read.
-template <typename T>
+template <typename T, int NT = kNT>
 __device__ __forceinline__ void rg_copy_lds(const T* src, T* dst, int tid) {
   static_assert(sizeof(T) % 4 == 0, "record of whole dwords");
-  constexpr int n = (int)(sizeof(T) / 4), NB = (n + kNT - 1) / kNT;
+  constexpr int n = (int)(sizeof(T) / 4), NB = (n + NT - 1) / NT;
   const int* sp = reinterpret_cast<const int*>(src);
   int* dp = reinterpret_cast<int*>(dst);
   int v[NB];
 #pragma unroll
   for (int q = 0; q < NB; ++q)
-    if (tid + q * kNT < n) v[q] = sp[tid + q * kNT];
+    if (tid + q * NT < n) v[q] = sp[tid + q * NT];
 #pragma unroll
   for (int q = 0; q < NB; ++q)
-    if (tid + q * kNT < n) dp[tid + q * kNT] = v[q];
+    if (tid + q * NT < n) dp[tid + q * NT] = v[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -730,226 +733,173 @@ __device__ __forceinline__ void rg_load_hist(const Agent& a, const double* hist,
 }
 
 
-// Stage 1: for positions ij in [r0, r1) (ij = i * d1 + j) and concatenated
-// last-dim histories n in [c0, c1) (at most 16):
-//   Z1[((n * (O+1) + o) * d0 + i) * d1 + j] = sum_k A[o, ij, k] q_n[k]   (o < O)
-//   Z1[((n * (O+1) + O) * d0 + i) * d1 + j] = sum_k (sum_o A ln A)[ij, k] q_n[k]
-// (Z1 in FP32).  The tile's A block is staged in LDS as floats, its rows of
-// the sum_o A ln A table (rg_ha_kernel) as doubles, all with coalesced loads
-// issued together;
-// each wave multiplies 16-row blocks of [A ; sum_o A ln A] by the 16
-// history columns with FP64 MFMA (v_mfma_f64_16x16x4f64: lane l holds
-// X[l % 16][k + l / 16] and Q[col l % 16][k + l / 16]; accumulator element v
-// of lane l is row 4 v + l / 16, column l % 16).
-constexpr int kE1A = 8704;    // floats of the A block
-constexpr int kE1H = 1024;    // doubles of the sum_o A ln A rows
-constexpr int kE1Q = 16 * 65;  // doubles of the history block
-
-// Workgroups are dispatched to the XCDs round-robin (workgroup w on XCD
-// w % 8), each XCD with its own L2.  Maps workgroups to tiles so that every
-// XCD runs chunks of 16 consecutive tiles (chunk c on XCD c % 8; the tail
-// past the last whole round of chunks keeps w -> w): neighbouring stage-1
-// tiles, whose outputs share cache lines, then meet in one L2 and their
-// partial lines merge there instead of being written back to HBM separately
-// by two XCDs (stage-1 HBM writes 964 -> 552 MB on the 4096-agent workload),
-// while the chunks stay interleaved across the XCDs for load balance.
-constexpr int kXcds = 8;
-constexpr int kXcdChunk = 16;
-__device__ __forceinline__ int rg_xcd_tile(int w, int n) {
-  const int full = (n / (kXcds * kXcdChunk)) * (kXcds * kXcdChunk);
-  if (w >= full) return w;
-  const int x = w % kXcds, i = w / kXcds;
-  return ((i / kXcdChunk) * kXcds + x) * kXcdChunk + (i % kXcdChunk);
-}
+// Stage 1 on FP64 vector FMAs, one row per thread.  Modality m's row space
+// is [O1][d01] (row e = o * d01 + pos, pos = i * d1 + j): row e of
+// [A ; sum_o A ln A] is A_m[pos, o, :] (FP32, position-major in the params
+// arena) for o < O and row pos of the modality's sum_o A ln A table (FP32)
+// for o = O.  For every concatenated last-axis history n of the modality:
+//   Z1[(n * O1 + o) * d01 + pos] = row_e . q_n   (summed in FP64, stored FP32)
+// A thread loads its row (at most 64 floats) into registers with 16-byte
+// loads of 4-byte alignment, all in flight together; the history vectors are
+// staged in LDS 16 columns at a time and read by every lane at the same
+// address (a broadcast); the stores of one column are one contiguous run of
+// the item's rows.  No MFMA: the FP64 vector rate of gfx950 equals its FP64
+// matrix rate, most modalities have few history columns (a 16-column MFMA
+// block would be mostly padding), and the row-per-thread form keeps every
+// operand load in flight at once instead of staging A through LDS.
+constexpr int kE1Cols = 16;  // history columns staged in LDS at a time
+// threads per stage-1 row: two for rows of more than 8 vectors (d2 > 32)
+__host__ __device__ constexpr int rg_e1_tpr(int d2) { return d2 > 32 ? 2 : 1; }
+constexpr int kE1Ld = 64;    // LDS row stride (doubles) of a staged history column
+typedef float rg_f4u __attribute__((ext_vector_type(4), aligned(4)));
 
 
-__host__ __device__ __forceinline__ int rg_e1_rows(int O, int d2, int d01, int abudget) {
-  // (i, j) positions of one tile: the A block (at most abudget <= kE1A LDS
-  // floats) and the sum_o A ln A rows fit
-  const int lda = ((d2 + 3) & ~3) + 1;
-  int nr = abudget / (O * lda);
-  nr = nr < kE1H / lda ? nr : kE1H / lda;
-  if (nr < 1) nr = 1;
-  return nr < d01 ? nr : d01;
-}
-
-// Stage-1 tile staging: the tile's A block (floats), its rows of the
-// sum_o A ln A table and its history columns (doubles) into LDS, by NP
-// threads (ptid in [0, NP)).  A_m is position-major, so the tile's rows
-// (position rl, outcome o) are one contiguous chunk: element e = (rl, o, k)
-// at Am[e], LDS row o nr + rl.  All loads are unconditional (past the
-// block's end they repeat its last element and are not stored) and issued
-// back to back before any LDS store.
-template <int NP>
-__device__ __forceinline__ void rg_e1_stage(const Bufs& b, const E1Rec& rc, float* s_a, double* s_h, double* s_q,
-                                            int ptid) {
-  const int O = rc.O, d2 = rc.d2, nr = rc.nr;
-  const int d2p = (d2 + 3) & ~3, lda = d2p + 1;
-  const float inv_d2 = 1.0f / (float)d2, inv_d2p = 1.0f / (float)d2p;
-  // the A block as 16-byte vectors from the aligned address at or below its
-  // first element (the arena is 16-byte aligned, agent blocks start at
-  // multiples of four floats and an A block is followed by at least four
-  // floats of the same agent, so every vector stays in the arena); vector j
-  // holds elements 4 j - shift .. 4 j - shift + 3 of the block
-  constexpr int NV = (kE1A / 4 + 1 + NP - 1) / NP;
-  constexpr int NQ = (16 * 64 + NP - 1) / NP;
-  typedef float f4 __attribute__((ext_vector_type(4)));
-  const int per = O * d2;  // floats per position
-  const int tot = nr * per;
-  const int shift = (int)(rc.a_off & 3);
-  const f4* Av = reinterpret_cast<const f4*>(b.params + (rc.a_off - shift));
-  const int nv = (shift + tot + 3) >> 2;
-  const float inv_per = 1.0f / (float)per;
-  f4 v[NV];
+// The item's rows with NV = ceil(d2 / 4) 16-byte vectors each (a
+// compile-time trip count: no per-vector guards), TPR threads per row
+// (adjacent lanes, NVT = ceil(NV / TPR) vectors each, the second thread's
+// last one zero when NV is odd; their partial sums are combined by a DPP
+// lane swap).  Rows of more than 8 vectors take two threads: a thread
+// holding up to 64 floats would leave too few registers for the LDS
+// operands at four waves per SIMD (the compiler spills).
+template <int NV, int NT, int TPR>
+__device__ __forceinline__ void rg_e1_rows(const Bufs& b, const E1Rec& r, double* s_q, int tid) {
+  static_assert(TPR == 1 || TPR == 2, "threads per row");
+  constexpr int NVT = (NV + TPR - 1) / TPR;  // vectors per thread
+  const int O = r.O, d2 = r.d2, d01 = r.d01, N1 = r.N1;
+  const int h = TPR == 1 ? 0 : (tid & 1);  // which part of the row
+  const int e = r.e0 + tid / TPR;
+  const bool live = e < r.e1;
+  const int ec = live ? e : r.e1 - 1;
+  const int o = ec / d01, pos = ec - o * d01;
+  // the row: A rows and the table's rows are both FP32 [.][d2]; the last
+  // vector may reach up to 3 floats past the row (the params arena holds at
+  // least 3 more floats of the same agent after every A block, the table 16
+  // bytes of tail padding); those lanes are zeroed below
+  const float* src = (o < O ? b.params + r.a_off + ((long long)pos * O + o) * d2 : b.ha + r.h_off + (long long)pos * d2) +
+                     4 * NVT * h;
+  rg_f4u v[NVT];
 #pragma unroll
-  for (int q = 0; q < NV; ++q) v[q] = __builtin_nontemporal_load(Av + min(ptid + q * NP, nv - 1));
-  // the tile's sum_o A ln A rows: [nr][d2] contiguous in the table
-  constexpr int NH = (kE1H + NP - 1) / NP;
-  const double* Hr = b.ha + rc.h_off;
-  const int toth = nr * d2;
-  double hh[NH];
-#pragma unroll
-  for (int q = 0; q < NH; ++q) hh[q] = Hr[min(ptid + q * NP, toth - 1)];
-  double hv[NQ];
-  bool hok[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    const int e = ptid + q * NP, cq = rg_quot(e, inv_d2p), c = min(cq, 15), kk = e - cq * d2p;
-    const long long cb = rc.cb[c];
-    hok[q] = e < 16 * d2p && cb >= 0 && kk < d2;
-    hv[q] = b.hist[hok[q] ? cb + kk : 0];
+  for (int c = 0; c < NVT; ++c) {
+    if (TPR == 1 || NV % 2 == 0 || c < NVT - 1 || h == 0)
+      v[c] = *reinterpret_cast<const rg_f4u*>(src + 4 * c);
+    else
+      v[c] = rg_f4u{0.0f, 0.0f, 0.0f, 0.0f};
   }
+  const long long zstride = (long long)(O + 1) * d01;  // between history columns of Z1
+  float* z = b.z1 + r.z_off + ec;
+  constexpr int kp = 4 * NVT * TPR;  // staged length of a history column
+  for (int c0 = 0; c0 < N1; c0 += kE1Cols) {
+    const int nb = min(kE1Cols, N1 - c0);
+    // history columns [c0, c0 + nb) into s_q[n][k], zero for k in [d2, kp)
+    constexpr int NQ = (kE1Cols * kp + NT - 1) / NT;
+    const int tot = nb * kp;
+    double hv[NQ];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q)
-    if (!hok[q]) hv[q] = 0.0;
+    for (int q = 0; q < NQ; ++q) {
+      const int idx = min(tid + q * NT, tot - 1), n = idx / kp, k = idx - n * kp;
+      const int col = c0 + n;
+      int t = 0;
 #pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    const int j = ptid + q * NP;
+      for (int l = 1; l < kT; ++l) t += col >= r.ts[l] ? 1 : 0;
+      hv[q] = b.hist[r.hb[t] + (long long)(col - r.ts[t]) * d2 + min(k, d2 - 1)];
+    }
+    if (c0 > 0) lds_barrier();  // the previous chunk's reads are done
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int e = 4 * j + c - shift;
-      if (j < nv && e >= 0 && e < tot) {
-        const int rl = rg_quot(e, inv_per), rem = e - rl * per;
-        const int o = rg_quot(rem, inv_d2), k = rem - o * d2;
-        s_a[(o * nr + rl) * lda + k] = v[q][c];
+    for (int q = 0; q < NQ; ++q) {
+      const int idx = tid + q * NT;
+      if (idx < tot) {
+        const int n = idx / kp, k = idx - n * kp;
+        s_q[n * kE1Ld + k] = k < d2 ? hv[q] : 0.0;
       }
     }
-  }
+    lds_barrier();
+    if (c0 == 0 && h == (NV - 1) / NVT) {
+      // lanes of the row's last vector past its end are zero (the staged
+      // history columns are zero there too, but a finite product needs a
+      // finite factor)
+      constexpr int cl = (NV - 1) % NVT;
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    const int e = ptid + q * NP, c = rg_quot(e, inv_d2p), kk = e - c * d2p;
-    if (e < 16 * d2p) s_q[c * lda + kk] = hv[q];
-  }
+      for (int i = 1; i < 4; ++i)
+        if (4 * (NV - 1) + i >= d2) v[cl][i] = 0.0f;
+    }
+    // two columns per pass (each row element is converted to FP64 once per
+    // pass), two partial sums per column.  The empty asm keeps the
+    // conversions inside the pass: hoisted out of the loop they would hold
+    // the whole row as doubles (twice the row's VGPRs).
+#pragma unroll 1
+    for (int n = 0; n < nb; n += 2) {
 #pragma unroll
-  for (int q = 0; q < NH; ++q) {
-    const int e = ptid + q * NP, rl2 = rg_quot(e, inv_d2), k2 = e - rl2 * d2;
-    if (e < toth) s_h[rl2 * lda + k2] = hh[q];
-  }
-  // zero the padding columns [d2, d2p) of every A and sum_o A ln A row, so
-  // the product's k loop reads them unconditionally (the history block's
-  // padding columns are zero too)
-  if (d2p > d2) {
-    const int npad = d2p - d2;
-    for (int e = ptid; e < (O + 1) * nr * npad; e += NP) {
-      const int r = e / npad, k2 = d2 + (e - r * npad);
-      if (r < O * nr)
-        s_a[r * lda + k2] = 0.0f;
-      else
-        s_h[(r - O * nr) * lda + k2] = 0.0;
-    }
-  }
-}
-
-// Stage-1 tile product from LDS by NW waves (wave index cw in [0, NW)):
-// MFMA over 16-row blocks of the O1 * nr rows, two blocks per wave at a
-// time (independent accumulators hide the MFMA latency).  The k loop has no
-// branches and no per-lane condition: each lane reads its row of both
-// operand blocks at clamped rows and selects between them; the padding
-// columns [d2, d2p) of X and Q are zero in LDS.
-template <int NW>
-__device__ __forceinline__ void rg_e1_mfma(const Bufs& b, const E1Rec& rc, const float* s_a, const double* s_h,
-                                           const double* s_q, int lane, int cw) {
-  const int O = rc.O, O1 = O + 1, d01 = rc.d01, d2 = rc.d2;
-  const int d2p = (d2 + 3) & ~3, lda = d2p + 1;
-  const int nr = rc.nr, nc = rc.nc;
-  const int R = O1 * nr, RA = O * nr;
-  const int nblk = (R + 15) >> 4;
-  const int li = lane & 15, lk = lane >> 4;
-  const float inv_nr = 1.0f / (float)nr;
-  // this lane's output column (history c0 + li) base in Z1
-  float* zc = b.z1 + rc.z_off + (long long)(rc.c0 + li) * O1 * d01;
-  const double* qrow = s_q + li * lda + lk;
-  for (int blk = cw; blk < nblk; blk += 2 * NW) {
-    const int blk1 = blk + NW;  // may be past the last block: computed, not stored
-    const int r0 = (blk << 4) + li, r1 = (blk1 << 4) + li;
-    const float* xa0 = s_a + (r0 < RA ? r0 : 0) * lda + lk;
-    const double* xh0 = s_h + (r0 >= RA && r0 < R ? r0 - RA : 0) * lda + lk;
-    const float* xa1 = s_a + (r1 < RA ? r1 : 0) * lda + lk;
-    const double* xh1 = s_h + (r1 >= RA && r1 < R ? r1 - RA : 0) * lda + lk;
-    const bool h0 = r0 >= RA, h1 = r1 >= RA;
-    rg_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-    // two k steps per iteration: their LDS reads are in flight together
-    int k0 = 0;
-    for (; k0 + 4 < d2p; k0 += 8) {
-      const double a0 = (double)xa0[k0], g0 = xh0[k0], a0b = (double)xa0[k0 + 4], g0b = xh0[k0 + 4];
-      const double a1 = (double)xa1[k0], g1 = xh1[k0], a1b = (double)xa1[k0 + 4], g1b = xh1[k0 + 4];
-      const double q = qrow[k0], qb = qrow[k0 + 4];
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(h0 ? g0 : a0, q, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(h1 ? g1 : a1, q, acc1, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(h0 ? g0b : a0b, qb, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(h1 ? g1b : a1b, qb, acc1, 0, 0, 0);
-    }
-    if (k0 < d2p) {
-      const double a0 = (double)xa0[k0], g0 = xh0[k0];
-      const double a1 = (double)xa1[k0], g1 = xh1[k0];
-      const double q = qrow[k0];
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(h0 ? g0 : a0, q, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(h1 ? g1 : a1, q, acc1, 0, 0, 0);
-    }
-    if (li < nc) {
+      for (int c = 0; c < NVT; ++c) asm volatile("" : "+v"(v[c]));
+      const double* q0 = s_q + n * kE1Ld + 4 * NVT * h;
+      const double* q1 = q0 + kE1Ld;
+      double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int rr0 = (blk << 4) + 4 * v + lk;
-        if (rr0 < R) {
-          const int o = rg_quot(rr0, inv_nr);
-          zc[o * d01 + (rr0 - o * nr)] = (float)acc0[v];
-        }
-        const int rr1 = (blk1 << 4) + 4 * v + lk;
-        if (rr1 < R) {
-          const int o = rg_quot(rr1, inv_nr);
-          zc[o * d01 + (rr1 - o * nr)] = (float)acc1[v];
-        }
+      for (int c = 0; c < NVT; ++c) {
+        const double x0 = (double)v[c][0], x1 = (double)v[c][1], x2 = (double)v[c][2], x3 = (double)v[c][3];
+        a0 = fma(x0, q0[4 * c], a0);
+        a1 = fma(x1, q0[4 * c + 1], a1);
+        a0 = fma(x2, q0[4 * c + 2], a0);
+        a1 = fma(x3, q0[4 * c + 3], a1);
+        b0 = fma(x0, q1[4 * c], b0);
+        b1 = fma(x1, q1[4 * c + 1], b1);
+        b0 = fma(x2, q1[4 * c + 2], b0);
+        b1 = fma(x3, q1[4 * c + 3], b1);
+      }
+      double za = a0 + a1, zb = b0 + b1;
+      if (TPR == 2) {
+        // lanes 2r and 2r + 1 hold the two halves of row r's sums; each adds
+        // its partner's (addition commutes, so both lanes get the same
+        // value).  No lane-dependent select around the lane swap: the
+        // compiler turns such a select into a branch and the swap inside it
+        // then reads the inactive partner.
+        za += dpp_mov_f64<0xB1>(za);
+        zb += dpp_mov_f64<0xB1>(zb);
+      } else {
+        // both sums formed here: the compiler would otherwise sink the
+        // second column's FMAs into its conditional store and keep every
+        // converted row element alive until then
+        asm volatile("" : "+v"(za), "+v"(zb));
+      }
+      if (live && h == 0) {
+        z[(long long)(c0 + n) * zstride] = (float)za;
+        if (n + 1 < nb) z[(long long)(c0 + n + 1) * zstride] = (float)zb;
       }
     }
   }
 }
 
-// One tile per workgroup of NT threads: every thread stages, then every
-// wave multiplies.  512 threads (eight waves) put twice the loads of a tile
-// in flight and, at three tiles per CU (the LDS bound), six waves per SIMD.
-constexpr int kNT1 = 512;
+// At least four waves per SIMD (at most 128 VGPRs): the row of a thread and
+// the loads in flight per CU are what set the stage's HBM rate.
 template <int NT>
-__global__ __launch_bounds__(NT) void rg_e1_kernel(Bufs b) {
-  constexpr int NW = NT / kWave;
-  __shared__ float s_a[kE1A];
-  __shared__ double s_h[kE1H];
-  __shared__ double s_q[kE1Q];
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void rg_e1_kernel(Bufs b) {
+  __shared__ double s_q[kE1Cols * kE1Ld];
   __shared__ E1Rec s_rec;
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
-  const int tile = rg_xcd_tile(blockIdx.x, gridDim.x);
-  long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_units + tile) : nullptr;
+  const int tid = threadIdx.x;
+  long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_units + blockIdx.x) : nullptr;
   if (tr && tid == 0) tr[0] = wall_clock64();
-  rg_copy_lds(b.e1 + tile, &s_rec, tid);
+  rg_copy_lds<E1Rec, NT>(b.e1 + blockIdx.x, &s_rec, tid);
   lds_barrier();
-  rg_e1_stage<NT>(b, s_rec, s_a, s_h, s_q, tid);
-  lds_barrier();
-  rg_e1_mfma<NW>(b, s_rec, s_a, s_h, s_q, lane, wave);
+  switch ((s_rec.d2 + 3) >> 2) {
+#define RG_E1_CASE(nv, tpr)                                  \
+  case nv:                                                   \
+    static_assert(rg_e1_tpr(4 * nv) == tpr, "threads per row"); \
+    rg_e1_rows<nv, NT, tpr>(b, s_rec, s_q, tid);             \
+    break;
+    // threads per row as rg_e1_tpr(d2), which the host tiling follows
+    RG_E1_CASE(1, 1) RG_E1_CASE(2, 1) RG_E1_CASE(3, 1) RG_E1_CASE(4, 1) RG_E1_CASE(5, 1) RG_E1_CASE(6, 1)
+    RG_E1_CASE(7, 1) RG_E1_CASE(8, 1) RG_E1_CASE(9, 2) RG_E1_CASE(10, 2) RG_E1_CASE(11, 2) RG_E1_CASE(12, 2)
+    RG_E1_CASE(13, 2) RG_E1_CASE(14, 2) RG_E1_CASE(15, 2) RG_E1_CASE(16, 2)
+#undef RG_E1_CASE
+    default:
+      break;
+  }
   if (tr && tid == 0) tr[1] = wall_clock64();
 }
 
-// sum_o A ln A of every (position, k) of a modality, summed in FP64 (the
-// ambiguity row of the stage-1 product): one thread per element, the
-// outcome loop with eight loads in flight.
+// sum_o A ln A of every (position, k) of a modality, summed in FP64 and
+// stored in FP32 (the ambiguity row of the stage-1 product, whose output Z1
+// is FP32 too): one thread per element, the outcome loop with eight loads in
+// flight.
 __global__ __launch_bounds__(kNT) void rg_ha_kernel(Bufs b) {
   const HaTask tk = b.hat[blockIdx.x];
   const int e = tk.e0 + (int)threadIdx.x;
@@ -970,7 +920,7 @@ __global__ __launch_bounds__(kNT) void rg_ha_kernel(Bufs b) {
     }
   }
   for (; o < tk.O; ++o) s0 += xlogx_fast(col[o * st]);
-  b.ha[tk.ha_off + e] = s0 + s1;
+  b.ha[tk.ha_off + e] = (float)(s0 + s1);
 }
 
 // Stage 2 at level t (FP64 MFMA): rows R = (h2, o, i) in [r0, r1) of Z1 (length d1),
@@ -1187,7 +1137,7 @@ __global__ __launch_bounds__(kNT) void rg_assemble_kernel(Bufs b) {
 // ---------------------------------------------------------------------------
 // EFE tile sizes (see build_tiles); the defaults are the LDS capacities
 struct TileCfg {
-  int e1_a = kE1A;            // A floats per stage-1 tile
+  int e1_rows = 256;          // rows per stage-1 work item (= its workgroup size)
   int e2_t = kTile2;          // Z1 doubles per stage-2 tile
   int e3_t = kTile;           // Z2 doubles per stage-3 tile
   int e3_combos = kE3Combos;  // history combinations scored per stage-3 tile
@@ -1195,7 +1145,7 @@ struct TileCfg {
 };
 
 // autotuner candidates (each at most its LDS capacity)
-constexpr int kTuneE1A[] = {4352, 6528, kE1A};
+constexpr int kTuneE1R[] = {128, 256};
 constexpr int kTuneE2T[] = {2304, kTile2};
 constexpr int kTuneE3T[] = {2304, kTile};
 constexpr int kTuneCombos[] = {64, kE3Combos};
@@ -1232,7 +1182,7 @@ struct Plan {
   E1Rec* d_e1 = nullptr;
   HaTask* d_hat = nullptr;
   RollTask* d_roll = nullptr;
-  double* d_ha = nullptr;
+  float* d_ha = nullptr;
   int32_t* d_iarena = nullptr;
   double *d_hist = nullptr, *d_score = nullptr, *d_part = nullptr;
   float *d_z1 = nullptr, *d_z2 = nullptr;
@@ -1276,10 +1226,9 @@ static double efe_cost(const std::vector<int>& dims, const std::vector<int>& Hs,
 using namespace pmh;
 using namespace pmh::rg;
 
-// EFE tiling from the agents' layouts.  Stage 1: blocks of (i, j) positions
-// whose A rows (at most e1_a LDS floats) and sum_o A ln A rows fit the
-// tile's LDS x blocks of at most 16 concatenated last-dim histories; stage
-// 2: blocks of Z1 rows (at most e2_t LDS doubles) x at most `cols` middle-dim
+// EFE tiling from the agents' layouts.  Stage 1: blocks of e1_rows rows of
+// one modality's row space [O1][d01], all its last-axis history columns;
+// stage 2: blocks of Z1 rows (at most e2_t LDS doubles) x at most `cols` middle-dim
 // histories of one level; stage 3: blocks of (h2, h1) pairs (at most e3_t
 // LDS doubles and e3_combos scored combinations) x at most `cols` first-dim
 // histories.  Every tile list is ordered by decreasing cost.
@@ -1298,13 +1247,11 @@ static void build_tiles(Plan* pl, const TileCfg& cfg) {
       const int d0 = a.ed[m][0], d1 = a.ed[m][1], d2 = a.ed[m][2];
       auto Hc = [&](int t, int p) { return a.ef[m][p] < 0 ? 1 : a.H[t][a.ef[m][p]]; };
       const int N1 = a.tstart[m][a.T];
-      const int d01 = d0 * d1;
-      const int nr1 = rg_e1_rows(a.O[m], d2, d01, cfg.e1_a);
-      for (int r0 = 0; r0 < d01; r0 += nr1)
-        for (int c0 = 0; c0 < N1; c0 += 16) {
-          const int r1 = std::min(d01, r0 + nr1), c1 = std::min(N1, c0 + 16);
-          t1.push_back(TileC{Tile{ai, m, 0, r0, r1, c0, c1, 0}, (double)(r1 - r0) * O1 * d2 * 16});
-        }
+      const int d01 = d0 * d1, rows1 = O1 * d01, rpi = cfg.e1_rows / rg_e1_tpr(d2);
+      for (int e0 = 0; e0 < rows1; e0 += rpi) {
+        const int e1 = std::min(rows1, e0 + rpi);
+        t1.push_back(TileC{Tile{ai, m, 0, e0, e1, 0, N1, 0}, (double)(e1 - e0) * d2 * (N1 + 4)});
+      }
       pl->efe_macs += (double)d01 * O1 * d2 * N1;
       for (int t = 0; t < a.T; ++t) {
         const int H0 = Hc(t, 0), H1 = Hc(t, 1), H2 = Hc(t, 2);
@@ -1343,24 +1290,17 @@ static void build_tiles(Plan* pl, const TileCfg& cfg) {
     const int m = tl.m, f = a.ef[m][2];
     E1Rec r;
     std::memset(&r, 0, sizeof(r));
+    r.a_off = a.A_off[m];
+    r.h_off = a.ha_off[m];
+    r.z_off = a.z1_off[m];
+    for (int t = 0; t < kT; ++t) r.hb[t] = t < a.T ? a.hist_off + a.hist_lvl[t][f] : 0;
+    for (int t = 0; t <= kT; ++t) r.ts[t] = a.tstart[m][t];
     r.O = a.O[m];
     r.d2 = a.ed[m][2];
     r.d01 = a.ed[m][0] * a.ed[m][1];
-    r.nr = tl.r1 - tl.r0;
-    r.nc = tl.c1 - tl.c0;
-    r.c0 = tl.c0;
-    r.a_off = a.A_off[m] + (long long)tl.r0 * a.O[m] * a.ed[m][2];
-    r.h_off = a.ha_off[m] + (long long)tl.r0 * a.ed[m][2];
-    r.z_off = a.z1_off[m] + tl.r0;
-    for (int c = 0; c < 16; ++c) {
-      r.cb[c] = -1;
-      if (c < r.nc) {
-        const int n = tl.c0 + c;
-        int t = 0;
-        while (t + 1 < a.T && n >= a.tstart[m][t + 1]) ++t;
-        r.cb[c] = a.hist_off + a.hist_lvl[t][f] + (long long)(n - a.tstart[m][t]) * a.S[f];
-      }
-    }
+    r.e0 = tl.r0;
+    r.e1 = tl.r1;
+    r.N1 = a.tstart[m][a.T];
     pl->e1.push_back(r);
   }
   for (auto& x : t2) pl->t2.push_back(x.t);
@@ -1700,7 +1640,6 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
       // columns; the modality's sum_o A ln A table ([pos][d2], rg_ha_kernel)
       {
         const int d01 = d0 * d1;
-        if (a.O[m] * (((d2 + 3) & ~3) + 1) > kE1A) return fail("EFE tile does not fit");
         const long long nel = (long long)d01 * d2;
         a.ha_off[m] = pl->ha_total;
         for (long long e0 = 0; e0 < nel; e0 += kNT)
@@ -1812,7 +1751,7 @@ static int upload(Plan* pl) {
   // the autotuner can re-tile without reallocating
   {
     const TileCfg keep = pl->tiles;
-    build_tiles(pl, TileCfg{kTuneE1A[0], kTuneE2T[0], kTuneE3T[0], kTuneCombos[0], kTuneCols[0]});
+    build_tiles(pl, TileCfg{kTuneE1R[0], kTuneE2T[0], kTuneE3T[0], kTuneCombos[0], kTuneCols[0]});
     pl->cap_t1 = pl->t1.size();
     pl->cap_t2 = pl->t2.size();
     pl->cap_t3 = pl->t3.size();
@@ -1829,7 +1768,8 @@ static int upload(Plan* pl) {
   if ((e = alloc((void**)&pl->d_hat, sizeof(HaTask) * pl->hat.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_roll, sizeof(RollTask) * pl->roll.size())) != hipSuccess)
     return fail_dev(e, "hipMalloc");
-  if ((e = alloc((void**)&pl->d_ha, sizeof(double) * pl->ha_total)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  // 16 bytes of tail padding: stage 1 reads the table's rows as 16-byte vectors
+  if ((e = alloc((void**)&pl->d_ha, sizeof(float) * pl->ha_total + 16)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_t2, sizeof(Tile) * pl->cap_t2)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_t3, sizeof(Tile) * pl->cap_t3)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_iarena, sizeof(int32_t) * pl->iarena.size())) != hipSuccess)
@@ -2000,7 +1940,12 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
     if (n)
       hipLaunchKernelGGL(rg_roll_kernel, dim3((unsigned)n), dim3(kNT), 0, s, b, pl->roll_start[t]);
   }
-  if (b.n_t1) hipLaunchKernelGGL(rg_e1_kernel<kNT1>, dim3(b.n_t1), dim3(kNT1), 0, s, b);
+  if (b.n_t1) {
+    if (pl->tiles.e1_rows == 128)
+      hipLaunchKernelGGL(rg_e1_kernel<128>, dim3(b.n_t1), dim3(128), 0, s, b);
+    else
+      hipLaunchKernelGGL(rg_e1_kernel<256>, dim3(b.n_t1), dim3(256), 0, s, b);
+  }
   if (b.n_t2) hipLaunchKernelGGL(rg_e2_kernel, dim3(b.n_t2), dim3(kNT2), 0, s, b);
   if (b.n_t3) hipLaunchKernelGGL(rg_e3_kernel, dim3(b.n_t3), dim3(kNT3), 0, s, b);
   hipLaunchKernelGGL(rg_assemble_kernel, dim3((unsigned)pl->n_agents), dim3(kNT), 0, s, b);
@@ -2100,10 +2045,10 @@ extern "C" int pymdp_hip_ragged_autotune(void* plan, const float* params, const 
       }
       return 0;
     };
-    for (int x : kTuneE1A) {
+    for (int x : kTuneE1R) {
       TileCfg c = bc;
-      c.e1_a = x;
-      if (x != bc.e1_a && (rc = trial(c))) return rc;
+      c.e1_rows = x;
+      if (x != bc.e1_rows && (rc = trial(c))) return rc;
     }
     for (int x : kTuneE2T) {
       TileCfg c = bc;
@@ -2155,7 +2100,7 @@ extern "C" int pymdp_hip_ragged_tile_config(void* plan, int32_t* out8) {
   Plan* pl = static_cast<Plan*>(plan);
   if (!pl || !out8) return einval("plan or out8 is NULL");
   const TileCfg& c = pl->tiles;
-  const int32_t v[8] = {c.e1_a, c.e2_t, c.e3_t, c.e3_combos, c.cols, pl->tuned ? 1 : 0, pl->grid1, 0};
+  const int32_t v[8] = {c.e1_rows, c.e2_t, c.e3_t, c.e3_combos, c.cols, pl->tuned ? 1 : 0, pl->grid1, 0};
   std::memcpy(out8, v, sizeof(v));
   return PYMDP_HIP_OK;
 }
@@ -2164,15 +2109,15 @@ extern "C" int pymdp_hip_ragged_set_tiles(void* plan, const int32_t* cfg5) {
   Plan* pl = static_cast<Plan*>(plan);
   if (!pl || !cfg5) return einval("plan or cfg5 is NULL");
   TileCfg c;
-  c.e1_a = cfg5[0];
+  c.e1_rows = cfg5[0];
   c.e2_t = cfg5[1];
   c.e3_t = cfg5[2];
   c.e3_combos = cfg5[3];
   c.cols = cfg5[4];
-  if (c.e1_a < kTuneE1A[0] || c.e1_a > kE1A || c.e2_t < kTuneE2T[0] || c.e2_t > kTile2 || c.e3_t < kTuneE3T[0] ||
+  if ((c.e1_rows != 128 && c.e1_rows != 256) || c.e2_t < kTuneE2T[0] || c.e2_t > kTile2 || c.e3_t < kTuneE3T[0] ||
       c.e3_t > kTile || c.e3_combos < kTuneCombos[0] || c.e3_combos > kE3Combos || c.cols < kTuneCols[0] ||
       c.cols > kCols)
-    return einval("tile sizes out of range (stage 1 A floats 4352..8704, stage 2 / 3 doubles 2304..4608, "
+    return einval("tile sizes out of range (stage 1 rows 128 or 256, stage 2 / 3 doubles 2304..4608, "
                   "combinations 64..128, columns 16..32)");
   build_tiles(pl, c);
   if (pl->d_ag && upload_tiles(pl) != hipSuccess) {

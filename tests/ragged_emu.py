@@ -19,7 +19,7 @@ kF, kM, kR, kT, kDim, kNT = 8, 8, 3, 4, 64, 256
 kCap, kRowsCap, kTeamMax = 6144, 1024, 128
 kTile, kTile2, kCols = 4608, 4608, 32
 kQTab = kCols * 65
-kE1A, kE1H, kE1Q = 8704, 1024, 16 * 65
+kE1Cols, kE1Ld = 16, 64  # stage-1 history columns staged at a time, their LDS row stride
 
 AGENT_DT = np.dtype([
     ("F", "i4"), ("M", "i4"), ("T", "i4"), ("P", "i4"), ("G", "i4"), ("iters", "i4"), ("totS", "i4"),
@@ -42,8 +42,14 @@ AGENT_DT = np.dtype([
 
 UNIT_DT = np.dtype([("agent", "i4"), ("slice", "i4"), ("nseg", "i4"), ("pad", "i4"), ("g", "i4", (kM,)),
                     ("r0", "i4", (kM,)), ("r1", "i4", (kM,)), ("lds", "i4", (kM,))])
-E1REC_DT = np.dtype([("a_off", "i8"), ("h_off", "i8"), ("z_off", "i8"), ("cb", "i8", (16,)), ("O", "i4"), ("d2", "i4"),
-                     ("d01", "i4"), ("nr", "i4"), ("nc", "i4"), ("c0", "i4"), ("pad0", "i4"), ("pad1", "i4")])
+E1REC_DT = np.dtype([("a_off", "i8"), ("h_off", "i8"), ("z_off", "i8"), ("hb", "i8", (kT,)), ("ts", "i4", (kT + 1,)),
+                     ("O", "i4"), ("d2", "i4"), ("d01", "i4"), ("e0", "i4"), ("e1", "i4"), ("N1", "i4"),
+                     ("pad", "i4")])
+
+
+def e1_tpr(d2):
+    """Threads per stage-1 row (rg_e1_tpr)."""
+    return 2 if d2 > 32 else 1
 ROLL_DT = np.dtype([("b_off", "i8"), ("out", "i8"), ("par", "i8", (3,)), ("ent", "i8"), ("Sf", "i4"), ("KB", "i4"),
                     ("rb", "i4"), ("nh", "i4"), ("t", "i4"), ("dd", "i4", (3,))], align=True)
 TILE_DT = np.dtype([("agent", "i4"), ("m", "i4"), ("t", "i4"), ("r0", "i4"), ("r1", "i4"), ("c0", "i4"),
@@ -314,55 +320,71 @@ class Emu:
         return q
 
     def e1(self, tl, rec):
-        ai, m, _, r0, r1, c0, c1 = (int(tl[k]) for k in ("agent", "m", "t", "r0", "r1", "c0", "c1"))
+        """rg_e1_kernel: rows [e0, e1) of the modality's row space [O1][d01]
+        (row e = o d01 + pos: A_m[pos, o, :] for o < O, the FP32 sum_o A ln A
+        table's row pos for o = O) times every last-axis history column."""
+        ai, m, e0, e1 = (int(tl[k]) for k in ("agent", "m", "r0", "r1"))
         a = self.ag[ai]
         O = int(a["O"][m])
         O1 = O + 1
         d0, d1, d2 = (int(x) for x in a["ed"][m])
-        d01, ld2 = d0 * d1, d2 | 1
-        nr, nc = r1 - r0, c1 - c0
-        assert 1 <= nc <= 16 and 1 <= nr and r1 <= d01 and d2 <= 64
-        lda = ((d2 + 3) & ~3) + 1
-        _chk(np.array([(O * nr - 1) * lda + d2 - 1]), kE1A, "s_a (E1)")
-        _chk(np.array([(nr - 1) * lda + d2 - 1]), kE1H, "s_h (E1)")
-        _chk(np.array([15 * lda + ((d2 + 3) & ~3) - 1]), kE1Q, "s_q (E1)")
-        _chk(np.array([nr * d2 - 1]), kE1H, "sum A ln A block (E1)")
-        K = int(a["K"][m])
-        assert K == d01 * d2
-        # the tile's record: sizes and addresses as derived from the agent
-        assert (int(rec["O"]), int(rec["d2"]), int(rec["d01"]), int(rec["nr"]), int(rec["nc"]), int(rec["c0"])) == \
-            (O, d2, d01, nr, nc, c0)
-        assert int(rec["a_off"]) == int(a["A_off"][m]) + r0 * O * d2
-        assert int(rec["z_off"]) == int(a["z1_off"][m]) + r0
-        assert int(rec["h_off"]) == int(a["ha_off"][m]) + r0 * d2
-        _chk(int(rec["h_off"]) + np.arange(nr * d2), self.info["ha_doubles"], "sum A ln A table (E1)")
-        # position-major A_m: (position, o, k) at (position O + o) d2 + k, from the record's offset
-        idx = int(rec["a_off"]) + ((np.arange(nr))[None, :, None] * O + np.arange(O)[:, None, None]) * d2 + \
-            np.arange(d2)[None, None, :]
-        At = self.params[_chk(idx, self.params.size, "A (E1)")]  # [O, nr, d2]
-        xl = np.where(At == 0, np.float32(0), At * np.log(np.maximum(At, MINVAL)).astype(np.float32))
-        ha = xl.astype(np.float64).sum(0)
-        X = np.concatenate([At.astype(np.float64), ha[None]], 0)  # [O1, nr, d2]
+        d01 = d0 * d1
         T = int(a["T"])
         f = int(a["ef"][m][2])
-        cols = []
-        for n in range(c0, c1):
-            t = 0
-            while t + 1 < T and n >= a["tstart"][m][t + 1]:
-                t += 1
-            h = n - int(a["tstart"][m][t])
-            assert 0 <= h < a["H"][t][f]
-            col = self._hist_table(a, m, t, 2, h, h + 1)[0]
-            cb = int(rec["cb"][n - c0])
-            assert cb == int(a["hist_off"]) + int(a["hist_lvl"][t][f]) + h * int(a["S"][f])
-            np.testing.assert_array_equal(self.hist[_chk(cb + np.arange(d2), self.hist.size, "hist (E1)")], col)
-            cols.append(col)
-        assert (rec["cb"][nc:] == -1).all()
-        Q = np.stack(cols)  # [nc, d2]
-        Y = np.einsum("ork,ck->cor", X, Q)  # [nc, O1, nr]
-        zi = int(a["z1_off"][m]) + ((c0 + np.arange(nc))[:, None, None] * O1 + np.arange(O1)[None, :, None]) * d01 + \
-            r0 + np.arange(nr)[None, None, :]
         N1 = int(a["tstart"][m][T])
+        tpr = e1_tpr(d2)
+        nv = (d2 + 3) // 4
+        nvt = (nv + tpr - 1) // tpr
+        kp = 4 * nvt * tpr
+        assert 1 <= nv <= 16 and kp <= kE1Ld
+        rows_per_item = self.plan.tile_config()["e1_rows"] // tpr
+        assert 0 <= e0 < e1 <= O1 * d01 and e1 - e0 <= rows_per_item
+        assert (int(tl["c0"]), int(tl["c1"])) == (0, N1)
+        assert int(a["K"][m]) == d01 * d2
+        # the item's record: sizes and addresses as derived from the agent
+        assert (int(rec["O"]), int(rec["d2"]), int(rec["d01"]), int(rec["e0"]), int(rec["e1"]), int(rec["N1"])) == \
+            (O, d2, d01, e0, e1, N1)
+        assert int(rec["a_off"]) == int(a["A_off"][m])
+        assert int(rec["h_off"]) == int(a["ha_off"][m])
+        assert int(rec["z_off"]) == int(a["z1_off"][m])
+        for t in range(kT + 1):
+            assert int(rec["ts"][t]) == int(a["tstart"][m][t])
+        for t in range(T):
+            assert int(rec["hb"][t]) == int(a["hist_off"]) + int(a["hist_lvl"][t][f])
+        e = np.arange(e0, e1)
+        o, pos = e // d01, e % d01
+        # every 16-byte row load (up to 3 floats past the row: the params
+        # arena keeps at least 3 floats of the agent after its last A block,
+        # the table 16 bytes of tail padding)
+        isA = o < O
+        start = np.where(isA, int(rec["a_off"]) + (pos * O + o) * d2, int(rec["h_off"]) + pos * d2)
+        last = start + 4 * nv - 1
+        _chk(np.concatenate([start[isA], last[isA]]), self.params.size, "A row (E1)")
+        _chk(np.concatenate([start[~isA], last[~isA]]), self.info["ha_floats"] + 4, "sum A ln A row (E1)")
+        # the rows: A in FP32, the table formed as rg_ha_kernel (FP32 xlogx
+        # terms summed in FP64, stored FP32)
+        pidx = start[:, None] + np.arange(d2)[None, :]
+        Xa = self.params[_chk(np.where(isA[:, None], pidx, 0), self.params.size, "A (E1)")]
+        Am = self.params[_chk(int(a["A_off"][m]) + np.arange(d01 * O * d2), self.params.size, "A block (E1)")]
+        Am = Am.reshape(d01, O, d2)
+        xl = np.where(Am == 0, np.float32(0), Am * np.log(np.maximum(Am, MINVAL)).astype(np.float32))
+        ha = xl.astype(np.float64).sum(1).astype(np.float32)  # [d01, d2]
+        X = np.where(isA[:, None], Xa, ha[pos]).astype(np.float64)  # [rows, d2]
+        # history columns, staged 16 at a time as s_q[n][k] (k < kp)
+        cols = []
+        for n in range(N1):
+            t = sum(1 for lvl in range(1, kT) if n >= int(rec["ts"][lvl]))
+            h = n - int(rec["ts"][t])
+            assert t < T and 0 <= h < a["H"][t][f]
+            base = int(rec["hb"][t]) + h * d2
+            col = self.hist[_chk(base + np.arange(d2), self.hist.size, "hist (E1)")]
+            np.testing.assert_array_equal(col, self._hist_table(a, m, t, 2, h, h + 1)[0])
+            assert np.isfinite(col).all()
+            cols.append(col)
+            _chk(np.array([(n % kE1Cols) * kE1Ld + kp - 1]), kE1Cols * kE1Ld, "s_q (E1)")
+        Q = np.stack(cols)  # [N1, d2]
+        Y = X @ Q.T  # [rows, N1]
+        zi = int(rec["z_off"]) + np.arange(N1)[None, :] * (O1 * d01) + e[:, None]
         _chk(zi - int(a["z1_off"][m]), N1 * O1 * d01, "Z1 (block)")
         self.z1[_chk(zi, self.z1.size, "Z1")] = Y
 
@@ -600,28 +622,41 @@ def check_bounds_only(plan):
             hi = 0 if int(rt["t"]) == 0 else int(ent[:, 2 + j].max())
             assert int(rt["par"][j]) + (hi + 1) * int(rt["dd"][j]) <= info["hist_doubles"]
         assert int(rt["out"]) + (int(ent[:, 0].max()) + 1) * Sf <= info["hist_doubles"]
+    e1_rows = plan.tile_config()["e1_rows"]
     for tl in t1:
         a = ag[int(tl["agent"])]
         m = int(tl["m"])
         O1 = int(a["O"][m]) + 1
         d0, d1, d2 = (int(x) for x in a["ed"][m])
-        nr, nc = int(tl["r1"]) - int(tl["r0"]), int(tl["c1"]) - int(tl["c0"])
-        assert 1 <= nr and int(tl["r1"]) <= d0 * d1 and 1 <= nc <= kCols
-        assert int(tl["c1"]) <= int(a["tstart"][m][int(a["T"])])
-        lda = ((d2 + 3) & ~3) + 1
-        assert nc <= 16 and int(a["O"][m]) * nr * lda <= kE1A and nr * lda <= kE1H
-    # stage-1 records: the A block, the history columns and the output block stay inside their buffers
+        e0, e1 = int(tl["r0"]), int(tl["r1"])
+        assert 0 <= e0 < e1 <= O1 * d0 * d1 and e1 - e0 <= e1_rows // e1_tpr(d2) and 1 <= d2 <= kDim
+        assert (int(tl["c0"]), int(tl["c1"])) == (0, int(a["tstart"][m][int(a["T"])]))
+    # stage-1 records: every row load (up to 3 floats past a row), history
+    # column and output element stays inside its buffer
     rec = e1_records(plan)
     assert len(rec) == len(t1)
     if len(rec):
         O_, d2_, d01_ = rec["O"].astype(np.int64), rec["d2"].astype(np.int64), rec["d01"].astype(np.int64)
-        assert (rec["a_off"] >= 0).all() and (rec["a_off"] + rec["nr"] * O_ * d2_ <= params_n).all()
-        valid = np.arange(16)[None, :] < rec["nc"][:, None]
-        assert ((rec["cb"] >= 0) == valid).all()
-        assert (rec["cb"][valid] + np.repeat(d2_, 16).reshape(-1, 16)[valid] <= info["hist_doubles"]).all()
-        last = rec["z_off"] + ((rec["c0"] + rec["nc"] - 1) * (O_ + 1) + O_) * d01_ + rec["nr"] - 1
+        e0_, e1_ = rec["e0"].astype(np.int64), rec["e1"].astype(np.int64)
+        nv4 = 4 * ((d2_ + 3) // 4)
+        assert (rec["e0"] == t1["r0"]).all() and (rec["e1"] == t1["r1"]).all()
+        # A rows: the largest row of the modality that an item touches (rows of
+        # a modality are laid out [pos][O][d2], the last is pos = d01 - 1,
+        # o = O - 1)
+        hasA = e0_ < O_ * d01_
+        last_a = rec["a_off"] + ((d01_ - 1) * O_ + O_ - 1) * d2_ + nv4 - 1
+        assert (rec["a_off"] >= 0).all() and (last_a[hasA] < params_n).all()
+        hasH = e1_ > O_ * d01_
+        last_h = rec["h_off"] + (e1_ - 1 - O_ * d01_) * d2_ + nv4 - 1
+        assert (rec["h_off"] >= 0).all() and (last_h[hasH] < info["ha_floats"] + 4).all()
+        N1 = rec["N1"].astype(np.int64)
+        for t in range(kT):
+            lvl = rec["ts"][:, t + 1] > rec["ts"][:, t]
+            last_hist = rec["hb"][:, t] + (rec["ts"][:, t + 1] - rec["ts"][:, t]) * d2_
+            assert (rec["hb"][lvl, t] >= 0).all() and (last_hist[lvl] <= info["hist_doubles"]).all()
+        assert (rec["ts"][:, 0] == 0).all() and (rec["ts"][:, kT] == N1).all()
+        last = rec["z_off"] + (N1 - 1) * (O_ + 1) * d01_ + e1_ - 1
         assert (rec["z_off"] >= 0).all() and (last < info["z1_doubles"]).all()
-        assert (rec["h_off"] >= 0).all() and (rec["h_off"] + rec["nr"] * d2_ <= info["ha_doubles"]).all()
     for tl in t2:
         a = ag[int(tl["agent"])]
         m, t = int(tl["m"]), int(tl["t"])
