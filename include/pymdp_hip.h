@@ -446,6 +446,13 @@ int pymdp_hip_ragged_tables(void* plan, int64_t* sizes16, void* agents_out,
  * sizes and the row range); count_out
  * / record_bytes receive the count and the record size; out may be NULL. */
 int pymdp_hip_ragged_e1_records(void* plan, void* out, int64_t* count_out, int64_t* record_bytes);
+/* Host copies of the stage-2 work-item records (same record layout: per
+ * item, a range of one level's stage-1 output rows, the z1 offset of the
+ * level's first row, the stage-2 output offset, the history offset of the
+ * level's middle-factor histories, their count, the row length and the rows
+ * per output block); count_out / record_bytes / out as for the stage-1
+ * records. */
+int pymdp_hip_ragged_e2_records(void* plan, void* out, int64_t* count_out, int64_t* record_bytes);
 /* Host copies of the rollout task records (per task: params offset of B_f,
  * history offsets of its outputs and of each dependency's parent level,
  * int-arena offset of its history records, sizes and level); level_start
@@ -471,17 +478,17 @@ int pymdp_hip_ragged_step(void* plan, const float* params, const int32_t* obs,
                           float* out, int32_t* actions, void* stream);
 /* The plan's autotuner (no counterpart in the reference): times the
  * persistent FPI grid (1, 2, ... workgroups per CU up to the occupancy
- * limit), then the EFE tile sizes one stage at a time (stage-1 rows per work
- * item 128 / 256, stage-2 and stage-3 doubles per tile 2304 / 4608,
- * stage-3 combinations 64 / 128, histories per stage-2 / 3 tile 16 / 32),
+ * limit), then the EFE tile sizes one stage at a time (stage-1 and stage-2
+ * rows per work item 128 / 256, stage-3 doubles per tile 2304 / 4608,
+ * stage-3 combinations 64 / 128, histories per stage-3 tile 16 / 32),
  * each over whole steps, and keeps the fastest; *best_ms = that step time.
  * Not while `stream` is being captured; graphs captured before the call
  * must be captured again (the tile tables are rewritten). */
 int pymdp_hip_ragged_autotune(void* plan, const float* params, const int32_t* obs,
                               float* out, int32_t* actions, void* stream, float* best_ms);
-/* out8 = (stage-1 rows per work item, stage-2 doubles per tile, stage-3
+/* out8 = (stage-1 rows per work item, stage-2 rows per work item, stage-3
  *         doubles per tile, stage-3 combinations per tile, histories per
- *         stage-2 / 3 tile, 1 if autotuned, FPI grid, 0). */
+ *         stage-3 tile, 1 if autotuned, FPI grid, 0). */
 int pymdp_hip_ragged_tile_config(void* plan, int32_t* out8);
 /* Sets the EFE tile sizes (cfg5 = the first five values of
  * pymdp_hip_ragged_tile_config, each within the autotuner's range; EINVAL

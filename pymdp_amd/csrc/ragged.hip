@@ -1,23 +1,27 @@
 // Batched ragged agents: one step of infer_states -> infer_policies ->
 // sample_action for thousands of agents whose factor, modality, control and
-// policy counts all differ, as five stream-ordered launches that a caller can
+// policy counts all differ, as stream-ordered launches that a caller can
 // capture in one hipGraph.
 //
+//   rg_ha_kernel        sum_o A ln A of every modality (once under static
+//                       parameters).
 //   rg_fpi_kernel       persistent; a workgroup takes the next FPI unit from
 //                       an atomic queue.  A unit holds (a row range of) an
 //                       agent's log-likelihood tensors in LDS for all sweeps
 //                       of the fused log-likelihood + Jacobi FPI of
 //                       pymdp/algos.py:88-155.  An agent too large for one
 //                       unit is split over a team of units whose workgroups
-//                       exchange partial marginals once per sweep; slice 0
-//                       then runs the action-history rollout of
-//                       pymdp/control.py:351-389 (B . q per distinct history).
-//   rg_e1_kernel        one tile per workgroup: rows (o, i, j) of
+//                       exchange partial marginals once per sweep.
+//   rg_roll_kernel      one launch per policy level: the action-history
+//                       rollout of pymdp/control.py:351-389 (B . q per
+//                       distinct history).
+//   rg_rows_kernel<1>   stage 1, one row per thread: rows (o, i, j) of
 //                       [A ; sum_o A ln A] of one modality contracted over
 //                       its last storage axis with the history vectors of
 //                       every level at once (A is read once per step).
-//   rg_e2_kernel        the middle axis contracted with the middle factor's
-//                       histories of one level.
+//   rg_rows_kernel<2>   stage 2, the same row streaming over the stage-1
+//                       output: the middle axis contracted with the middle
+//                       factor's histories of one level.
 //   rg_e3_kernel        the first axis contracted with the first factor's
 //                       histories and every history combination scored:
 //                       entropy of qo + ambiguity + qo . C
@@ -61,7 +65,6 @@ constexpr int kCap = 6144;        // log-likelihood doubles one FPI unit keeps i
 constexpr int kRowsCap = 1024;    // rows of one FPI segment
 constexpr int kTeamMax = 128;     // FPI units of one agent
 constexpr int kTile = 4608;       // doubles of a stage-3 operand tile in LDS
-constexpr int kTile2 = 4608;      // doubles of a stage-2 operand tile in LDS
 constexpr int kCols = 32;         // histories per stage-2 / stage-3 tile
 constexpr int kQTab = kCols * 65;  // doubles of a stage-2 / stage-3 history table in LDS
 constexpr int kE3Combos = 128;    // history combinations scored per E3 tile
@@ -112,17 +115,20 @@ struct Tile {
   int agent, m, t, r0, r1, c0, c1, pad;
 };
 
-// Stage-1 work item: rows [e0, e1) of one modality's row space (see
-// rg_e1_kernel); every address the item needs comes from this record
-// (history column n of level t, ts[t] <= n < ts[t + 1], is the vector at
-// hb[t] + (n - ts[t]) * d2 of the history buffer).
+// Stage-1 / stage-2 work item: rows [e0, e1) of one row space (see
+// rg_rows); every address the item needs comes from this record (history
+// column n of level t, ts[t] <= n < ts[t + 1], is the vector at
+// hb[t] + (n - ts[t]) * d2 of the history buffer).  Stage 1: a_off / h_off
+// locate A_m and its sum_o A ln A table, blk = O1 d01 (one output block);
+// stage 2: a_off is the z1 offset of the level's first row (rows of d2
+// floats, contiguous), blk = O1 d0 rows per output block of N1 columns.
 struct E1Rec {
   long long a_off;   // params offset of A_m
   long long h_off;   // ha offset of the modality's sum_o A ln A table ([pos][d2])
   long long z_off;   // z1 offset of the modality
   long long hb[kT];  // hist offset of level t's first last-axis history vector
   int ts[kT + 1];    // first concatenated column of each level (ts[T..kT] = N1)
-  int O, d2, d01, e0, e1, N1, pad;
+  int O, d2, d01, e0, e1, N1, blk;
 };
 static_assert(sizeof(E1Rec) == 104, "stage-1 record layout");
 
@@ -152,6 +158,7 @@ struct Bufs {
   const Unit* units;
   const Tile* t1;
   const E1Rec* e1;
+  const E1Rec* e2;
   const HaTask* hat;
   const RollTask* roll;
   const Tile* t2;
@@ -761,8 +768,8 @@ typedef float rg_f4u __attribute__((ext_vector_type(4), aligned(4)));
 // lane swap).  Rows of more than 8 vectors take two threads: a thread
 // holding up to 64 floats would leave too few registers for the LDS
 // operands at four waves per SIMD (the compiler spills).
-template <int NV, int NT, int TPR>
-__device__ __forceinline__ void rg_e1_rows(const Bufs& b, const E1Rec& r, double* s_q, int tid) {
+template <int STAGE, int NV, int NT, int TPR>
+__device__ __forceinline__ void rg_rows(const Bufs& b, const E1Rec& r, double* s_q, int tid) {
   static_assert(TPR == 1 || TPR == 2, "threads per row");
   constexpr int NVT = (NV + TPR - 1) / TPR;  // vectors per thread
   const int O = r.O, d2 = r.d2, d01 = r.d01, N1 = r.N1;
@@ -770,13 +777,22 @@ __device__ __forceinline__ void rg_e1_rows(const Bufs& b, const E1Rec& r, double
   const int e = r.e0 + tid / TPR;
   const bool live = e < r.e1;
   const int ec = live ? e : r.e1 - 1;
-  const int o = ec / d01, pos = ec - o * d01;
-  // the row: A rows and the table's rows are both FP32 [.][d2]; the last
-  // vector may reach up to 3 floats past the row (the params arena holds at
-  // least 3 more floats of the same agent after every A block, the table 16
-  // bytes of tail padding); those lanes are zeroed below
-  const float* src = (o < O ? b.params + r.a_off + ((long long)pos * O + o) * d2 : b.ha + r.h_off + (long long)pos * d2) +
-                     4 * NVT * h;
+  // the row: A rows, the table's rows and Z1 rows are all FP32 [.][d2]; the
+  // last vector may reach up to 3 floats past the row (the params arena
+  // holds at least 3 more floats of the same agent after every A block, the
+  // table and Z1 16 bytes of tail padding); those lanes are zeroed below
+  const float* src;
+  float* z;
+  if (STAGE == 1) {
+    const int o = ec / d01, pos = ec - o * d01;
+    src = o < O ? b.params + r.a_off + ((long long)pos * O + o) * d2 : b.ha + r.h_off + (long long)pos * d2;
+    z = b.z1 + r.z_off + ec;
+  } else {
+    const int blk = r.blk, hb2 = ec / blk;
+    src = b.z1 + r.a_off + (long long)ec * d2;
+    z = b.z2 + r.z_off + (long long)hb2 * N1 * blk + (ec - hb2 * blk);
+  }
+  src += 4 * NVT * h;
   rg_f4u v[NVT];
 #pragma unroll
   for (int c = 0; c < NVT; ++c) {
@@ -785,8 +801,7 @@ __device__ __forceinline__ void rg_e1_rows(const Bufs& b, const E1Rec& r, double
     else
       v[c] = rg_f4u{0.0f, 0.0f, 0.0f, 0.0f};
   }
-  const long long zstride = (long long)(O + 1) * d01;  // between history columns of Z1
-  float* z = b.z1 + r.z_off + ec;
+  const long long zstride = r.blk;  // between history columns of the output
   constexpr int kp = 4 * NVT * TPR;  // staged length of a history column
   for (int c0 = 0; c0 < N1; c0 += kE1Cols) {
     const int nb = min(kE1Cols, N1 - c0);
@@ -868,22 +883,23 @@ __device__ __forceinline__ void rg_e1_rows(const Bufs& b, const E1Rec& r, double
   }
 }
 
-// At least four waves per SIMD (at most 128 VGPRs): the row of a thread and
-// the loads in flight per CU are what set the stage's HBM rate.
-template <int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void rg_e1_kernel(Bufs b) {
+// One work item per workgroup (stage 1: records b.e1, stage 2: b.e2).  At
+// least four waves per SIMD (at most 128 VGPRs): the row of a thread and the
+// loads in flight per CU are what set the stage's HBM rate.
+template <int STAGE, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void rg_rows_kernel(Bufs b) {
   __shared__ double s_q[kE1Cols * kE1Ld];
   __shared__ E1Rec s_rec;
   const int tid = threadIdx.x;
-  long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_units + blockIdx.x) : nullptr;
+  long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_units + (STAGE == 1 ? 0 : b.n_t1) + blockIdx.x) : nullptr;
   if (tr && tid == 0) tr[0] = wall_clock64();
-  rg_copy_lds<E1Rec, NT>(b.e1 + blockIdx.x, &s_rec, tid);
+  rg_copy_lds<E1Rec, NT>((STAGE == 1 ? b.e1 : b.e2) + blockIdx.x, &s_rec, tid);
   lds_barrier();
   switch ((s_rec.d2 + 3) >> 2) {
 #define RG_E1_CASE(nv, tpr)                                  \
   case nv:                                                   \
     static_assert(rg_e1_tpr(4 * nv) == tpr, "threads per row"); \
-    rg_e1_rows<nv, NT, tpr>(b, s_rec, s_q, tid);             \
+    rg_rows<STAGE, nv, NT, tpr>(b, s_rec, s_q, tid);         \
     break;
     // threads per row as rg_e1_tpr(d2), which the host tiling follows
     RG_E1_CASE(1, 1) RG_E1_CASE(2, 1) RG_E1_CASE(3, 1) RG_E1_CASE(4, 1) RG_E1_CASE(5, 1) RG_E1_CASE(6, 1)
@@ -921,37 +937,6 @@ __global__ __launch_bounds__(kNT) void rg_ha_kernel(Bufs b) {
   }
   for (; o < tk.O; ++o) s0 += xlogx_fast(col[o * st]);
   b.ha[tk.ha_off + e] = (float)(s0 + s1);
-}
-
-// Stage 2 at level t (FP64 MFMA): rows R = (h2, o, i) in [r0, r1) of Z1 (length d1),
-// columns = middle-dim histories [c0, c1).  Z2[((h2 * H1 + h1) * (O+1) + o) * d0 + i].
-// 512 threads: the tile is LDS-bound at 3 per CU, so more waves per tile hide
-// more of its load latency.
-constexpr int kNT2 = 512;
-__global__ __launch_bounds__(kNT2) void rg_e2_kernel(Bufs b) {
-  __shared__ double s_t[kTile2];
-  __shared__ double s_q[kQTab];
-  const int tid = threadIdx.x;
-  const Tile tl = b.t2[blockIdx.x];
-  const Agent& a = b.ag[tl.agent];
-  long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_units + b.n_t1 + blockIdx.x) : nullptr;
-  if (tr && tid == 0) tr[0] = wall_clock64();
-  const int m = tl.m, t = tl.t, O1 = a.O[m] + 1;
-  const int d0 = a.ed[m][0], d1 = a.ed[m][1], ld1 = d1 | 1;
-  const int nr = tl.r1 - tl.r0, nc = tl.c1 - tl.c0;
-  const int H1 = rg_hcount(a, m, t, 1);
-  const float* src = b.z1 + a.z1_off[m] + ((long long)a.tstart[m][t] * O1 * d0 + tl.r0) * d1;
-  rg_stage_rows_v<double, (kTile2 / 4 + 1 + kNT2 - 1) / kNT2, float, kNT2>(src, nr, d1, ld1, s_t, tid);
-  rg_load_hist<kNT2>(a, b.hist + a.hist_off, m, t, 1, tl.c0, tl.c1, s_q, tid);
-  __syncthreads();
-  float* Z = b.z2 + a.z2_off[m][t];
-  const int r0 = tl.r0, c0 = tl.c0, od = O1 * d0;
-  rg_gemm_mfma<kNT2 / kWave>(s_t, nr, s_q, nc, d1, tid & (kWave - 1), tid >> 6, [&](int r, int c, double v) {
-    const int R = r0 + r;
-    const int h2 = R / od, oi = R - h2 * od;
-    Z[((long long)h2 * H1 + c0 + c) * od + oi] = (float)v;
-  });
-  if (tr && tid == 0) tr[1] = wall_clock64();
 }
 
 // Stage 3 at level t: pairs (h2, h1) in [r0, r1) (pair = h2 * H1 + h1),
@@ -1138,15 +1123,15 @@ __global__ __launch_bounds__(kNT) void rg_assemble_kernel(Bufs b) {
 // EFE tile sizes (see build_tiles); the defaults are the LDS capacities
 struct TileCfg {
   int e1_rows = 256;          // rows per stage-1 work item (= its workgroup size)
-  int e2_t = kTile2;          // Z1 doubles per stage-2 tile
+  int e2_rows = 256;          // rows per stage-2 work item (= its workgroup size)
   int e3_t = kTile;           // Z2 doubles per stage-3 tile
   int e3_combos = kE3Combos;  // history combinations scored per stage-3 tile
-  int cols = kCols;           // histories per stage-2 / stage-3 tile
+  int cols = kCols;           // histories per stage-3 tile
 };
 
 // autotuner candidates (each at most its LDS capacity)
 constexpr int kTuneE1R[] = {128, 256};
-constexpr int kTuneE2T[] = {2304, kTile2};
+constexpr int kTuneE2R[] = {128, 256};
 constexpr int kTuneE3T[] = {2304, kTile};
 constexpr int kTuneCombos[] = {64, kE3Combos};
 constexpr int kTuneCols[] = {16, kCols};
@@ -1159,7 +1144,7 @@ struct Plan {
   std::vector<Agent> agents;
   std::vector<Unit> units;
   std::vector<Tile> t1, t2, t3;
-  std::vector<E1Rec> e1;
+  std::vector<E1Rec> e1, e2;
   std::vector<HaTask> hat;
   std::vector<RollTask> roll;     // rollout tasks, level by level
   int roll_start[kT + 1] = {};    // first task of each level
@@ -1179,7 +1164,7 @@ struct Plan {
   Agent* d_ag = nullptr;
   Unit* d_units = nullptr;
   Tile *d_t1 = nullptr, *d_t2 = nullptr, *d_t3 = nullptr;
-  E1Rec* d_e1 = nullptr;
+  E1Rec *d_e1 = nullptr, *d_e2 = nullptr;
   HaTask* d_hat = nullptr;
   RollTask* d_roll = nullptr;
   float* d_ha = nullptr;
@@ -1199,7 +1184,8 @@ static void free_plan(Plan* p) {
   int prev = 0;
   (void)hipGetDevice(&prev);
   (void)hipSetDevice(p->device);
-  for (void* x : {(void*)p->d_ag, (void*)p->d_units, (void*)p->d_t1, (void*)p->d_e1, (void*)p->d_hat, (void*)p->d_roll,
+  for (void* x : {(void*)p->d_ag, (void*)p->d_units, (void*)p->d_t1, (void*)p->d_e1, (void*)p->d_e2, (void*)p->d_hat,
+                  (void*)p->d_roll,
                   (void*)p->d_ha, (void*)p->d_t2, (void*)p->d_t3,
                   (void*)p->d_iarena, (void*)p->d_hist, (void*)p->d_score, (void*)p->d_z1, (void*)p->d_z2,
                   (void*)p->d_part, (void*)p->d_cnt})
@@ -1228,8 +1214,8 @@ using namespace pmh::rg;
 
 // EFE tiling from the agents' layouts.  Stage 1: blocks of e1_rows rows of
 // one modality's row space [O1][d01], all its last-axis history columns;
-// stage 2: blocks of Z1 rows (at most e2_t LDS doubles) x at most `cols` middle-dim
-// histories of one level; stage 3: blocks of (h2, h1) pairs (at most e3_t
+// stage 2: blocks of e2_rows rows of one level's Z1 rows (h2, o, i), all
+// the level's middle-dim histories; stage 3: blocks of (h2, h1) pairs (at most e3_t
 // LDS doubles and e3_combos scored combinations) x at most `cols` first-dim
 // histories.  Every tile list is ordered by decreasing cost.
 static void build_tiles(Plan* pl, const TileCfg& cfg) {
@@ -1256,13 +1242,11 @@ static void build_tiles(Plan* pl, const TileCfg& cfg) {
       for (int t = 0; t < a.T; ++t) {
         const int H0 = Hc(t, 0), H1 = Hc(t, 1), H2 = Hc(t, 2);
         if (R != 1) {
-          const int rows = H2 * O1 * d0, ld1 = d1 | 1;
-          const int nr = std::max(1, std::min(rows, cfg.e2_t / ld1));
-          for (int r0 = 0; r0 < rows; r0 += nr)
-            for (int c0 = 0; c0 < H1; c0 += cfg.cols) {
-              const int r1 = std::min(rows, r0 + nr), c1 = std::min(H1, c0 + cfg.cols);
-              t2.push_back(TileC{Tile{ai, m, t, r0, r1, c0, c1, 0}, (double)(r1 - r0) * d1 * (c1 - c0)});
-            }
+          const int rows = H2 * O1 * d0, rpi = cfg.e2_rows / rg_e1_tpr(d1);
+          for (int r0 = 0; r0 < rows; r0 += rpi) {
+            const int r1 = std::min(rows, r0 + rpi);
+            t2.push_back(TileC{Tile{ai, m, t, r0, r1, 0, H1, 0}, (double)(r1 - r0) * d1 * (H1 + 4)});
+          }
           pl->efe_macs += (double)rows * d1 * H1;
         }
         // E3 tiles: (h2, h1) pairs x first-dim histories
@@ -1301,9 +1285,33 @@ static void build_tiles(Plan* pl, const TileCfg& cfg) {
     r.e0 = tl.r0;
     r.e1 = tl.r1;
     r.N1 = a.tstart[m][a.T];
+    r.blk = (a.O[m] + 1) * r.d01;
     pl->e1.push_back(r);
   }
   for (auto& x : t2) pl->t2.push_back(x.t);
+  pl->e2.clear();
+  for (const Tile& tl : pl->t2) {
+    // rows (h2, o, i) of level t's Z1 block, contracted over the middle dim
+    // with that level's middle-factor histories
+    const Agent& a = pl->agents[tl.agent];
+    const int m = tl.m, t = tl.t, f = a.ef[m][1];
+    E1Rec r;
+    std::memset(&r, 0, sizeof(r));
+    const int O1 = a.O[m] + 1, d0 = a.ed[m][0], d1 = a.ed[m][1];
+    r.a_off = a.z1_off[m] + (long long)a.tstart[m][t] * O1 * d0 * d1;
+    r.z_off = a.z2_off[m][t];
+    r.hb[0] = a.hist_off + a.hist_lvl[t][f];
+    r.ts[0] = 0;
+    for (int l = 1; l <= kT; ++l) r.ts[l] = a.H[t][f];
+    r.O = a.O[m];
+    r.d2 = d1;
+    r.d01 = d0;
+    r.e0 = tl.r0;
+    r.e1 = tl.r1;
+    r.N1 = a.H[t][f];
+    r.blk = O1 * d0;
+    pl->e2.push_back(r);
+  }
   for (auto& x : t3) pl->t3.push_back(x.t);
   pl->tiles = cfg;
 }
@@ -1718,6 +1726,7 @@ static hipError_t upload_tiles(Plan* pl) {
   if ((e = up(pl->d_t1, pl->t1.data(), sizeof(Tile) * pl->t1.size())) != hipSuccess) return e;
   if ((e = up(pl->d_e1, pl->e1.data(), sizeof(E1Rec) * pl->e1.size())) != hipSuccess) return e;
   if ((e = up(pl->d_t2, pl->t2.data(), sizeof(Tile) * pl->t2.size())) != hipSuccess) return e;
+  if ((e = up(pl->d_e2, pl->e2.data(), sizeof(E1Rec) * pl->e2.size())) != hipSuccess) return e;
   return up(pl->d_t3, pl->t3.data(), sizeof(Tile) * pl->t3.size());
 }
 
@@ -1751,7 +1760,7 @@ static int upload(Plan* pl) {
   // the autotuner can re-tile without reallocating
   {
     const TileCfg keep = pl->tiles;
-    build_tiles(pl, TileCfg{kTuneE1R[0], kTuneE2T[0], kTuneE3T[0], kTuneCombos[0], kTuneCols[0]});
+    build_tiles(pl, TileCfg{kTuneE1R[0], kTuneE2R[0], kTuneE3T[0], kTuneCombos[0], kTuneCols[0]});
     pl->cap_t1 = pl->t1.size();
     pl->cap_t2 = pl->t2.size();
     pl->cap_t3 = pl->t3.size();
@@ -1765,6 +1774,7 @@ static int upload(Plan* pl) {
   if ((e = alloc((void**)&pl->d_units, sizeof(Unit) * pl->units.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_t1, sizeof(Tile) * pl->cap_t1)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_e1, sizeof(E1Rec) * pl->cap_t1)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_e2, sizeof(E1Rec) * pl->cap_t2)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_hat, sizeof(HaTask) * pl->hat.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_roll, sizeof(RollTask) * pl->roll.size())) != hipSuccess)
     return fail_dev(e, "hipMalloc");
@@ -1849,6 +1859,15 @@ extern "C" int pymdp_hip_ragged_e1_records(void* plan, void* out, int64_t* count
   return PYMDP_HIP_OK;
 }
 
+extern "C" int pymdp_hip_ragged_e2_records(void* plan, void* out, int64_t* count_out, int64_t* record_bytes) {
+  Plan* pl = static_cast<Plan*>(plan);
+  if (!pl) return einval("plan is NULL");
+  if (count_out) *count_out = (int64_t)pl->e2.size();
+  if (record_bytes) *record_bytes = (int64_t)sizeof(E1Rec);
+  if (out) std::memcpy(out, pl->e2.data(), sizeof(E1Rec) * pl->e2.size());
+  return PYMDP_HIP_OK;
+}
+
 extern "C" int pymdp_hip_ragged_roll_tasks(void* plan, void* out, int64_t* count_out, int64_t* record_bytes,
                                            int64_t* level_start) {
   Plan* pl = static_cast<Plan*>(plan);
@@ -1902,6 +1921,7 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
   b.units = pl->d_units;
   b.t1 = pl->d_t1;
   b.e1 = pl->d_e1;
+  b.e2 = pl->d_e2;
   b.hat = pl->d_hat;
   b.roll = pl->d_roll;
   b.ha = pl->d_ha;
@@ -1942,11 +1962,16 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
   }
   if (b.n_t1) {
     if (pl->tiles.e1_rows == 128)
-      hipLaunchKernelGGL(rg_e1_kernel<128>, dim3(b.n_t1), dim3(128), 0, s, b);
+      hipLaunchKernelGGL((rg_rows_kernel<1, 128>), dim3(b.n_t1), dim3(128), 0, s, b);
     else
-      hipLaunchKernelGGL(rg_e1_kernel<256>, dim3(b.n_t1), dim3(256), 0, s, b);
+      hipLaunchKernelGGL((rg_rows_kernel<1, 256>), dim3(b.n_t1), dim3(256), 0, s, b);
   }
-  if (b.n_t2) hipLaunchKernelGGL(rg_e2_kernel, dim3(b.n_t2), dim3(kNT2), 0, s, b);
+  if (b.n_t2) {
+    if (pl->tiles.e2_rows == 128)
+      hipLaunchKernelGGL((rg_rows_kernel<2, 128>), dim3(b.n_t2), dim3(128), 0, s, b);
+    else
+      hipLaunchKernelGGL((rg_rows_kernel<2, 256>), dim3(b.n_t2), dim3(256), 0, s, b);
+  }
   if (b.n_t3) hipLaunchKernelGGL(rg_e3_kernel, dim3(b.n_t3), dim3(kNT3), 0, s, b);
   hipLaunchKernelGGL(rg_assemble_kernel, dim3((unsigned)pl->n_agents), dim3(kNT), 0, s, b);
   PMH_HIP_CHECK(hipGetLastError(), "ragged: ");
@@ -2050,10 +2075,10 @@ extern "C" int pymdp_hip_ragged_autotune(void* plan, const float* params, const 
       c.e1_rows = x;
       if (x != bc.e1_rows && (rc = trial(c))) return rc;
     }
-    for (int x : kTuneE2T) {
+    for (int x : kTuneE2R) {
       TileCfg c = bc;
-      c.e2_t = x;
-      if (x != bc.e2_t && (rc = trial(c))) return rc;
+      c.e2_rows = x;
+      if (x != bc.e2_rows && (rc = trial(c))) return rc;
     }
     for (int x : kTuneE3T)
       for (int y : kTuneCombos) {
@@ -2100,7 +2125,7 @@ extern "C" int pymdp_hip_ragged_tile_config(void* plan, int32_t* out8) {
   Plan* pl = static_cast<Plan*>(plan);
   if (!pl || !out8) return einval("plan or out8 is NULL");
   const TileCfg& c = pl->tiles;
-  const int32_t v[8] = {c.e1_rows, c.e2_t, c.e3_t, c.e3_combos, c.cols, pl->tuned ? 1 : 0, pl->grid1, 0};
+  const int32_t v[8] = {c.e1_rows, c.e2_rows, c.e3_t, c.e3_combos, c.cols, pl->tuned ? 1 : 0, pl->grid1, 0};
   std::memcpy(out8, v, sizeof(v));
   return PYMDP_HIP_OK;
 }
@@ -2110,14 +2135,14 @@ extern "C" int pymdp_hip_ragged_set_tiles(void* plan, const int32_t* cfg5) {
   if (!pl || !cfg5) return einval("plan or cfg5 is NULL");
   TileCfg c;
   c.e1_rows = cfg5[0];
-  c.e2_t = cfg5[1];
+  c.e2_rows = cfg5[1];
   c.e3_t = cfg5[2];
   c.e3_combos = cfg5[3];
   c.cols = cfg5[4];
-  if ((c.e1_rows != 128 && c.e1_rows != 256) || c.e2_t < kTuneE2T[0] || c.e2_t > kTile2 || c.e3_t < kTuneE3T[0] ||
+  if ((c.e1_rows != 128 && c.e1_rows != 256) || (c.e2_rows != 128 && c.e2_rows != 256) || c.e3_t < kTuneE3T[0] ||
       c.e3_t > kTile || c.e3_combos < kTuneCombos[0] || c.e3_combos > kE3Combos || c.cols < kTuneCols[0] ||
       c.cols > kCols)
-    return einval("tile sizes out of range (stage 1 rows 128 or 256, stage 2 / 3 doubles 2304..4608, "
+    return einval("tile sizes out of range (stage 1 / 2 rows 128 or 256, stage 3 doubles 2304..4608, "
                   "combinations 64..128, columns 16..32)");
   build_tiles(pl, c);
   if (pl->d_ag && upload_tiles(pl) != hipSuccess) {

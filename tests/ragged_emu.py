@@ -17,7 +17,7 @@ from pymdp_amd import _native
 MINVAL = np.float32(2.220446049250313e-16)
 kF, kM, kR, kT, kDim, kNT = 8, 8, 3, 4, 64, 256
 kCap, kRowsCap, kTeamMax = 6144, 1024, 128
-kTile, kTile2, kCols = 4608, 4608, 32
+kTile, kCols = 4608, 32
 kQTab = kCols * 65
 kE1Cols, kE1Ld = 16, 64  # stage-1 history columns staged at a time, their LDS row stride
 
@@ -44,7 +44,7 @@ UNIT_DT = np.dtype([("agent", "i4"), ("slice", "i4"), ("nseg", "i4"), ("pad", "i
                     ("r0", "i4", (kM,)), ("r1", "i4", (kM,)), ("lds", "i4", (kM,))])
 E1REC_DT = np.dtype([("a_off", "i8"), ("h_off", "i8"), ("z_off", "i8"), ("hb", "i8", (kT,)), ("ts", "i4", (kT + 1,)),
                      ("O", "i4"), ("d2", "i4"), ("d01", "i4"), ("e0", "i4"), ("e1", "i4"), ("N1", "i4"),
-                     ("pad", "i4")])
+                     ("blk", "i4")])
 
 
 def e1_tpr(d2):
@@ -74,15 +74,15 @@ def tables(plan):
     return ag, units, (tiles[:n1], tiles[n1:n1 + n2], tiles[n1 + n2:]), iarena[:int(sizes[7])]
 
 
-def e1_records(plan):
-    """Stage-1 tile records copied from the plan."""
+def e1_records(plan, stage=1):
+    """Stage-1 (or stage-2) work-item records copied from the plan."""
     L = _native.lib()
+    fn = L.pymdp_hip_ragged_e1_records if stage == 1 else L.pymdp_hip_ragged_e2_records
     cnt, size = np.zeros(1, np.int64), np.zeros(1, np.int64)
-    _native.check(L.pymdp_hip_ragged_e1_records(plan.handle, None, cnt.ctypes.data_as(_native._i64p),
-                                                 size.ctypes.data_as(_native._i64p)))
+    _native.check(fn(plan.handle, None, cnt.ctypes.data_as(_native._i64p), size.ctypes.data_as(_native._i64p)))
     assert int(size[0]) == E1REC_DT.itemsize, (int(size[0]), E1REC_DT.itemsize)
     rec = np.zeros(int(cnt[0]), E1REC_DT)
-    _native.check(L.pymdp_hip_ragged_e1_records(plan.handle, rec.ctypes.data, None, None))
+    _native.check(fn(plan.handle, rec.ctypes.data, None, None))
     return rec
 
 
@@ -128,6 +128,8 @@ class Emu:
         self.info = plan.info()
         self.ag, self.units, (self.t1, self.t2, self.t3), self.iarena = tables(plan)
         self.e1rec = e1_records(plan)
+        self.e2rec = e1_records(plan, 2)
+        assert len(self.e2rec) == len(self.t2)
         self.rolls, self.roll_start = roll_tasks(plan)
         assert len(self.e1rec) == len(self.t1)
         self.params = params.astype(np.float32)
@@ -388,27 +390,45 @@ class Emu:
         _chk(zi - int(a["z1_off"][m]), N1 * O1 * d01, "Z1 (block)")
         self.z1[_chk(zi, self.z1.size, "Z1")] = Y
 
-    def e2(self, tl):
-        ai, m, t, r0, r1, c0, c1 = (int(tl[k]) for k in ("agent", "m", "t", "r0", "r1", "c0", "c1"))
+    def e2(self, tl, rec):
+        """rg_rows_kernel<2>: rows [r0, r1) of level t's stage-1 output rows
+        (h2, o, i) (each d1 floats, contiguous) times the level's middle-dim
+        histories; output block h2 of Z2 holds H1 columns of O1 d0 rows."""
+        ai, m, t, r0, r1 = (int(tl[k]) for k in ("agent", "m", "t", "r0", "r1"))
         a = self.ag[ai]
         O1 = int(a["O"][m]) + 1
         d0, d1, _ = (int(x) for x in a["ed"][m])
         assert int(a["gR"][a["mod_group"][m]]) >= 2
-        ld1 = d1 | 1
-        nr, nc = r1 - r0, c1 - c0
         H1, H2 = self._hcount(a, m, t, 1), self._hcount(a, m, t, 2)
-        assert r1 <= H2 * O1 * d0 and c1 <= H1 and nc <= kCols
-        _chk(np.array([(nr - 1) * ld1 + d1 - 1]), kTile2, "s_t (E2)")
-        src = int(a["z1_off"][m]) + (int(a["tstart"][m][t]) * O1 * d0 + r0) * d1 + np.arange(nr * d1)
-        Z = self.z1[_chk(src, self.z1.size, "Z1 (E2 read)")].reshape(nr, d1)
-        assert np.isfinite(Z).all(), "E2 reads an unset stage-1 value"
-        Q = self._hist_table(a, m, t, 1, c0, c1)  # [nc, d1]
-        Y = Z @ Q.T  # [nr, nc]
-        Rr = r0 + np.arange(nr)
+        f = int(a["ef"][m][1])
         od = O1 * d0
+        tpr = e1_tpr(d1)
+        nv = (d1 + 3) // 4
+        assert 0 <= r0 < r1 <= H2 * od and r1 - r0 <= self.plan.tile_config()["e2_rows"] // tpr
+        assert (int(tl["c0"]), int(tl["c1"])) == (0, H1)
+        z1base = int(a["z1_off"][m]) + int(a["tstart"][m][t]) * od * d1
+        assert (int(rec["a_off"]), int(rec["z_off"]), int(rec["d2"]), int(rec["d01"]), int(rec["e0"]), int(rec["e1"]),
+                int(rec["N1"]), int(rec["blk"])) == (z1base, int(a["z2_off"][m][t]), d1, d0, r0, r1, H1, od)
+        assert int(rec["hb"][0]) == int(a["hist_off"]) + int(a["hist_lvl"][t][f])
+        assert int(rec["ts"][0]) == 0 and (rec["ts"][1:] == H1).all()
+        Rr = r0 + np.arange(r1 - r0)
+        # row loads (up to 3 floats past a row: Z1 has 16 bytes of tail padding)
+        _chk(np.array([z1base + r0 * d1, z1base + (r1 - 1) * d1 + 4 * nv - 1]), self.z1.size + 4, "Z1 row (E2)")
+        src = z1base + Rr[:, None] * d1 + np.arange(d1)[None, :]
+        Z = self.z1[_chk(src, self.z1.size, "Z1 (E2 read)")]
+        assert np.isfinite(Z).all(), "E2 reads an unset stage-1 value"
+        # the level's middle-dim histories, staged 16 at a time as s_q[n][k]
+        kp = 4 * ((nv + tpr - 1) // tpr) * tpr
+        assert kp <= kE1Ld
+        _chk(np.array([(kE1Cols - 1) * kE1Ld + kp - 1]), kE1Cols * kE1Ld, "s_q (E2)")
+        hidx = int(rec["hb"][0]) + np.arange(H1)[:, None] * d1 + np.arange(d1)[None, :]
+        assert int(rec["hb"][0]) == int(a["hist_off"]) + int(a["hist_lvl"][t][f])
+        Q = self.hist[_chk(hidx, self.hist.size, "hist (E2)")]  # [H1, d1]
+        assert np.isfinite(Q).all()
+        Y = Z.astype(np.float64) @ Q.T  # [nr, H1]
         h2 = Rr // od
         oi = Rr - h2 * od
-        zi = int(a["z2_off"][m][t]) + ((h2[:, None] * H1 + c0 + np.arange(nc)[None, :]) * od + oi[:, None])
+        zi = int(a["z2_off"][m][t]) + ((h2[:, None] * H1 + np.arange(H1)[None, :]) * od + oi[:, None])
         self.z2[_chk(zi, self.z2.size, "Z2")] = Y
 
     def e3(self, tl):
@@ -490,8 +510,8 @@ class Emu:
                 self.roll(rt)
         for tl, rec in zip(self.t1, self.e1rec):
             self.e1(tl, rec)
-        for tl in self.t2:
-            self.e2(tl)
+        for tl, rec in zip(self.t2, self.e2rec):
+            self.e2(tl, rec)
         for tl in self.t3:
             self.e3(tl)
         for ai in range(self.plan.n):
@@ -657,15 +677,26 @@ def check_bounds_only(plan):
         assert (rec["ts"][:, 0] == 0).all() and (rec["ts"][:, kT] == N1).all()
         last = rec["z_off"] + (N1 - 1) * (O_ + 1) * d01_ + e1_ - 1
         assert (rec["z_off"] >= 0).all() and (last < info["z1_doubles"]).all()
-    for tl in t2:
+    e2_rows = plan.tile_config()["e2_rows"]
+    rec2 = e1_records(plan, 2)
+    assert len(rec2) == len(t2)
+    for tl, rc in zip(t2, rec2):
         a = ag[int(tl["agent"])]
         m, t = int(tl["m"]), int(tl["t"])
         O1 = int(a["O"][m]) + 1
         d0, d1, _ = (int(x) for x in a["ed"][m])
         H = [1 if a["ef"][m][p] < 0 else int(a["H"][t][a["ef"][m][p]]) for p in range(3)]
-        nr, nc = int(tl["r1"]) - int(tl["r0"]), int(tl["c1"]) - int(tl["c0"])
-        assert int(tl["r1"]) <= H[2] * O1 * d0 and int(tl["c1"]) <= H[1] and 1 <= nc <= kCols
-        assert nr * (d1 | 1) <= kTile2 and nc * (d1 | 1) <= kQTab
+        r0, r1 = int(tl["r0"]), int(tl["r1"])
+        od = O1 * d0
+        assert 0 <= r0 < r1 <= H[2] * od and r1 - r0 <= e2_rows // e1_tpr(d1)
+        assert (int(tl["c0"]), int(tl["c1"])) == (0, H[1])
+        assert (int(rc["e0"]), int(rc["e1"]), int(rc["d2"]), int(rc["N1"]), int(rc["blk"])) == (r0, r1, d1, H[1], od)
+        last_row = int(rc["a_off"]) + (r1 - 1) * d1 + 4 * ((d1 + 3) // 4) - 1
+        assert int(rc["a_off"]) >= 0 and last_row < info["z1_doubles"] + 4
+        assert int(rc["hb"][0]) >= 0 and int(rc["hb"][0]) + H[1] * d1 <= info["hist_doubles"]
+        h2 = (r1 - 1) // od
+        last = int(rc["z_off"]) + h2 * H[1] * od + (H[1] - 1) * od + (r1 - 1 - h2 * od)
+        assert int(rc["z_off"]) >= 0 and last < info["z2_doubles"]
     for tl in t3:
         a = ag[int(tl["agent"])]
         m, t = int(tl["m"]), int(tl["t"])
