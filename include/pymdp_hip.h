@@ -453,6 +453,13 @@ int pymdp_hip_ragged_e1_records(void* plan, void* out, int64_t* count_out, int64
  * per output block); count_out / record_bytes / out as for the stage-1
  * records. */
 int pymdp_hip_ragged_e2_records(void* plan, void* out, int64_t* count_out, int64_t* record_bytes);
+/* Host copies of the stage-3 work-item records (same layout: per item, a
+ * range of whole (h2, h1) pairs of one level's stage-2 rows, their offset and
+ * buffer (d01 = 1: the stage-1 output of a rank-1 modality), the params
+ * offset of C at that level, the score offset, the history offset of the
+ * level's first-factor histories (-1: padding axis, q = 1), their count, the
+ * row length and O + 1 rows per pair). */
+int pymdp_hip_ragged_e3_records(void* plan, void* out, int64_t* count_out, int64_t* record_bytes);
 /* Host copies of the rollout task records (per task: params offset of B_f,
  * history offsets of its outputs and of each dependency's parent level,
  * int-arena offset of its history records, sizes and level); level_start
@@ -478,23 +485,19 @@ int pymdp_hip_ragged_step(void* plan, const float* params, const int32_t* obs,
                           float* out, int32_t* actions, void* stream);
 /* The plan's autotuner (no counterpart in the reference): times the
  * persistent FPI grid (1, 2, ... workgroups per CU up to the occupancy
- * limit), then the EFE tile sizes one stage at a time (stage-1 and stage-2
- * rows per work item 128 / 256, stage-3 doubles per tile 2304 / 4608,
- * stage-3 combinations 64 / 128, histories per stage-3 tile 16 / 32),
+ * limit), then the rows per stage-1 and per stage-2 work item (128 / 256),
  * each over whole steps, and keeps the fastest; *best_ms = that step time.
  * Not while `stream` is being captured; graphs captured before the call
  * must be captured again (the tile tables are rewritten). */
 int pymdp_hip_ragged_autotune(void* plan, const float* params, const int32_t* obs,
                               float* out, int32_t* actions, void* stream, float* best_ms);
 /* out8 = (stage-1 rows per work item, stage-2 rows per work item, stage-3
- *         doubles per tile, stage-3 combinations per tile, histories per
- *         stage-3 tile, 1 if autotuned, FPI grid, 0). */
+ *         rows per work item (fixed), 1 if autotuned, FPI grid, 0, 0, 0). */
 int pymdp_hip_ragged_tile_config(void* plan, int32_t* out8);
-/* Sets the EFE tile sizes (cfg5 = the first five values of
- * pymdp_hip_ragged_tile_config, each within the autotuner's range; EINVAL
- * otherwise) and rewrites the tile tables.  Results do not depend on the
- * tiling.  Not while a graph that replays the plan may run. */
-int pymdp_hip_ragged_set_tiles(void* plan, const int32_t* cfg5);
+/* Sets the rows per stage-1 and per stage-2 work item (cfg2, each 128 or
+ * 256; EINVAL otherwise) and rewrites the work-item tables.  Results do not
+ * depend on them.  Not while a graph that replays the plan may run. */
+int pymdp_hip_ragged_set_tiles(void* plan, const int32_t* cfg2);
 /* *flag_out = 1 if a team exchange (the per-sweep wait of a large agent's
  * workgroups for each other's partial marginals) of an earlier step gave up
  * waiting (the posteriors of that agent are NaN), else 0.  Synchronises the

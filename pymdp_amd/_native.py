@@ -142,6 +142,7 @@ SIGNATURES: dict[str, tuple] = {
     "pymdp_hip_ragged_tables": (ctypes.c_int, [_vp, _i64p, _vp, _vp, _vp, _vp]),
     "pymdp_hip_ragged_e1_records": (ctypes.c_int, [_vp, _vp, _i64p, _i64p]),
     "pymdp_hip_ragged_e2_records": (ctypes.c_int, [_vp, _vp, _i64p, _i64p]),
+    "pymdp_hip_ragged_e3_records": (ctypes.c_int, [_vp, _vp, _i64p, _i64p]),
     "pymdp_hip_ragged_roll_tasks": (ctypes.c_int, [_vp, _vp, _i64p, _i64p, _i64p]),
     "pymdp_hip_ragged_set_params_static": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "pymdp_hip_ragged_tile_config": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),

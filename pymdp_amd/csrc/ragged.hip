@@ -64,10 +64,6 @@ constexpr int kNW = kNT / kWave;
 constexpr int kCap = 6144;        // log-likelihood doubles one FPI unit keeps in LDS
 constexpr int kRowsCap = 1024;    // rows of one FPI segment
 constexpr int kTeamMax = 128;     // FPI units of one agent
-constexpr int kTile = 4608;       // doubles of a stage-3 operand tile in LDS
-constexpr int kCols = 32;         // histories per stage-2 / stage-3 tile
-constexpr int kQTab = kCols * 65;  // doubles of a stage-2 / stage-3 history table in LDS
-constexpr int kE3Combos = 128;    // history combinations scored per E3 tile
 constexpr long long kSpinMax = 1LL << 22;
 // Team exchange tags: the low kTagBits mantissa bits of every published
 // partial carry the sweep's sequence number (step * iters + sweep), cleared
@@ -159,6 +155,7 @@ struct Bufs {
   const Tile* t1;
   const E1Rec* e1;
   const E1Rec* e2;
+  const E1Rec* e3;
   const HaTask* hat;
   const RollTask* roll;
   const Tile* t2;
@@ -636,110 +633,6 @@ __global__ __launch_bounds__(kNT) void rg_roll_kernel(Bufs b, int task0) {
 // ---------------------------------------------------------------------------
 // EFE stages
 // ---------------------------------------------------------------------------
-// Y[row, col] = X[row, :] . Q[col, :] on FP64 MFMA (v_mfma_f64_16x16x4f64):
-// X (rows x nk) and Q (cols x nk) in LDS with row stride nk | 1; each wave
-// takes 16 x 16 blocks of Y.  Lane l feeds X row l % 16 and Q row l % 16 at
-// k = k0 + l / 16 (k >= nk and rows / columns past the block read index 0
-// and feed zeros) and holds element j of the block at row 4 j + l / 16,
-// column l % 16.
-typedef double rg_d4 __attribute__((ext_vector_type(4)));
-template <int NW, typename Store>
-__device__ __forceinline__ void rg_gemm_mfma(const double* X, int rows, const double* Q, int cols, int nk, int lane,
-                                             int wave, Store store) {
-  const int ld = nk | 1, nkp = (nk + 3) & ~3;
-  const int li = lane & 15, lk = lane >> 4;
-  const int nrb = (rows + 15) >> 4, ncb = (cols + 15) >> 4;
-  for (int blk = wave; blk < nrb * ncb; blk += NW) {
-    const int rb = blk % nrb, cb = blk / nrb;
-    const int r = rb * 16 + li, c = cb * 16 + li;
-    const double* xr = X + (r < rows ? r : 0) * ld;
-    const double* qr = Q + (c < cols ? c : 0) * ld;
-    rg_d4 acc = {0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < nkp; k0 += 4) {
-      const int k = k0 + lk;
-      const bool kv = k < nk;
-      const double x = xr[kv ? k : 0], q = qr[kv ? k : 0];
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(kv ? x : 0.0, kv ? q : 0.0, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int rr = rb * 16 + 4 * j + lk;
-      if (rr < rows && c < cols) store(rr, c, acc[j]);
-    }
-  }
-}
-
-// floor(n / d) for 0 <= n, 1 <= d, n + d <= 2^22, from inv_d = 1.0f / d: the
-// quotient of n + 1/2 lies at least 1/(2d) from an integer, and the two float
-// roundings move it by at most (n/d + 1) 2^-23 < 1/(2d).
-__device__ __forceinline__ int rg_quot(int n, float inv_d) { return (int)(((float)n + 0.5f) * inv_d); }
-
-// Copies a row-major block [rows x w] (contiguous in global memory) into LDS
-// with row stride ld, converted to T, with 16-byte vector loads: every
-// thread issues NV loads before storing any element.  The block is read as
-// vectors from the 16-byte aligned address at or below its first element,
-// so up to 16 bytes before and after the block are read (the buffers staged
-// this way are 16-byte aligned allocations with 16 bytes of tail padding)
-// and only the block's elements are stored.  n = rows * w <= 2^22 - w.
-template <typename S>
-struct rg_vec16;
-template <>
-struct rg_vec16<float> {
-  typedef float type __attribute__((ext_vector_type(4)));
-};
-template <>
-struct rg_vec16<double> {
-  typedef double type __attribute__((ext_vector_type(2)));
-};
-template <typename T, int NV, typename Src, int NT = kNT>
-__device__ __forceinline__ void rg_stage_rows_v(const Src* src, int rows, int w, int ld, T* dst, int tid) {
-  constexpr int E = 16 / (int)sizeof(Src);
-  typedef typename rg_vec16<Src>::type V;
-  const int n = rows * w;
-  if (n <= 0) return;
-  const int shift = (int)((reinterpret_cast<uintptr_t>(src) / sizeof(Src)) & (E - 1));
-  const V* sv = reinterpret_cast<const V*>(src - shift);
-  const int nv = (shift + n + E - 1) / E;
-  const float inv_w = 1.0f / (float)w;
-  for (int j0 = tid; j0 < nv; j0 += NV * NT) {
-    V v[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) v[q] = sv[min(j0 + q * NT, nv - 1)];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) {
-      const int j = j0 + q * NT;
-#pragma unroll
-      for (int c = 0; c < E; ++c) {
-        const int e = j * E + c - shift;
-        if (j < nv && e >= 0 && e < n) {
-          const int r = rg_quot(e, inv_w);
-          dst[r * ld + (e - r * w)] = (T)v[q][c];
-        }
-      }
-    }
-  }
-}
-
-__device__ __forceinline__ int rg_hcount(const Agent& a, int m, int t, int p) {
-  const int f = a.ef[m][p];
-  return f < 0 ? 1 : a.H[t][f];
-}
-
-// History vectors [c0, c1) of padded dim p of modality m at level t into
-// Q [c][k] (row stride d | 1); padding dims have one history, q = 1.
-template <int NT = kNT>
-__device__ __forceinline__ void rg_load_hist(const Agent& a, const double* hist, int m, int t, int p, int c0,
-                                             int c1, double* Q, int tid) {
-  const int d = a.ed[m][p], ld = d | 1, f = a.ef[m][p];
-  if (f < 0) {
-    for (int e = tid; e < c1 - c0; e += NT) Q[e * ld] = 1.0;
-    return;
-  }
-  const double* src = hist + a.hist_lvl[t][f] + (long long)c0 * a.S[f];
-  rg_stage_rows_v<double, (kCols * 64 / 2 + 1 + NT - 1) / NT, double, NT>(src, c1 - c0, d, ld, Q, tid);
-}
-
-
 // Stage 1 on FP64 vector FMAs, one row per thread.  Modality m's row space
 // is [O1][d01] (row e = o * d01 + pos, pos = i * d1 + j): row e of
 // [A ; sum_o A ln A] is A_m[pos, o, :] (FP32, position-major in the params
@@ -754,7 +647,8 @@ __device__ __forceinline__ void rg_load_hist(const Agent& a, const double* hist,
 // matrix rate, most modalities have few history columns (a 16-column MFMA
 // block would be mostly padding), and the row-per-thread form keeps every
 // operand load in flight at once instead of staging A through LDS.
-constexpr int kE1Cols = 16;  // history columns staged in LDS at a time
+constexpr int kE1Cols = 16;  // history columns staged in LDS at a time (stages 1, 2)
+constexpr int kE3Cols = 8;   // the same for stage 3
 // threads per stage-1 row: two for rows of more than 8 vectors (d2 > 32)
 __host__ __device__ constexpr int rg_e1_tpr(int d2) { return d2 > 32 ? 2 : 1; }
 constexpr int kE1Ld = 64;    // LDS row stride (doubles) of a staged history column
@@ -769,9 +663,12 @@ typedef float rg_f4u __attribute__((ext_vector_type(4), aligned(4)));
 // holding up to 64 floats would leave too few registers for the LDS
 // operands at four waves per SIMD (the compiler spills).
 template <int STAGE, int NV, int NT, int TPR>
-__device__ __forceinline__ void rg_rows(const Bufs& b, const E1Rec& r, double* s_q, int tid) {
+__device__ __forceinline__ void rg_rows(const Bufs& b, const E1Rec& r, double* s_q, double* s_t, int tid) {
   static_assert(TPR == 1 || TPR == 2, "threads per row");
   constexpr int NVT = (NV + TPR - 1) / TPR;  // vectors per thread
+  // history columns per chunk: 16, stage 3 8 (its score terms of a chunk
+  // sit in LDS, 8 x 256 doubles)
+  constexpr int CC = STAGE == 3 ? kE3Cols : kE1Cols;
   const int O = r.O, d2 = r.d2, d01 = r.d01, N1 = r.N1;
   const int h = TPR == 1 ? 0 : (tid & 1);  // which part of the row
   const int e = r.e0 + tid / TPR;
@@ -782,15 +679,21 @@ __device__ __forceinline__ void rg_rows(const Bufs& b, const E1Rec& r, double* s
   // holds at least 3 more floats of the same agent after every A block, the
   // table and Z1 16 bytes of tail padding); those lanes are zeroed below
   const float* src;
-  float* z;
+  float* z = nullptr;
+  double cval = 0.0;  // stage 3: C[o] of the row's outcome (0 for the ambiguity row)
   if (STAGE == 1) {
     const int o = ec / d01, pos = ec - o * d01;
     src = o < O ? b.params + r.a_off + ((long long)pos * O + o) * d2 : b.ha + r.h_off + (long long)pos * d2;
     z = b.z1 + r.z_off + ec;
-  } else {
+  } else if (STAGE == 2) {
     const int blk = r.blk, hb2 = ec / blk;
     src = b.z1 + r.a_off + (long long)ec * d2;
     z = b.z2 + r.z_off + (long long)hb2 * N1 * blk + (ec - hb2 * blk);
+  } else {
+    // rows (pair, o) of Z2 (of Z1 for a rank-1 modality, d01 = 1)
+    src = (d01 ? b.z1 : b.z2) + r.a_off + (long long)ec * d2;
+    const int o = ec % r.blk;
+    cval = o < O ? (double)b.params[r.h_off + o] : 0.0;
   }
   src += 4 * NVT * h;
   rg_f4u v[NVT];
@@ -803,10 +706,10 @@ __device__ __forceinline__ void rg_rows(const Bufs& b, const E1Rec& r, double* s
   }
   const long long zstride = r.blk;  // between history columns of the output
   constexpr int kp = 4 * NVT * TPR;  // staged length of a history column
-  for (int c0 = 0; c0 < N1; c0 += kE1Cols) {
-    const int nb = min(kE1Cols, N1 - c0);
+  for (int c0 = 0; c0 < N1; c0 += CC) {
+    const int nb = min(CC, N1 - c0);
     // history columns [c0, c0 + nb) into s_q[n][k], zero for k in [d2, kp)
-    constexpr int NQ = (kE1Cols * kp + NT - 1) / NT;
+    constexpr int NQ = (CC * kp + NT - 1) / NT;
     const int tot = nb * kp;
     double hv[NQ];
 #pragma unroll
@@ -816,7 +719,8 @@ __device__ __forceinline__ void rg_rows(const Bufs& b, const E1Rec& r, double* s
       int t = 0;
 #pragma unroll
       for (int l = 1; l < kT; ++l) t += col >= r.ts[l] ? 1 : 0;
-      hv[q] = b.hist[r.hb[t] + (long long)(col - r.ts[t]) * d2 + min(k, d2 - 1)];
+      // stage 3 with a padding first axis (hb < 0): one history, q = 1
+      hv[q] = STAGE == 3 && r.hb[0] < 0 ? 1.0 : b.hist[r.hb[t] + (long long)(col - r.ts[t]) * d2 + min(k, d2 - 1)];
     }
     if (c0 > 0) lds_barrier();  // the previous chunk's reads are done
 #pragma unroll
@@ -875,31 +779,116 @@ __device__ __forceinline__ void rg_rows(const Bufs& b, const E1Rec& r, double* s
         // converted row element alive until then
         asm volatile("" : "+v"(za), "+v"(zb));
       }
-      if (live && h == 0) {
+      if (STAGE == 3) {
+        // the row's term of the combination score: qo C[o] - qo ln qo for an
+        // outcome row, qo for the ambiguity row (pymdp/control.py:420-509)
+        if (live && h == 0) {
+          const int rl = tid / TPR, o = ec % r.blk;
+          auto term = [&](double qo) {
+            double w = qo;
+            if (o < O) {
+              w = qo * cval;
+              if (qo != 0.0) w -= qo * ln_fast(fmax(qo, (double)kMinVal));
+            }
+            return w;
+          };
+          s_t[n * (NT / TPR) + rl] = term(za);
+          if (n + 1 < nb) s_t[(n + 1) * (NT / TPR) + rl] = term(zb);
+        }
+      } else if (live && h == 0) {
         z[(long long)(c0 + n) * zstride] = (float)za;
         if (n + 1 < nb) z[(long long)(c0 + n + 1) * zstride] = (float)zb;
       }
     }
+    if (STAGE == 3) {
+      // score[pair, h0] = sum over the pair's O1 rows, in row order
+      lds_barrier();
+      const int O1 = r.blk, np = (r.e1 - r.e0) / O1, pair0 = r.e0 / O1;
+      for (int idx = tid; idx < np * nb; idx += NT) {
+        const int pl = idx / nb, n = idx - pl * nb;
+        const double* tp = s_t + n * (NT / TPR) + pl * O1;
+        double sum = 0.0;
+        for (int oo = 0; oo < O1; ++oo) sum += tp[oo];
+        b.score[r.z_off + (long long)(pair0 + pl) * N1 + c0 + n] = sum;
+      }
+      lds_barrier();  // s_t is rewritten by the next chunk
+    }
   }
 }
 
-// One work item per workgroup (stage 1: records b.e1, stage 2: b.e2).  At
+// Stage 3 of a modality whose first axis is padding (rank <= 2: one history,
+// q = 1), so qo is the stage-2 output itself: a group of 2^lg >= O1 lanes
+// (at most the wave) per (h2, h1) pair, a lane per outcome (a lane of a
+// 64-lane group takes outcome 64 too), the terms of the combination score
+// summed by a fixed-order DPP group sum; each wave takes groups of pairs
+// four batches at a time (loads in flight together), up to 256 pairs per
+// work item.
+template <int NT>
+__device__ __forceinline__ void rg_score_pairs(const Bufs& b, const E1Rec& r, double* s_c, int tid) {
+  constexpr int NW = NT / kWave, NB = 4;
+  const int O = r.O, O1 = O + 1;
+  for (int o = tid; o < O; o += NT) s_c[o] = (double)b.params[r.h_off + o];
+  lds_barrier();
+  const int lg = ceil_log2(min(O1, kWave)), gl = 1 << lg;
+  const int lane = tid & (kWave - 1), wave = tid / kWave;
+  const int gpw = kWave >> lg, grp = lane >> lg, sub = lane & (gl - 1);
+  const int p0 = r.e0 / O1, np = (r.e1 - r.e0) / O1;
+  const float* zb = (r.d01 ? b.z1 : b.z2) + r.a_off + (long long)p0 * O1;
+  // the loop bounds are uniform per wave: every lane reaches the group sums
+  for (int pb = wave * gpw; pb < np; pb += NB * NW * gpw) {
+    double w[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int p = pb + u * NW * gpw + grp;
+      w[u] = 0.0;
+      if (p < np) {
+        const float* z = zb + (long long)p * O1;
+        for (int o = sub; o < O1; o += gl) {
+          const double qo = (double)z[o];
+          double t = qo;
+          if (o < O) {
+            t = qo * s_c[o];
+            if (qo != 0.0) t -= qo * ln_fast(fmax(qo, (double)kMinVal));
+          }
+          w[u] += t;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int p = pb + u * NW * gpw + grp;
+      const double v = group_sum_low_dpp(w[u], lg);
+      if (sub == 0 && p < np) b.score[r.z_off + p0 + p] = v;
+    }
+  }
+}
+
+// One work item per workgroup (stage 1: records b.e1, stage 2: b.e2, stage 3:
+// b.e3, always 256 threads so that an item holds at least one pair's 65
+// rows).  At
 // least four waves per SIMD (at most 128 VGPRs): the row of a thread and the
 // loads in flight per CU are what set the stage's HBM rate.
 template <int STAGE, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void rg_rows_kernel(Bufs b) {
   __shared__ double s_q[kE1Cols * kE1Ld];
+  __shared__ double s_t[STAGE == 3 ? kE3Cols * NT : 1];  // stage 3: per-row score terms [column][row]
   __shared__ E1Rec s_rec;
   const int tid = threadIdx.x;
-  long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_units + (STAGE == 1 ? 0 : b.n_t1) + blockIdx.x) : nullptr;
+  const long long tbase = STAGE == 1 ? 0 : STAGE == 2 ? b.n_t1 : (long long)b.n_t1 + b.n_t2;
+  long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_units + tbase + blockIdx.x) : nullptr;
   if (tr && tid == 0) tr[0] = wall_clock64();
-  rg_copy_lds<E1Rec, NT>((STAGE == 1 ? b.e1 : b.e2) + blockIdx.x, &s_rec, tid);
+  rg_copy_lds<E1Rec, NT>((STAGE == 1 ? b.e1 : STAGE == 2 ? b.e2 : b.e3) + blockIdx.x, &s_rec, tid);
   lds_barrier();
+  if (STAGE == 3 && s_rec.hb[0] < 0) {
+    rg_score_pairs<NT>(b, s_rec, s_q, tid);
+    if (tr && tid == 0) tr[1] = wall_clock64();
+    return;
+  }
   switch ((s_rec.d2 + 3) >> 2) {
 #define RG_E1_CASE(nv, tpr)                                  \
   case nv:                                                   \
     static_assert(rg_e1_tpr(4 * nv) == tpr, "threads per row"); \
-    rg_rows<STAGE, nv, NT, tpr>(b, s_rec, s_q, tid);         \
+    rg_rows<STAGE, nv, NT, tpr>(b, s_rec, s_q, s_t, tid);    \
     break;
     // threads per row as rg_e1_tpr(d2), which the host tiling follows
     RG_E1_CASE(1, 1) RG_E1_CASE(2, 1) RG_E1_CASE(3, 1) RG_E1_CASE(4, 1) RG_E1_CASE(5, 1) RG_E1_CASE(6, 1)
@@ -937,59 +926,6 @@ __global__ __launch_bounds__(kNT) void rg_ha_kernel(Bufs b) {
   }
   for (; o < tk.O; ++o) s0 += xlogx_fast(col[o * st]);
   b.ha[tk.ha_off + e] = (float)(s0 + s1);
-}
-
-// Stage 3 at level t: pairs (h2, h1) in [r0, r1) (pair = h2 * H1 + h1),
-// first-dim histories h0 in [c0, c1): qo[o] = Z2[pair, o, :] . q0[h0], then
-// score = sum_{o<O} (qo C[o] - qo ln qo) + qo[O]  (one wave per combination).
-// 512 threads: twice the waves of the other stages per tile at the same LDS
-// (the tile is LDS-bound at 3 per CU; measured 0.44 -> 0.37 ms).
-constexpr int kNT3 = 512, kNW3 = kNT3 / kWave;
-__global__ __launch_bounds__(kNT3) void rg_e3_kernel(Bufs b) {
-  __shared__ double s_t[kTile];
-  __shared__ double s_q[kQTab];
-  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid >> 6;
-  const Tile tl = b.t3[blockIdx.x];
-  const Agent& a = b.ag[tl.agent];
-  long long* tr = b.trace ? b.trace + 2 * ((long long)b.n_units + b.n_t1 + b.n_t2 + blockIdx.x) : nullptr;
-  if (tr && tid == 0) tr[0] = wall_clock64();
-  const int m = tl.m, t = tl.t, O = a.O[m], O1 = O + 1;
-  const int d0 = a.ed[m][0], ld0 = d0 | 1;
-  const int np = tl.r1 - tl.r0, nh = tl.c1 - tl.c0;
-  const int H0 = rg_hcount(a, m, t, 0);
-  // a rank-1 modality's stage-1 output already has the stage-2 layout
-  const float* zb = a.gR[a.mod_group[m]] == 1 ? b.z1 : b.z2;
-  const float* src = zb + a.z2_off[m][t] + (long long)tl.r0 * O1 * d0;
-  rg_stage_rows_v<double, (kTile / 4 + 1 + kNT3 - 1) / kNT3, float, kNT3>(src, np * O1, d0, ld0, s_t, tid);
-  rg_load_hist<kNT3>(a, b.hist + a.hist_off, m, t, 0, tl.c0, tl.c1, s_q, tid);
-  const float* Cm = b.params + a.C_off[m] + (a.C_time[m] ? (long long)t * O : 0);
-  const float c_lo = lane < O ? Cm[lane] : 0.0f;
-  const float c_hi = lane + kWave < O ? Cm[lane + kWave] : 0.0f;
-  __syncthreads();
-  double* sc = b.score + a.score_off + a.score_base[t][m];
-  for (int cc = wave; cc < np * nh; cc += kNW3) {
-    const int pl = cc / nh, hl = cc - pl * nh;
-    const double* q0 = s_q + hl * ld0;
-    double v = 0.0;
-#pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      const int o = lane + half * kWave;
-      if (o > O) continue;
-      const double* z = s_t + (pl * O1 + o) * ld0;
-      double qo = 0.0;
-      for (int i = 0; i < d0; ++i) qo = fma(z[i], q0[i], qo);
-      if (o < O) {
-        double w = qo * (double)(half ? c_hi : c_lo);
-        if (qo != 0.0) w -= qo * ln_fast(fmax(qo, (double)kMinVal));
-        v += w;
-      } else {
-        v += qo;
-      }
-    }
-    v = wave_sum(v);
-    if (lane == 0) sc[(long long)(tl.r0 + pl) * H0 + tl.c0 + hl] = v;
-  }
-  if (tr && tid == 0) tr[1] = wall_clock64();
 }
 
 // ---------------------------------------------------------------------------
@@ -1122,19 +1058,14 @@ __global__ __launch_bounds__(kNT) void rg_assemble_kernel(Bufs b) {
 // ---------------------------------------------------------------------------
 // EFE tile sizes (see build_tiles); the defaults are the LDS capacities
 struct TileCfg {
-  int e1_rows = 256;          // rows per stage-1 work item (= its workgroup size)
-  int e2_rows = 256;          // rows per stage-2 work item (= its workgroup size)
-  int e3_t = kTile;           // Z2 doubles per stage-3 tile
-  int e3_combos = kE3Combos;  // history combinations scored per stage-3 tile
-  int cols = kCols;           // histories per stage-3 tile
+  int e1_rows = 256;  // rows per stage-1 work item (= its workgroup size)
+  int e2_rows = 256;  // rows per stage-2 work item (= its workgroup size)
 };
+constexpr int kE3Rows = 256;  // row slots of a stage-3 work item (its workgroup size)
 
-// autotuner candidates (each at most its LDS capacity)
+// autotuner candidates
 constexpr int kTuneE1R[] = {128, 256};
 constexpr int kTuneE2R[] = {128, 256};
-constexpr int kTuneE3T[] = {2304, kTile};
-constexpr int kTuneCombos[] = {64, kE3Combos};
-constexpr int kTuneCols[] = {16, kCols};
 
 struct Plan {
   int device = 0;
@@ -1144,7 +1075,7 @@ struct Plan {
   std::vector<Agent> agents;
   std::vector<Unit> units;
   std::vector<Tile> t1, t2, t3;
-  std::vector<E1Rec> e1, e2;
+  std::vector<E1Rec> e1, e2, e3;
   std::vector<HaTask> hat;
   std::vector<RollTask> roll;     // rollout tasks, level by level
   int roll_start[kT + 1] = {};    // first task of each level
@@ -1164,7 +1095,7 @@ struct Plan {
   Agent* d_ag = nullptr;
   Unit* d_units = nullptr;
   Tile *d_t1 = nullptr, *d_t2 = nullptr, *d_t3 = nullptr;
-  E1Rec *d_e1 = nullptr, *d_e2 = nullptr;
+  E1Rec *d_e1 = nullptr, *d_e2 = nullptr, *d_e3 = nullptr;
   HaTask* d_hat = nullptr;
   RollTask* d_roll = nullptr;
   float* d_ha = nullptr;
@@ -1184,7 +1115,7 @@ static void free_plan(Plan* p) {
   int prev = 0;
   (void)hipGetDevice(&prev);
   (void)hipSetDevice(p->device);
-  for (void* x : {(void*)p->d_ag, (void*)p->d_units, (void*)p->d_t1, (void*)p->d_e1, (void*)p->d_e2, (void*)p->d_hat,
+  for (void* x : {(void*)p->d_ag, (void*)p->d_units, (void*)p->d_t1, (void*)p->d_e1, (void*)p->d_e2, (void*)p->d_e3, (void*)p->d_hat,
                   (void*)p->d_roll,
                   (void*)p->d_ha, (void*)p->d_t2, (void*)p->d_t3,
                   (void*)p->d_iarena, (void*)p->d_hist, (void*)p->d_score, (void*)p->d_z1, (void*)p->d_z2,
@@ -1215,8 +1146,8 @@ using namespace pmh::rg;
 // EFE tiling from the agents' layouts.  Stage 1: blocks of e1_rows rows of
 // one modality's row space [O1][d01], all its last-axis history columns;
 // stage 2: blocks of e2_rows rows of one level's Z1 rows (h2, o, i), all
-// the level's middle-dim histories; stage 3: blocks of (h2, h1) pairs (at most e3_t
-// LDS doubles and e3_combos scored combinations) x at most `cols` first-dim
+// the level's middle-dim histories; stage 3: whole (h2, h1) pairs of a level
+// (O1 rows each, at most 256 / (threads per row) rows), all first-dim
 // histories.  Every tile list is ordered by decreasing cost.
 static void build_tiles(Plan* pl, const TileCfg& cfg) {
   struct TileC {
@@ -1249,15 +1180,15 @@ static void build_tiles(Plan* pl, const TileCfg& cfg) {
           }
           pl->efe_macs += (double)rows * d1 * H1;
         }
-        // E3 tiles: (h2, h1) pairs x first-dim histories
-        const int pairs = H2 * H1, ld0 = d0 | 1;
-        const int hb = std::min(H0, cfg.cols);
-        const int np = std::max(1, std::min({pairs, cfg.e3_t / (O1 * ld0), std::max(1, cfg.e3_combos / hb)}));
-        for (int r0 = 0; r0 < pairs; r0 += np)
-          for (int c0 = 0; c0 < H0; c0 += cfg.cols) {
-            const int r1 = std::min(pairs, r0 + np), c1 = std::min(H0, c0 + cfg.cols);
-            t3.push_back(TileC{Tile{ai, m, t, r0, r1, c0, c1, 0}, (double)(r1 - r0) * (c1 - c0) * O1 * d0});
-          }
+        // stage-3 items: whole (h2, h1) pairs of O1 rows each, every
+        // first-dim history (tile r0 / r1 count rows); a padding first axis
+        // (rank <= 2) takes a thread per pair
+        const int pairs = H2 * H1;
+        const int ppi = a.ef[m][0] < 0 ? kE3Rows : std::max(1, (kE3Rows / rg_e1_tpr(d0)) / O1);
+        for (int p0 = 0; p0 < pairs; p0 += ppi) {
+          const int p1 = std::min(pairs, p0 + ppi);
+          t3.push_back(TileC{Tile{ai, m, t, p0 * O1, p1 * O1, 0, H0, 0}, (double)(p1 - p0) * O1 * d0 * (H0 + 8)});
+        }
         pl->efe_macs += (double)pairs * H0 * O1 * d0;
       }
     }
@@ -1313,6 +1244,31 @@ static void build_tiles(Plan* pl, const TileCfg& cfg) {
     pl->e2.push_back(r);
   }
   for (auto& x : t3) pl->t3.push_back(x.t);
+  pl->e3.clear();
+  for (const Tile& tl : pl->t3) {
+    // rows (pair, o) of level t's stage-2 output (stage-1 output for a
+    // rank-1 modality) contracted over the first dim with that level's
+    // first-factor histories, scored per (pair, history)
+    const Agent& a = pl->agents[tl.agent];
+    const int m = tl.m, t = tl.t, f = a.ef[m][0];
+    E1Rec r;
+    std::memset(&r, 0, sizeof(r));
+    const int H0 = f < 0 ? 1 : a.H[t][f];
+    r.a_off = a.z2_off[m][t];
+    r.h_off = a.C_off[m] + (a.C_time[m] ? (long long)t * a.O[m] : 0);
+    r.z_off = a.score_off + a.score_base[t][m];
+    r.hb[0] = f < 0 ? -1 : a.hist_off + a.hist_lvl[t][f];
+    r.ts[0] = 0;
+    for (int l = 1; l <= kT; ++l) r.ts[l] = H0;
+    r.O = a.O[m];
+    r.d2 = a.ed[m][0];
+    r.d01 = a.gR[a.mod_group[m]] == 1 ? 1 : 0;
+    r.e0 = tl.r0;
+    r.e1 = tl.r1;
+    r.N1 = H0;
+    r.blk = a.O[m] + 1;
+    pl->e3.push_back(r);
+  }
   pl->tiles = cfg;
 }
 
@@ -1663,7 +1619,6 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
           a.z2_off[m][t] = z2;
           z2 += (long long)H2 * H1 * O1 * d0;
         }
-        if (O1 * ((d0 | 1)) > kTile) return fail("EFE tile does not fit");
       }
     }
     pl->totS_max = std::max(pl->totS_max, a.totS);
@@ -1727,6 +1682,7 @@ static hipError_t upload_tiles(Plan* pl) {
   if ((e = up(pl->d_e1, pl->e1.data(), sizeof(E1Rec) * pl->e1.size())) != hipSuccess) return e;
   if ((e = up(pl->d_t2, pl->t2.data(), sizeof(Tile) * pl->t2.size())) != hipSuccess) return e;
   if ((e = up(pl->d_e2, pl->e2.data(), sizeof(E1Rec) * pl->e2.size())) != hipSuccess) return e;
+  if ((e = up(pl->d_e3, pl->e3.data(), sizeof(E1Rec) * pl->e3.size())) != hipSuccess) return e;
   return up(pl->d_t3, pl->t3.data(), sizeof(Tile) * pl->t3.size());
 }
 
@@ -1760,7 +1716,7 @@ static int upload(Plan* pl) {
   // the autotuner can re-tile without reallocating
   {
     const TileCfg keep = pl->tiles;
-    build_tiles(pl, TileCfg{kTuneE1R[0], kTuneE2R[0], kTuneE3T[0], kTuneCombos[0], kTuneCols[0]});
+    build_tiles(pl, TileCfg{kTuneE1R[0], kTuneE2R[0]});
     pl->cap_t1 = pl->t1.size();
     pl->cap_t2 = pl->t2.size();
     pl->cap_t3 = pl->t3.size();
@@ -1775,6 +1731,7 @@ static int upload(Plan* pl) {
   if ((e = alloc((void**)&pl->d_t1, sizeof(Tile) * pl->cap_t1)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_e1, sizeof(E1Rec) * pl->cap_t1)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_e2, sizeof(E1Rec) * pl->cap_t2)) != hipSuccess) return fail_dev(e, "hipMalloc");
+  if ((e = alloc((void**)&pl->d_e3, sizeof(E1Rec) * pl->cap_t3)) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_hat, sizeof(HaTask) * pl->hat.size())) != hipSuccess) return fail_dev(e, "hipMalloc");
   if ((e = alloc((void**)&pl->d_roll, sizeof(RollTask) * pl->roll.size())) != hipSuccess)
     return fail_dev(e, "hipMalloc");
@@ -1868,6 +1825,15 @@ extern "C" int pymdp_hip_ragged_e2_records(void* plan, void* out, int64_t* count
   return PYMDP_HIP_OK;
 }
 
+extern "C" int pymdp_hip_ragged_e3_records(void* plan, void* out, int64_t* count_out, int64_t* record_bytes) {
+  Plan* pl = static_cast<Plan*>(plan);
+  if (!pl) return einval("plan is NULL");
+  if (count_out) *count_out = (int64_t)pl->e3.size();
+  if (record_bytes) *record_bytes = (int64_t)sizeof(E1Rec);
+  if (out) std::memcpy(out, pl->e3.data(), sizeof(E1Rec) * pl->e3.size());
+  return PYMDP_HIP_OK;
+}
+
 extern "C" int pymdp_hip_ragged_roll_tasks(void* plan, void* out, int64_t* count_out, int64_t* record_bytes,
                                            int64_t* level_start) {
   Plan* pl = static_cast<Plan*>(plan);
@@ -1922,6 +1888,7 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
   b.t1 = pl->d_t1;
   b.e1 = pl->d_e1;
   b.e2 = pl->d_e2;
+  b.e3 = pl->d_e3;
   b.hat = pl->d_hat;
   b.roll = pl->d_roll;
   b.ha = pl->d_ha;
@@ -1972,7 +1939,7 @@ static int ragged_launch(Plan* pl, const float* params, const int32_t* obs, floa
     else
       hipLaunchKernelGGL((rg_rows_kernel<2, 256>), dim3(b.n_t2), dim3(256), 0, s, b);
   }
-  if (b.n_t3) hipLaunchKernelGGL(rg_e3_kernel, dim3(b.n_t3), dim3(kNT3), 0, s, b);
+  if (b.n_t3) hipLaunchKernelGGL((rg_rows_kernel<3, kE3Rows>), dim3(b.n_t3), dim3(kE3Rows), 0, s, b);
   hipLaunchKernelGGL(rg_assemble_kernel, dim3((unsigned)pl->n_agents), dim3(kNT), 0, s, b);
   PMH_HIP_CHECK(hipGetLastError(), "ragged: ");
   return 0;
@@ -2080,18 +2047,6 @@ extern "C" int pymdp_hip_ragged_autotune(void* plan, const float* params, const 
       c.e2_rows = x;
       if (x != bc.e2_rows && (rc = trial(c))) return rc;
     }
-    for (int x : kTuneE3T)
-      for (int y : kTuneCombos) {
-        TileCfg c = bc;
-        c.e3_t = x;
-        c.e3_combos = y;
-        if ((x != bc.e3_t || y != bc.e3_combos) && (rc = trial(c))) return rc;
-      }
-    for (int x : kTuneCols) {
-      TileCfg c = bc;
-      c.cols = x;
-      if (x != bc.cols && (rc = trial(c))) return rc;
-    }
     build_tiles(pl, bc);
     if (upload_tiles(pl) != hipSuccess) {
       set_error("ragged: tile upload failed");
@@ -2125,25 +2080,19 @@ extern "C" int pymdp_hip_ragged_tile_config(void* plan, int32_t* out8) {
   Plan* pl = static_cast<Plan*>(plan);
   if (!pl || !out8) return einval("plan or out8 is NULL");
   const TileCfg& c = pl->tiles;
-  const int32_t v[8] = {c.e1_rows, c.e2_rows, c.e3_t, c.e3_combos, c.cols, pl->tuned ? 1 : 0, pl->grid1, 0};
+  const int32_t v[8] = {c.e1_rows, c.e2_rows, kE3Rows, pl->tuned ? 1 : 0, pl->grid1, 0, 0, 0};
   std::memcpy(out8, v, sizeof(v));
   return PYMDP_HIP_OK;
 }
 
-extern "C" int pymdp_hip_ragged_set_tiles(void* plan, const int32_t* cfg5) {
+extern "C" int pymdp_hip_ragged_set_tiles(void* plan, const int32_t* cfg2) {
   Plan* pl = static_cast<Plan*>(plan);
-  if (!pl || !cfg5) return einval("plan or cfg5 is NULL");
+  if (!pl || !cfg2) return einval("plan or cfg2 is NULL");
   TileCfg c;
-  c.e1_rows = cfg5[0];
-  c.e2_rows = cfg5[1];
-  c.e3_t = cfg5[2];
-  c.e3_combos = cfg5[3];
-  c.cols = cfg5[4];
-  if ((c.e1_rows != 128 && c.e1_rows != 256) || (c.e2_rows != 128 && c.e2_rows != 256) || c.e3_t < kTuneE3T[0] ||
-      c.e3_t > kTile || c.e3_combos < kTuneCombos[0] || c.e3_combos > kE3Combos || c.cols < kTuneCols[0] ||
-      c.cols > kCols)
-    return einval("tile sizes out of range (stage 1 / 2 rows 128 or 256, stage 3 doubles 2304..4608, "
-                  "combinations 64..128, columns 16..32)");
+  c.e1_rows = cfg2[0];
+  c.e2_rows = cfg2[1];
+  if ((c.e1_rows != 128 && c.e1_rows != 256) || (c.e2_rows != 128 && c.e2_rows != 256))
+    return einval("tile sizes out of range (stage 1 / 2 rows per work item: 128 or 256)");
   build_tiles(pl, c);
   if (pl->d_ag && upload_tiles(pl) != hipSuccess) {
     set_error("ragged: tile upload failed");

@@ -106,16 +106,16 @@ class RaggedPlan:
         return dict(zip(keys, (int(x) for x in out)))
 
     def tile_config(self) -> dict:
-        """EFE tile sizes in use (pymdp_hip_ragged_tile_config)."""
+        """EFE work-item sizes in use (pymdp_hip_ragged_tile_config)."""
         out = (ctypes.c_int32 * 8)()
         _native.check(_native.lib().pymdp_hip_ragged_tile_config(self.handle, out))
-        keys = ["e1_rows", "e2_rows", "e3_doubles", "e3_combos", "cols", "autotuned", "grid_fpi"]
+        keys = ["e1_rows", "e2_rows", "e3_rows", "autotuned", "grid_fpi"]
         return dict(zip(keys, (int(x) for x in out)))
 
-    def set_tiles(self, e1_rows: int, e2_rows: int, e3_doubles: int, e3_combos: int, cols: int):
-        """Sets the EFE tile sizes (the autotuner's ranges); results do not
-        depend on them."""
-        cfg = (ctypes.c_int32 * 5)(e1_rows, e2_rows, e3_doubles, e3_combos, cols)
+    def set_tiles(self, e1_rows: int, e2_rows: int):
+        """Sets the stage-1 and stage-2 rows per work item (128 or 256);
+        results do not depend on them."""
+        cfg = (ctypes.c_int32 * 2)(e1_rows, e2_rows)
         _native.check(_native.lib().pymdp_hip_ragged_set_tiles(self.handle, cfg))
 
     def upload(self):

@@ -17,9 +17,8 @@ from pymdp_amd import _native
 MINVAL = np.float32(2.220446049250313e-16)
 kF, kM, kR, kT, kDim, kNT = 8, 8, 3, 4, 64, 256
 kCap, kRowsCap, kTeamMax = 6144, 1024, 128
-kTile, kCols = 4608, 32
-kQTab = kCols * 65
-kE1Cols, kE1Ld = 16, 64  # stage-1 history columns staged at a time, their LDS row stride
+kE1Cols, kE1Ld = 16, 64  # stage-1 / 2 history columns staged at a time, their LDS row stride
+kE3Cols, kE3Rows = 8, 256  # stage-3 columns staged at a time, row slots of a stage-3 item
 
 AGENT_DT = np.dtype([
     ("F", "i4"), ("M", "i4"), ("T", "i4"), ("P", "i4"), ("G", "i4"), ("iters", "i4"), ("totS", "i4"),
@@ -75,9 +74,9 @@ def tables(plan):
 
 
 def e1_records(plan, stage=1):
-    """Stage-1 (or stage-2) work-item records copied from the plan."""
+    """Stage-1, -2 or -3 work-item records copied from the plan."""
     L = _native.lib()
-    fn = L.pymdp_hip_ragged_e1_records if stage == 1 else L.pymdp_hip_ragged_e2_records
+    fn = {1: L.pymdp_hip_ragged_e1_records, 2: L.pymdp_hip_ragged_e2_records, 3: L.pymdp_hip_ragged_e3_records}[stage]
     cnt, size = np.zeros(1, np.int64), np.zeros(1, np.int64)
     _native.check(fn(plan.handle, None, cnt.ctypes.data_as(_native._i64p), size.ctypes.data_as(_native._i64p)))
     assert int(size[0]) == E1REC_DT.itemsize, (int(size[0]), E1REC_DT.itemsize)
@@ -130,6 +129,8 @@ class Emu:
         self.e1rec = e1_records(plan)
         self.e2rec = e1_records(plan, 2)
         assert len(self.e2rec) == len(self.t2)
+        self.e3rec = e1_records(plan, 3)
+        assert len(self.e3rec) == len(self.t3)
         self.rolls, self.roll_start = roll_tasks(plan)
         assert len(self.e1rec) == len(self.t1)
         self.params = params.astype(np.float32)
@@ -308,9 +309,10 @@ class Emu:
         return 1 if f < 0 else int(a["H"][t][f])
 
     def _hist_table(self, a, m, t, p, c0, c1):
+        """History vectors [c0, c1) of padded dim p of modality m at level t
+        from the agent record (padding dims: one history, q = 1)."""
         d = int(a["ed"][m][p])
         f = int(a["ef"][m][p])
-        _chk(np.arange((c1 - c0) * (d | 1)), kQTab, "s_q")
         if f < 0:
             assert d == 1
             return np.ones((c1 - c0, 1))
@@ -431,32 +433,62 @@ class Emu:
         zi = int(a["z2_off"][m][t]) + ((h2[:, None] * H1 + np.arange(H1)[None, :]) * od + oi[:, None])
         self.z2[_chk(zi, self.z2.size, "Z2")] = Y
 
-    def e3(self, tl):
-        ai, m, t, r0, r1, c0, c1 = (int(tl[k]) for k in ("agent", "m", "t", "r0", "r1", "c0", "c1"))
+    def e3(self, tl, rec):
+        """rg_rows_kernel<3>: whole (h2, h1) pairs [r0 / O1, r1 / O1) of level
+        t (rows (pair, o) of d0 floats) times the level's first-dim histories,
+        each row's term of the combination score in LDS, summed per
+        (pair, history) over the pair's O1 rows."""
+        ai, m, t, r0, r1 = (int(tl[k]) for k in ("agent", "m", "t", "r0", "r1"))
         a = self.ag[ai]
         O = int(a["O"][m])
         O1 = O + 1
         d0 = int(a["ed"][m][0])
-        ld0 = d0 | 1
-        np_, nh = r1 - r0, c1 - c0
         H0, H1, H2 = (self._hcount(a, m, t, p) for p in range(3))
-        assert r1 <= H2 * H1 and c1 <= H0 and nh <= kCols
-        _chk(np.array([(np_ * O1 - 1) * ld0 + d0 - 1]), kTile, "s_t (E3)")
+        f = int(a["ef"][m][0])
+        tpr = e1_tpr(d0)
+        nv = (d0 + 3) // 4
+        rs = kE3Rows // tpr  # row slots of an item
+        # a padding first axis: a thread per pair, up to 256 pairs
+        assert r0 % O1 == 0 and r1 % O1 == 0 and 0 <= r0 < r1 <= H2 * H1 * O1
+        assert r1 - r0 <= (kE3Rows * O1 if f < 0 else rs)
+        assert (int(tl["c0"]), int(tl["c1"])) == (0, H0)
         R = int(a["gR"][a["mod_group"][m]])
-        buf = self.z1 if R == 1 else self.z2
-        src = int(a["z2_off"][m][t]) + r0 * O1 * d0 + np.arange(np_ * O1 * d0)
-        Z = buf[_chk(src, buf.size, "Z2 (E3 read)")].reshape(np_, O1, d0)
-        assert np.isfinite(Z).all(), "E3 reads an unset stage-2 value"
-        Q = self._hist_table(a, m, t, 0, c0, c1)  # [nh, d0]
-        qo = np.einsum("poi,hi->pho", Z, Q)  # [np, nh, O1]
         coff = int(a["C_off"][m]) + (t * O if a["C_time"][m] else 0)
+        base = int(a["score_off"]) + int(a["score_base"][t][m])
+        assert (int(rec["a_off"]), int(rec["h_off"]), int(rec["z_off"]), int(rec["d2"]), int(rec["d01"]),
+                int(rec["e0"]), int(rec["e1"]), int(rec["N1"]), int(rec["blk"]), int(rec["O"])) == \
+            (int(a["z2_off"][m][t]), coff, base, d0, 1 if R == 1 else 0, r0, r1, H0, O1, O)
+        assert int(rec["hb"][0]) == (-1 if f < 0 else int(a["hist_off"]) + int(a["hist_lvl"][t][f]))
+        assert int(rec["ts"][0]) == 0 and (rec["ts"][1:] == H0).all()
+        # LDS: staged columns and the score terms [column][row slot]
+        kp = 4 * ((nv + tpr - 1) // tpr) * tpr
+        assert kp <= kE1Ld
+        _chk(np.array([(kE3Cols - 1) * kE1Ld + kp - 1]), kE1Cols * kE1Ld, "s_q (E3)")
+        if f < 0:
+            _chk(np.array([O - 1]), kE1Cols * kE1Ld, "s_c (E3 pairs)")
+        else:
+            _chk(np.array([(kE3Cols - 1) * rs + (r1 - r0) - 1]), kE3Cols * kE3Rows, "s_t (E3)")
+        buf = self.z1 if R == 1 else self.z2
+        a_off = int(rec["a_off"])
+        _chk(np.array([a_off + r0 * d0, a_off + (r1 - 1) * d0 + 4 * nv - 1]), buf.size + 4, "row (E3)")
+        np_ = (r1 - r0) // O1
+        src = a_off + r0 * d0 + np.arange(np_ * O1 * d0)
+        Z = buf[_chk(src, buf.size, "Z2 (E3 read)")].reshape(np_, O1, d0).astype(np.float64)
+        assert np.isfinite(Z).all(), "E3 reads an unset stage-2 value"
+        if f < 0:
+            assert d0 == 1 and H0 == 1
+            Q = np.ones((1, 1))
+        else:
+            hidx = int(rec["hb"][0]) + np.arange(H0)[:, None] * d0 + np.arange(d0)[None, :]
+            Q = self.hist[_chk(hidx, self.hist.size, "hist (E3)")]
+            assert np.isfinite(Q).all()
+        qo = np.einsum("poi,hi->pho", Z, Q)  # [np, H0, O1]
         C = self.params[_chk(coff + np.arange(O), self.params.size, "C")].astype(np.float64)
         q = qo[..., :O]
         qf = q.astype(np.float32)
         lq = np.log(np.maximum(qf, MINVAL)).astype(np.float32).astype(np.float64)
         v = (q * C).sum(-1) - np.where(qf != 0, q * lq, 0.0).sum(-1) + qo[..., O]
-        base = int(a["score_off"]) + int(a["score_base"][t][m])
-        si = base + (r0 + np.arange(np_))[:, None] * H0 + c0 + np.arange(nh)[None, :]
+        si = base + (r0 // O1 + np.arange(np_))[:, None] * H0 + np.arange(H0)[None, :]
         _chk(si - base, H0 * H1 * H2, "score (block)")
         self.score[_chk(si, self.score.size, "score")] = v
 
@@ -512,8 +544,8 @@ class Emu:
             self.e1(tl, rec)
         for tl, rec in zip(self.t2, self.e2rec):
             self.e2(tl, rec)
-        for tl in self.t3:
-            self.e3(tl)
+        for tl, rec in zip(self.t3, self.e3rec):
+            self.e3(tl, rec)
         for ai in range(self.plan.n):
             self.assemble(ai)
         return self
@@ -697,13 +729,25 @@ def check_bounds_only(plan):
         h2 = (r1 - 1) // od
         last = int(rc["z_off"]) + h2 * H[1] * od + (H[1] - 1) * od + (r1 - 1 - h2 * od)
         assert int(rc["z_off"]) >= 0 and last < info["z2_doubles"]
-    for tl in t3:
+    rec3 = e1_records(plan, 3)
+    assert len(rec3) == len(t3)
+    for tl, rc in zip(t3, rec3):
         a = ag[int(tl["agent"])]
         m, t = int(tl["m"]), int(tl["t"])
         O1 = int(a["O"][m]) + 1
         d0 = int(a["ed"][m][0])
         H = [1 if a["ef"][m][p] < 0 else int(a["H"][t][a["ef"][m][p]]) for p in range(3)]
-        np_, nh = int(tl["r1"]) - int(tl["r0"]), int(tl["c1"]) - int(tl["c0"])
-        assert int(tl["r1"]) <= H[2] * H[1] and int(tl["c1"]) <= H[0] and 1 <= nh <= kCols
-        assert np_ * O1 * (d0 | 1) <= kTile and nh * (d0 | 1) <= kQTab
+        r0, r1 = int(tl["r0"]), int(tl["r1"])
+        assert r0 % O1 == 0 and r1 % O1 == 0 and 0 <= r0 < r1 <= H[2] * H[1] * O1
+        pad = a["ef"][m][0] < 0
+        assert r1 - r0 <= (kE3Rows * O1 if pad else kE3Rows // e1_tpr(d0)) and (int(tl["c0"]), int(tl["c1"])) == (0, H[0])
+        assert (int(rc["e0"]), int(rc["e1"]), int(rc["d2"]), int(rc["N1"]), int(rc["blk"])) == (r0, r1, d0, H[0], O1)
+        lim = info["z1_doubles"] if int(rc["d01"]) else info["z2_doubles"]
+        assert int(rc["a_off"]) >= 0 and int(rc["a_off"]) + (r1 - 1) * d0 + 4 * ((d0 + 3) // 4) - 1 < lim + 4
+        assert int(rc["h_off"]) >= 0 and int(rc["h_off"]) + O1 - 1 <= params_n
+        if int(rc["hb"][0]) >= 0:
+            assert int(rc["hb"][0]) + H[0] * d0 <= info["hist_doubles"]
+        else:
+            assert d0 == 1 and H[0] == 1
+        assert int(rc["z_off"]) >= 0 and int(rc["z_off"]) + (r1 // O1 - 1) * H[0] + H[0] - 1 < info["score_doubles"]
     return info

@@ -251,8 +251,7 @@ def test_ragged_tilings_bitwise_equal(dev):
     rb.step()
     torch.cuda.synchronize()
     ref_out, ref_act = rb.out.clone(), rb.act.clone()
-    for cfg in [(128, 128, 2304, 64, 16), (256, 256, 2304, 128, 32), (128, 256, 4608, 64, 32),
-                (256, 128, 4608, 128, 16)]:
+    for cfg in [(128, 128), (256, 256), (128, 256), (256, 128)]:
         rb.plan.set_tiles(*cfg)
         rb.out.zero_()
         rb.act.zero_()
@@ -260,4 +259,4 @@ def test_ragged_tilings_bitwise_equal(dev):
         torch.cuda.synchronize()
         assert torch.equal(ref_out, rb.out) and torch.equal(ref_act, rb.act), cfg
     with pytest.raises(ValueError, match="tile sizes out of range"):
-        rb.plan.set_tiles(256, 200, 4608, 128, 32)
+        rb.plan.set_tiles(256, 200)

@@ -252,7 +252,7 @@ def test_emulated_rollout_carries_prior():
                 assert np.abs(x - y).max() <= 1e-6, (i, t)
 
 
-@pytest.mark.parametrize("cfg", [(128, 128, 2304, 64, 16), (256, 256, 4608, 128, 32)])
+@pytest.mark.parametrize("cfg", [(128, 128), (256, 256), (128, 256)])
 def test_tilings_bounds_and_emulation(cfg):
     """Every tiling of the autotuner's range keeps every access of the full
     4096-agent plan in bounds, and the emulated step is bitwise the same as
