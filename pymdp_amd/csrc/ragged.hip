@@ -576,7 +576,7 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
 // state s of entry i = o / S_f, entry i being history h with action u:
 //   q_f[t][h][s] = sum_k B_f[u][k][s] w_i[k],  w_i = x_j q_{dep_j}[t-1][parent_j]
 // (t = 0: the parents are the posterior), one thread per output, loads in
-// flight eight k steps at a time (one-dependency B).  B_f is stored [U_f][K_B][S_f], so the lanes
+// flight sixteen k steps at a time (one-dependency B).  B_f is stored [U_f][K_B][S_f], so the lanes
 // of one history read consecutive states (coalesced) and the same w_i[k] (one
 // broadcast).  Measured on the 4096-agent workload: 74 us per level; staging
 // the B blocks and parents in LDS (three load rounds per task instead of ten)
@@ -595,16 +595,16 @@ __global__ __launch_bounds__(kNT) void rg_roll_kernel(Bufs b, int task0) {
   if (rb == 1) {
     const double* w = b.hist + tk.par[0] + (tk.t == 0 ? 0 : (long long)en[2] * tk.dd[0]);
     int k = 0;
-    for (; k + 8 <= KB; k += 8) {
-      float bv[8];
-      double wv[8];
+    for (; k + 16 <= KB; k += 16) {
+      float bv[16];
+      double wv[16];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < 16; ++j) {
         bv[j] = Bu[(long long)(k + j) * Sf];
         wv[j] = w[k + j];
       }
 #pragma unroll
-      for (int j = 0; j < 8; j += 2) {
+      for (int j = 0; j < 16; j += 2) {
         acc0 = fma((double)bv[j], wv[j], acc0);
         acc1 = fma((double)bv[j + 1], wv[j + 1], acc1);
       }
