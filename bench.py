@@ -39,7 +39,8 @@ CPU baseline timed on a bounded sample on this host.
         --pmc-dirs FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR
                                        # HBM bytes per launch of the kernel
                                        # whose name contains SUBSTR (comma-
-                                       # separated: summed), from four
+                                       # separated: summed; SUBSTR:N sums
+                                       # its last N dispatches), from four
                                        # rocprofv3 --pmc csv runs, scaled by
                                        # the copy calibration; stored under KEY
                                        # in profiles/traffic.json
@@ -138,11 +139,15 @@ def write_traffic(a):
     plain 1 GiB device copy measured the same way."""
     fdir, wdir, cfdir, cwdir = a.pmc_dirs
 
-    def last(d, counter, sub):
+    def last(d, counter, spec):
+        # "name" = the last dispatch of the kernel, "name:N" = its last N
+        # dispatches summed (a kernel launched N times per step)
+        sub, _, n = spec.partition(":")
+        n = int(n) if n else 1
         vals = [v for k, v in _pmc_rows(d, counter) if sub in k]
-        if not vals:
-            raise SystemExit(f"no {counter} rows for kernels matching {sub!r} under {d}")
-        return vals[-1]
+        if len(vals) < n:
+            raise SystemExit(f"fewer than {n} {counter} rows for kernels matching {sub!r} under {d}")
+        return sum(vals[-n:])
 
     def biggest(d, counter):
         vals = [v for _, v in _pmc_rows(d, counter)]
@@ -705,7 +710,8 @@ def bench_ragged(a, torch):
                    "efe_macs_per_step": info["efe_macs"], "fpi_macs_per_step": info["fpi_macs"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("ragged step"),
-                     "kernel": "rg_fpi + rg_e1 + rg_e2 + rg_e3 + rg_assemble kernels (one graph, static parameters)",
+                     "kernel": "rg_fpi + rg_roll (one per level) + rg_rows stages 1, 2, 3 + rg_assemble kernels "
+                               "(one graph, static parameters)",
                      "kernel_ms": ms_events, "alg_bytes_per_launch": alg},
         "plain_per_agent": {"value": plain_rate, "unit": "agent-steps/s",
                             "sample": f"every 16th agent ({len(sample)}), batch-1 infer_states_ffi + "

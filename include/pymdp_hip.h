@@ -409,9 +409,10 @@ int pymdp_hip_ragged_plan_upload(void* plan);
 
 /* Packed layout chosen by the plan, as int64 values:
  *   [params_floats, obs_ints, out_floats, act_ints]
- *   A_off [sum M]            offset of A_m in the params arena, storage
- *                            axes in the plan's order, position-major:
- *                            [leading storage axes][O_m][last storage axis]
+ *   A_off [sum M]            offset of A_m in the params arena,
+ *                            outcome-major, storage axes in the plan's
+ *                            order: [O_m][leading storage axes][last
+ *                            storage axis]
  *   A_perm [sum M * 3]       storage axis j = position A_perm[j] of the
  *                            modality's given dependency list (-1 = unused)
  *   B_off [sum F]            B_f stored [U_f][S_deps...][S_f] (control axis

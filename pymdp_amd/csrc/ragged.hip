@@ -449,9 +449,10 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
         for (int m = 0; m < kM; ++m) {
           const int o = m < a.M ? s_obs[m] : -1;
           if (m < a.M && a.mod_group[m] == g && o >= 0 && o < a.O[m]) {
-            // A_m is position-major: row (position r, outcome o) at (r O + o) n
-            const float* Ar = b.params + a.A_off[m] + ((long long)r0 * a.O[m] + o) * n;
-            const long long rs = (long long)a.O[m] * n;
+            // A_m is outcome-major: row (outcome o, position r) at (o P + r) n
+            // with P positions (K_m / n), so the observed rows are contiguous
+            const float* Ar = b.params + a.A_off[m] + ((long long)o * (a.K[m] / n) + r0) * n;
+            const long long rs = n;
 #pragma unroll
             for (int q = 0; q < 4; ++q) v[m][q] = Ar[rl[q] * rs + kk[q]];
           } else {
@@ -635,8 +636,9 @@ __global__ __launch_bounds__(kNT) void rg_roll_kernel(Bufs b, int task0) {
 // ---------------------------------------------------------------------------
 // Stage 1 on FP64 vector FMAs, one row per thread.  Modality m's row space
 // is [O1][d01] (row e = o * d01 + pos, pos = i * d1 + j): row e of
-// [A ; sum_o A ln A] is A_m[pos, o, :] (FP32, position-major in the params
-// arena) for o < O and row pos of the modality's sum_o A ln A table (FP32)
+// [A ; sum_o A ln A] is A_m[o, pos, :] (FP32, outcome-major in the params
+// arena, so row e sits at A_m + e d2) for o < O and row pos of the
+// modality's sum_o A ln A table (FP32)
 // for o = O.  For every concatenated last-axis history n of the modality:
 //   Z1[(n * O1 + o) * d01 + pos] = row_e . q_n   (summed in FP64, stored FP32)
 // A thread loads its row (at most 64 floats) into registers with 16-byte
@@ -682,8 +684,9 @@ __device__ __forceinline__ void rg_rows(const Bufs& b, const E1Rec& r, double* s
   float* z = nullptr;
   double cval = 0.0;  // stage 3: C[o] of the row's outcome (0 for the ambiguity row)
   if (STAGE == 1) {
-    const int o = ec / d01, pos = ec - o * d01;
-    src = o < O ? b.params + r.a_off + ((long long)pos * O + o) * d2 : b.ha + r.h_off + (long long)pos * d2;
+    // rows e < O d01 are A's rows in storage order, the rest the table's
+    const long long ea = (long long)O * d01;
+    src = ec < ea ? b.params + r.a_off + (long long)ec * d2 : b.ha + r.h_off + (ec - ea) * d2;
     z = b.z1 + r.z_off + ec;
   } else if (STAGE == 2) {
     const int blk = r.blk, hb2 = ec / blk;
@@ -909,9 +912,9 @@ __global__ __launch_bounds__(kNT) void rg_ha_kernel(Bufs b) {
   const HaTask tk = b.hat[blockIdx.x];
   const int e = tk.e0 + (int)threadIdx.x;
   if (e >= tk.n) return;
-  const int pos = e / tk.d2, k = e - pos * tk.d2;
-  const long long st = tk.d2;
-  const float* col = b.params + tk.a_off + (long long)pos * tk.O * st + k;
+  // element (o, pos, k) of the outcome-major A_m at o n + e, e = pos d2 + k
+  const long long st = tk.n;
+  const float* col = b.params + tk.a_off + e;
   double s0 = 0.0, s1 = 0.0;
   int o = 0;
   for (; o + 8 <= tk.O; o += 8) {

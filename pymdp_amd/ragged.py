@@ -16,9 +16,10 @@ no counterpart in the reference; the plain per-agent entry points stay the
 drop-in (pymdp_amd.ffi).
 
 Layout: one float32 parameter arena (every agent's A, B, C, D, E at offsets
-the plan chooses; A_m is stored with its dependency axes in the plan's
-contraction order and position-major: [positions of the leading axes][O_m]
-[last axis], so that every outcome of a block of positions is contiguous;
+the plan chooses; A_m is stored outcome-major with its dependency axes in
+the plan's contraction order: [O_m][positions of the leading axes][last
+axis], so that the rows of one outcome are contiguous (the FPI reads the
+observed outcome's rows, the EFE stream reads every row in storage order);
 B_f as [U_f][dependency states][S_f]), one int32 observation arena, one float32 output arena
 (qs, G, q_pi, predicted prior per agent) and one int32 action arena.  A step allocates nothing
 and does not synchronise, so it can be captured in a HIP graph
@@ -142,11 +143,9 @@ def pack_params(plan: RaggedPlan, agents: Sequence[dict], device) -> torch.Tenso
     for ai, ag in enumerate(agents):
         for m, A in enumerate(ag["A"]):
             perm = [int(p) for p in lay["A_perm"][mi] if p >= 0]
-            # storage axes in the plan's order, then position-major:
-            # [positions of all but the last storage axis][O][last axis]
-            As = A.permute(0, *[1 + p for p in perm])
-            O_m, last = int(As.shape[0]), int(As.shape[-1])
-            At = As.reshape(O_m, -1, last).transpose(0, 1).contiguous().reshape(-1)
+            # outcome-major, then the storage axes in the plan's order:
+            # [O][positions of all but the last storage axis][last axis]
+            At = A.permute(0, *[1 + p for p in perm]).contiguous().reshape(-1)
             o = int(lay["A_off"][mi])
             arena[o:o + At.numel()] = At.to(device)
             C = ag["C"][m].reshape(-1)

@@ -233,9 +233,9 @@ class Emu:
                         continue
                     Km = int(a["K"][m])
                     assert ej.max() < Km
-                    # position-major A_m: (position, o, k) at (position O + o) n + k
+                    # outcome-major A_m: (o, position, k) at (o P + position) n + k
                     pos = (r0 + np.arange(nr))[:, None]
-                    addr = (pos * int(a["O"][m]) + o) * n + np.arange(n)[None, :]
+                    addr = (o * (Km // n) + pos) * n + np.arange(n)[None, :]
                     v = P[_chk(int(a["A_off"][m]) + addr, P.size, "A (ll)")]
                     prod *= np.maximum(v, MINVAL).astype(np.float64)
                 dst = int(u["lds"][sg]) + np.arange(nr)[:, None] * ld + np.arange(n)[None, :]
@@ -361,7 +361,8 @@ class Emu:
         # arena keeps at least 3 floats of the agent after its last A block,
         # the table 16 bytes of tail padding)
         isA = o < O
-        start = np.where(isA, int(rec["a_off"]) + (pos * O + o) * d2, int(rec["h_off"]) + pos * d2)
+        # outcome-major A_m: row e = o d01 + pos at e d2
+        start = np.where(isA, int(rec["a_off"]) + e * d2, int(rec["h_off"]) + pos * d2)
         last = start + 4 * nv - 1
         _chk(np.concatenate([start[isA], last[isA]]), self.params.size, "A row (E1)")
         _chk(np.concatenate([start[~isA], last[~isA]]), self.info["ha_floats"] + 4, "sum A ln A row (E1)")
@@ -370,9 +371,9 @@ class Emu:
         pidx = start[:, None] + np.arange(d2)[None, :]
         Xa = self.params[_chk(np.where(isA[:, None], pidx, 0), self.params.size, "A (E1)")]
         Am = self.params[_chk(int(a["A_off"][m]) + np.arange(d01 * O * d2), self.params.size, "A block (E1)")]
-        Am = Am.reshape(d01, O, d2)
+        Am = Am.reshape(O, d01, d2)
         xl = np.where(Am == 0, np.float32(0), Am * np.log(np.maximum(Am, MINVAL)).astype(np.float32))
-        ha = xl.astype(np.float64).sum(1).astype(np.float32)  # [d01, d2]
+        ha = xl.astype(np.float64).sum(0).astype(np.float32)  # [d01, d2]
         X = np.where(isA[:, None], Xa, ha[pos]).astype(np.float64)  # [rows, d2]
         # history columns, staged 16 at a time as s_q[n][k] (k < kp)
         cols = []
@@ -692,11 +693,10 @@ def check_bounds_only(plan):
         e0_, e1_ = rec["e0"].astype(np.int64), rec["e1"].astype(np.int64)
         nv4 = 4 * ((d2_ + 3) // 4)
         assert (rec["e0"] == t1["r0"]).all() and (rec["e1"] == t1["r1"]).all()
-        # A rows: the largest row of the modality that an item touches (rows of
-        # a modality are laid out [pos][O][d2], the last is pos = d01 - 1,
-        # o = O - 1)
+        # A rows: the last A row an item touches (A_m is outcome-major, row
+        # e = o d01 + pos at e d2)
         hasA = e0_ < O_ * d01_
-        last_a = rec["a_off"] + ((d01_ - 1) * O_ + O_ - 1) * d2_ + nv4 - 1
+        last_a = rec["a_off"] + (np.minimum(e1_, O_ * d01_) - 1) * d2_ + nv4 - 1
         assert (rec["a_off"] >= 0).all() and (last_a[hasA] < params_n).all()
         hasH = e1_ > O_ * d01_
         last_h = rec["h_off"] + (e1_ - 1 - O_ * d01_) * d2_ + nv4 - 1
