@@ -323,6 +323,32 @@ def bench_plain_fpi_large(a, torch):
             e32 = max(e32, float(np.abs(g - r32[f]).max()))
             e64 = max(e64, float(np.abs(g - r64[f]).max()))
 
+    # the latency floor: one more launch with the kernel's timing trace
+    # (per agent: start, log-likelihood rows formed, end of each sweep, end;
+    # wall-clock ticks of 100 MHz), outside the timed region
+    import ctypes
+    it = w["num_iter"]
+    trbuf = torch.zeros(B * (it + 3), dtype=torch.int64, device=dev)
+    _native.check(_native.lib().pymdp_hip_fpi_set_trace(ctypes.c_void_p(trbuf.data_ptr())))
+    try:
+        step()
+        torch.cuda.synchronize()
+    finally:
+        _native.check(_native.lib().pymdp_hip_fpi_set_trace(None))
+    t = trbuf.cpu().numpy().reshape(B, it + 3).astype(np.float64) / 100.0  # us
+    t -= t[:, 0].min()
+    sweeps = np.diff(t[:, 1:2 + it], axis=1)
+    starts = np.sort(t[:, 0])
+    sweep_trace = {
+        "unit": "us", "kernel_span": float(t[:, 2 + it].max()),
+        "loglik_rows_mean": float((t[:, 1] - t[:, 0]).mean()),
+        "sweep_mean": float(sweeps.mean()), "sweep_median": float(np.median(sweeps)),
+        "sweep_min": float(sweeps.min()), "sweep_max": float(sweeps.max()),
+        "agent_mean": float((t[:, 2 + it] - t[:, 0]).mean()),
+        "agents_started_within_1us": int((starts < starts[0] + 1.0).sum()),
+        "how": "pymdp_hip_fpi_set_trace on one launch after the timed region",
+    }
+
     cpu = None
     if not a.no_cpu_baseline:
         cpu = cpu_baseline_fpi(w)
@@ -345,6 +371,7 @@ def bench_plain_fpi_large(a, torch):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic("fpi plain_fpi_large"),
                      "kernel": kernel, "kernel_ms": ms_events, "alg_bytes_per_launch": alg_bytes},
         "parity": {"agents_checked": 3, "max_abs_err_vs_fp32_ref": e32, "max_abs_err_vs_fp64": e64},
+        "sweep_trace": sweep_trace,
         "cpu_baseline": cpu,
     }
     return res
@@ -880,7 +907,7 @@ def bench_headline(a, torch):
     res = bench_ragged(a, torch)
     res["metric"] = HEADLINE_METRIC
     keep = ("value", "unit", "ms_per_step", "steps", "config", "roofline", "parity", "cpu_baseline",
-            "autograd_baseline", "speedup_vs_autograd", "grads_per_s")
+            "autograd_baseline", "speedup_vs_autograd", "grads_per_s", "sweep_trace")
     b = copy.copy(a)
     b.steps, b.warmup = 50, 5
     _log("plain_fpi_small")

@@ -90,6 +90,13 @@ int pymdp_hip_fpi(const float* ll_flat, const float* lp_flat, float* q_out,
                   const int64_t* A_dep_flat, const int64_t* A_dep_offsets,
                   int32_t num_iter, void* stream);
 
+/* Diagnostic timing trace of the fast FPI kernel (no counterpart in the
+ * reference): a device buffer of batch * (num_iter + 3) int64 values that
+ * later launches of the one-dependency-tuple fast path fill with wall-clock
+ * ticks (100 MHz) per agent: start, log-likelihood rows formed, end of each
+ * sweep, end.  NULL switches it off.  Process-wide, not for concurrent
+ * callers. */
+int pymdp_hip_fpi_set_trace(int64_t* trace);
 /*
  * Fused state inference: log-likelihood build + FPI in one launch.
  * Replaces the body of inference.update_posterior_states's one-step `fpi`

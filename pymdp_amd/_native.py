@@ -152,6 +152,7 @@ SIGNATURES: dict[str, tuple] = {
     "pymdp_hip_ragged_status": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
     "pymdp_hip_ragged_set_prior_source": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "pymdp_hip_ragged_set_trace": (ctypes.c_int, [_vp, _vp]),
+    "pymdp_hip_fpi_set_trace": (ctypes.c_int, [_vp]),
     "pymdp_hip_ragged_plan_destroy": (ctypes.c_int, [_vp]),
 }
 

@@ -86,6 +86,10 @@ struct FpiPtrs {
   double* ws;          // global workspace (JMAX == 0 only)
   const float* A[kMaxM];
   const float* prior[kMaxF];  // mode 1: per-factor prior probabilities
+  // optional timing trace of fpi_fast_kernel (pymdp_hip_fpi_set_trace):
+  // per agent iters + 3 wall-clock ticks (100 MHz): start, log-likelihood
+  // rows formed, end of each sweep, end
+  long long* trace;
 };
 
 // Rows of a group's stage whose column position is c: prod_{j<c} dim[j].
@@ -393,7 +397,9 @@ __device__ void lds_stage(const double* X, int rows, int n, const double* qc, co
   }
 }
 
-template <int N0, int RPL, int NT>
+// TRACE: the launch records the timing trace (a separate instantiation, so
+// that ordinary launches carry no trace code in the sweep loop).
+template <int N0, int RPL, int NT, bool TRACE = false>
 __global__ __launch_bounds__(NT) void fpi_fast_kernel(const FpiDesc d, const FpiPtrs p) {
   constexpr int kFastNW = NT / kWave;
   constexpr int LGN = N0 == 4 ? 2 : N0 == 8 ? 3 : 4;  // log2(N0)
@@ -402,6 +408,13 @@ __global__ __launch_bounds__(NT) void fpi_fast_kernel(const FpiDesc d, const Fpi
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
   const int wave = tid >> 6;
+  // timing trace (diagnostic), written with relaxed atomic stores
+  long long* const tr = TRACE ? p.trace + (long long)b * (d.iters + 3) : nullptr;
+  auto stamp = [&](int i) {
+    if (TRACE && tid == 0)
+      __hip_atomic_store(tr + i, (long long)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  };
+  stamp(0);
 
   double* ltab = smd;  // ln / reciprocal tables for log_f64d
   double* log_q = ltab + 256;
@@ -514,6 +527,10 @@ __global__ __launch_bounds__(NT) void fpi_fast_kernel(const FpiDesc d, const Fpi
     }
   }
 
+  if (TRACE) {
+    __syncthreads();  // (trace only) the rows' loads have landed
+    stamp(1);
+  }
   // ---- Jacobi sweeps ---------------------------------------------------------
   const int f_last = d.fac[0][R - 1];
   const int f_prev = (R >= 2) ? d.fac[0][R - 2] : 0;
@@ -605,11 +622,13 @@ __global__ __launch_bounds__(NT) void fpi_fast_kernel(const FpiDesc d, const Fpi
       src = dst;
     }
     __syncthreads();
+    stamp(2 + it);
   }
 
   softmax_factors<kFastNW>(d, log_q, qv, lane, wave);
   __syncthreads();
   for (int i = tid; i < d.totS; i += NT) p.q[(long long)b * d.totS + i] = (float)qv[i];
+  stamp(2 + d.iters);
 }
 
 // ---------------------------------------------------------------------------
@@ -817,7 +836,10 @@ static int launch_fast(const FpiDesc& d, const FpiPtrs& p, const FastChoice& fc,
   const size_t lds = fast_lds(d, fc.N0, fc.NT);
 #define PMH_FAST(n0, rpl, nt)                                                                    \
   if (fc.N0 == n0 && fc.RPL == rpl && fc.NT == nt) {                                             \
-    hipLaunchKernelGGL((fpi_fast_kernel<n0, rpl, nt>), grid, dim3(nt), lds, stream, d, p);       \
+    if (p.trace)                                                                                 \
+      hipLaunchKernelGGL((fpi_fast_kernel<n0, rpl, nt, true>), grid, dim3(nt), lds, stream, d, p); \
+    else                                                                                         \
+      hipLaunchKernelGGL((fpi_fast_kernel<n0, rpl, nt>), grid, dim3(nt), lds, stream, d, p);     \
     return 0;                                                                                    \
   }
   PMH_FAST_INSTANCES(PMH_FAST)
@@ -885,10 +907,15 @@ static int autotune_fast(const FpiDesc& d, const FpiPtrs& p, const std::vector<F
 // fixed_ws / fixed_cap: a caller-provided workspace (the allocation-free
 // entry point's fixed scratch) instead of the library's stream scratch;
 // shapes that need more than fixed_cap bytes are rejected.
+// Diagnostic timing trace of the fast FPI kernel (pymdp_hip_fpi_set_trace):
+// later launches of fpi_fast_kernel record per-agent timestamps into it.
+static long long* g_fpi_trace = nullptr;
+
 static int launch_fpi(const FpiDesc& d0, FpiPtrs p, int /*unused*/, int64_t batch, hipStream_t stream,
                       double* fixed_ws = nullptr, size_t fixed_cap = 0) {
   const char* pfx = "fpi_ffi: ";
   FpiDesc d = d0;
+  p.trace = g_fpi_trace;
   if (batch == 0) return 0;
   if (ensure_log_table()) {
     set_error("fpi_ffi: cannot initialise the log table");
@@ -1120,6 +1147,11 @@ static int infer_states_impl(const float* const* A_ptrs, const int64_t* A_batch_
     d.prior_stride[f] = prior_batch_stride[f];
   }
   return launch_fpi(d, p, jslots, batch, (hipStream_t)stream, fixed_ws, fixed_cap);
+}
+
+extern "C" int pymdp_hip_fpi_set_trace(int64_t* trace) {
+  g_fpi_trace = reinterpret_cast<long long*>(trace);
+  return PYMDP_HIP_OK;
 }
 
 extern "C" int pymdp_hip_infer_states(const float* const* A_ptrs, const int64_t* A_batch_stride,

@@ -204,3 +204,29 @@ def test_algos_unbatched_entry(dev):
     qv = algos.run_vanilla_fpi(A1, obs1, D1, num_iter=16)
     for a, b in zip(qv, q):
         assert torch.allclose(a, b, atol=1e-6)
+
+
+def test_fast_kernel_timing_trace(dev):
+    """The fast kernel's diagnostic trace leaves the posterior bitwise
+    unchanged and records, per agent, non-decreasing stamps: start,
+    log-likelihood rows formed, the end of each sweep, end."""
+    import ctypes
+
+    from pymdp_amd import _native
+    w = workloads.plain_fpi_large(seed=2, device=dev, batch=64)
+    B, it = w["batch"], w["num_iter"]
+    obs = torch.stack([o.to(torch.int32) for o in w["obs"]], -1).contiguous()
+    ref = [q.clone() for q in infer_states_ffi(w["A"], obs, w["D"], w["A_dependencies"], it, distr_obs=False)]
+    tr = torch.zeros(B * (it + 3), dtype=torch.int64, device=dev)
+    L = _native.lib()
+    _native.check(L.pymdp_hip_fpi_set_trace(ctypes.c_void_p(tr.data_ptr())))
+    try:
+        got = infer_states_ffi(w["A"], obs, w["D"], w["A_dependencies"], it, distr_obs=False)
+        torch.cuda.synchronize()
+    finally:
+        _native.check(L.pymdp_hip_fpi_set_trace(None))
+    assert "fpi_fast_kernel" in L.pymdp_hip_last_launch().decode()
+    for a, b in zip(ref, got):
+        assert torch.equal(a, b)
+    t = tr.cpu().numpy().reshape(B, it + 3)
+    assert (t > 0).all() and (np.diff(t, axis=1) >= 0).all()
