@@ -99,9 +99,12 @@ struct Agent {
 
 // One FPI unit: up to kM segments, each a row range [r0, r1) of the stage-0
 // matrix [rows x n] (n = last storage dim) of dependency group g, held in
-// LDS at offset lds with row stride n | 1.
+// LDS at offset lds with row stride n | 1.  x32 = 0: the log-likelihood is
+// held as doubles (kCap of them); x32 = 1: as floats (2 kCap), for agents
+// whose log-likelihood does not fit one unit as doubles (offsets then count
+// floats).
 struct Unit {
-  int agent, slice, nseg, pad;
+  int agent, slice, nseg, x32;
   int g[kM], r0[kM], r1[kM], lds[kM];
 };
 
@@ -242,8 +245,10 @@ __device__ __forceinline__ int rows_lg(int n) { return n <= 8 ? 0 : ceil_log2(n)
 //   R = 2: rows i: m_1[k] += sum_i q_0[i] X[i, k];  m_0[i] += X[i, :] . q_1
 //   R = 3: rows r = i*d1 + j: m_2[k] += sum_r q_0[i] q_1[j] X[r, k];
 //          y_r = X[r, :] . q_2;  m_1[j] += sum_i q_0[i] y_ij;  m_0[i] += sum_j q_1[j] y_ij
-// Two barriers; every sum has a fixed order (bitwise reproducible).
-__device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const double* X, double* s_y,
+// Two barriers; every sum has a fixed order (bitwise reproducible).  X is
+// double or float (the unit's x32); products and sums are FP64 either way.
+template <typename TX>
+__device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const TX* X, double* s_y,
                                double* s_red, const double* s_qv, double* s_part, int tid) {
   const int R = a.gR[g];
   const int n = a.gdim[g][R - 1], ld = n | 1;
@@ -262,16 +267,16 @@ __device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const doub
     const int tpr = 1 << lt;
     for (int e = tid; e < nr * tpr; e += kNT) {
       const int rl = e >> lt, sub = e & (tpr - 1);
-      const double* x = X + rl * ld;
+      const TX* x = X + rl * ld;
       double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
       int k = sub;
       for (; k + 3 * tpr < n; k += 4 * tpr) {
-        v0 = fma(x[k], qc[k], v0);
-        v1 = fma(x[k + tpr], qc[k + tpr], v1);
-        v2 = fma(x[k + 2 * tpr], qc[k + 2 * tpr], v2);
-        v3 = fma(x[k + 3 * tpr], qc[k + 3 * tpr], v3);
+        v0 = fma((double)x[k], qc[k], v0);
+        v1 = fma((double)x[k + tpr], qc[k + tpr], v1);
+        v2 = fma((double)x[k + 2 * tpr], qc[k + 2 * tpr], v2);
+        v3 = fma((double)x[k + 3 * tpr], qc[k + 3 * tpr], v3);
       }
-      for (; k < n; k += tpr) v0 = fma(x[k], qc[k], v0);
+      for (; k < n; k += tpr) v0 = fma((double)x[k], qc[k], v0);
       double v = group_sum_low_dpp((v0 + v1) + (v2 + v3), lt);
       if (sub == 0) {
         if (R == 2)
@@ -291,16 +296,16 @@ __device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const doub
   if (rg < nrg) {
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
     if (R == 1) {
-      if (rg == 0) acc0 = X[col];
+      if (rg == 0) acc0 = (double)X[col];
     } else if (R == 2) {
       int rl = rg;
       for (; rl + 3 * nrg < nr; rl += 4 * nrg) {
-        acc0 = fma(q0[r0 + rl], X[rl * ld + col], acc0);
-        acc1 = fma(q0[r0 + rl + nrg], X[(rl + nrg) * ld + col], acc1);
-        acc2 = fma(q0[r0 + rl + 2 * nrg], X[(rl + 2 * nrg) * ld + col], acc2);
-        acc3 = fma(q0[r0 + rl + 3 * nrg], X[(rl + 3 * nrg) * ld + col], acc3);
+        acc0 = fma(q0[r0 + rl], (double)X[rl * ld + col], acc0);
+        acc1 = fma(q0[r0 + rl + nrg], (double)X[(rl + nrg) * ld + col], acc1);
+        acc2 = fma(q0[r0 + rl + 2 * nrg], (double)X[(rl + 2 * nrg) * ld + col], acc2);
+        acc3 = fma(q0[r0 + rl + 3 * nrg], (double)X[(rl + 3 * nrg) * ld + col], acc3);
       }
-      for (; rl < nr; rl += nrg) acc0 = fma(q0[r0 + rl], X[rl * ld + col], acc0);
+      for (; rl < nr; rl += nrg) acc0 = fma(q0[r0 + rl], (double)X[rl * ld + col], acc0);
     } else {
       // two rows per step: (i, j) of rows rl and rl + nrg advance by 2 nrg
       const int di = nrg / d1, dj = nrg - di * d1;
@@ -313,8 +318,8 @@ __device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const doub
       const int ei = (2 * nrg) / d1, ej = 2 * nrg - ei * d1;
       int rl = rg;
       for (; rl + nrg < nr; rl += 2 * nrg) {
-        acc0 = fma(q0[i] * q1[j], X[rl * ld + col], acc0);
-        acc1 = fma(q0[i2] * q1[j2], X[(rl + nrg) * ld + col], acc1);
+        acc0 = fma(q0[i] * q1[j], (double)X[rl * ld + col], acc0);
+        acc1 = fma(q0[i2] * q1[j2], (double)X[(rl + nrg) * ld + col], acc1);
         i += ei;
         j += ej;
         if (j >= d1) {
@@ -328,7 +333,7 @@ __device__ void rg_fpi_segment(const Agent& a, int g, int r0, int nr, const doub
           ++i2;
         }
       }
-      if (rl < nr) acc0 = fma(q0[i] * q1[j], X[rl * ld + col], acc0);
+      if (rl < nr) acc0 = fma(q0[i] * q1[j], (double)X[rl * ld + col], acc0);
     }
     s_red[tid] = (acc0 + acc1) + (acc2 + acc3);
   }
@@ -431,6 +436,7 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
       const int n = a.gdim[g][R - 1], ld = n | 1;
       const int r0 = u.r0[sg], nr = u.r1[sg] - r0;
       double* X = s_x + u.lds[sg];
+      float* X32 = reinterpret_cast<float*>(s_x) + u.lds[sg];
       // four elements per thread and every modality of the group in flight:
       // all loads of a batch are issued (unconditionally, at clamped
       // addresses; the modality guards are uniform) before any product is
@@ -470,7 +476,11 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
           const int e = e0 + q * kNT;
           if (e < nr * n) {
             const int rl = e / n, k = e - rl * n;
-            X[rl * ld + k] = log_f64d(prod[q], s_ltab);
+            const double l = log_f64d(prod[q], s_ltab);
+            if (u.x32)
+              X32[rl * ld + k] = (float)l;
+            else
+              X[rl * ld + k] = l;
           }
         }
       }
@@ -486,8 +496,13 @@ __global__ __launch_bounds__(kNT) void rg_fpi_kernel(Bufs b) {
       else
         rg_softmax(a, s_logq, nullptr, s_qv, lane, wave);
       __syncthreads();
-      for (int sg = 0; sg < u.nseg; ++sg)
-        rg_fpi_segment(a, u.g[sg], u.r0[sg], u.r1[sg] - u.r0[sg], s_x + u.lds[sg], s_y, s_red, s_qv, s_part, tid);
+      for (int sg = 0; sg < u.nseg; ++sg) {
+        if (u.x32)
+          rg_fpi_segment(a, u.g[sg], u.r0[sg], u.r1[sg] - u.r0[sg], reinterpret_cast<const float*>(s_x) + u.lds[sg],
+                         s_y, s_red, s_qv, s_part, tid);
+        else
+          rg_fpi_segment(a, u.g[sg], u.r0[sg], u.r1[sg] - u.r0[sg], s_x + u.lds[sg], s_y, s_red, s_qv, s_part, tid);
+      }
       if (a.team > 1) {
         // exchange partial marginals with the other units of this agent
         // (double-buffered by sweep parity: a unit overwrites its slot of
@@ -1411,43 +1426,57 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
     }
     a.iters = (F == 1 && num_iter > 0) ? 1 : num_iter;
     // FPI units: segments of the groups' stage-0 rows, packed into LDS slices
+    // of cap elements (kCap doubles; an agent that needs more than one such
+    // unit is packed again into slices of 2 kCap floats, x32 = 1: half as
+    // many units exchanging partial marginals every sweep)
     {
-      std::vector<Unit> us;
-      Unit cur;
-      std::memset(&cur, 0, sizeof(cur));
-      int used = 0;
       double fc = 0.0;
-      auto flush = [&]() {
-        if (cur.nseg > 0) {
-          us.push_back(cur);
-          std::memset(&cur, 0, sizeof(cur));
-          used = 0;
-        }
-      };
       for (int g = 0; g < a.G; ++g) {
         const int R = a.gR[g];
-        const int n = a.gdim[g][R - 1], ld = n | 1;
         int rows = 1;
         for (int j = 0; j + 1 < R; ++j) rows *= a.gdim[g][j];
-        fc += (double)rows * n * (1.0 + 2.0 * a.iters);
-        int r = 0;
-        while (r < rows) {
-          const int cap = std::min((kCap - used) / ld, kRowsCap);
-          if (cap <= 0 || cur.nseg == kM) {
-            flush();
-            continue;
-          }
-          const int take = std::min(rows - r, cap);
-          cur.g[cur.nseg] = g;
-          cur.r0[cur.nseg] = r;
-          cur.r1[cur.nseg] = r + take;
-          cur.lds[cur.nseg] = used;
-          cur.nseg++;
-          used += take * ld;
-          r += take;
-        }
+        fc += (double)rows * a.gdim[g][R - 1] * (1.0 + 2.0 * a.iters);
       }
-      flush();
+      auto pack = [&](int cap_el, int x32) {
+        std::vector<Unit> us;
+        Unit cur;
+        std::memset(&cur, 0, sizeof(cur));
+        int used = 0;
+        auto flush = [&]() {
+          if (cur.nseg > 0) {
+            cur.x32 = x32;
+            us.push_back(cur);
+            std::memset(&cur, 0, sizeof(cur));
+            used = 0;
+          }
+        };
+        for (int g = 0; g < a.G; ++g) {
+          const int R = a.gR[g];
+          const int n = a.gdim[g][R - 1], ld = n | 1;
+          int rows = 1;
+          for (int j = 0; j + 1 < R; ++j) rows *= a.gdim[g][j];
+          int r = 0;
+          while (r < rows) {
+            const int cap = std::min((cap_el - used) / ld, kRowsCap);
+            if (cap <= 0 || cur.nseg == kM) {
+              flush();
+              continue;
+            }
+            const int take = std::min(rows - r, cap);
+            cur.g[cur.nseg] = g;
+            cur.r0[cur.nseg] = r;
+            cur.r1[cur.nseg] = r + take;
+            cur.lds[cur.nseg] = used;
+            cur.nseg++;
+            used += take * ld;
+            r += take;
+          }
+        }
+        flush();
+        return us;
+      };
+      std::vector<Unit> us = pack(kCap, 0);
+      if (us.size() > 1) us = pack(2 * kCap, 1);
       if (us.empty()) return fail("no FPI work");
       if ((int)us.size() > kTeamMax) return fail("log-likelihood too large for one agent (more than 128 units)");
       a.team = (int)us.size();

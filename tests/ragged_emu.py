@@ -39,7 +39,7 @@ AGENT_DT = np.dtype([
     ("pred_off", "i8"),
 ], align=True)
 
-UNIT_DT = np.dtype([("agent", "i4"), ("slice", "i4"), ("nseg", "i4"), ("pad", "i4"), ("g", "i4", (kM,)),
+UNIT_DT = np.dtype([("agent", "i4"), ("slice", "i4"), ("nseg", "i4"), ("x32", "i4"), ("g", "i4", (kM,)),
                     ("r0", "i4", (kM,)), ("r1", "i4", (kM,)), ("lds", "i4", (kM,))])
 E1REC_DT = np.dtype([("a_off", "i8"), ("h_off", "i8"), ("z_off", "i8"), ("hb", "i8", (kT,)), ("ts", "i4", (kT + 1,)),
                      ("O", "i4"), ("d2", "i4"), ("d01", "i4"), ("e0", "i4"), ("e1", "i4"), ("N1", "i4"),
@@ -214,8 +214,12 @@ class Emu:
             lp[S_off[f]:S_off[f] + S[f]] = np.log(np.maximum(d, MINVAL).astype(np.float64))
         o_idx = self.obs[_chk(int(a["obs_off"]) + np.arange(M), self.obs.size, "obs")]
         lds = []
+        # x32 units hold the log-likelihood as floats, 2 kCap of them
+        x32 = int(units[0]["x32"])
+        assert all(int(u["x32"]) == x32 for u in units) and x32 in (0, 1)
+        cap = 2 * kCap if x32 else kCap
         for u in units:
-            x = np.full(kCap, np.nan)
+            x = np.full(cap, np.nan)
             for sg in range(int(u["nseg"])):
                 g = int(u["g"][sg])
                 R = int(a["gR"][g])
@@ -239,7 +243,8 @@ class Emu:
                     v = P[_chk(int(a["A_off"][m]) + addr, P.size, "A (ll)")]
                     prod *= np.maximum(v, MINVAL).astype(np.float64)
                 dst = int(u["lds"][sg]) + np.arange(nr)[:, None] * ld + np.arange(n)[None, :]
-                x[_chk(dst, kCap, "s_x (ll)")] = np.log(prod)
+                ll = np.log(prod)
+                x[_chk(dst, cap, "s_x (ll)")] = ll.astype(np.float32).astype(np.float64) if x32 else ll
             lds.append(x)
 
         def softmax(src):
@@ -261,7 +266,7 @@ class Emu:
                     base = int(u["lds"][sg])
 
                     def X(idx, base=base, x=x):
-                        v = x[_chk(base + idx, kCap, "s_x")]
+                        v = x[_chk(base + idx, cap, "s_x")]
                         assert np.isfinite(v).all(), "read of an unset LDS log-likelihood entry"
                         return v
 
@@ -605,7 +610,7 @@ def check_units(ag, units, info):
             r0, r1 = int(u["r0"][sg]), int(u["r1"][sg])
             assert 0 <= r0 < r1 <= rows and r1 - r0 <= kRowsCap
             lo, hi = int(u["lds"][sg]), int(u["lds"][sg]) + (r1 - r0) * ld
-            assert 0 <= lo and hi <= kCap
+            assert 0 <= lo and hi <= (2 * kCap if int(u["x32"]) else kCap)
             used.append((lo, hi))
             seen.setdefault((ai, g), []).append((r0, r1))
         used.sort()
@@ -623,6 +628,18 @@ def check_units(ag, units, info):
                 assert x1 == y0
         if int(a["team"]) > 1:
             assert int(a["part_off"]) + 2 * int(a["team"]) * int(a["totS"]) <= info["part_doubles"]
+        # float slices only for an agent whose stage-0 rows do not fit one
+        # slice of kCap doubles
+        x32 = {int(u["x32"]) for u in units if int(u["agent"]) == ai}
+        assert len(x32) == 1
+        if x32 == {1}:
+            need, longest = 0, 0
+            for g in range(int(a["G"])):
+                R = int(a["gR"][g])
+                rows = int(np.prod(a["gdim"][g][:R - 1])) if R > 1 else 1
+                need += rows * (int(a["gdim"][g][R - 1]) | 1)
+                longest = max(longest, rows)
+            assert need > kCap or longest > kRowsCap or int(a["G"]) > kM, ai
 
 
 def check_bounds_only(plan):

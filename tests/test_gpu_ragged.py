@@ -11,6 +11,7 @@ import torch
 from oracle import dedup_ref as DR
 from oracle import numpy_ref as R
 from pymdp_amd import ragged, workloads
+from tests import ragged_emu as EM
 from tests.test_ragged_host import _oracle_rollout
 
 pytestmark = pytest.mark.gpu
@@ -157,6 +158,23 @@ def test_ragged_full_batch(dev):
                                                                        for a in w["agents"]]))]
     for i in sample:
         _check_agent(rb, int(i), use_dedup=True)
+    # every agent whose FPI units hold the log-likelihood as floats (those
+    # too large for one unit of doubles): qs to the same bar
+    _, units, _, _ = EM.tables(rb.plan)
+    x32 = sorted({int(u["agent"]) for u in units if int(u["x32"])})
+    assert len(x32) > 100
+    worst = [0.0, 0.0]
+    for i in x32:
+        a = w["agents"][i]
+        A = [_np(x) for x in a["A"]]
+        D = [_np(x) for x in a["D"]]
+        ob = [np.eye(o, dtype=np.float32)[x] for o, x in zip(a["O"], a["obs"])]
+        q32 = R.run_factorized_fpi(A, ob, D, a["A_dependencies"], num_iter=16, dt=np.float32)
+        q64 = R.run_factorized_fpi(A, ob, D, a["A_dependencies"], num_iter=16, dt=np.float64)
+        for q, y32, y64 in zip(rb.agent_outputs(i)[0], q32, q64):
+            e = _bar(_np(q), y32, y64)
+            worst = [max(worst[0], e[0]), max(worst[1], e[1])]
+    print(f"ragged full, {len(x32)} float-slice agents: qs max|x-x32|={worst[0]:.3e} max|x-x64|={worst[1]:.3e}")
 
 
 def test_ragged_graph_replay_and_autotune(dev):

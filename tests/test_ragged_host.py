@@ -162,13 +162,15 @@ def test_plan_layout_is_disjoint():
 
 def test_emulated_teams_match_oracle():
     """Agents whose log-likelihood spans several FPI units (a team exchanging
-    partial marginals every sweep), with rank-1, -2 and -3 modalities."""
+    partial marginals every sweep, the log-likelihood held as floats), with
+    rank-1, -2 and -3 modalities."""
     w = workloads.ragged(seed=0, n_agents=5, state_range=(20, 40), obs_range=(2, 6), max_policies=64,
                          max_A_bytes=1 << 22, max_modalities=3)
     ags = w["agents"]
     plan, em = _emulate(ags)
     teams = [int(a["team"]) for a in em.ag]
     assert max(teams) > 2, teams
+    assert {int(u["x32"]) for u in em.units} == {0, 1}
     ranks = {len(d) for a in ags for d in a["A_dependencies"]}
     assert {1, 2, 3} <= ranks, ranks
     lay = plan.layout
