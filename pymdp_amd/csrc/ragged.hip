@@ -1144,15 +1144,31 @@ static void free_plan(Plan* p) {
 }
 
 // Stage costs of contracting axes last-to-first in storage order `ax`.
+// Row length as the row kernels and the FPI sweeps pay for it: whole 4-float
+// vectors, and at least 8 (a row of 2 costs a thread and its loads, stores
+// and reductions nearly as much as a row of 8).
+static double row_len(int d) { return std::max(8.0, 4.0 * ((d + 3) / 4)); }
+
+// Cost of one level's EFE stages for a storage order (dims outermost
+// first, Hs the level's history counts per axis, O outcomes): stage j
+// contracts axis j with its Hs[j] histories over rows of that axis.
 static double efe_cost(const std::vector<int>& dims, const std::vector<int>& Hs, int O) {
   double rows = O + 1.0, c = 0.0, mult = 1.0;
   for (int d : dims) rows *= d;
   for (int j = (int)dims.size() - 1; j >= 0; --j) {
     mult *= Hs[j];
-    c += mult * rows;
     rows /= dims[j];
+    c += mult * rows * row_len(dims[j]);
   }
   return c + mult * O;
+}
+
+// Cost of the FPI sweeps for a storage order: two passes per sweep over the
+// rows of the last axis.
+static double fpi_cost(const std::vector<int>& dims, int iters) {
+  double rows = 1.0;
+  for (size_t j = 0; j + 1 < dims.size(); ++j) rows *= dims[j];
+  return 2.0 * iters * rows * row_len(dims.back());
 }
 
 }  // namespace rg
@@ -1393,7 +1409,7 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
         do {
           std::vector<int> dims;
           for (int x : perm) dims.push_back(a.S[x]);
-          double c = 0;
+          double c = fpi_cost(dims, num_iter);
           for (int t = 0; t < T; ++t) {
             std::vector<int> Hs;
             for (int x : perm) Hs.push_back(hp.fac[t][x].n);
@@ -1485,13 +1501,19 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
         us[i].slice = i;
       }
       pl->fpi_macs += fc;
+      // queue order (longest first) by estimated time: every segment of a
+      // unit costs about as much per sweep as 800 log-likelihood elements
+      // (its barriers, reductions and setup), which dominates small units
+      int nseg = 0;
+      for (const Unit& un : us) nseg += un.nseg;
+      const double est = fc + 800.0 * nseg * (1.0 + 2.0 * a.iters);
       if (a.team > 1) {
         a.part_off = part;
         part += 2LL * a.team * a.totS;
-        teams.push_back(UnitC{us, fc});
+        teams.push_back(UnitC{us, est});
       } else {
         a.part_off = 0;
-        singles.push_back(UnitC{us, fc});
+        singles.push_back(UnitC{us, est});
       }
       pl->team_max = std::max(pl->team_max, a.team);
     }
