@@ -883,11 +883,11 @@ __device__ __forceinline__ void rg_score_pairs(const Bufs& b, const E1Rec& r, do
 
 // One work item per workgroup (stage 1: records b.e1, stage 2: b.e2, stage 3:
 // b.e3, always 256 threads so that an item holds at least one pair's 65
-// rows).  At
-// least four waves per SIMD (at most 128 VGPRs): the row of a thread and the
-// loads in flight per CU are what set the stage's HBM rate.
+// rows).  Six
+// waves per SIMD (at most 80 VGPRs, no spills; eight spill): the row of a
+// thread and the loads in flight per CU are what set the stage's HBM rate.
 template <int STAGE, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void rg_rows_kernel(Bufs b) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(6))) void rg_rows_kernel(Bufs b) {
   __shared__ double s_q[kE1Cols * kE1Ld];
   __shared__ double s_t[STAGE == 3 ? kE3Cols * NT : 1];  // stage 3: per-row score terms [column][row]
   __shared__ E1Rec s_rec;
