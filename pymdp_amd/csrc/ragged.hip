@@ -22,8 +22,8 @@
 //   rg_rows_kernel<2>   stage 2, the same row streaming over the stage-1
 //                       output: the middle axis contracted with the middle
 //                       factor's histories of one level.
-//   rg_e3_kernel        the first axis contracted with the first factor's
-//                       histories and every history combination scored:
+//   rg_rows_kernel<3>   stage 3, the first axis contracted with the first
+//                       factor's histories and every history combination scored:
 //                       entropy of qo + ambiguity + qo . C
 //                       (pymdp/control.py:420-509).
 //   rg_assemble_kernel  one workgroup per agent: G[p] = sum_t sum_m of the
