@@ -1493,6 +1493,19 @@ extern "C" int pymdp_hip_ragged_plan_create(int64_t n_agents, const int64_t* hdr
       };
       std::vector<Unit> us = pack(kCap, 0);
       if (us.size() > 1) us = pack(2 * kCap, 1);
+      if (us.size() > 1) {
+        // a team waits every sweep for its largest unit: re-pack with the
+        // smallest slice (in steps of 1/64 of a unit) that keeps the unit
+        // count, so the units carry about equal shares
+        const size_t k = us.size();
+        for (int c = 2 * kCap / 2; c < 2 * kCap; c += 2 * kCap / 64) {
+          std::vector<Unit> v = pack(c, 1);
+          if (v.size() <= k) {
+            us = std::move(v);
+            break;
+          }
+        }
+      }
       if (us.empty()) return fail("no FPI work");
       if ((int)us.size() > kTeamMax) return fail("log-likelihood too large for one agent (more than 128 units)");
       a.team = (int)us.size();
